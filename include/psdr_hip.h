@@ -1,0 +1,195 @@
+/*
+ * psdr_hip.h — C ABI of the MI355X (gfx950) differentiable path tracer.
+ *
+ * This is the drop-in boundary for torch_psdr's hot path.  The reference binds
+ * its C++ classes to Python with pybind11 (module `pypsdr`, src/psdr.cc:15-137);
+ * these entry points replace that module for the render path.  All buffers are
+ * device pointers owned by the caller (torch tensors); the library owns only what
+ * it derives from them (triangle records, edges, samplers, LBVH, wavefront
+ * queues) inside an opaque scene handle.  Every call is asynchronous on the
+ * given HIP stream (NULL = the null stream) and never synchronises the host
+ * unless stated.  Every call returns 0 on success or a negative code; the text
+ * of the last error on the calling thread is returned by psdr_last_error().
+ * No C++ exception crosses this boundary.
+ */
+#ifndef PSDR_HIP_H
+#define PSDR_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PSDR_ABI_VERSION 1
+
+/* status codes */
+#define PSDR_OK 0
+#define PSDR_ERR_ARG -1      /* bad argument (shape, size, null pointer) */
+#define PSDR_ERR_HIP -2      /* a HIP runtime call failed */
+#define PSDR_ERR_STATE -3    /* call out of order (e.g. backward without forward) */
+#define PSDR_ERR_NOMEM -4    /* device allocation failed */
+
+/* Material kinds (src/material.h:24-58). */
+#define PSDR_MAT_DIFFUSE 0      /* DiffuseMaterial: albedo texture / pi      */
+#define PSDR_MAT_DIFFUSE_BSDF 1 /* DiffuseBsdfMaterial: color, roughness     */
+#define PSDR_MAT_LIGHT 2        /* DiffuseLightMaterial of an AreaLight mesh */
+
+/* One object = one mesh (scene objects are meshes first, then the meshes of area
+ * lights, as in Scene::Scene, src/scene.cc:8-20).  Textures live in one packed
+ * texel buffer; a texture with h == w == 0 is constant (ConstantTexture). */
+typedef struct psdr_object {
+  int32_t material;        /* PSDR_MAT_* */
+  int32_t light;           /* index into the light arrays, or -1 */
+  int32_t albedo_offset;   /* float offset of the 3-channel albedo / color texture */
+  int32_t albedo_h, albedo_w;
+  int32_t rough_offset;    /* float offset of the 1-channel roughness texture */
+  int32_t rough_h, rough_w;
+  int32_t tri_begin, tri_count; /* triangle range of this object */
+  int32_t grad_mask;       /* PSDR_GRAD_* bits: which of this object's inputs need gradients */
+} psdr_object;
+
+/* grad_mask bits (mirror `requires_grad` of the wrapped tensors,
+ * src/python/torch_psdr.py:84-103) */
+#define PSDR_GRAD_VERTICES 1
+#define PSDR_GRAD_ALBEDO 2
+#define PSDR_GRAD_ROUGHNESS 4
+#define PSDR_GRAD_EMIT 8
+
+/* Scene description.  Device arrays: vertices, faces, tri_uv, tri_obj, texels,
+ * light_emit.  Host arrays: objects, light_obj. */
+typedef struct psdr_scene_desc {
+  int32_t n_vertices;
+  const float* vertices;   /* [n_vertices*3]                           */
+  int32_t n_tris;
+  const int32_t* faces;    /* [n_tris*3] global vertex ids             */
+  const float* tri_uv;     /* [n_tris*6] corner uvs                    */
+  const int32_t* tri_obj;  /* [n_tris]                                 */
+  int32_t n_objects;
+  const psdr_object* objects; /* host [n_objects]                     */
+  int32_t n_texels;        /* floats in texels                          */
+  const float* texels;
+  int32_t n_lights;
+  const float* light_emit; /* [n_lights*3]                              */
+  const int32_t* light_obj;/* host [n_lights]                           */
+} psdr_scene_desc;
+
+/* PathIntegrator::Config (src/path_integrator.h:8-21; pybind fields
+ * src/psdr.cc:108-131).  Defaults as in the Python wrapper
+ * (src/python/torch_psdr.py:478-491). */
+typedef struct psdr_config {
+  uint32_t n_pass;
+  uint32_t spp_interior;
+  uint32_t enable_light_visable;
+  uint32_t spp_primary_edge;
+  uint32_t spp_secondary_edge;
+  uint32_t max_bounce;
+  uint32_t mis_light_samples;
+  uint32_t mis_bsdf_samples;
+  uint32_t primary_edge_preprocess_rounds;
+  uint32_t secondary_edge_preprocess_rounds;
+  uint32_t primary_edge_hypercube_resolution;
+  uint32_t secondary_edge_hypercube_resolution[3];
+} psdr_config;
+
+/* PerspectiveCamera(look_from, look_at, up, vfov[rad], height, width)
+ * (src/psdr.cc:22-29, src/camera.h:208-277). */
+typedef struct psdr_camera {
+  float look_from[3];
+  float look_at[3];
+  float up[3];
+  float vfov;
+  int32_t height;
+  int32_t width;
+} psdr_camera;
+
+typedef struct psdr_scene psdr_scene; /* opaque */
+
+/* ---- errors / version ---------------------------------------------------- */
+const char* psdr_last_error(void);
+int psdr_abi_version(void);
+
+/* ---- scene ---------------------------------------------------------------- */
+/* Replaces pypsdr.TriangleMesh / AreaLight / Scene construction
+ * (src/psdr.cc:79-106 -> src/mesh.cc:5-95, src/emitter.cc:4-14, src/scene.cc:3-104,
+ * src/scene_optix.cc:10-238).  Copies the topology, builds the edge list, derives
+ * triangle records and samplers, and builds the LBVH.  Synchronises the stream
+ * once (topology is read back to build edges). */
+int psdr_scene_create(const psdr_scene_desc* desc, psdr_scene** out, void* stream);
+
+/* Re-derive everything that depends on vertex positions, texels and emission
+ * (triangle records, edge geometry, samplers, LBVH rebuild) from the buffers in
+ * desc, which must have the same topology as at creation.  Fully asynchronous. */
+int psdr_scene_update(psdr_scene* scene, const psdr_scene_desc* desc, void* stream);
+
+int psdr_scene_destroy(psdr_scene* scene);
+
+/* Counts: triangles, edges, LBVH depth (root = 0). */
+int psdr_scene_info(const psdr_scene* scene, int32_t* n_tris, int32_t* n_edges, int32_t* bvh_depth);
+
+/* ---- rendering ------------------------------------------------------------ */
+/* PathIntegrator.renderC (src/path_integrator.cc:408-410 for one camera):
+ * writes the (height, width, 3) fp32 image, row 0 = top.  With keep != 0 the
+ * per-lane hit records are kept in the scene for psdr_render_backward
+ * (the forward half of renderD, src/path_integrator.cc:412-414). */
+int psdr_render_forward(psdr_scene* scene, const psdr_config* cfg, const psdr_camera* cam, float* out_rgb,
+                        int32_t keep, void* stream);
+
+/* Backward of renderD (Enoki's FloatD.backward, src/python/torch_psdr.py:47-72)
+ * for the image of the last psdr_render_forward(keep=1) with the same camera and
+ * config: accumulates dLoss/d(vertices), d(texels), d(light emit) given dL_dI
+ * (height, width, 3).  The gradient buffers are ADDED to (zero them first).
+ * Any gradient pointer may be NULL.  Interior term by path replay; primary and
+ * direct-shadow boundary terms by edge sampling (src/path_integrator.cc:188-353). */
+int psdr_render_backward(psdr_scene* scene, const psdr_config* cfg, const psdr_camera* cam, const float* dL_dI,
+                         float* grad_vertices, float* grad_texels, float* grad_emit, void* stream);
+
+/* ---- component entry points (tests, benchmarks) --------------------------- */
+/* First n_draws PCG32 outputs of lanes [lane0, lane0+n_lanes) of Sampler
+ * (src/sampler.h:11-17); out is device [n_lanes*n_draws]. */
+int psdr_pcg_lane_draws(uint32_t lane0, int32_t n_lanes, int32_t n_draws, uint32_t* out, void* stream);
+
+/* Camera rays of every lane of Sampler(spp*H*W) (src/camera.h:235-258); device
+ * outputs o, d [N*3] and pixel index [N]. */
+int psdr_camera_rays(const psdr_camera* cam, uint32_t spp, float* o, float* d, int32_t* pix, void* stream);
+
+/* Host copies of the camera matrices (row-major 4x4): camera2world,
+ * world2camera, camera2raster, raster2camera; and the camera position. */
+int psdr_camera_matrices(const psdr_camera* cam, float* c2w, float* w2c, float* c2r, float* r2c, float* pos);
+
+/* Closest hit (src/scene_optix.cc:240-277 + device_programs.cu:13-47) of n rays
+ * against the scene: global triangle id (-1 on miss), t, barycentrics u, v
+ * (weights of p1, p2; -1 on miss).  All device arrays. */
+int psdr_scene_trace(psdr_scene* scene, const float* ro, const float* rd, int32_t n, int32_t* tri, float* t,
+                     float* u, float* v, void* stream);
+
+/* Derived data for parity tests (device outputs):
+ * tri7 [n_tris*7] = normal(3), tangent(3), area;
+ * edge_topo [n_edges*6] = v0, v1, left face, right face, left third vertex, right third vertex;
+ * edge_geom [n_edges*16] = p0(3), p1(3), out(3), n_left(3), n_right(3), length. */
+int psdr_scene_derived(const psdr_scene* scene, float* tri7, int32_t* edge_topo, float* edge_geom, void* stream);
+
+/* Sampler tables (device outputs): light pick pmf/cmf over objects [n_objects],
+ * edge pmf/cmf [n_edges], light power and area [n_lights]. */
+int psdr_scene_samplers(const psdr_scene* scene, float* light_pmf, float* light_cmf, float* edge_pmf,
+                        float* edge_cmf, float* light_power, float* light_area, void* stream);
+
+/* LBVH of the scene (device outputs), as built by psdr_scene_create/update:
+ * morton [n] by triangle, order [n] (sorted position -> triangle),
+ * child [2(n-1)] (leaf k encoded as k | 0x80000000), parent [2n-1] (internal
+ * nodes then leaves, root -1), box [(n-1)*12] (left lo/hi, right lo/hi). */
+int psdr_scene_bvh(const psdr_scene* scene, uint32_t* morton, int32_t* order, int32_t* child, int32_t* parent,
+                   float* box, void* stream);
+
+/* Per-lane interior radiance of the last forward render (device [N*3]). */
+int psdr_last_lane_radiance(const psdr_scene* scene, float* out, void* stream);
+
+/* Counters of the last forward render: active paths entering each depth
+ * (counts[0] = camera lanes, counts[d+1] = paths alive at depth d), and rays
+ * traced; counts needs max_bounce + 2 entries.  Synchronises the stream. */
+int psdr_last_render_stats(const psdr_scene* scene, int64_t* counts, int32_t n_counts, int64_t* rays_traced);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PSDR_HIP_H */

@@ -1,0 +1,70 @@
+"""Scene builders shared by the tests (CPU or GPU tensors)."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import torch
+
+import torch_psdr_amd as dr
+from torch_psdr_amd import workloads as W
+from torch_psdr_amd.flat import flatten
+
+
+def objects(device, level=1, tex_res=None, tex_seed=1, sphere_vertices=None, light_y=4.0, emit=10.0,
+            bsdf_rough=None, requires_grad=False):
+    """Sphere (+ optional UV texture), ground quad, quad light: (meshes, lights, leaves)."""
+    Vs, Fs = W.icosphere(level)
+    if sphere_vertices is not None:
+        Vs = sphere_vertices
+    vs = torch.tensor(Vs, device=device, requires_grad=requires_grad)
+    fs = torch.tensor(Fs, device=device)
+    leaves = {"sphere_v": vs}
+    if tex_res is None:
+        color = torch.tensor([0.5, 0.5, 0.5], device=device, requires_grad=requires_grad)
+        uv, uvi = None, None
+    else:
+        g = torch.Generator().manual_seed(tex_seed)
+        color = (0.2 + 0.6 * torch.rand((tex_res, tex_res, 3), generator=g)).to(device)
+        color.requires_grad_(requires_grad)
+        uv = torch.tensor(W.sphere_uv(W.icosphere(level)[0]), device=device)
+        uvi = fs
+    leaves["color"] = color
+    if bsdf_rough is None:
+        mat = dr.DiffuseMaterial(color)
+    else:
+        rough = torch.tensor([bsdf_rough], device=device, requires_grad=requires_grad)
+        leaves["rough"] = rough
+        mat = dr.DiffuseBsdfMaterial(color, rough)
+    sphere = dr.Mesh(vs, uv, fs, uvi, mat)
+    Vg, Fg = W.quad(10.0, -1.0)
+    gv = torch.tensor(Vg, device=device, requires_grad=requires_grad)
+    ground_col = torch.tensor([0.5, 0.5, 0.5], device=device, requires_grad=requires_grad)
+    ground = dr.Mesh(gv, None, torch.tensor(Fg, device=device), None, dr.DiffuseMaterial(ground_col))
+    Vl, Fl = W.quad(2.0, light_y, facing_down=True)
+    lv = torch.tensor(Vl, device=device, requires_grad=requires_grad)
+    le = torch.tensor([emit, emit, emit], device=device, requires_grad=requires_grad)
+    light = dr.AreaLight(lv, torch.tensor(Fl, device=device), le)
+    leaves.update(ground_v=gv, ground_color=ground_col, light_v=lv, emit=le)
+    return [sphere, ground], [light], leaves
+
+
+def cam_params(res, look_from=(0.0, 1.0, 5.0), vfov=math.pi / 4):
+    return list(look_from) + [0.0, 0.0, 0.0, 0.0, 1.0, 0.0, vfov, float(res), float(res)]
+
+
+def camera(device, res, look_from=(0.0, 1.0, 5.0), vfov=math.pi / 4):
+    return dr.PerspectiveCamera(torch.tensor(look_from, device=device), torch.tensor([0.0, 0.0, 0.0], device=device),
+                                torch.tensor([0.0, 1.0, 0.0], device=device), torch.tensor([vfov], device=device),
+                                res, res)
+
+
+def cpu_flat(**kw):
+    meshes, lights, leaves = objects(torch.device("cpu"), **kw)
+    return flatten(meshes, lights, torch.device("cpu")), leaves
+
+
+def tri9(flat) -> np.ndarray:
+    V = flat.vertices.detach().cpu().numpy()
+    F = flat.faces.cpu().numpy()
+    return V[F].reshape(-1, 9).astype(np.float32)

@@ -1,0 +1,45 @@
+"""The C-ABI library loads and exports every entry point include/psdr_hip.h declares (no GPU calls)."""
+import ctypes
+import re
+from pathlib import Path
+
+import torch_psdr_amd._native as N
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def declared_symbols():
+    text = (ROOT / "include" / "psdr_hip.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(psdr_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_library_exports_declared_symbols():
+    lib = N.load()
+    names = declared_symbols()
+    assert len(names) >= 15
+    for name in names:
+        assert hasattr(lib, name), name
+    assert set(names) == set(N.EXPORTED)
+
+
+def test_abi_version_and_error_string():
+    lib = N.load()
+    assert lib.psdr_abi_version() == N.ABI_VERSION
+    assert isinstance(lib.psdr_last_error(), bytes)
+
+
+def test_struct_sizes_match_header():
+    # psdr_object: 11 int32; psdr_config: 14 uint32; psdr_camera: 10 floats + 2 int32
+    assert ctypes.sizeof(N.PsdrObject) == 44
+    assert ctypes.sizeof(N.PsdrConfig) == 56
+    assert ctypes.sizeof(N.PsdrCamera) == 48
+
+
+def test_argument_errors_without_gpu():
+    lib = N.load()
+    out = ctypes.c_void_p()
+    assert lib.psdr_scene_create(None, ctypes.byref(out), None) == -1
+    assert b"NULL" in lib.psdr_last_error()
+    cam = N.PsdrCamera()
+    assert lib.psdr_camera_matrices(ctypes.byref(cam), None, None, None, None, None) == -1  # height 0

@@ -1,0 +1,374 @@
+"""Pins of the CPU oracle against published vectors, closed forms and brute force.
+
+Each test is chosen so that a plausible slip in the oracle (a dropped term, a
+wrong sign or index, a transposed operand) fails it.  Nothing here needs a GPU.
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from tests import helpers as H
+
+M32 = 0xFFFFFFFF
+
+
+# ---------------------------------------------------------------- PCG32 / TEA
+def test_pcg32_published_vector(oracle):
+    # pcg32_srandom(42, 54) from the public pcg-c-basic demo (O'Neill 2014)
+    out = oracle.pcg_kat(42, 54, 6)
+    assert [int(x) for x in out] == [0xA15C02B7, 0x7B47F409, 0xBA1D3330, 0x83D2F293, 0xBFA4784B, 0xCBED606E]
+
+
+def _tea(v0, v1, k, rounds):
+    s = 0
+    for _ in range(rounds):
+        s = (s + 0x9E3779B9) & M32
+        v0 = (v0 + ((((v1 << 4) & M32) + k[0]) ^ (v1 + s) ^ ((v1 >> 5) + k[1]))) & M32
+        v1 = (v1 + ((((v0 << 4) & M32) + k[2]) ^ (v0 + s) ^ ((v0 >> 5) + k[3]))) & M32
+    return v0, v1
+
+
+def test_tea_round_function_matches_published_vector():
+    # standard TEA (Wheeler & Needham 1994): key 0, plaintext 0, 32 rounds
+    assert _tea(0, 0, [0, 0, 0, 0], 32) == (0x41EA3A0A, 0x94BAA940)
+
+
+def _pcg_seed(initstate, initseq):
+    mult = 0x5851F42D4C957F2D
+    M64 = (1 << 64) - 1
+    inc = ((initseq << 1) | 1) & M64
+    st = 0
+    st = (st * mult + inc) & M64
+    st = (st + initstate) & M64
+    st = (st * mult + inc) & M64
+    return st, inc
+
+
+def test_lane_seeding_follows_sampler_ctor(oracle):
+    # src/sampler.h:11-17: seed_value = i + PCG32_DEFAULT_STATE narrowed to u32;
+    # initstate = TEA4(seed_value, i), initseq = TEA4(i, seed_value) with the
+    # reference's key (src/sampler.h:70-80); the TEA round is pinned above.
+    key = [0xA341316C, 0xC8013EA4, 0xAD90777D, 0x7E95761E]
+    lanes = [0, 1, 2, 77, 65535, 262143, 4194303, 123456789]
+    for lane in lanes:
+        st, inc = oracle.lane_seed(lane, 1)
+        s = (lane + 0x853C49E6748FEA9B) & M32
+        a0, a1 = _tea(s, lane, key, 4)
+        b0, b1 = _tea(lane, s, key, 4)
+        want = _pcg_seed(a0 + (a1 << 32), b0 + (b1 << 32))
+        assert (int(st[0]), int(inc[0])) == want
+
+
+def test_lane_streams_and_float_conversion(oracle):
+    d = oracle.lane_draws(1000, 4, 8)
+    st, inc = oracle.lane_seed(1000, 4)
+    mult = 0x5851F42D4C957F2D
+    for l in range(4):
+        s = int(st[l])
+        for k in range(8):
+            old = s
+            s = (old * mult + int(inc[l])) & ((1 << 64) - 1)
+            xs = (((old >> 18) ^ old) >> 27) & M32
+            rot = old >> 59
+            want = ((xs >> rot) | (xs << ((-rot) & 31))) & M32
+            assert int(d[l, k]) == want
+    # next_float32: (u >> 9 | 0x3f800000) - 1 lies in [0, 1) with 23-bit resolution
+    f = ((d >> 9) | 0x3F800000).view(np.float32) - 1.0
+    assert (f >= 0).all() and (f < 1).all()
+
+
+# ---------------------------------------------------------------- sin / cos
+def test_sincos_against_libm(oracle):
+    x = np.linspace(0.0, 2 * math.pi, 200001, dtype=np.float32)
+    x = np.concatenate([x, np.float32(6.2831855) * np.random.default_rng(0).random(100000).astype(np.float32)])
+    s, c = oracle.sincos(x)
+    xd = x.astype(np.float64)
+    assert np.abs(s - np.sin(xd)).max() < 2.5e-7
+    assert np.abs(c - np.cos(xd)).max() < 2.5e-7
+
+
+# ---------------------------------------------------------------- camera
+def test_camera_closed_form(oracle):
+    cp = H.cam_params(64)
+    (c2w, w2c, c2r, r2c), pos = oracle.camera_matrices(cp)
+    np.testing.assert_allclose(c2w @ w2c, np.eye(4), atol=2e-6)
+    np.testing.assert_allclose(c2r @ r2c, np.eye(4), atol=2e-6)
+    np.testing.assert_allclose(pos, [0, 1, 5], atol=1e-6)
+    look = -np.array([0, 1, 5.0]) / np.linalg.norm([0, 1, 5.0])
+
+    def ray(x, y):
+        p = r2c @ np.array([x, y, 1.0, 1.0])
+        p = p[:3] / p[3]
+        d = c2w[:3, :3] @ p
+        return d / np.linalg.norm(d)
+
+    np.testing.assert_allclose(ray(32, 32), look, atol=1e-6)
+    # look_at (src/camera.h:33-53) does not normalise right = cross(normalize(up), dir):
+    # |right| = |up'| = sin(angle(up, dir)), which shrinks the image-plane axes, so
+    # the half field of view is atan(sin(angle) * tan(vfov / 2)).
+    s = math.sqrt(1.0 - (1.0 / math.sqrt(26.0)) ** 2)
+    half = math.atan(s * math.tan(math.pi / 8))
+    for (x, y) in [(32, 0), (32, 64), (0, 32), (64, 32)]:
+        assert abs(math.acos(float(np.dot(ray(x, y), look))) - half) < 1e-5
+    # raster row 0 is the top of the image
+    assert ray(32, 0)[1] > ray(32, 64)[1]
+
+
+def test_camera_rays_lane_to_pixel(oracle):
+    cp = H.cam_params(8)
+    o, d, pix = oracle.camera_rays(cp, 2)
+    assert (pix == np.arange(128) // 2).all()
+    np.testing.assert_allclose(np.linalg.norm(d, axis=1), 1.0, atol=1e-6)
+    np.testing.assert_array_equal(o, np.tile([0, 1, 5.0], (128, 1)).astype(np.float32))
+    # first lane of pixel (x=3, y=5) points inside that pixel's frustum
+    (c2w, w2c, c2r, r2c), _ = oracle.camera_matrices(cp)
+    lane = 2 * (5 * 8 + 3)
+    p = c2r @ w2c @ np.append(o[lane] + d[lane], 1.0)
+    p = p[:2] / p[3]
+    assert 3 <= p[0] < 4 and 5 <= p[1] < 6
+
+
+def test_project_points(oracle):
+    cp = H.cam_params(64)
+    xy, valid = oracle.project_points(cp, np.array([[0, 0, 0], [0, 1, 10], [100, 0, 0]], np.float32))
+    assert valid.tolist() == [True, False, False]
+    assert xy[0].tolist() == [32, 39] or (31 <= xy[0][0] <= 32)
+
+
+# ---------------------------------------------------------------- mesh prep / edges
+def _flat_of(V, F, uv=None, uvi=None):
+    import torch_psdr_amd as dr
+    from torch_psdr_amd.flat import flatten
+    m = dr.Mesh(torch.tensor(V, dtype=torch.float32), None if uv is None else torch.tensor(uv, dtype=torch.float32),
+                torch.tensor(F, dtype=torch.int32), None if uvi is None else torch.tensor(uvi, dtype=torch.int32),
+                dr.DiffuseMaterial(torch.tensor([0.5, 0.5, 0.5])))
+    return flatten([m], [], torch.device("cpu"))
+
+
+def test_triangle_record_hand_computed(oracle):
+    flat = _flat_of([[0, 0, 0], [1, 0, 0], [0, 1, 0]], [[0, 1, 2]])
+    tri, topo, geom = oracle.prepare(oracle.Scene(flat))
+    # normal = normalize(cross(p1-p0, p0-p2)) = (0,0,-1); default uvs -> tangent = normalize(p0-p2)
+    np.testing.assert_allclose(tri[0, :3], [0, 0, -1], atol=1e-7)
+    np.testing.assert_allclose(tri[0, 3:6], [0, -1, 0], atol=1e-7)
+    assert abs(tri[0, 6] - 0.5) < 1e-7
+    flat = _flat_of([[0, 0, 0], [1, 0, 0], [0, 1, 0]], [[0, 1, 2]], uv=[[0, 0], [1, 0], [0, 1]], uvi=[[0, 1, 2]])
+    tri, _, _ = oracle.prepare(oracle.Scene(flat))
+    np.testing.assert_allclose(tri[0, 3:6], [1, 0, 0], atol=1e-7)  # duv1=(1,0), duv2=(0,1) -> p1-p0
+
+
+def test_tetrahedron_edges(oracle):
+    V = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], np.float32)
+    F = [[0, 2, 1], [0, 1, 3], [0, 3, 2], [1, 2, 3]]
+    tri, topo, geom = oracle.prepare(oracle.Scene(_flat_of(V, F)))
+    assert topo.shape[0] == 6
+    assert sorted(map(tuple, topo[:, :2].tolist())) == [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)]
+    assert (topo[:, 2] != topo[:, 3]).all()  # every edge shared
+    np.testing.assert_allclose(sorted(tri[:, 6]), sorted([0.5, 0.5, 0.5, math.sqrt(3) / 2]), atol=1e-6)
+    cen = V.mean(0)
+    for e in range(6):
+        p0, p1, out = geom[e, 0:3], geom[e, 3:6], geom[e, 6:9]
+        assert np.dot(out, (p0 + p1) / 2 - cen) > 0  # out direction points away from the solid
+        assert abs(geom[e, 15] - np.linalg.norm(p1 - p0)) < 1e-6
+        np.testing.assert_allclose(np.linalg.norm(geom[e, 9:12]), 1, atol=1e-6)
+
+
+def test_open_quad_edges(oracle):
+    V = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0]], np.float32)
+    tri, topo, geom = oracle.prepare(oracle.Scene(_flat_of(V, [[0, 1, 2], [0, 2, 3]])))
+    assert topo.shape[0] == 5
+    shared = topo[:, 2] != topo[:, 3]
+    assert shared.sum() == 1 and tuple(topo[shared][0, :2]) == (0, 2)
+    for e in np.nonzero(~shared)[0]:
+        # unshared: right face = left face, out = normalize(cross(p1-p0, n_left)), in-plane
+        assert topo[e, 4] == topo[e, 5]
+        p0, p1, out, nl = geom[e, 0:3], geom[e, 3:6], geom[e, 6:9], geom[e, 9:12]
+        want = np.cross(p1 - p0, nl)
+        np.testing.assert_allclose(out, want / np.linalg.norm(want), atol=1e-6)
+        assert abs(out[2]) < 1e-7
+
+
+# ---------------------------------------------------------------- closest hit
+def _random_rays(n, seed, lo=-6, hi=6):
+    rng = np.random.default_rng(seed)
+    o = rng.uniform(lo, hi, (n, 3)).astype(np.float32)
+    d = rng.normal(size=(n, 3))
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    return o, d
+
+
+def test_oracle_bvh_matches_brute_force(oracle):
+    flat, _ = H.cpu_flat(level=3)
+    t9 = H.tri9(flat)
+    o, d = _random_rays(20000, 1)
+    # aim half of the rays at the sphere so that hits are common
+    tgt = np.random.default_rng(2).normal(size=(10000, 3)).astype(np.float32) * 0.5
+    d[:10000] = (tgt - o[:10000]) / np.linalg.norm(tgt - o[:10000], axis=1, keepdims=True)
+    a = oracle.closest_hit(t9, o, d, brute=False)
+    b = oracle.closest_hit(t9, o, d, brute=True)
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x, y)
+    assert (a[0] >= 0).mean() > 0.3
+
+
+def test_ray_triangle_barycentrics(oracle):
+    # u, v are the weights of p1, p2 (OptiX convention); tmin = 1e-3
+    t9 = np.array([[0, 0, 0, 1, 0, 0, 0, 1, 0]], np.float32)
+    o = np.array([[0.25, 0.5, 1], [0.1, 0.1, 0.0005], [0.9, 0.9, 1]], np.float32)
+    d = np.array([[0, 0, -1]] * 3, np.float32)
+    tri, t, u, v = oracle.closest_hit(t9, o, d, brute=True)
+    assert tri.tolist() == [0, -1, -1]
+    assert abs(t[0] - 1) < 1e-7 and abs(u[0] - 0.25) < 1e-7 and abs(v[0] - 0.5) < 1e-7
+
+
+# ---------------------------------------------------------------- LBVH restatement
+def test_lbvh_structure(oracle):
+    flat, _ = H.cpu_flat(level=3)
+    t9 = H.tri9(flat)
+    n = t9.shape[0]
+    morton, order, child, parent, box = oracle.lbvh(t9)
+    assert sorted(order.tolist()) == list(range(n))
+    ms = morton[order]
+    assert (np.diff(ms.astype(np.int64)) >= 0).all()
+    # stable: equal codes keep triangle order
+    eq = np.diff(ms.astype(np.int64)) == 0
+    assert (np.diff(order)[eq] > 0).all()
+    leaves = child[child < 0] & 0x7FFFFFFF
+    assert sorted(leaves.tolist()) == list(range(n))
+    internal = child[child >= 0]
+    assert sorted(internal.tolist()) == list(range(1, n - 1))
+    assert parent[0] == -1
+    for i in range(n - 1):
+        for c in child[i]:
+            idx = (n - 1 + (c & 0x7FFFFFFF)) if c < 0 else c
+            assert parent[idx] == i
+
+    # each child box contains its triangle / subtree
+    def subtree_box(c):
+        if c < 0:
+            tri = t9[order[c & 0x7FFFFFFF]].reshape(3, 3)
+            return tri.min(0), tri.max(0)
+        lo = np.minimum(box[c, 0:3], box[c, 6:9])
+        hi = np.maximum(box[c, 3:6], box[c, 9:12])
+        return lo, hi
+
+    for i in range(n - 1):
+        for s, c in enumerate(child[i]):
+            lo, hi = subtree_box(c)
+            assert (box[i, 6 * s:6 * s + 3] <= lo).all() and (box[i, 6 * s + 3:6 * s + 6] >= hi).all()
+
+
+# ---------------------------------------------------------------- radiance: closed forms
+def _lambert_form_factor(x, n, poly):
+    """Point-to-polygon form factor (Lambert): F = 1/(2 pi) sum_i theta_i (n . gamma_i)."""
+    F = 0.0
+    k = len(poly)
+    for i in range(k):
+        a = poly[i] - x
+        b = poly[(i + 1) % k] - x
+        a = a / np.linalg.norm(a)
+        b = b / np.linalg.norm(b)
+        theta = math.acos(np.clip(np.dot(a, b), -1, 1))
+        g = np.cross(a, b)
+        g = g / np.linalg.norm(g)
+        F += theta * np.dot(n, g)
+    return abs(F) / (2 * math.pi)
+
+
+def _ground_light_scene(res, emit=10.0):
+    import torch_psdr_amd as dr
+    from torch_psdr_amd import workloads as W
+    from torch_psdr_amd.flat import flatten
+    cpu = torch.device("cpu")
+    Vg, Fg = W.quad(10.0, -1.0)
+    ground = dr.Mesh(torch.tensor(Vg), None, torch.tensor(Fg), None, dr.DiffuseMaterial(torch.tensor([0.5] * 3)))
+    Vl, Fl = W.quad(2.0, 4.0, facing_down=True)
+    light = dr.AreaLight(torch.tensor(Vl), torch.tensor(Fl), torch.tensor([emit] * 3))
+    return flatten([ground], [light], cpu), Vl
+
+
+def _ground_hits(oracle, cp, spp):
+    o, d, pix = oracle.camera_rays(cp, spp)
+    t = (-1.0 - o[:, 1]) / d[:, 1]
+    x = o + t[:, None] * d
+    ok = (t > 0) & (np.abs(x[:, 0]) < 4.9) & (np.abs(x[:, 2]) < 4.9)
+    return x, ok, pix
+
+
+@pytest.mark.parametrize("nL,nB", [(1, 0), (1, 1), (2, 1)])
+def test_direct_light_matches_lambert_formula(oracle, nL, nB):
+    flat, Vl = _ground_light_scene(32)
+    cp = H.cam_params(32, look_from=(0.0, 3.0, 4.0))
+    spp = 64
+    cfg = [1, spp, 0, 1, 1, 1, nL, nB, 1, 1, 10000, 10000, 6, 6]
+    img, lane_L = oracle.render(oracle.Scene(flat), cfg, cp, mode=0, lane_L=True)
+    x, ok, pix = _ground_hits(oracle, cp, spp)
+    poly = Vl[[0, 1, 2, 3]].astype(np.float64)
+    want = np.array([0.5 * 10.0 * _lambert_form_factor(xi.astype(np.float64), np.array([0, 1.0, 0]), poly)
+                     if k else 0.0 for xi, k in zip(x, ok)])
+    got = lane_L[:, 0].astype(np.float64)
+    diff = got[ok] - want[ok]
+    se = got[ok].std() / math.sqrt(ok.sum())
+    assert abs(diff.mean()) < 5 * se + 1e-6, (diff.mean(), se)
+    # the image is the per-pixel mean of lane radiance
+    np.testing.assert_allclose(img[..., 0].reshape(-1), lane_L[:, 0].reshape(-1, spp).sum(1) / spp, rtol=1e-6,
+                               atol=1e-7)
+
+
+def test_emission_linearity_jvp(oracle):
+    flat, _ = H.cpu_flat(level=2)
+    cp = H.cam_params(24)
+    cfg = [1, 4, 0, 1, 1, 3, 1, 1, 1, 1, 10000, 10000, 6, 6]
+    img, d_int, _, _ = oracle.render(oracle.Scene(flat, d_emit=np.ones(3, np.float32)), cfg, cp, mode=1,
+                                     boundary=False)
+    np.testing.assert_allclose(d_int, img / 10.0, rtol=2e-5, atol=1e-7)
+
+
+def test_texel_jvp_matches_central_difference(oracle):
+    flat, _ = H.cpu_flat(level=2, tex_res=8)
+    cp = H.cam_params(24)
+    cfg = [1, 4, 0, 1, 1, 2, 1, 1, 1, 1, 10000, 10000, 6, 6]
+    w = np.random.default_rng(5).normal(size=flat.texels.numel()).astype(np.float32)
+    w[-6:] = 0  # ground + light constants untouched
+    _, d_int, _, _ = oracle.render(oracle.Scene(flat, d_texels=w), cfg, cp, mode=1, boundary=False)
+    h = 0.01
+    base = flat.texels.clone()
+    flat.texels = base + h * torch.from_numpy(w)
+    ip = oracle.render(oracle.Scene(flat), cfg, cp).astype(np.float64)
+    flat.texels = base - h * torch.from_numpy(w)
+    im = oracle.render(oracle.Scene(flat), cfg, cp).astype(np.float64)
+    flat.texels = base
+    fd = (ip - im) / (2 * h)
+    assert np.abs(fd - d_int).max() < 2e-3 * max(1.0, np.abs(fd).max())
+    assert np.abs(d_int).max() > 1e-3
+
+
+def test_vertex_jvp_matches_lambert_derivative(oracle):
+    # NEE only, depth 1: moving the ground plane by delta*y moves every ground hit
+    # with the surface, so E[JVP] = rho * Le * dF/dy at the hit point.
+    flat, Vl = _ground_light_scene(24)
+    cp = H.cam_params(24, look_from=(0.0, 3.0, 4.0))
+    spp = 256
+    cfg = [1, spp, 0, 1, 1, 1, 1, 0, 1, 1, 10000, 10000, 6, 6]
+    dv = np.zeros((flat.vertices.shape[0], 3), np.float32)
+    dv[:4, 1] = 1.0  # ground vertices move up
+    img, d_int, _, _ = oracle.render(oracle.Scene(flat, d_vertices=dv), cfg, cp, mode=1, boundary=False)
+    x, ok, pix = _ground_hits(oracle, cp, spp)
+    poly = Vl.astype(np.float64)
+    eps = 1e-4
+
+    def F(p):
+        return _lambert_form_factor(p, np.array([0, 1.0, 0]), poly)
+
+    want = np.zeros(x.shape[0])
+    for i in np.nonzero(ok)[0]:
+        p = x[i].astype(np.float64)
+        want[i] = 0.5 * 10.0 * (F(p + [0, eps, 0]) - F(p - [0, eps, 0])) / (2 * eps)
+    want_pix = want.reshape(-1, spp).mean(1)
+    okp = ok.reshape(-1, spp).all(1)
+    got = d_int[..., 0].reshape(-1)
+    assert okp.sum() > 100
+    rel = np.abs(got[okp].mean() - want_pix[okp].mean()) / np.abs(want_pix[okp].mean())
+    assert rel < 0.02, rel

@@ -1,0 +1,294 @@
+"""Drop-in Python API of torch_psdr (src/python/torch_psdr.py) on the gfx950 library.
+
+Same class names, constructor arguments, shapes, dtypes, defaults and
+assertions as the reference wrapper; `PathIntegrator.renderC` / `renderD`
+return a list of (H, W, 3) fp32 images, one per camera (row 0 = top), and
+`renderD`'s images are differentiable w.r.t. mesh / light vertices, albedo and
+roughness texels and light emission through torch autograd.
+
+Differences, each deliberate:
+  * 2-D (H, W) textures are accepted as 1-channel maps (the reference's `==`
+    typo at src/python/torch_psdr.py:233 rejects them);
+  * OrthographicCamera, EnvLight, SH9Light and normal-mapped DiffuseBsdfMaterial
+    raise NotImplementedError (no workload of this build uses them);
+  * camera parameters are constants: they get no gradient (a warning is given
+    when one requires grad);
+  * the stray `from turtle import forward` (src/python/torch_psdr.py:2) is gone.
+"""
+from __future__ import annotations
+
+import warnings
+from typing import List, Union
+
+import torch
+from torch import Tensor
+
+from . import ops
+from .flat import flatten
+
+
+class VarContex:
+    """Bookkeeping of tensors that require grad (src/python/torch_psdr.py:84-103)."""
+
+    def __init__(self):
+        self.torch_vars: List[Tensor] = []
+
+    def append_variable(self, torch_var: Tensor, native_var=None):
+        if torch_var.requires_grad:
+            self.torch_vars.append(torch_var)
+
+    def get_variables(self):
+        return [], self.torch_vars
+
+    def merge_variables(self, native_vars, torch_vars):
+        for t in torch_vars:
+            if t.requires_grad:
+                self.torch_vars.append(t)
+
+
+class BaseCamera(VarContex):
+    """Base camera class (src/python/torch_psdr.py:106-110)."""
+
+
+class PerspectiveCamera(BaseCamera):
+    """PerspectiveCamera(look_from (3,), look_at (3,), up (3,), vfov (1,) [rad], height, width)
+    (src/python/torch_psdr.py:113-158; matrices src/camera.h:7-53, near -0.01, far -1000)."""
+
+    def __init__(self, look_from: Tensor, look_at: Tensor, up: Tensor, vfov: Tensor, height: int, width: int):
+        super().__init__()
+        assert look_from.size() == (3,)
+        assert look_at.size() == (3,)
+        assert up.size() == (3,)
+        assert vfov.size() == (1,)
+        assert height > 0 and width > 0
+        for t in (look_from, look_at, up, vfov):
+            self.append_variable(t)
+        if self.torch_vars:
+            warnings.warn("torch_psdr_amd: camera parameters are treated as constants (no camera gradients)")
+        self.look_from, self.look_at, self.up, self.vfov = look_from, look_at, up, vfov
+        self.height = int(height)
+        self.width = int(width)
+
+    def params(self) -> List[float]:
+        vals = torch.cat([self.look_from.detach().reshape(3), self.look_at.detach().reshape(3),
+                          self.up.detach().reshape(3), self.vfov.detach().reshape(1)]).float().cpu().tolist()
+        return vals + [float(self.height), float(self.width)]
+
+
+class OrthographicCamera(BaseCamera):
+    def __init__(self, look_from: Tensor, look_at: Tensor, up: Tensor, view_width: Tensor, height: int, width: int):
+        raise NotImplementedError("OrthographicCamera is not implemented by torch_psdr_amd")
+
+
+class Texture(VarContex):
+    """Constant or UV texture (src/python/torch_psdr.py:209-269).
+
+    Shapes: (1,) / (3,) constant, (H, W) / (H, W, 1) / (H, W, 3) maps.  The data
+    is kept in the caller's layout (row 0 = image top); lookups flip rows exactly
+    as the reference's `tex.flip(0)` (nearest texel, src/texture.h:65-74)."""
+
+    def __init__(self, tex: Tensor):
+        super().__init__()
+        tex_type = None
+        dims = tex.dim()
+        if dims == 1:
+            if tex.size() == (1,):
+                tex_type = "Constant1f"
+            elif tex.size() == (3,):
+                tex_type = "Constant3f"
+        elif dims == 2:
+            tex_type = "UV1f"
+            tex = tex.unsqueeze(-1)
+        elif dims == 3:
+            if tex.size(-1) == 1:
+                tex_type = "UV1f"
+            elif tex.size(-1) == 3:
+                tex_type = "UV3f"
+        assert tex_type is not None
+        self.tex_type = tex_type
+        self.data = tex if tex.dtype == torch.float32 else tex.float()
+        self.is_constant = tex_type.startswith("Constant")
+        self.height = 0 if self.is_constant else int(tex.size(0))
+        self.width = 0 if self.is_constant else int(tex.size(1))
+        self.append_variable(tex)
+
+
+class Material(VarContex):
+    """Base class of materials (src/python/torch_psdr.py:272-277)."""
+
+
+class DiffuseMaterial(Material):
+    """Lambertian albedo texture (src/python/torch_psdr.py:280-295; src/diffuse_material.cc)."""
+
+    kind = "diffuse"
+
+    def __init__(self, tex: Union[Tensor, Texture]):
+        super().__init__()
+        if isinstance(tex, Tensor):
+            tex = Texture(tex)
+        assert tex.tex_type == "Constant3f" or tex.tex_type == "UV3f"
+        self.color = tex
+        self.merge_variables(*tex.get_variables())
+
+
+class DiffuseBsdfMaterial(Material):
+    """Lambertian / Oren-Nayar-style diffuse (src/python/torch_psdr.py:298-335; src/diffuse_bsdf.cc)."""
+
+    kind = "diffuse_bsdf"
+
+    def __init__(self, color: Union[Tensor, Texture], roughness: Union[Tensor, Texture],
+                 normal: Union[Tensor, Texture, None] = None):
+        super().__init__()
+        if isinstance(color, Tensor):
+            color = Texture(color)
+        if isinstance(roughness, Tensor):
+            roughness = Texture(roughness)
+        if isinstance(normal, Tensor):
+            normal = Texture(normal)
+        assert color.tex_type == "Constant3f" or color.tex_type == "UV3f"
+        assert roughness.tex_type == "Constant1f" or roughness.tex_type == "UV1f"
+        assert normal is None or normal.tex_type == "Constant3f" or normal.tex_type == "UV3f"
+        if normal is not None:
+            raise NotImplementedError("normal maps are not implemented by torch_psdr_amd")
+        self.color = color
+        self.roughness = roughness
+        self.merge_variables(*color.get_variables())
+        self.merge_variables(*roughness.get_variables())
+
+
+class PrincipledMaterial(Material):
+    """Placeholder kept for name compatibility (empty in the reference too)."""
+
+
+class Mesh(VarContex):
+    """Triangle mesh + material (src/python/torch_psdr.py:343-379; src/mesh.cc:5-95)."""
+
+    def __init__(self, vertices: Tensor, uvs: Union[None, Tensor], face_indices: Tensor,
+                 uv_indices: Union[None, Tensor], material: Material):
+        super().__init__()
+        assert vertices.dim() == 2 and vertices.size(-1) == 3
+        assert uvs is None or (uvs.dim() == 2 and uvs.size(-1) == 2)
+        assert face_indices.dim() == 2 and face_indices.size(-1) == 3
+        assert uv_indices is None or (uv_indices.dim() == 2 and uv_indices.size(-1) == 3)
+        no_uvs = uvs is None or uv_indices is None
+        self.vertices = vertices
+        self.uvs = None if no_uvs else uvs
+        self.face_indices = face_indices
+        self.uv_indices = None if no_uvs else uv_indices
+        self.material = material
+        self.append_variable(vertices)
+        self.merge_variables(*material.get_variables())
+
+
+class Light(VarContex):
+    """Base class of lights (src/python/torch_psdr.py:382-385)."""
+
+
+class _LightMesh:
+    def __init__(self, vertices, face_indices):
+        self.vertices = vertices
+        self.face_indices = face_indices
+        self.uvs = None
+        self.uv_indices = None
+        self.material = None
+
+
+class AreaLight(Light):
+    """Triangle-mesh light with constant radiance (src/python/torch_psdr.py:388-414; src/emitter.cc:4-92)."""
+
+    def __init__(self, vertices: Tensor, face_indices: Tensor, emit: Tensor):
+        super().__init__()
+        assert vertices.dim() == 2 and vertices.size(-1) == 3
+        assert face_indices.dim() == 2 and face_indices.size(-1) == 3
+        assert emit.dim() == 1 and emit.size(0) == 3
+        self.vertices = vertices
+        self.face_indices = face_indices
+        self.emit = emit
+        self.append_variable(vertices)
+        self.append_variable(emit)
+
+    def mesh_proxy(self):
+        return _LightMesh(self.vertices, self.face_indices)
+
+
+class EnvLight(Light):
+    def __init__(self, cube_map: Texture):
+        raise NotImplementedError("EnvLight is not implemented by torch_psdr_amd")
+
+
+class SH9Light(Light):
+    def __init__(self, sh9_coeff: Tensor):
+        raise NotImplementedError("SH9Light is not implemented by torch_psdr_amd")
+
+
+class Scene(VarContex):
+    """Cameras, meshes and lights (src/python/torch_psdr.py:450-474; src/scene.cc:3-104).
+
+    The scene's buffers are captured at construction (as the reference copies
+    tensors into Enoki arrays), flattened into global vertex / texel / emission
+    buffers on the device."""
+
+    def __init__(self, cameras: List[BaseCamera], meshes: List[Mesh], lights: List[Light]):
+        super().__init__()
+        for c in cameras:
+            self.merge_variables(*c.get_variables())
+        for m in meshes:
+            self.merge_variables(*m.get_variables())
+        for l in lights:
+            self.merge_variables(*l.get_variables())
+        for l in lights:
+            if not isinstance(l, AreaLight):
+                raise NotImplementedError("only AreaLight is implemented by torch_psdr_amd")
+        if not meshes and not lights:
+            raise ValueError("Scene needs at least one mesh or light")
+        device = (meshes[0].vertices if meshes else lights[0].vertices).device
+        if device.type != "cuda":
+            raise RuntimeError("torch_psdr_amd renders on the GPU: scene tensors must be on a HIP device")
+        self.cameras = cameras
+        self.meshes = meshes
+        self.lights = lights
+        self.flat = flatten(meshes, lights, device)
+        self.token = ops.register(self.flat)
+
+    def __del__(self):
+        try:
+            ops.unregister(self.token)
+        except Exception:
+            pass
+
+
+class PathIntegrator:
+    """Path-space differentiable integrator (src/python/torch_psdr.py:477-556; src/path_integrator.cc)."""
+
+    def __init__(self, enable_light_visable: bool = False, n_pass: int = 1, spp_interior: int = 1,
+                 max_bounce: int = 3, mis_light_samples: int = 1, mis_bsdf_samples: int = 1,
+                 spp_primary_edge: int = 1, spp_secondary_edge: int = 1, primary_edge_preprocess_rounds: int = 1,
+                 secondary_edge_preprocess_rounds: int = 1, primary_edge_hypercube_resolution: int = 10000,
+                 secondary_edge_hypercube_resolution: List[int] = [10000, 6, 6]):
+        assert n_pass > 0
+        assert spp_interior > 0
+        assert spp_primary_edge > 0
+        assert spp_secondary_edge > 0
+        assert max_bounce > 0
+        assert mis_light_samples >= 0
+        assert mis_bsdf_samples >= 0
+        assert primary_edge_preprocess_rounds > 0
+        assert secondary_edge_preprocess_rounds > 0
+        assert primary_edge_hypercube_resolution > 0
+        assert len(secondary_edge_hypercube_resolution) == 3
+        for reso in secondary_edge_hypercube_resolution:
+            assert reso > 0
+        self.cfg = [int(n_pass), int(spp_interior), int(bool(enable_light_visable)), int(spp_primary_edge),
+                    int(spp_secondary_edge), int(max_bounce), int(mis_light_samples), int(mis_bsdf_samples),
+                    int(primary_edge_preprocess_rounds), int(secondary_edge_preprocess_rounds),
+                    int(primary_edge_hypercube_resolution)] + [int(r) for r in secondary_edge_hypercube_resolution]
+
+    def renderC(self, scene: Scene) -> List[Tensor]:
+        ns, flat = ops.entry(scene.token)
+        with torch.no_grad():
+            return [ops.forward_native(ns, flat, cam.params(), self.cfg, False) for cam in scene.cameras]
+
+    def renderD(self, scene: Scene) -> List[Tensor]:
+        f = scene.flat
+        return [torch.ops.psdr.render(scene.token, cam.params(), self.cfg, f.vertices, f.texels, f.light_emit, True)
+                for cam in scene.cameras]

@@ -1,0 +1,17 @@
+// Host-side entry points of the renderer kernels (render.hip, adjoint.hip).
+#pragma once
+
+#include "scene.h"
+
+namespace psdr {
+int trace_rays(const SceneView& sv, const float* ro, const float* rd, int n, const int* count_dev, int* tri,
+               float* t, float* u, float* v, int* spill, int grid, hipStream_t st);
+int trace_grid(int64_t max_rays);
+size_t spill_bytes(int grid);
+int lane_draws(uint32_t lane0, int n_lanes, int n_draws, uint32_t* out, hipStream_t st);
+int camera_rays(const CameraK& cam, uint32_t spp, float* o, float* d, int* pix, hipStream_t st);
+int render_forward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& cam, float* out, int keep,
+                   hipStream_t st);
+int render_backward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& cam, const float* dL_dI,
+                    float* grad_v, float* grad_t, float* grad_e, hipStream_t st);
+}  // namespace psdr

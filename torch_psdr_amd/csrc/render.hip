@@ -1,0 +1,536 @@
+// Wavefront forward path tracer (PathIntegrator::render_interior +
+// trace2light, src/path_integrator.cc:80-186, 355-372) on gfx950.
+//
+// Per camera the stages are
+//   k_cam      : TEA-seeded PCG32 lane streams, 2 draws, camera rays (all N lanes)
+//   k_trace    : closest hit of the ray queue (LDS-stack LBVH traversal)
+//   k_shade(d) : resolve depth d-1 (NEE + BSDF-MIS contributions, throughput
+//                update from the extension hit) and generate depth d's rays
+//                (light-sample, BSDF-sample and extension rays); paths that stay
+//                alive are compacted into the next queue with one atomic per wave
+//   k_film     : per-pixel sum over the pixel's spp lanes in lane order, / spp
+// The last depth draws its extension sample (the stream advances exactly as in
+// the reference) but traces no extension ray: its hit cannot change L.
+// Samples are re-derived from the stored per-depth PCG state in the resolve
+// step, so the queue carries only (lane, rng, T, L, hit, origin).
+// With keep != 0 every traced hit is also stored per lane (hit cache) for the
+// path-replay adjoint.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+
+#include "render.h"
+#include "shading.h"
+#include "traverse.h"
+
+namespace psdr {
+
+namespace {
+
+constexpr int TRACE_THREADS = 256;
+constexpr int STACK_LDS = 16;
+
+__global__ void k_lane_draws(uint32_t lane0, int n_lanes, int n_draws, uint32_t* out) {
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= n_lanes) return;
+  Rng r = lane_rng(lane0 + static_cast<uint32_t>(l));
+  for (int k = 0; k < n_draws; ++k) out[static_cast<int64_t>(l) * n_draws + k] = r.next_u32();
+}
+
+__global__ void k_camera_rays(CameraK cam, uint32_t spp, int64_t N, float* o, float* d, int* pix) {
+  const int64_t lane = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (lane >= N) return;
+  Rng r = lane_rng(static_cast<uint32_t>(lane));
+  const float u0 = r.next(), u1 = r.next();
+  f3 oo, dd;
+  int p;
+  camera_ray(cam, static_cast<uint32_t>(lane), spp, u0, u1, &oo, &dd, &p);
+  o[3 * lane] = oo.x; o[3 * lane + 1] = oo.y; o[3 * lane + 2] = oo.z;
+  d[3 * lane] = dd.x; d[3 * lane + 1] = dd.y; d[3 * lane + 2] = dd.z;
+  pix[lane] = p;
+}
+
+// Closest hit of rays [0, *count) (or n if count is null); rays SoA-of-xyz
+// AoS (ro[3*i+k]); outputs per ray.
+__global__ __launch_bounds__(TRACE_THREADS) void k_trace(SceneView sv, const float* __restrict__ ro,
+                                                         const float* __restrict__ rd, int n,
+                                                         const int* __restrict__ count, int* __restrict__ otri,
+                                                         float* __restrict__ ot, float* __restrict__ ou,
+                                                         float* __restrict__ ov, int* __restrict__ spill) {
+  __shared__ int stack[STACK_LDS * TRACE_THREADS];
+  const int cnt = count ? *count : n;
+  const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int stride = gridDim.x * blockDim.x;
+  for (int i = gtid; i < cnt; i += stride) {
+    const f3 o = f3{ro[3 * i], ro[3 * i + 1], ro[3 * i + 2]};
+    const f3 d = f3{rd[3 * i], rd[3 * i + 1], rd[3 * i + 2]};
+    const HitRec h = closest_hit<STACK_LDS>(sv.node, sv.leaf, sv.n_tris, o, d, stack, spill + gtid, stride);
+    otri[i] = h.tri;
+    if (ot) ot[i] = h.tri >= 0 ? h.t : -1.f;
+    ou[i] = h.u;
+    ov[i] = h.v;
+  }
+}
+
+struct QueueView {
+  uint32_t* lane;
+  uint64_t* rng;
+  float* T;  // [3][cap]
+  float* L;  // [3][cap]
+  int* tri;
+  float* u;
+  float* v;
+  float* O;  // [3][cap]
+  float* J0;
+  int cap;
+};
+
+struct RaysView {
+  float* o;  // [3*cap] xyz interleaved
+  float* d;
+  int* tri;
+  float* u;
+  float* v;
+};
+
+struct CacheView {
+  int* tri;  // [slots][N]
+  float* u;
+  float* v;
+  int64_t N;
+};
+
+struct ShadeParams {
+  SceneView sv;
+  CameraK cam;
+  int depth;  // vertex depth being generated (resolve of depth-1 first when > 0)
+  int D;
+  uint32_t nL, nB;
+  int elv;
+  int ad_jac;  // renderD semantics: jacobi = A / detach(A)
+  int K_in, K_out, Kmax;
+  int keep;
+};
+
+__device__ __forceinline__ void store3(float* a, int cap, int j, f3 v) {
+  a[j] = v.x;
+  a[cap + j] = v.y;
+  a[2 * cap + j] = v.z;
+}
+__device__ __forceinline__ f3 load3(const float* a, int cap, int j) { return f3{a[j], a[cap + j], a[2 * cap + j]}; }
+__device__ __forceinline__ void put_ray(RaysView r, int i, f3 o, f3 d) {
+  r.o[3 * i] = o.x; r.o[3 * i + 1] = o.y; r.o[3 * i + 2] = o.z;
+  r.d[3 * i] = d.x; r.d[3 * i + 1] = d.y; r.d[3 * i + 2] = d.z;
+}
+
+__global__ void k_zero_i32(int* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+__global__ void k_ray_count(const int* paths, int k, int* rays) { *rays = *paths * k; }
+
+__global__ void k_cam(CameraK cam, uint32_t spp, int N, QueueView q, RaysView rays, int* count_out) {
+  const int lane = blockIdx.x * blockDim.x + threadIdx.x;
+  if (lane == 0) *count_out = N;
+  if (lane >= N) return;
+  Rng r = lane_rng(static_cast<uint32_t>(lane));
+  const float u0 = r.next(), u1 = r.next();
+  f3 o, d;
+  int p;
+  camera_ray(cam, static_cast<uint32_t>(lane), spp, u0, u1, &o, &d, &p);
+  put_ray(rays, lane, o, d);
+  q.lane[lane] = static_cast<uint32_t>(lane);
+  q.rng[lane] = r.state;
+}
+
+__global__ __launch_bounds__(256) void k_shade(ShadeParams P, QueueView qi, const int* __restrict__ count_in,
+                                               RaysView rin, QueueView qo, int* __restrict__ count_out,
+                                               RaysView rout, CacheView cache, float* __restrict__ lane_L,
+                                               uint64_t* __restrict__ lane_rng_out) {
+  const SceneView& sv = P.sv;
+  const int cnt = *count_in;
+  const int d = P.depth;
+  for (int base = blockIdx.x * blockDim.x; base < cnt; base += gridDim.x * blockDim.x) {
+    const int j = base + threadIdx.x;
+    const bool have = j < cnt;
+    bool alive = false;
+    uint32_t lane = 0;
+    Rng rng{0, 0};
+    f3 T = splat(1.f), L = splat(0.f);
+    float J0 = 1.f;
+    Its its;
+    float cu = 0.f, cv = 0.f;
+    f3 O = P.cam.pos;
+    if (have) {
+      lane = qi.lane[j];
+      rng.state = qi.rng[j];
+      rng.inc = lane_inc(lane);
+      if (d == 0) {
+        const int ht = rin.tri[j];
+        const float hu = rin.u[j], hv = rin.v[j];
+        if (P.keep) {
+          cache.tri[lane] = ht;
+          cache.u[lane] = hu;
+          cache.v[lane] = hv;
+        }
+        its = reconstruct(sv, ht, hu, hv, O, true);
+        cu = hu;
+        cv = hv;
+        alive = its.valid;
+        if (alive && P.ad_jac) J0 = its.area / its.area;
+      } else {
+        T = load3(qi.T, qi.cap, j);
+        L = load3(qi.L, qi.cap, j);
+        J0 = qi.J0[j];
+        O = load3(qi.O, qi.cap, j);
+        const Its ip = reconstruct(sv, qi.tri[j], qi.u[j], qi.v[j], O, true);
+        const int rb = j * P.K_in;
+        const int64_t cslot = 1 + static_cast<int64_t>(d - 1) * P.Kmax;
+        int k = 0;
+        // next-event estimation (src/path_integrator.cc:103-131)
+        for (uint32_t i = 0; i < P.nL; ++i, ++k) {
+          float ru = rng.next();
+          const float rv = rng.next();
+          const int ht = rin.tri[rb + k];
+          const float hu = rin.u[rb + k], hv = rin.v[rb + k];
+          if (P.keep) {
+            const int64_t ci = (cslot + k) * cache.N + lane;
+            cache.tri[ci] = ht;
+            cache.u[ci] = hu;
+            cache.v[ci] = hv;
+          }
+          float pdf_choose;
+          const int obj = discrete_sample(sv.obj_pmf, sv.obj_cmf, sv.n_objects, &ru, &pdf_choose);
+          const int l = sv.objs[obj].light;
+          if (l < 0) continue;
+          const LightPoint lp = sample_light_point(sv, l, ru, rv);
+          const float pdf_light = lp.pdf * pdf_choose;
+          const f3 dir = normalize(lp.p - ip.p);
+          const Its itl = reconstruct(sv, ht, hu, hv, ip.p, true);
+          const bool is_hit = itl.valid && itl.is_light && sqnorm(itl.p - lp.p) < REAL_EPS;
+          if (!is_hit) continue;
+          const f3 emit = f3{sv.light_emit[3 * l], sv.light_emit[3 * l + 1], sv.light_emit[3 * l + 2]};
+          const f3 bsdf = mat_eval(sv, ip, dir);
+          const float tmp = dot(itl.n, -dir) / itl.d2;
+          const float G = dot(ip.n, dir) * tmp;
+          f3 value = (emit * G) * bsdf;
+          if (P.ad_jac) value = value * (itl.area / itl.area);
+          value = value / pdf_light;
+          float weight = 1.0f / static_cast<float>(P.nL);
+          if (P.nB > 0) weight = weight * mis_weight(pdf_light, mat_pdf(ip, dir) * tmp, P.nL, P.nB);
+          L = L + (T * weight) * value;
+        }
+        // BSDF sampling with MIS (src/path_integrator.cc:134-160)
+        for (uint32_t i = 0; i < P.nB; ++i, ++k) {
+          const float ru = rng.next();
+          const float rv = rng.next();
+          const int ht = rin.tri[rb + k];
+          const float hu = rin.u[rb + k], hv = rin.v[rb + k];
+          if (P.keep) {
+            const int64_t ci = (cslot + k) * cache.N + lane;
+            cache.tri[ci] = ht;
+            cache.u[ci] = hu;
+            cache.v[ci] = hv;
+          }
+          const Sampled smp = mat_sample(sv, ip, ru, rv);
+          const Its itl = reconstruct(sv, ht, hu, hv, ip.p, true);
+          if (!(itl.valid && itl.is_light)) continue;
+          const int l = itl.light;
+          const f3 emit = f3{sv.light_emit[3 * l], sv.light_emit[3 * l + 1], sv.light_emit[3 * l + 2]};
+          const float tmp = dot(itl.n, -smp.wo) / itl.d2;
+          const float G = dot(ip.n, smp.wo) * tmp;
+          const float pdf_bsdf = smp.pdf * tmp;
+          f3 value = (emit * G) * smp.bsdf;
+          if (P.ad_jac) value = value * (itl.area / itl.area);
+          value = value / pdf_bsdf;
+          float weight = 1.0f / static_cast<float>(P.nB);
+          if (P.nL > 0) {
+            const float pdf_light = (sv.light_power[l] / *sv.lights_total) * (1.0f / sv.light_area[l]);
+            weight = weight * mis_weight(pdf_bsdf, pdf_light, P.nB, P.nL);
+          }
+          L = L + (T * weight) * value;
+        }
+        // extension (src/path_integrator.cc:163-181)
+        {
+          const float ru = rng.next();
+          const float rv = rng.next();
+          if (d - 1 < P.D - 1) {
+            const int ht = rin.tri[rb + k];
+            const float hu = rin.u[rb + k], hv = rin.v[rb + k];
+            if (P.keep) {
+              const int64_t ci = (cslot + k) * cache.N + lane;
+              cache.tri[ci] = ht;
+              cache.u[ci] = hu;
+              cache.v[ci] = hv;
+            }
+            const Sampled smp = mat_sample(sv, ip, ru, rv);
+            its = reconstruct(sv, ht, hu, hv, ip.p, true);
+            cu = hu;
+            cv = hv;
+            alive = smp.valid && its.valid;
+            if (alive) {
+              const float tmp = dot(its.n, -smp.wo) / its.d2;
+              const float G = dot(ip.n, smp.wo) * tmp;
+              const float pdf = smp.pdf * tmp;
+              f3 f = smp.bsdf;
+              if (P.ad_jac) f = f * (its.area / its.area);
+              T = T * ((f * G) / pdf);
+              O = ip.p;
+            }
+          }
+        }
+      }
+      // generate depth d at `its` (src/path_integrator.cc:92-99 first)
+      if (alive && d < P.D) {
+        if (its.is_light) {
+          if (P.elv && d == 0) {
+            const int l = its.light;
+            L = L + T * f3{sv.light_emit[3 * l], sv.light_emit[3 * l + 1], sv.light_emit[3 * l + 2]};
+          }
+          alive = false;
+        }
+      } else {
+        alive = false;
+      }
+    }
+    const int slot = wave_alloc(alive, count_out);
+    if (have && alive) {
+      const uint64_t rng_start = rng.state;
+      const int rb = slot * P.K_out;
+      int k = 0;
+      for (uint32_t i = 0; i < P.nL; ++i, ++k) {
+        float ru = rng.next();
+        const float rv = rng.next();
+        float pdf_choose;
+        const int obj = discrete_sample(sv.obj_pmf, sv.obj_cmf, sv.n_objects, &ru, &pdf_choose);
+        const int l = sv.objs[obj].light;
+        f3 dir = splat(__builtin_nanf(""));
+        if (l >= 0) {
+          const LightPoint lp = sample_light_point(sv, l, ru, rv);
+          dir = normalize(lp.p - its.p);
+        }
+        put_ray(rout, rb + k, its.p, dir);
+      }
+      for (uint32_t i = 0; i < P.nB; ++i, ++k) {
+        const float ru = rng.next();
+        const float rv = rng.next();
+        const Sampled smp = mat_sample(sv, its, ru, rv);
+        put_ray(rout, rb + k, its.p, smp.wo);
+      }
+      {
+        const float ru = rng.next();
+        const float rv = rng.next();
+        if (d < P.D - 1) {
+          const Sampled smp = mat_sample(sv, its, ru, rv);
+          put_ray(rout, rb + k, its.p, smp.wo);
+        }
+      }
+      qo.lane[slot] = lane;
+      qo.rng[slot] = rng_start;
+      store3(qo.T, qo.cap, slot, T);
+      store3(qo.L, qo.cap, slot, L);
+      store3(qo.O, qo.cap, slot, O);
+      qo.J0[slot] = J0;
+      qo.tri[slot] = its.tri;
+      qo.u[slot] = cu;
+      qo.v[slot] = cv;
+    } else if (have) {
+      if (!isfinite(sqnorm(L))) L = splat(0.f);
+      L = L * J0;
+      lane_L[3 * static_cast<int64_t>(lane)] = L.x;
+      lane_L[3 * static_cast<int64_t>(lane) + 1] = L.y;
+      lane_L[3 * static_cast<int64_t>(lane) + 2] = L.z;
+      if (lane_rng_out) lane_rng_out[lane] = rng.state;
+    }
+  }
+}
+
+// Film::add + n_pass averaging (src/camera.h:75-87, src/path_integrator.cc:385-400)
+__global__ void k_film(const float* __restrict__ lane_L, int P_, uint32_t spp, uint32_t n_pass,
+                       float* __restrict__ out) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P_) return;
+  for (int c = 0; c < 3; ++c) {
+    float s = 0.f;
+    for (uint32_t k = 0; k < spp; ++k) s = s + lane_L[3 * (static_cast<int64_t>(p) * spp + k) + c];
+    if (spp > 1) s = s / static_cast<float>(spp);
+    float acc = 0.f;
+    for (uint32_t n = 0; n < n_pass; ++n) acc = acc + s;
+    if (n_pass > 1) acc = acc / static_cast<float>(n_pass);
+    out[3 * p + c] = acc;
+  }
+}
+
+inline int blocks(int64_t n, int t) { return static_cast<int>((n + t - 1) / t); }
+
+int g_cus = 0;
+int num_cus() {
+  if (g_cus == 0) {
+    int dev = 0;
+    hipGetDevice(&dev);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) == hipSuccess) g_cus = prop.multiProcessorCount;
+    if (g_cus <= 0) g_cus = 256;
+  }
+  return g_cus;
+}
+
+QueueView carve_queue(DevBuf& b, int cap) {
+  QueueView q;
+  char* p = static_cast<char*>(b.p);
+  q.cap = cap;
+  q.rng = reinterpret_cast<uint64_t*>(p);
+  p += sizeof(uint64_t) * cap;
+  q.lane = reinterpret_cast<uint32_t*>(p);
+  p += 4 * static_cast<size_t>(cap);
+  q.tri = reinterpret_cast<int*>(p);
+  p += 4 * static_cast<size_t>(cap);
+  q.u = reinterpret_cast<float*>(p);
+  p += 4 * static_cast<size_t>(cap);
+  q.v = reinterpret_cast<float*>(p);
+  p += 4 * static_cast<size_t>(cap);
+  q.J0 = reinterpret_cast<float*>(p);
+  p += 4 * static_cast<size_t>(cap);
+  q.T = reinterpret_cast<float*>(p);
+  p += 12 * static_cast<size_t>(cap);
+  q.L = reinterpret_cast<float*>(p);
+  p += 12 * static_cast<size_t>(cap);
+  q.O = reinterpret_cast<float*>(p);
+  return q;
+}
+constexpr size_t QUEUE_BYTES_PER = 8 + 4 * 5 + 12 * 3;
+
+RaysView carve_rays(DevBuf& rb, DevBuf& hb, int64_t cap) {
+  RaysView r;
+  r.o = rb.as<float>();
+  r.d = r.o + 3 * cap;
+  r.tri = hb.as<int>();
+  r.u = reinterpret_cast<float*>(r.tri + cap);
+  r.v = r.u + cap;
+  return r;
+}
+
+}  // namespace
+
+int trace_rays(const SceneView& sv, const float* ro, const float* rd, int n, const int* count_dev, int* tri,
+               float* t, float* u, float* v, int* spill, int grid, hipStream_t st) {
+  hipLaunchKernelGGL(k_trace, dim3(grid), dim3(TRACE_THREADS), 0, st, sv, ro, rd, n, count_dev, tri, t, u, v, spill);
+  PSDR_HIP_CHECK(hipGetLastError());
+  return PSDR_OK;
+}
+
+int trace_grid(int64_t max_rays) {
+  const int g = std::max(1, std::min(blocks(max_rays, TRACE_THREADS), num_cus() * 8));
+  return g;
+}
+
+size_t spill_bytes(int grid) { return static_cast<size_t>(grid) * TRACE_THREADS * (STACK_TOTAL - STACK_LDS) * 4; }
+
+int lane_draws(uint32_t lane0, int n_lanes, int n_draws, uint32_t* out, hipStream_t st) {
+  if (n_lanes <= 0) return PSDR_OK;
+  hipLaunchKernelGGL(k_lane_draws, dim3(blocks(n_lanes, 256)), dim3(256), 0, st, lane0, n_lanes, n_draws, out);
+  PSDR_HIP_CHECK(hipGetLastError());
+  return PSDR_OK;
+}
+
+int camera_rays(const CameraK& cam, uint32_t spp, float* o, float* d, int* pix, hipStream_t st) {
+  const int64_t N = static_cast<int64_t>(spp) * cam.H * cam.W;
+  if (N <= 0) return PSDR_OK;
+  hipLaunchKernelGGL(k_camera_rays, dim3(blocks(N, 256)), dim3(256), 0, st, cam, spp, N, o, d, pix);
+  PSDR_HIP_CHECK(hipGetLastError());
+  return PSDR_OK;
+}
+
+int render_forward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& camd, float* out, int keep,
+                   hipStream_t st) {
+  const CameraK cam = make_camera(camd);
+  const int64_t N64 = static_cast<int64_t>(cfg.spp_interior) * cam.H * cam.W;
+  if (N64 <= 0 || N64 > 0x7fffffff / 4) {
+    set_error(PSDR_ERR_ARG, "render: spp*H*W out of range");
+    return PSDR_ERR_ARG;
+  }
+  const int N = static_cast<int>(N64);
+  const int D = static_cast<int>(cfg.max_bounce);
+  const uint32_t nL = cfg.mis_light_samples, nB = cfg.mis_bsdf_samples;
+  const int Kmax = static_cast<int>(nL + nB + 1);
+  const int64_t ray_cap = static_cast<int64_t>(N) * std::max(Kmax, 1);
+  // buffers
+  if (s->lane_L.ensure(12ull * N) || s->q_a.ensure(QUEUE_BYTES_PER * N) || s->q_b.ensure(QUEUE_BYTES_PER * N) ||
+      s->rays.ensure(24ull * ray_cap) || s->hits.ensure(12ull * ray_cap) || s->counters_r.ensure(256))
+    return PSDR_ERR_NOMEM;
+  const int tgrid = trace_grid(ray_cap);
+  if (s->spill.ensure(spill_bytes(tgrid))) return PSDR_ERR_NOMEM;
+  const int64_t slots = 1 + static_cast<int64_t>(D) * Kmax;
+  CacheView cache{nullptr, nullptr, nullptr, N};
+  if (keep) {
+    if (s->hit_cache.ensure(12ull * slots * N)) return PSDR_ERR_NOMEM;
+    cache.tri = s->hit_cache.as<int>();
+    cache.u = reinterpret_cast<float*>(cache.tri + slots * N);
+    cache.v = cache.u + slots * N;
+  }
+  QueueView qa = carve_queue(s->q_a, N), qb = carve_queue(s->q_b, N);
+  RaysView rays_a = carve_rays(s->rays, s->hits, ray_cap);
+  // second ray set: the shade kernel reads the hits of the previous depth while
+  // writing the next depth's rays; rays and hits are double-buffered by halves.
+  int* counters = s->counters_r.as<int>();
+  hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, counters, 64);
+  const SceneView sv = s->view();
+  float* lane_L = s->lane_L.as<float>();
+
+  // stage 0: camera rays + primary hits
+  hipLaunchKernelGGL(k_cam, dim3(blocks(N, 256)), dim3(256), 0, st, cam, cfg.spp_interior, N, qa, rays_a,
+                     counters + 0);
+  PSDR_HIP_CHECK(hipGetLastError());
+  int rc = trace_rays(sv, rays_a.o, rays_a.d, N, counters + 0, rays_a.tri, nullptr, rays_a.u, rays_a.v,
+                      s->spill.as<int>(), tgrid, st);
+  if (rc) return rc;
+  // ray buffers ping-pong: a depth's shade reads rays_in (hits) and writes rays_out
+  if (s->rays2.ensure(24ull * ray_cap) || s->hits2.ensure(12ull * ray_cap)) return PSDR_ERR_NOMEM;
+  RaysView rays_b = carve_rays(s->rays2, s->hits2, ray_cap);
+  QueueView* qin = &qa;
+  QueueView* qout = &qb;
+  RaysView* rin = &rays_a;
+  RaysView* rout = &rays_b;
+  const int sgrid = std::max(1, std::min(blocks(N, 256), num_cus() * 8));
+  for (int d = 0; d <= D; ++d) {
+    ShadeParams P;
+    P.sv = sv;
+    P.cam = cam;
+    P.depth = d;
+    P.D = D;
+    P.nL = nL;
+    P.nB = nB;
+    P.elv = cfg.enable_light_visable ? 1 : 0;
+    P.ad_jac = keep ? 1 : 0;
+    P.K_in = d == 0 ? 1 : static_cast<int>(nL + nB) + ((d - 1 < D - 1) ? 1 : 0);
+    P.K_out = d < D ? static_cast<int>(nL + nB) + ((d < D - 1) ? 1 : 0) : 0;
+    P.Kmax = Kmax;
+    P.keep = keep;
+    // counters: [d] = paths entering shade(d); [d+1] = survivors; [64 + d] = rays out
+    hipLaunchKernelGGL(k_shade, dim3(sgrid), dim3(256), 0, st, P, *qin, counters + d, *rin, *qout, counters + d + 1,
+                       *rout, cache, lane_L, static_cast<uint64_t*>(nullptr));
+    PSDR_HIP_CHECK(hipGetLastError());
+    if (d < D && P.K_out > 0) {
+      hipLaunchKernelGGL(k_ray_count, dim3(1), dim3(1), 0, st, counters + d + 1, P.K_out, counters + 64 + d);
+      rc = trace_rays(sv, rout->o, rout->d, static_cast<int>(ray_cap), counters + 64 + d, rout->tri, nullptr,
+                      rout->u, rout->v, s->spill.as<int>(), tgrid, st);
+      if (rc) return rc;
+    }
+    std::swap(qin, qout);
+    std::swap(rin, rout);
+  }
+  const int P_ = cam.H * cam.W;
+  hipLaunchKernelGGL(k_film, dim3(blocks(P_, 256)), dim3(256), 0, st, lane_L, P_, cfg.spp_interior, cfg.n_pass, out);
+  PSDR_HIP_CHECK(hipGetLastError());
+  s->last_N = N;
+  s->last_K = Kmax;
+  s->last_D = D;
+  s->have_cache = keep != 0;
+  s->last_cfg = cfg;
+  s->last_cam = camd;
+  return PSDR_OK;
+}
+
+}  // namespace psdr

@@ -1,0 +1,129 @@
+// Internal scene representation of the gfx950 path tracer (device data layout).
+//
+// HBM layout (all SoA / 16-byte records, global triangle order = object order):
+//   tri_geom  [n_tris*5] float4 : {p0, area} {p1, -} {p2, -} {normal, -} {tangent, -}
+//   tri_uv    [n_tris*6] float  : corner uvs (non-differentiable, src/mesh.cc:29-36)
+//   leaf      [n_tris*3] float4 : LBVH-ordered {p0, tri id} {p1-p0, -} {p2-p0, -}
+//   node      [(n-1)*4]  float4 : {lo_l.xyz, hi_l.x} {hi_l.yz, lo_r.xy} {lo_r.z, hi_r.xyz}
+//                                 {left, right, -, -} (child ids; leaf k -> k | 0x80000000)
+//   edge_geom [n_edges*16] float: p0 p1 out n_left n_right length (src/mesh.h:28-42)
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/psdr_hip.h"
+#include "common.h"
+
+namespace psdr {
+
+// Error reporting: HIP failures become a PSDR_ERR_HIP status with a message.
+struct Status {
+  int code = PSDR_OK;
+  std::string msg;
+};
+void set_error(int code, const std::string& msg);
+#define PSDR_HIP_CHECK(expr)                                                              \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess) {                                                               \
+      ::psdr::set_error(PSDR_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+      return PSDR_ERR_HIP;                                                                \
+    }                                                                                     \
+  } while (0)
+
+// Grow-only device buffer.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t n);
+  void release();
+  template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+// Object record as seen by kernels.
+struct ObjectK {
+  int mat, light;
+  int alb_off, alb_h, alb_w;
+  int rough_off, rough_h, rough_w;
+  int tri_begin, tri_count;
+  int grad_mask;
+};
+
+// Light record as seen by kernels.
+struct LightK {
+  int obj;
+  int tri_begin, tri_count;  // global triangle range of the light mesh
+  int pmf_off;               // offset into light_tri_pmf/cmf
+};
+
+// Read-only view of a scene passed by value to render kernels.
+struct SceneView {
+  const float4* tri_geom;
+  const float* tri_uv;
+  const int* tri_obj;
+  const int* faces;
+  const float4* leaf;
+  const float4* node;
+  int n_tris;
+  const ObjectK* objs;
+  int n_objects;
+  const LightK* lights;
+  int n_lights;
+  const float* light_emit;   // [n_lights*3]
+  const float* light_power;  // [n_lights]
+  const float* light_area;   // [n_lights]
+  const float* lights_total; // [1]
+  const float* light_tri_pmf;
+  const float* light_tri_cmf;
+  const float* obj_pmf;  // light pick over objects
+  const float* obj_cmf;
+  const float* texels;
+  const float* vertices;
+  const int* edge_topo;
+  const float* edge_geom;
+  const float* edge_pmf;
+  const float* edge_cmf;
+  int n_edges;
+};
+
+}  // namespace psdr
+
+struct psdr_scene {
+  int n_vertices = 0, n_tris = 0, n_objects = 0, n_lights = 0, n_texels = 0, n_edges = 0;
+  std::vector<psdr_object> objects;
+  std::vector<int> light_obj;
+  // caller-owned device buffers of the last create/update
+  const float* vertices = nullptr;
+  const float* texels = nullptr;
+  const float* light_emit = nullptr;
+  // topology copies
+  psdr::DevBuf faces, tri_uv, tri_obj, objs, lights, edge_topo;
+  // derived
+  psdr::DevBuf tri_geom, edge_geom, edge_pmf, edge_cmf, light_tri_pmf, light_tri_cmf, light_power, light_area,
+      lights_total, obj_pmf, obj_cmf, scan_tmp;
+  // LBVH
+  psdr::DevBuf morton, order, child, parent, box, leaf, node, counters, sort_keys, sort_vals, sort_hist, bounds;
+  // renderer scratch
+  psdr::DevBuf lane_L, q_a, q_b, rays, hits, rays2, hits2, spill, hit_cache, counters_r, hc_mass, hc_pmf,
+      hc_cmf, stats;
+  int light_tri_total = 0;
+  // last forward (for backward / stats)
+  uint64_t last_N = 0;
+  int last_K = 0, last_D = 0;
+  bool have_cache = false;
+  psdr_config last_cfg{};
+  psdr_camera last_cam{};
+
+  psdr::SceneView view() const;
+};
+
+namespace psdr {
+// scene.hip
+int scene_derive(psdr_scene* s, hipStream_t st);
+int build_lbvh(psdr_scene* s, hipStream_t st);
+// camera (host, capi.cpp)
+CameraK make_camera(const psdr_camera& c);
+}  // namespace psdr

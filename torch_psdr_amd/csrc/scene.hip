@@ -1,0 +1,544 @@
+// Scene derivation and LBVH construction on gfx950.
+//
+//  * k_tri_prep       : per-triangle record (src/mesh.cc:5-67).
+//  * k_edge_geom      : per-edge geometry (src/mesh.cc:97-204) over a canonical
+//                       (i<j)-sorted edge list built once at scene creation.
+//  * k_sampler        : DiscreteSampler tables (src/sampler.h:86-96) with the
+//                       project's fixed fp32 order (chunks of 256 summed left to
+//                       right, then chunk totals left to right).
+//  * LBVH (replaces the OptiX GAS, src/scene_optix.cc:187-238): 30-bit Morton
+//    codes of padded triangle-box centres, a stable LSD radix sort (LDS digit
+//    histograms + wave-ballot multisplit ranking), the Karras (2012) hierarchy
+//    over keys (morton << 32 | sorted position), and an atomic bottom-up refit.
+//    The integer output is deterministic and matches the CPU restatement exactly.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "scene.h"
+
+namespace psdr {
+
+namespace {
+
+constexpr int SCAN_CHUNK = 256;
+constexpr int SORT_THREADS = 256;
+constexpr int SORT_ITEMS = 8;
+constexpr int SORT_TILE = SORT_THREADS * SORT_ITEMS;
+
+__device__ __forceinline__ f3 ld3(const float* p) { return f3{p[0], p[1], p[2]}; }
+__device__ __forceinline__ f3 xyz(float4 a) { return f3{a.x, a.y, a.z}; }
+
+// --- triangle records ------------------------------------------------------
+__global__ void k_tri_prep(const float* __restrict__ V, const int* __restrict__ faces,
+                           const float* __restrict__ tri_uv, int n, float4* __restrict__ geom) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= n) return;
+  const f3 p0 = ld3(V + 3 * faces[3 * f]);
+  const f3 p1 = ld3(V + 3 * faces[3 * f + 1]);
+  const f3 p2 = ld3(V + 3 * faces[3 * f + 2]);
+  const float* uv = tri_uv + 6 * f;
+  const f3 a = p2 - p1;
+  const f3 b = p0 - p2;
+  const f3 c = p1 - p0;
+  const f3 nrm = normalize(cross(c, b));
+  const float duv1y = uv[3] - uv[1];
+  const float duv2y = uv[5] - uv[1];
+  const f3 tg = normalize(duv2y * c + duv1y * b);
+  const float al = norm(a), bl = norm(b), cl = norm(c);
+  const float p = ((al + bl) + cl) / 2.0f;
+  const float area = safe_sqrt(((p * (p - al)) * (p - bl)) * (p - cl));
+  geom[5 * f + 0] = make_float4(p0.x, p0.y, p0.z, area);
+  geom[5 * f + 1] = make_float4(p1.x, p1.y, p1.z, 0.f);
+  geom[5 * f + 2] = make_float4(p2.x, p2.y, p2.z, 0.f);
+  geom[5 * f + 3] = make_float4(nrm.x, nrm.y, nrm.z, 0.f);
+  geom[5 * f + 4] = make_float4(tg.x, tg.y, tg.z, 0.f);
+}
+
+// --- edges -------------------------------------------------------------------
+__global__ void k_edge_geom(const float* __restrict__ V, const int* __restrict__ topo, int n,
+                            float* __restrict__ geom, float* __restrict__ len) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const int* t = topo + 6 * e;
+  const f3 p0 = ld3(V + 3 * t[0]);
+  const f3 p1 = ld3(V + 3 * t[1]);
+  const f3 p2l = ld3(V + 3 * t[4]);
+  const f3 p2r = ld3(V + 3 * t[5]);
+  f3 nl = normalize(cross(p1 - p0, p2l - p1));
+  f3 nr = normalize(cross(p2r - p1, p1 - p0));
+  const bool shared = t[2] != t[3];
+  f3 out = shared ? normalize(nl + nr) : normalize(cross(p1 - p0, nl));
+  const f3 cp = normalize((p0 + p1) - (p2l + p2r));
+  if (dot(cp, out) < 0.f && shared && norm(nl - nr) > REAL_EPS) {
+    nl = -nl;
+    nr = -nr;
+    out = -out;
+  }
+  float* g = geom + 16 * e;
+  g[0] = p0.x; g[1] = p0.y; g[2] = p0.z;
+  g[3] = p1.x; g[4] = p1.y; g[5] = p1.z;
+  g[6] = out.x; g[7] = out.y; g[8] = out.z;
+  g[9] = nl.x; g[10] = nl.y; g[11] = nl.z;
+  g[12] = nr.x; g[13] = nr.y; g[14] = nr.z;
+  const float l = norm(p0 - p1);
+  g[15] = l;
+  len[e] = l;
+}
+
+// --- discrete sampler tables ------------------------------------------------
+// One block per segment [seg_begin, seg_begin + seg_count) of w.  tmp holds two
+// floats per chunk per segment (at 2*seg_begin/SCAN_CHUNK + ...).
+__global__ void k_sampler(const float* __restrict__ w, const int* __restrict__ seg_begin,
+                          const int* __restrict__ seg_count, float* __restrict__ pmf, float* __restrict__ cmf,
+                          float* __restrict__ totals, float* __restrict__ tmp) {
+  const int sgi = blockIdx.x;
+  const int b = seg_begin ? seg_begin[sgi] : 0;
+  const int n = seg_count[sgi];
+  const int nchunk = (n + SCAN_CHUNK - 1) / SCAN_CHUNK;
+  float* csum = tmp + 2 * ((b + SCAN_CHUNK - 1) / SCAN_CHUNK + sgi);
+  float* coff = csum + nchunk;
+  __shared__ float s_total;
+  for (int c = threadIdx.x; c < nchunk; c += blockDim.x) {
+    float s = 0.f;
+    const int e = min(n, (c + 1) * SCAN_CHUNK);
+    for (int i = c * SCAN_CHUNK; i < e; ++i) s = s + w[b + i];
+    csum[c] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float run = 0.f;
+    for (int c = 0; c < nchunk; ++c) run = run + csum[c];
+    s_total = run;
+    if (totals) totals[sgi] = run;
+  }
+  __syncthreads();
+  const float total = s_total;
+  for (int c = threadIdx.x; c < nchunk; c += blockDim.x) {
+    float s = 0.f;
+    const int e = min(n, (c + 1) * SCAN_CHUNK);
+    for (int i = c * SCAN_CHUNK; i < e; ++i) {
+      const float p = w[b + i] / total;
+      pmf[b + i] = p;
+      s = s + p;
+    }
+    csum[c] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float run = 0.f;
+    for (int c = 0; c < nchunk; ++c) {
+      coff[c] = run;
+      run = run + csum[c];
+    }
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < nchunk; c += blockDim.x) {
+    float s = 0.f;
+    const float off = coff[c];
+    const int e = min(n, (c + 1) * SCAN_CHUNK);
+    for (int i = c * SCAN_CHUNK; i < e; ++i) {
+      s = s + pmf[b + i];
+      cmf[b + i] = off + s;
+    }
+  }
+}
+
+// light-triangle areas gathered contiguously per light, then per-light sampler.
+__global__ void k_light_areas(const float4* __restrict__ geom, const LightK* __restrict__ lights, int n_lights,
+                              float* __restrict__ out, int* __restrict__ seg_begin, int* __restrict__ seg_count) {
+  const int l = blockIdx.x;
+  if (l >= n_lights) return;
+  const LightK L = lights[l];
+  for (int k = threadIdx.x; k < L.tri_count; k += blockDim.x) out[L.pmf_off + k] = geom[5 * (L.tri_begin + k)].w;
+  if (threadIdx.x == 0) {
+    seg_begin[l] = L.pmf_off;
+    seg_count[l] = L.tri_count;
+  }
+}
+
+// AreaLight power (src/emitter.cc:10-11) and the per-object light weights.
+__global__ void k_light_power(const ObjectK* __restrict__ objs, int n_objects, const float* __restrict__ emit,
+                              const float* __restrict__ light_area, float* __restrict__ light_power,
+                              float* __restrict__ obj_w) {
+  const int o = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= n_objects) return;
+  const int l = objs[o].light;
+  float w = 0.f;
+  if (l >= 0) {
+    const float lum = (emit[3 * l] * 0.2126f + emit[3 * l + 1] * 0.7152f) + emit[3 * l + 2] * 0.0722f;
+    w = (REAL_2PI * lum) * light_area[l];
+    light_power[l] = w;
+  }
+  obj_w[o] = w;
+}
+
+// --- LBVH --------------------------------------------------------------------
+__device__ __forceinline__ void tri_box(const float4* geom, int f, float lo[3], float hi[3]) {
+  const float4 a = geom[5 * f], b = geom[5 * f + 1], c = geom[5 * f + 2];
+  const float xs[3][3] = {{a.x, b.x, c.x}, {a.y, b.y, c.y}, {a.z, b.z, c.z}};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float l = fminf(xs[k][0], fminf(xs[k][1], xs[k][2]));
+    const float h = fmaxf(xs[k][0], fmaxf(xs[k][1], xs[k][2]));
+    const float pad = (fabsf(l) + fabsf(h)) * 1e-6f + 1e-30f;
+    lo[k] = l - pad;
+    hi[k] = h + pad;
+  }
+}
+__device__ __forceinline__ void tri_center(const float4* geom, int f, float c[3]) {
+  const float4 a = geom[5 * f], b = geom[5 * f + 1], d = geom[5 * f + 2];
+  const float xs[3][3] = {{a.x, b.x, d.x}, {a.y, b.y, d.y}, {a.z, b.z, d.z}};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float l = fminf(xs[k][0], fminf(xs[k][1], xs[k][2]));
+    const float h = fmaxf(xs[k][0], fmaxf(xs[k][1], xs[k][2]));
+    c[k] = (l + h) * 0.5f;
+  }
+}
+
+// centre bounds, phase 1: per-block min/max -> part[6*block]
+__global__ void k_center_bounds(const float4* __restrict__ geom, int n, float* __restrict__ part) {
+  __shared__ float s[6][256];
+  float mn[3] = {F_INF, F_INF, F_INF}, mx[3] = {-F_INF, -F_INF, -F_INF};
+  for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < n; f += gridDim.x * blockDim.x) {
+    float c[3];
+    tri_center(geom, f, c);
+    for (int k = 0; k < 3; ++k) {
+      mn[k] = fminf(mn[k], c[k]);
+      mx[k] = fmaxf(mx[k], c[k]);
+    }
+  }
+  for (int k = 0; k < 3; ++k) {
+    s[k][threadIdx.x] = mn[k];
+    s[3 + k][threadIdx.x] = mx[k];
+  }
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st)
+      for (int k = 0; k < 3; ++k) {
+        s[k][threadIdx.x] = fminf(s[k][threadIdx.x], s[k][threadIdx.x + st]);
+        s[3 + k][threadIdx.x] = fmaxf(s[3 + k][threadIdx.x], s[3 + k][threadIdx.x + st]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) part[6 * blockIdx.x + threadIdx.x] = s[threadIdx.x][0];
+}
+// phase 2 (one block): reduce parts -> bounds[6]
+__global__ void k_center_bounds2(const float* __restrict__ part, int nparts, float* __restrict__ bounds) {
+  if (threadIdx.x >= 6) return;
+  const int k = threadIdx.x;
+  float v = k < 3 ? F_INF : -F_INF;
+  for (int p = 0; p < nparts; ++p) v = k < 3 ? fminf(v, part[6 * p + k]) : fmaxf(v, part[6 * p + k]);
+  bounds[k] = v;
+}
+
+__device__ __forceinline__ uint32_t expand_bits(uint32_t v) {
+  v = (v * 0x00010001u) & 0xFF0000FFu;
+  v = (v * 0x00000101u) & 0x0F00F00Fu;
+  v = (v * 0x00000011u) & 0xC30C30C3u;
+  v = (v * 0x00000005u) & 0x49249249u;
+  return v;
+}
+
+__global__ void k_morton(const float4* __restrict__ geom, int n, const float* __restrict__ bounds,
+                         uint32_t* __restrict__ morton, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= n) return;
+  float c[3];
+  tri_center(geom, f, c);
+  uint32_t q[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float ext = bounds[3 + k] - bounds[k];
+    const float x = ext > 0.f ? (c[k] - bounds[k]) / ext : 0.5f;
+    q[k] = static_cast<uint32_t>(fminf(fmaxf(x * 1024.0f, 0.0f), 1023.0f));
+  }
+  const uint32_t m = (expand_bits(q[0]) << 2) | (expand_bits(q[1]) << 1) | expand_bits(q[2]);
+  morton[f] = m;
+  keys[f] = m;
+  vals[f] = static_cast<uint32_t>(f);
+}
+
+__global__ __launch_bounds__(SORT_THREADS) void k_radix_hist(const uint32_t* __restrict__ keys, int n, int shift,
+                                                            uint32_t* __restrict__ hist) {
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  for (int r = 0; r < SORT_ITEMS; ++r) {
+    const int i = blockIdx.x * SORT_TILE + r * SORT_THREADS + threadIdx.x;
+    if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  hist[threadIdx.x * gridDim.x + blockIdx.x] = h[threadIdx.x];
+}
+
+// exclusive scan of m uint32 values, one block of 1024 threads
+__global__ __launch_bounds__(1024) void k_scan_u32(uint32_t* __restrict__ a, int m) {
+  __shared__ uint32_t s[1024];
+  const int per = (m + 1023) / 1024;
+  const int b = threadIdx.x * per, e = min(m, b + per);
+  uint32_t sum = 0;
+  for (int i = b; i < e; ++i) sum += a[i];
+  s[threadIdx.x] = sum;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const uint32_t v = threadIdx.x >= off ? s[threadIdx.x - off] : 0u;
+    __syncthreads();
+    s[threadIdx.x] += v;
+    __syncthreads();
+  }
+  uint32_t run = s[threadIdx.x] - sum;
+  for (int i = b; i < e; ++i) {
+    const uint32_t x = a[i];
+    a[i] = run;
+    run += x;
+  }
+}
+
+__global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(const uint32_t* __restrict__ kin,
+                                                               const uint32_t* __restrict__ vin,
+                                                               uint32_t* __restrict__ kout,
+                                                               uint32_t* __restrict__ vout, int n, int shift,
+                                                               const uint32_t* __restrict__ hist) {
+  __shared__ uint32_t s_base[256];
+  __shared__ uint32_t s_cnt[SORT_THREADS / 64][256];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  s_base[tid] = hist[tid * gridDim.x + blockIdx.x];
+  for (int w = 0; w < SORT_THREADS / 64; ++w) s_cnt[w][tid] = 0;
+  __syncthreads();
+  const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+  for (int r = 0; r < SORT_ITEMS; ++r) {
+    const int i = blockIdx.x * SORT_TILE + r * SORT_THREADS + tid;
+    const bool valid = i < n;
+    const uint32_t key = valid ? kin[i] : 0u;
+    const uint32_t val = valid ? vin[i] : 0u;
+    const uint32_t digit = (key >> shift) & 255u;
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int bit = 0; bit < 8; ++bit) {
+      const bool on = (digit >> bit) & 1u;
+      const uint64_t bb = __ballot(on);
+      peers &= on ? bb : ~bb;
+    }
+    const uint32_t rank = __popcll(peers & lt);
+    if (valid && rank == 0) s_cnt[wave][digit] = __popcll(peers);
+    __syncthreads();
+    if (valid) {
+      uint32_t off = s_base[digit];
+      for (int w = 0; w < wave; ++w) off += s_cnt[w][digit];
+      kout[off + rank] = key;
+      vout[off + rank] = val;
+    }
+    __syncthreads();
+    uint32_t tot = 0;
+    for (int w = 0; w < SORT_THREADS / 64; ++w) {
+      tot += s_cnt[w][tid];
+      s_cnt[w][tid] = 0;
+    }
+    s_base[tid] += tot;
+    __syncthreads();
+  }
+}
+
+__device__ __forceinline__ int kdelta(const uint32_t* ms, int n, int i, int j) {
+  if (j < 0 || j >= n) return -1;
+  const uint64_t a = (static_cast<uint64_t>(ms[i]) << 32) | static_cast<uint32_t>(i);
+  const uint64_t b = (static_cast<uint64_t>(ms[j]) << 32) | static_cast<uint32_t>(j);
+  return __clzll(static_cast<long long>(a ^ b));
+}
+
+__global__ void k_karras(const uint32_t* __restrict__ ms, int n, int* __restrict__ child, int* __restrict__ parent) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n - 1) return;
+  const int d = (kdelta(ms, n, i, i + 1) - kdelta(ms, n, i, i - 1)) >= 0 ? 1 : -1;
+  const int dmin = kdelta(ms, n, i, i - d);
+  int lmax = 2;
+  while (kdelta(ms, n, i, i + lmax * d) > dmin) lmax *= 2;
+  int l = 0;
+  for (int t = lmax / 2; t >= 1; t /= 2)
+    if (kdelta(ms, n, i, i + (l + t) * d) > dmin) l += t;
+  const int j = i + l * d;
+  const int dnode = kdelta(ms, n, i, j);
+  int s = 0, div = 2, t;
+  do {
+    t = (l + div - 1) / div;
+    if (kdelta(ms, n, i, i + (s + t) * d) > dnode) s += t;
+    div *= 2;
+  } while (t > 1);
+  const int g = i + s * d + min(d, 0);
+  const int left = (min(i, j) == g) ? (g | static_cast<int>(0x80000000u)) : g;
+  const int right = (max(i, j) == g + 1) ? ((g + 1) | static_cast<int>(0x80000000u)) : g + 1;
+  child[2 * i] = left;
+  child[2 * i + 1] = right;
+  parent[left < 0 ? (n - 1 + (left & 0x7fffffff)) : left] = i;
+  parent[right < 0 ? (n - 1 + (right & 0x7fffffff)) : right] = i;
+  if (i == 0) parent[0] = -1;
+}
+
+__global__ void k_leaf(const float4* __restrict__ geom, const uint32_t* __restrict__ order, int n,
+                       float4* __restrict__ leaf, int* __restrict__ counters) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n - 1) counters[k] = 0;
+  if (k >= n) return;
+  const int f = order ? static_cast<int>(order[k]) : k;
+  const float4 a = geom[5 * f], b = geom[5 * f + 1], c = geom[5 * f + 2];
+  const f3 p0 = xyz(a), p1 = xyz(b), p2 = xyz(c);
+  const f3 e1 = p1 - p0, e2 = p2 - p0;
+  leaf[3 * k] = make_float4(p0.x, p0.y, p0.z, __int_as_float(f));
+  leaf[3 * k + 1] = make_float4(e1.x, e1.y, e1.z, 0.f);
+  leaf[3 * k + 2] = make_float4(e2.x, e2.y, e2.z, 0.f);
+}
+
+__global__ void k_refit(const float4* __restrict__ geom, const uint32_t* __restrict__ order, int n,
+                        const int* __restrict__ child, const int* __restrict__ parent, float* box,
+                        int* counters) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  float lo[3], hi[3];
+  tri_box(geom, static_cast<int>(order[k]), lo, hi);
+  int c = k | static_cast<int>(0x80000000u);
+  int p = parent[n - 1 + k];
+  while (p >= 0) {
+    const int slot = (child[2 * p] == c) ? 0 : 1;
+    float* b = box + 12 * p + 6 * slot;
+    for (int a = 0; a < 3; ++a) {
+      __hip_atomic_store(b + a, lo[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(b + 3 + a, hi[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __threadfence();
+    const int old = atomicAdd(counters + p, 1);
+    if (old == 0) return;
+    __threadfence();
+    const float* sb = box + 12 * p + 6 * (1 - slot);
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = fminf(lo[a], __hip_atomic_load(sb + a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+      hi[a] = fmaxf(hi[a], __hip_atomic_load(sb + 3 + a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    }
+    c = p;
+    p = parent[p];
+  }
+}
+
+__global__ void k_pack_nodes(const float* __restrict__ box, const int* __restrict__ child, int n,
+                             float4* __restrict__ node) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n - 1) return;
+  const float* b = box + 12 * i;
+  node[4 * i] = make_float4(b[0], b[1], b[2], b[3]);
+  node[4 * i + 1] = make_float4(b[4], b[5], b[6], b[7]);
+  node[4 * i + 2] = make_float4(b[8], b[9], b[10], b[11]);
+  node[4 * i + 3] = make_float4(__int_as_float(child[2 * i]), __int_as_float(child[2 * i + 1]), 0.f, 0.f);
+}
+
+inline int blocks(long long n, int t) { return static_cast<int>((n + t - 1) / t); }
+
+}  // namespace
+
+// Sampler tables over `nseg` segments (device arrays seg_begin/seg_count).
+static int launch_sampler(const float* w, const int* seg_begin, const int* seg_count, int nseg, float* pmf,
+                          float* cmf, float* totals, float* tmp, hipStream_t st) {
+  if (nseg <= 0) return PSDR_OK;
+  hipLaunchKernelGGL(k_sampler, dim3(nseg), dim3(256), 0, st, w, seg_begin, seg_count, pmf, cmf, totals, tmp);
+  PSDR_HIP_CHECK(hipGetLastError());
+  return PSDR_OK;
+}
+
+int scene_derive(psdr_scene* s, hipStream_t st) {
+  const int nt = s->n_tris;
+  if (nt > 0) {
+    hipLaunchKernelGGL(k_tri_prep, dim3(blocks(nt, 256)), dim3(256), 0, st, s->vertices, s->faces.as<int>(),
+                       s->tri_uv.as<float>(), nt, s->tri_geom.as<float4>());
+    PSDR_HIP_CHECK(hipGetLastError());
+  }
+  // scratch layout in scan_tmp: [ints: seg_begin(nl), seg_count(nl), one(1)] [floats...]
+  const int nl = s->n_lights;
+  const int ne = s->n_edges;
+  const int nobj = s->n_objects;
+  int* segi = s->scan_tmp.as<int>();
+  float* wbuf = reinterpret_cast<float*>(segi + 2 * nl + 4);
+  const size_t wmax = std::max<size_t>({static_cast<size_t>(s->light_tri_total), static_cast<size_t>(ne),
+                                        static_cast<size_t>(nobj), 1});
+  float* tmp = wbuf + wmax;
+  // lights: areas -> per-light tri samplers and total areas
+  if (nl > 0) {
+    hipLaunchKernelGGL(k_light_areas, dim3(nl), dim3(256), 0, st, s->tri_geom.as<float4>(),
+                       s->lights.as<LightK>(), nl, wbuf, segi, segi + nl);
+    PSDR_HIP_CHECK(hipGetLastError());
+    int rc = launch_sampler(wbuf, segi, segi + nl, nl, s->light_tri_pmf.as<float>(), s->light_tri_cmf.as<float>(),
+                            s->light_area.as<float>(), tmp, st);
+    if (rc) return rc;
+  }
+  // light power + light pick over objects
+  if (nobj > 0) {
+    hipLaunchKernelGGL(k_light_power, dim3(blocks(nobj, 64)), dim3(64), 0, st, s->objs.as<ObjectK>(), nobj,
+                       s->light_emit, s->light_area.as<float>(), s->light_power.as<float>(), wbuf);
+    PSDR_HIP_CHECK(hipGetLastError());
+    int* cnt = segi + 2 * nl;  // holds nobj (uploaded at creation)
+    int rc = launch_sampler(wbuf, nullptr, cnt, 1, s->obj_pmf.as<float>(), s->obj_cmf.as<float>(),
+                            s->lights_total.as<float>(), tmp, st);
+    if (rc) return rc;
+  }
+  // edges
+  if (ne > 0) {
+    hipLaunchKernelGGL(k_edge_geom, dim3(blocks(ne, 256)), dim3(256), 0, st, s->vertices, s->edge_topo.as<int>(),
+                       ne, s->edge_geom.as<float>(), wbuf);
+    PSDR_HIP_CHECK(hipGetLastError());
+    int* cnt = segi + 2 * nl + 1;  // holds ne
+    int rc = launch_sampler(wbuf, nullptr, cnt, 1, s->edge_pmf.as<float>(), s->edge_cmf.as<float>(), nullptr, tmp,
+                            st);
+    if (rc) return rc;
+  }
+  return build_lbvh(s, st);
+}
+
+int build_lbvh(psdr_scene* s, hipStream_t st) {
+  const int n = s->n_tris;
+  if (n < 2) {
+    if (n == 1) {
+      hipLaunchKernelGGL(k_leaf, dim3(1), dim3(64), 0, st, s->tri_geom.as<float4>(), nullptr, 1,
+                         s->leaf.as<float4>(), s->counters.as<int>());
+      PSDR_HIP_CHECK(hipGetLastError());
+    }
+    return PSDR_OK;
+  }
+  const int nb = std::min(blocks(n, 256), 1024);
+  float* part = s->bounds.as<float>() + 8;
+  hipLaunchKernelGGL(k_center_bounds, dim3(nb), dim3(256), 0, st, s->tri_geom.as<float4>(), n, part);
+  hipLaunchKernelGGL(k_center_bounds2, dim3(1), dim3(64), 0, st, part, nb, s->bounds.as<float>());
+  uint32_t* k0 = s->sort_keys.as<uint32_t>();
+  uint32_t* k1 = k0 + n;
+  uint32_t* v0 = s->sort_vals.as<uint32_t>();
+  uint32_t* v1 = v0 + n;
+  hipLaunchKernelGGL(k_morton, dim3(blocks(n, 256)), dim3(256), 0, st, s->tri_geom.as<float4>(), n,
+                     s->bounds.as<float>(), s->morton.as<uint32_t>(), k0, v0);
+  PSDR_HIP_CHECK(hipGetLastError());
+  const int tiles = blocks(n, SORT_TILE);
+  for (int pass = 0; pass < 4; ++pass) {
+    const int shift = 8 * pass;
+    hipLaunchKernelGGL(k_radix_hist, dim3(tiles), dim3(SORT_THREADS), 0, st, k0, n, shift,
+                       s->sort_hist.as<uint32_t>());
+    hipLaunchKernelGGL(k_scan_u32, dim3(1), dim3(1024), 0, st, s->sort_hist.as<uint32_t>(), 256 * tiles);
+    hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(SORT_THREADS), 0, st, k0, v0, k1, v1, n, shift,
+                       s->sort_hist.as<uint32_t>());
+    PSDR_HIP_CHECK(hipGetLastError());
+    std::swap(k0, k1);
+    std::swap(v0, v1);
+  }
+  // after an even number of passes the sorted data is back in k0/v0 (= buffer start)
+  PSDR_HIP_CHECK(hipMemcpyAsync(s->order.p, v0, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, st));
+  hipLaunchKernelGGL(k_karras, dim3(blocks(n - 1, 256)), dim3(256), 0, st, k0, n, s->child.as<int>(),
+                     s->parent.as<int>());
+  hipLaunchKernelGGL(k_leaf, dim3(blocks(n, 256)), dim3(256), 0, st, s->tri_geom.as<float4>(),
+                     s->order.as<uint32_t>(), n, s->leaf.as<float4>(), s->counters.as<int>());
+  hipLaunchKernelGGL(k_refit, dim3(blocks(n, 256)), dim3(256), 0, st, s->tri_geom.as<float4>(),
+                     s->order.as<uint32_t>(), n, s->child.as<int>(), s->parent.as<int>(), s->box.as<float>(),
+                     s->counters.as<int>());
+  hipLaunchKernelGGL(k_pack_nodes, dim3(blocks(n - 1, 256)), dim3(256), 0, st, s->box.as<float>(),
+                     s->child.as<int>(), n, s->node.as<float4>());
+  PSDR_HIP_CHECK(hipGetLastError());
+  return PSDR_OK;
+}
+
+}  // namespace psdr

@@ -1,0 +1,187 @@
+// Device-side scene access: intersection reconstruction, textures, materials
+// and area-light sampling.  Each routine restates the reference expression it
+// cites, in the same fp32 operation order.
+#pragma once
+
+#include "scene.h"
+
+namespace psdr {
+
+struct Its {
+  bool valid, is_light;
+  f3 p, n, t, wi;
+  float d2;
+  f2 uv;
+  float area;
+  int tri, obj, light;
+};
+
+__device__ __forceinline__ f3 xyz4(float4 a) { return f3{a.x, a.y, a.z}; }
+
+// Scene::hit reconstruction (src/scene.cc:106-161) from (tri, u, v) and the ray
+// origin O.
+__device__ __forceinline__ Its reconstruct(const SceneView& sv, int tri, float u, float v, f3 O, bool valid_in) {
+  Its it;
+  it.valid = valid_in && tri >= 0;
+  it.is_light = false;
+  it.tri = tri;
+  it.obj = -1;
+  it.light = -1;
+  if (!it.valid) {
+    it.p = it.n = it.t = it.wi = splat(0.f);
+    it.d2 = 0.f;
+    it.uv = f2{0.f, 0.f};
+    it.area = 0.f;
+    return it;
+  }
+  const float4 g0 = sv.tri_geom[5 * tri], g1 = sv.tri_geom[5 * tri + 1], g2 = sv.tri_geom[5 * tri + 2];
+  const float4 g3 = sv.tri_geom[5 * tri + 3], g4 = sv.tri_geom[5 * tri + 4];
+  const f3 P0 = xyz4(g0), P1 = xyz4(g1), P2 = xyz4(g2), N = xyz4(g3), T = xyz4(g4);
+  it.area = g0.w;
+  it.p = (P0 + u * (P1 - P0)) + v * (P2 - P0);
+  const float* uv = sv.tri_uv + 6 * tri;
+  it.uv.x = (uv[0] + u * (uv[2] - uv[0])) + v * (uv[4] - uv[0]);
+  it.uv.y = (uv[1] + u * (uv[3] - uv[1])) + v * (uv[5] - uv[1]);
+  const f3 wu = it.p - O;
+  it.wi = -normalize(wu);
+  it.d2 = sqnorm(wu);
+  const bool front = dot(it.wi, N) > 0.f;
+  it.n = front ? N : -N;
+  it.t = front ? T : -T;
+  it.obj = sv.tri_obj[tri];
+  it.light = sv.objs[it.obj].light;
+  it.is_light = it.light >= 0;
+  return it;
+}
+
+// UVTexture::at_nearest (src/texture.h:65-74); the Python wrapper flips rows,
+// so data row v0 is image row H-1-v0 of the caller's (H, W, C) tensor.
+__device__ __forceinline__ int texel_index(int h, int w, f2 uv) {
+  const float u = uv.x * static_cast<float>(w) - 0.5f;
+  const float v = uv.y * static_cast<float>(h) - 0.5f;
+  const float u0 = fminf(fmaxf(rintf(u), 0.f), static_cast<float>(w - 1));
+  const float v0 = fminf(fmaxf(rintf(v), 0.f), static_cast<float>(h - 1));
+  return (h - 1 - static_cast<int>(v0)) * w + static_cast<int>(u0);
+}
+// float offset of the 3-channel texel for uv
+__device__ __forceinline__ int tex3_offset(int off, int h, int w, f2 uv) {
+  return h == 0 ? off : off + 3 * texel_index(h, w, uv);
+}
+__device__ __forceinline__ int tex1_offset(int off, int h, int w, f2 uv) {
+  return h == 0 ? off : off + texel_index(h, w, uv);
+}
+__device__ __forceinline__ f3 tex3(const float* tex, int o) { return f3{tex[o], tex[o + 1], tex[o + 2]}; }
+
+// DiffuseBsdfMaterial value (src/diffuse_bsdf.cc:42-75) for N = its.n
+__device__ __forceinline__ f3 diffuse_bsdf_value(f3 color, float sigma, const Its& it, f3 L) {
+  const float div = 1.0f / (REAL_PI + ((3.0f * REAL_PI - 4.0f) / 6.0f) * sigma);
+  const float a = div;
+  const float b = sigma * div;
+  float NdotL = dot(it.n, L);
+  if (NdotL < 0.f) NdotL = 0.f;
+  float NdotV = dot(it.n, it.wi);
+  if (NdotV < 0.f) NdotV = 0.f;
+  const float VdotL = dot(it.wi, L);
+  float t = VdotL - NdotL * NdotV;
+  if (t > 0.f) t = (NdotL >= NdotV ? NdotL : NdotV) + REAL_EPS;
+  return color * (a + b * t);
+}
+
+// Material::eval (diffuse_material.cc:29-34, diffuse_bsdf.cc:42-75)
+__device__ __forceinline__ f3 mat_eval(const SceneView& sv, const Its& it, f3 wo) {
+  const ObjectK& ob = sv.objs[it.obj];
+  const f3 color = tex3(sv.texels, tex3_offset(ob.alb_off, ob.alb_h, ob.alb_w, it.uv));
+  if (ob.mat == PSDR_MAT_DIFFUSE_BSDF) {
+    const float sigma = sv.texels[tex1_offset(ob.rough_off, ob.rough_h, ob.rough_w, it.uv)];
+    return diffuse_bsdf_value(color, sigma, it, wo);
+  }
+  return color * REAL_INV_PI;
+}
+
+// Material::pdf
+__device__ __forceinline__ float mat_pdf(const Its& it, f3 wo) { return dot(it.n, wo) * REAL_INV_PI; }
+
+struct Sampled {
+  f3 wo, bsdf;
+  float pdf, cos_t, u, v;
+  bool valid;
+};
+// Material::sample (diffuse_material.cc:3-27, diffuse_bsdf.cc:3-40)
+__device__ __forceinline__ Sampled mat_sample(const SceneView& sv, const Its& it, float ru, float rv) {
+  Sampled s;
+  const ObjectK& ob = sv.objs[it.obj];
+  const f3 B = cross(it.n, it.t);
+  const float cos_t = safe_sqrt(ru);
+  const float sin_t = safe_sqrt(1.0f - ru);
+  const float phi = rv * REAL_2PI;
+  float sp, cp;
+  psdr_sincos(phi, &sp, &cp);
+  const float u = cp * sin_t;
+  const float v = sp * sin_t;
+  s.wo = normalize((u * it.t + v * B) + cos_t * it.n);
+  s.pdf = cos_t * REAL_INV_PI;
+  s.cos_t = cos_t;
+  s.u = u;
+  s.v = v;
+  s.valid = it.valid;
+  const f3 color = tex3(sv.texels, tex3_offset(ob.alb_off, ob.alb_h, ob.alb_w, it.uv));
+  if (ob.mat == PSDR_MAT_DIFFUSE_BSDF) {
+    s.valid = s.valid && dot(it.n, it.wi) > 0.f;
+    const float sigma = sv.texels[tex1_offset(ob.rough_off, ob.rough_h, ob.rough_w, it.uv)];
+    s.bsdf = diffuse_bsdf_value(color, sigma, it, s.wo);
+  } else {
+    s.bsdf = color * REAL_INV_PI;
+  }
+  return s;
+}
+
+// AreaLight::sample_light_point_ (src/emitter.cc:30-59).  Returns the light
+// triangle (global id) and the barycentric weights used for the point.
+struct LightPoint {
+  f3 p;
+  float pdf;
+  int tri;
+  float bu, bv;
+};
+__device__ __forceinline__ LightPoint sample_light_point(const SceneView& sv, int l, float ru, float rv) {
+  const LightK L = sv.lights[l];
+  float pt;
+  const int k = discrete_sample(sv.light_tri_pmf + L.pmf_off, sv.light_tri_cmf + L.pmf_off, L.tri_count, &ru, &pt);
+  const int tri = L.tri_begin + k;
+  const float4 g0 = sv.tri_geom[5 * tri], g1 = sv.tri_geom[5 * tri + 1], g2 = sv.tri_geom[5 * tri + 2];
+  const f3 P0 = xyz4(g0), P1 = xyz4(g1), P2 = xyz4(g2);
+  const float svv = safe_sqrt(rv);
+  const float u = ru * svv;
+  const float v = 1.0f - svv;
+  LightPoint lp;
+  lp.pdf = 1.0f / sv.light_area[l];
+  lp.p = (P0 + (P1 - P0) * u) + (P2 - P0) * v;
+  lp.tri = tri;
+  lp.bu = u;
+  lp.bv = v;
+  return lp;
+}
+
+// PathIntegrator::mis_weight (src/path_integrator.cc:3-9)
+__device__ __forceinline__ float mis_weight(float w1, float w2, uint32_t n1, uint32_t n2) {
+  const float a = static_cast<float>(n1) * w1;
+  const float b = static_cast<float>(n2) * w2;
+  const float sa = a * a, sb = b * b;
+  return sa / (sa + sb);
+}
+
+// Wave-aggregated slot allocation (all lanes of the wave must call it).
+__device__ __forceinline__ int wave_alloc(bool want, int* counter) {
+  const uint64_t m = __ballot(want);
+  const int lane = __lane_id();
+  int base = 0;
+  if (m != 0ull) {
+    const int leader = __ffsll(static_cast<long long>(m)) - 1;
+    if (lane == leader) base = atomicAdd(counter, __popcll(m));
+    base = __shfl(base, leader);
+  }
+  const uint64_t lt = lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
+  return base + __popcll(m & lt);
+}
+
+}  // namespace psdr

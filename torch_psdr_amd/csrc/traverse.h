@@ -1,0 +1,116 @@
+// Closest-hit LBVH traversal for gfx950 (replaces optixTrace with the
+// raygen/closesthit/miss programs of src/cuda/device_programs.cu:13-47).
+//
+// BVH2 with both child boxes stored in the parent (one 64-byte node fetch tests
+// two boxes); leaves are single triangles stored in LBVH order as
+// {p0, id}{e1}{e2}.  The traversal stack lives in LDS (STACK_LDS entries per
+// lane, laid out [entry][lane] so a wave's pushes hit distinct banks) and
+// overflows into a per-thread global area.  A Karras tree over 62-bit keys has at
+// most 62 internal levels and the stack holds at most one entry per level, so a
+// total of 64 entries never overflows.
+//
+// Result contract: smallest t >= 1e-3 (Moller-Trumbore, fixed op order); an
+// exactly equal t keeps the smaller global triangle id, so the answer does not
+// depend on traversal order.  Boxes are padded and the slab test is widened by
+// 4e-7 relative, so pruning is conservative.
+#pragma once
+
+#include "common.h"
+
+namespace psdr {
+
+constexpr int STACK_TOTAL = 64;
+
+struct HitRec {
+  int tri;
+  float t, u, v;
+};
+
+__device__ __forceinline__ float slab_entry(float4 lo_hi_a, float4 lo_hi_b, float4 lo_hi_c, int which, f3 o, f3 inv,
+                                            float tbest) {
+  float lx, ly, lz, hx, hy, hz;
+  if (which == 0) {
+    lx = lo_hi_a.x; ly = lo_hi_a.y; lz = lo_hi_a.z;
+    hx = lo_hi_a.w; hy = lo_hi_b.x; hz = lo_hi_b.y;
+  } else {
+    lx = lo_hi_b.z; ly = lo_hi_b.w; lz = lo_hi_c.x;
+    hx = lo_hi_c.y; hy = lo_hi_c.z; hz = lo_hi_c.w;
+  }
+  const float t0x = (lx - o.x) * inv.x, t1x = (hx - o.x) * inv.x;
+  const float t0y = (ly - o.y) * inv.y, t1y = (hy - o.y) * inv.y;
+  const float t0z = (lz - o.z) * inv.z, t1z = (hz - o.z) * inv.z;
+  const float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), 0.f));
+  const float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
+  const bool hit = tn <= tf * 1.0000004f && tn <= tbest * 1.0000004f;
+  return hit ? tn : F_INF;
+}
+
+__device__ __forceinline__ void leaf_test(const float4* __restrict__ leaf, int k, f3 o, f3 d, HitRec& h) {
+  const float4 a = leaf[3 * k], b = leaf[3 * k + 1], c = leaf[3 * k + 2];
+  float t, u, v;
+  if (ray_tri(o, d, f3{a.x, a.y, a.z}, f3{b.x, b.y, b.z}, f3{c.x, c.y, c.z}, &t, &u, &v)) {
+    const int id = __float_as_int(a.w);
+    if (t < h.t || (t == h.t && id < h.tri)) {
+      h.t = t;
+      h.tri = id;
+      h.u = u;
+      h.v = v;
+    }
+  }
+}
+
+// stack_lds: this block's LDS stack base ([STACK_LDS][blockDim]); spill: this
+// thread's global overflow area with stride `spill_stride` between entries.
+template <int STACK_LDS>
+__device__ __forceinline__ HitRec closest_hit(const float4* __restrict__ node, const float4* __restrict__ leaf,
+                                              int n_tris, f3 o, f3 d, int* stack_lds, int* spill,
+                                              int spill_stride) {
+  HitRec h{-1, F_INF, -1.f, -1.f};
+  if (!(d.x == d.x) || n_tris <= 0) return h;
+  if (n_tris == 1) {
+    leaf_test(leaf, 0, o, d, h);
+    if (h.tri < 0) h.t = F_INF;
+    return h;
+  }
+  const f3 inv = f3{1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+  const int lane = threadIdx.x, nthr = blockDim.x;
+  int sp = 0;
+  int ni = 0;
+  for (;;) {
+    const float4 a = node[4 * ni], b = node[4 * ni + 1], c = node[4 * ni + 2], e = node[4 * ni + 3];
+    int cl = __float_as_int(e.x), cr = __float_as_int(e.y);
+    float tl = slab_entry(a, b, c, 0, o, inv, h.t);
+    float tr = slab_entry(a, b, c, 1, o, inv, h.t);
+    if (tl != F_INF && cl < 0) {
+      leaf_test(leaf, cl & 0x7fffffff, o, d, h);
+      tl = F_INF;
+    }
+    if (tr != F_INF && cr < 0) {
+      leaf_test(leaf, cr & 0x7fffffff, o, d, h);
+      tr = F_INF;
+    }
+    const bool hl = tl != F_INF, hr = tr != F_INF;
+    if (hl && hr) {
+      int nearc = cl, farc = cr;
+      if (tr < tl) {
+        nearc = cr;
+        farc = cl;
+      }
+      if (sp < STACK_LDS) stack_lds[sp * nthr + lane] = farc;
+      else spill[(sp - STACK_LDS) * spill_stride] = farc;
+      ++sp;
+      ni = nearc;
+    } else if (hl) {
+      ni = cl;
+    } else if (hr) {
+      ni = cr;
+    } else {
+      if (sp == 0) break;
+      --sp;
+      ni = (sp < STACK_LDS) ? stack_lds[sp * nthr + lane] : spill[(sp - STACK_LDS) * spill_stride];
+    }
+  }
+  return h;
+}
+
+}  // namespace psdr

@@ -1,0 +1,145 @@
+"""Flatten a Scene's objects into the global arrays the native library consumes.
+
+Object order is meshes first, then the meshes of area lights (Scene::Scene,
+src/scene.cc:8-20); triangle ids are global in that order (the OptiX SBT offset
+convention of src/scene_optix.cc:160-168).  Only gathers and concatenations
+happen here (no arithmetic), and they are torch ops, so gradients of the global
+vertex / texel / emission buffers flow back to the user's tensors.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import torch
+
+from . import _native as N
+
+# default corner uvs when a mesh has none (src/mesh.cc:31-36)
+_DEFAULT_UV = (0.0, 0.0, 0.0, 1.0, 1.0, 0.0)
+
+
+@dataclass
+class FlatScene:
+    vertices: torch.Tensor          # (V, 3) f32, autograd-connected
+    faces: torch.Tensor             # (F, 3) i32 global vertex ids
+    tri_uv: torch.Tensor            # (F, 6) f32
+    tri_obj: torch.Tensor           # (F,) i32
+    texels: torch.Tensor            # (T,) f32, autograd-connected
+    light_emit: torch.Tensor        # (L, 3) f32, autograd-connected
+    objects: List[dict]             # psdr_object fields per object
+    light_obj: List[int]
+    obj_vertex_range: List[tuple] = field(default_factory=list)
+    topo_key: tuple = ()
+
+    @property
+    def n_tris(self) -> int:
+        return int(self.faces.shape[0])
+
+    def desc(self, keep_alive: list) -> N.PsdrSceneDesc:
+        """ctypes psdr_scene_desc over this scene's device buffers."""
+        objs = (N.PsdrObject * max(len(self.objects), 1))()
+        for i, o in enumerate(self.objects):
+            for k, v in o.items():
+                setattr(objs[i], k, int(v))
+        lobj = (N.C.c_int32 * max(len(self.light_obj), 1))(*self.light_obj)
+        keep_alive += [objs, lobj]
+        d = N.PsdrSceneDesc()
+        d.n_vertices = int(self.vertices.shape[0])
+        d.vertices = self.vertices.data_ptr()
+        d.n_tris = self.n_tris
+        d.faces = self.faces.data_ptr()
+        d.tri_uv = self.tri_uv.data_ptr()
+        d.tri_obj = self.tri_obj.data_ptr()
+        d.n_objects = len(self.objects)
+        d.objects = N.C.cast(objs, N.C.POINTER(N.PsdrObject))
+        d.n_texels = int(self.texels.numel())
+        d.texels = self.texels.data_ptr()
+        d.n_lights = len(self.light_obj)
+        d.light_emit = self.light_emit.data_ptr() if self.light_emit.numel() else None
+        d.light_obj = N.C.cast(lobj, N.C.POINTER(N.C.c_int32))
+        return d
+
+
+def _tex_entry(tex, pieces: list, offset: int):
+    """Append a texture's data; returns (offset, h, w, new_offset)."""
+    data = tex.data
+    if tex.is_constant:
+        flat = data.reshape(-1)
+        pieces.append(flat)
+        return offset, 0, 0, offset + flat.numel()
+    flat = data.reshape(-1)
+    pieces.append(flat)
+    return offset, tex.height, tex.width, offset + flat.numel()
+
+
+def flatten(meshes, lights, device) -> FlatScene:
+    """meshes: list of api.Mesh; lights: list of api.AreaLight."""
+    objs = [(m, None) for m in meshes] + [(l.mesh_proxy(), l) for l in lights]
+    verts, faces, uvs, tobj = [], [], [], []
+    objects, light_obj = [], []
+    tex_pieces: list = []
+    tex_off = 0
+    voff = 0
+    toff = 0
+    vranges = []
+    key = []
+    emits = []
+    for oi, (m, light) in enumerate(objs):
+        v = m.vertices
+        f = m.face_indices
+        if v.device != device or f.device != device:
+            raise RuntimeError("all scene tensors must live on the same GPU device")
+        nv, nf = int(v.shape[0]), int(f.shape[0])
+        verts.append(v.to(torch.float32))
+        faces.append(f.to(torch.int32) + voff)
+        if m.uvs is not None and m.uv_indices is not None:
+            corner = m.uvs.to(torch.float32)[m.uv_indices.long()]  # (F, 3, 2)
+            uvs.append(corner.reshape(nf, 6))
+        else:
+            uvs.append(torch.tensor(_DEFAULT_UV, dtype=torch.float32, device=device).expand(nf, 6))
+        tobj.append(torch.full((nf,), oi, dtype=torch.int32, device=device))
+        rec = dict(material=N.PSDR_MAT_DIFFUSE, light=-1, albedo_offset=0, albedo_h=0, albedo_w=0,
+                   rough_offset=0, rough_h=0, rough_w=0, tri_begin=toff, tri_count=nf, grad_mask=0)
+        if v.requires_grad:
+            rec["grad_mask"] |= N.PSDR_GRAD_VERTICES
+        if light is not None:
+            rec["material"] = N.PSDR_MAT_LIGHT
+            rec["light"] = len(light_obj)
+            light_obj.append(oi)
+            emits.append(light.emit.to(torch.float32).reshape(3))
+            if light.emit.requires_grad:
+                rec["grad_mask"] |= N.PSDR_GRAD_EMIT
+        else:
+            mat = m.material
+            a_off, a_h, a_w, tex_off = _tex_entry(mat.color, tex_pieces, tex_off)
+            rec.update(albedo_offset=a_off, albedo_h=a_h, albedo_w=a_w)
+            if mat.color.data.requires_grad:
+                rec["grad_mask"] |= N.PSDR_GRAD_ALBEDO
+            if mat.kind == "diffuse_bsdf":
+                rec["material"] = N.PSDR_MAT_DIFFUSE_BSDF
+                r_off, r_h, r_w, tex_off = _tex_entry(mat.roughness, tex_pieces, tex_off)
+                rec.update(rough_offset=r_off, rough_h=r_h, rough_w=r_w)
+                if mat.roughness.data.requires_grad:
+                    rec["grad_mask"] |= N.PSDR_GRAD_ROUGHNESS
+        objects.append(rec)
+        vranges.append((voff, voff + nv))
+        key.append((f.data_ptr(), f._version, tuple(f.shape), nv,
+                    None if m.uv_indices is None else (m.uv_indices.data_ptr(), m.uv_indices._version),
+                    None if m.uvs is None else (m.uvs.data_ptr(), m.uvs._version)))
+        voff += nv
+        toff += nf
+    texels = torch.cat(tex_pieces) if tex_pieces else torch.zeros(1, dtype=torch.float32, device=device)
+    emit = torch.stack(emits) if emits else torch.zeros((0, 3), dtype=torch.float32, device=device)
+    return FlatScene(
+        vertices=torch.cat(verts).contiguous() if verts else torch.zeros((0, 3), device=device),
+        faces=torch.cat(faces).contiguous() if faces else torch.zeros((0, 3), dtype=torch.int32, device=device),
+        tri_uv=torch.cat(uvs).contiguous() if uvs else torch.zeros((0, 6), device=device),
+        tri_obj=torch.cat(tobj).contiguous() if tobj else torch.zeros((0,), dtype=torch.int32, device=device),
+        texels=texels.contiguous(),
+        light_emit=emit.contiguous(),
+        objects=objects,
+        light_obj=light_obj,
+        obj_vertex_range=vranges,
+        topo_key=tuple(key),
+    )

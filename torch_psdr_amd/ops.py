@@ -1,0 +1,177 @@
+"""PyTorch custom ops over the C ABI: psdr::render and psdr::render_backward.
+
+`psdr::render(token, cam, cfg, vertices, texels, emit, keep)` renders one
+camera into a (H, W, 3) fp32 tensor on the current HIP stream; its registered
+autograd formula calls `psdr::render_backward`, the hand-written adjoint
+(path replay + edge sampling).  This replaces the reference's Wrap2torch
+(src/python/torch_psdr.py:29-72), which bridged Enoki's tape to torch.
+
+A token names one flattened Scene (its device buffers and host object table);
+native scene handles are shared between Scene objects of the same topology, so
+an optimisation loop that rebuilds its Scene every iteration re-derives the
+geometry and LBVH on the GPU instead of re-creating the handle.
+"""
+from __future__ import annotations
+
+import itertools
+import threading
+from typing import Dict, List, Sequence, Tuple
+
+import torch
+
+from . import _native as N
+from .flat import FlatScene
+
+_lock = threading.Lock()
+_ids = itertools.count(1)
+_entries: Dict[int, Tuple["NativeScene", FlatScene]] = {}
+_by_topo: Dict[tuple, "NativeScene"] = {}
+_MAX_CACHED = 8
+
+
+class NativeScene:
+    """Owns a psdr_scene handle bound to one FlatScene at a time."""
+
+    def __init__(self, flat: FlatScene):
+        self.lib = N.load()
+        self.keep_alive: list = []
+        self.handle = N.C.c_void_p()
+        desc = flat.desc(self.keep_alive)
+        N.check(self.lib.psdr_scene_create(N.C.byref(desc), N.C.byref(self.handle), N.current_stream()))
+        self.flat = flat
+        self.last_forward = None  # (cam, cfg, keep) of the last forward on self.flat
+
+    def bind(self, flat: FlatScene) -> None:
+        if flat is self.flat:
+            return
+        keep_alive: list = []
+        desc = flat.desc(keep_alive)
+        N.check(self.lib.psdr_scene_update(self.handle, N.C.byref(desc), N.current_stream()))
+        self.keep_alive = keep_alive
+        self.flat = flat
+        self.last_forward = None
+
+    def info(self):
+        nt, ne, dep = N.C.c_int32(), N.C.c_int32(), N.C.c_int32()
+        N.check(self.lib.psdr_scene_info(self.handle, N.C.byref(nt), N.C.byref(ne), N.C.byref(dep)))
+        return nt.value, ne.value, dep.value
+
+    def __del__(self):
+        try:
+            if self.handle:
+                self.lib.psdr_scene_destroy(self.handle)
+        except Exception:
+            pass
+
+
+def register(flat: FlatScene) -> int:
+    """Register a flattened scene; returns its token."""
+    with _lock:
+        ns = _by_topo.get(flat.topo_key)
+    if ns is None:
+        ns = NativeScene(flat)
+        with _lock:
+            if len(_by_topo) >= _MAX_CACHED:
+                _by_topo.pop(next(iter(_by_topo)))
+            _by_topo[flat.topo_key] = ns
+    else:
+        ns.bind(flat)
+    tok = next(_ids)
+    with _lock:
+        _entries[tok] = (ns, flat)
+    return tok
+
+
+def unregister(tok: int) -> None:
+    with _lock:
+        _entries.pop(tok, None)
+
+
+def entry(tok: int) -> Tuple[NativeScene, FlatScene]:
+    return _entries[tok]
+
+
+def make_config(cfg: Sequence[int]) -> N.PsdrConfig:
+    c = N.PsdrConfig()
+    (c.n_pass, c.spp_interior, c.enable_light_visable, c.spp_primary_edge, c.spp_secondary_edge, c.max_bounce,
+     c.mis_light_samples, c.mis_bsdf_samples, c.primary_edge_preprocess_rounds, c.secondary_edge_preprocess_rounds,
+     c.primary_edge_hypercube_resolution) = [int(x) for x in cfg[:11]]
+    for i in range(3):
+        c.secondary_edge_hypercube_resolution[i] = int(cfg[11 + i])
+    return c
+
+
+def make_camera(cam: Sequence[float]) -> N.PsdrCamera:
+    c = N.PsdrCamera()
+    for i in range(3):
+        c.look_from[i] = float(cam[i])
+        c.look_at[i] = float(cam[3 + i])
+        c.up[i] = float(cam[6 + i])
+    c.vfov = float(cam[9])
+    c.height = int(cam[10])
+    c.width = int(cam[11])
+    return c
+
+
+def forward_native(ns: NativeScene, flat: FlatScene, cam: Sequence[float], cfg: Sequence[int],
+                   keep: bool) -> torch.Tensor:
+    ns.bind(flat)
+    H, W = int(cam[10]), int(cam[11])
+    out = torch.empty((H, W, 3), dtype=torch.float32, device=flat.vertices.device)
+    c, k = make_config(cfg), make_camera(cam)
+    N.check(ns.lib.psdr_render_forward(ns.handle, N.C.byref(c), N.C.byref(k), out.data_ptr(), int(keep),
+                                       N.current_stream()))
+    ns.last_forward = (tuple(cam), tuple(cfg), bool(keep))
+    return out
+
+
+@torch.library.custom_op("psdr::render", mutates_args=())
+def render(token: int, cam: List[float], cfg: List[int], vertices: torch.Tensor, texels: torch.Tensor,
+           emit: torch.Tensor, keep: bool) -> torch.Tensor:
+    ns, flat = entry(token)
+    return forward_native(ns, flat, cam, cfg, keep)
+
+
+@render.register_fake
+def _render_fake(token, cam, cfg, vertices, texels, emit, keep):
+    return vertices.new_empty((int(cam[10]), int(cam[11]), 3))
+
+
+@torch.library.custom_op("psdr::render_backward", mutates_args=())
+def render_backward(token: int, cam: List[float], cfg: List[int], grad: torch.Tensor, vertices: torch.Tensor,
+                    texels: torch.Tensor, emit: torch.Tensor) -> List[torch.Tensor]:
+    ns, flat = entry(token)
+    ns.bind(flat)
+    if ns.last_forward != (tuple(cam), tuple(cfg), True):
+        forward_native(ns, flat, cam, cfg, True)
+    gv = torch.zeros_like(vertices)
+    gt = torch.zeros_like(texels)
+    ge = torch.zeros((max(int(emit.shape[0]), 1), 3), dtype=torch.float32, device=emit.device)
+    c, k = make_config(cfg), make_camera(cam)
+    g = grad.contiguous().to(torch.float32)
+    N.check(ns.lib.psdr_render_backward(ns.handle, N.C.byref(c), N.C.byref(k), g.data_ptr(), gv.data_ptr(),
+                                        gt.data_ptr(), ge.data_ptr(), N.current_stream()))
+    return [gv, gt, ge[: emit.shape[0]].clone()]
+
+
+@render_backward.register_fake
+def _render_backward_fake(token, cam, cfg, grad, vertices, texels, emit):
+    return [torch.empty_like(vertices), torch.empty_like(texels), torch.empty_like(emit)]
+
+
+def _setup(ctx, inputs, output):
+    token, cam, cfg, vertices, texels, emit, keep = inputs
+    ctx.token, ctx.cam, ctx.cfg = token, cam, cfg
+    ctx.entry = entry(token)  # keeps the scene alive until backward
+    ctx.save_for_backward(vertices, texels, emit)
+
+
+def _backward(ctx, grad):
+    vertices, texels, emit = ctx.saved_tensors
+    with _lock:
+        _entries.setdefault(ctx.token, ctx.entry)
+    gv, gt, ge = render_backward(ctx.token, ctx.cam, ctx.cfg, grad, vertices, texels, emit)
+    return None, None, None, gv, gt, ge, None
+
+
+render.register_autograd(_backward, setup_context=_setup)

@@ -1,0 +1,130 @@
+"""Synthetic scenes of the benchmark configurations, built through the drop-in API.
+
+  C0  icosphere level 3 (r=1, origin), 10x10 ground quad at y=-1, 2x2 quad light
+      at y=4 with radiance 10, albedo 0.5, camera (0,1,5) -> origin, vfov 45 deg,
+      256x256, 4 spp, max_bounce 2
+  C1  C0 + a 256x256 UV-sphere-mapped texture on the sphere, texels U[0.2, 0.8]
+      (seed 1); target texture seed 2; 8 spp, depth 2
+  C2  icosphere level 5 (20480 tris), vertices radially perturbed by N(0, 0.02)
+      (seed 3), 512x512 texture, ground + light as C0; 512x512, 16 spp, depth 3,
+      spp_primary_edge = spp_secondary_edge = 4 (1M boundary samples)
+Geometry is generated with numpy (float64, then rounded to fp32 once).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+import torch
+
+
+def icosphere(level: int, radius: float = 1.0):
+    """Subdivided icosahedron: (V, 3) float32 vertices, (F, 3) int32 faces."""
+    t = (1.0 + math.sqrt(5.0)) / 2.0
+    verts = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+             (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    verts = [np.array(v, dtype=np.float64) / np.linalg.norm(v) for v in verts]
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+             (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11),
+             (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    for _ in range(level):
+        cache = {}
+        nf = []
+
+        def mid(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in cache:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                cache[key] = len(verts) - 1
+            return cache[key]
+
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nf
+    V = (np.stack(verts) * radius).astype(np.float32)
+    return V, np.array(faces, dtype=np.int32)
+
+
+def sphere_uv(V: np.ndarray) -> np.ndarray:
+    """UV-sphere map of unit-sphere directions: u = 0.5 + atan2(z, x)/2pi, v = 0.5 + asin(y)/pi."""
+    v = V.astype(np.float64)
+    v = v / np.linalg.norm(v, axis=1, keepdims=True)
+    u = 0.5 + np.arctan2(v[:, 2], v[:, 0]) / (2 * math.pi)
+    w = 0.5 + np.arcsin(np.clip(v[:, 1], -1, 1)) / math.pi
+    return np.stack([u, w], 1).astype(np.float32)
+
+
+def quad(size: float, y: float, facing_down: bool = False):
+    h = size / 2.0
+    V = np.array([[-h, y, -h], [h, y, -h], [h, y, h], [-h, y, h]], dtype=np.float32)
+    F = np.array([[0, 2, 1], [0, 3, 2]] if not facing_down else [[0, 1, 2], [0, 2, 3]], dtype=np.int32)
+    return V, F
+
+
+@dataclass
+class Workload:
+    name: str
+    scene: object
+    integrator: object
+    params: dict
+
+
+def _t(x, device, requires_grad=False):
+    return torch.tensor(x, device=device, requires_grad=requires_grad)
+
+
+def camera(dr, device, height, width):
+    return dr.PerspectiveCamera(
+        look_from=torch.tensor([0.0, 1.0, 5.0], device=device),
+        look_at=torch.tensor([0.0, 0.0, 0.0], device=device),
+        up=torch.tensor([0.0, 1.0, 0.0], device=device),
+        vfov=torch.tensor([math.pi / 4.0], device=device),
+        height=height, width=width)
+
+
+def texture(seed: int, res: int, device) -> torch.Tensor:
+    g = torch.Generator().manual_seed(seed)
+    return (0.2 + 0.6 * torch.rand((res, res, 3), generator=g, dtype=torch.float32)).to(device)
+
+
+def build_scene(dr, device, *, level=3, res=256, tex=None, sphere_vertices=None, spp=4, max_bounce=2,
+                spp_edge=1, vertices_grad=False, light_emit=10.0, albedo=0.5):
+    """Sphere + ground quad + quad light, as the configurations describe."""
+    Vs, Fs = icosphere(level)
+    if sphere_vertices is not None:
+        Vs = sphere_vertices
+    vs = torch.tensor(Vs, device=device, requires_grad=vertices_grad)
+    fs = torch.tensor(Fs, device=device)
+    if tex is None:
+        mat = dr.DiffuseMaterial(torch.tensor([albedo] * 3, device=device))
+        sphere = dr.Mesh(vs, None, fs, None, mat)
+    else:
+        uv = torch.tensor(sphere_uv(icosphere(level)[0]), device=device)
+        sphere = dr.Mesh(vs, uv, fs, fs, dr.DiffuseMaterial(tex))
+    Vg, Fg = quad(10.0, -1.0)
+    ground = dr.Mesh(torch.tensor(Vg, device=device), None, torch.tensor(Fg, device=device), None,
+                     dr.DiffuseMaterial(torch.tensor([albedo] * 3, device=device)))
+    Vl, Fl = quad(2.0, 4.0, facing_down=True)
+    light = dr.AreaLight(torch.tensor(Vl, device=device), torch.tensor(Fl, device=device),
+                         torch.tensor([light_emit] * 3, device=device))
+    cam = camera(dr, device, res, res)
+    scene = dr.Scene([cam], [sphere, ground], [light])
+    integ = dr.PathIntegrator(spp_interior=spp, max_bounce=max_bounce, spp_primary_edge=spp_edge,
+                              spp_secondary_edge=spp_edge)
+    return scene, integ, (vs, tex)
+
+
+def c0(dr, device, res=256, spp=4):
+    scene, integ, _ = build_scene(dr, device, level=3, res=res, spp=spp, max_bounce=2)
+    return Workload("c0_forward_256_4spp_d2", scene, integ, {})
+
+
+def perturbed_sphere(level: int, seed: int, sigma: float = 0.02) -> np.ndarray:
+    V, _ = icosphere(level)
+    rng = np.random.default_rng(seed)
+    r = 1.0 + sigma * rng.standard_normal(V.shape[0])
+    return (V.astype(np.float64) * r[:, None]).astype(np.float32)
