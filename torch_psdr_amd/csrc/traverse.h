@@ -76,17 +76,19 @@ __device__ __forceinline__ HitRec closest_hit(const float4* __restrict__ node, c
   const int lane = threadIdx.x, nthr = blockDim.x;
   int sp = 0;
   int ni = 0;
+  const int n_internal = n_tris - 1;
   for (;;) {
+    if (static_cast<unsigned>(ni) >= static_cast<unsigned>(n_internal)) break;  // malformed tree guard
     const float4 a = node[4 * ni], b = node[4 * ni + 1], c = node[4 * ni + 2], e = node[4 * ni + 3];
     int cl = __float_as_int(e.x), cr = __float_as_int(e.y);
     float tl = slab_entry(a, b, c, 0, o, inv, h.t);
     float tr = slab_entry(a, b, c, 1, o, inv, h.t);
     if (tl != F_INF && cl < 0) {
-      leaf_test(leaf, cl & 0x7fffffff, o, d, h);
+      if ((cl & 0x7fffffff) < n_tris) leaf_test(leaf, cl & 0x7fffffff, o, d, h);
       tl = F_INF;
     }
     if (tr != F_INF && cr < 0) {
-      leaf_test(leaf, cr & 0x7fffffff, o, d, h);
+      if ((cr & 0x7fffffff) < n_tris) leaf_test(leaf, cr & 0x7fffffff, o, d, h);
       tr = F_INF;
     }
     const bool hl = tl != F_INF, hr = tr != F_INF;
@@ -97,8 +99,8 @@ __device__ __forceinline__ HitRec closest_hit(const float4* __restrict__ node, c
         farc = cl;
       }
       if (sp < STACK_LDS) stack_lds[sp * nthr + lane] = farc;
-      else spill[(sp - STACK_LDS) * spill_stride] = farc;
-      ++sp;
+      else if (sp < STACK_TOTAL) spill[(sp - STACK_LDS) * spill_stride] = farc;
+      if (sp < STACK_TOTAL) ++sp;
       ni = nearc;
     } else if (hl) {
       ni = cl;
