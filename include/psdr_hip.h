@@ -135,14 +135,22 @@ int psdr_scene_info(const psdr_scene* scene, int32_t* n_tris, int32_t* n_edges, 
 int psdr_render_forward(psdr_scene* scene, const psdr_config* cfg, const psdr_camera* cam, float* out_rgb,
                         int32_t keep, void* stream);
 
+/* Terms of renderD's image derivative (src/path_integrator.cc:381-406). */
+#define PSDR_TERM_INTERIOR 1 /* render_interior (path replay)                  */
+#define PSDR_TERM_PRIMARY 2  /* render_primary_boundary (silhouette edges)     */
+#define PSDR_TERM_DIRECT 4   /* render_direct_boundary (shadow edges)          */
+#define PSDR_TERM_ALL 7
+
 /* Backward of renderD (Enoki's FloatD.backward, src/python/torch_psdr.py:47-72)
  * for the image of the last psdr_render_forward(keep=1) with the same camera and
  * config: accumulates dLoss/d(vertices), d(texels), d(light emit) given dL_dI
  * (height, width, 3).  The gradient buffers are ADDED to (zero them first).
- * Any gradient pointer may be NULL.  Interior term by path replay; primary and
- * direct-shadow boundary terms by edge sampling (src/path_integrator.cc:188-353). */
+ * Any gradient pointer may be NULL.  `terms` selects PSDR_TERM_* (renderD uses
+ * PSDR_TERM_ALL).  Interior term by path replay; primary and direct-shadow
+ * boundary terms by edge sampling with their hypercube preprocessing
+ * (src/path_integrator.cc:188-353). */
 int psdr_render_backward(psdr_scene* scene, const psdr_config* cfg, const psdr_camera* cam, const float* dL_dI,
-                         float* grad_vertices, float* grad_texels, float* grad_emit, void* stream);
+                         float* grad_vertices, float* grad_texels, float* grad_emit, int32_t terms, void* stream);
 
 /* ---- component entry points (tests, benchmarks) --------------------------- */
 /* First n_draws PCG32 outputs of lanes [lane0, lane0+n_lanes) of Sampler

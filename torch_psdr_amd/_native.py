@@ -20,6 +20,11 @@ PSDR_GRAD_ALBEDO = 2
 PSDR_GRAD_ROUGHNESS = 4
 PSDR_GRAD_EMIT = 8
 
+PSDR_TERM_INTERIOR = 1
+PSDR_TERM_PRIMARY = 2
+PSDR_TERM_DIRECT = 4
+PSDR_TERM_ALL = 7
+
 ABI_VERSION = 1
 
 
@@ -95,7 +100,8 @@ _SIGS = {
     "psdr_scene_destroy": (C.c_int, [_VP]),
     "psdr_scene_info": (C.c_int, [_VP, C.POINTER(_I32), C.POINTER(_I32), C.POINTER(_I32)]),
     "psdr_render_forward": (C.c_int, [_VP, C.POINTER(PsdrConfig), C.POINTER(PsdrCamera), _VP, _I32, _VP]),
-    "psdr_render_backward": (C.c_int, [_VP, C.POINTER(PsdrConfig), C.POINTER(PsdrCamera), _VP, _VP, _VP, _VP, _VP]),
+    "psdr_render_backward": (C.c_int, [_VP, C.POINTER(PsdrConfig), C.POINTER(PsdrCamera), _VP, _VP, _VP, _VP, _I32,
+                                       _VP]),
     "psdr_pcg_lane_draws": (C.c_int, [C.c_uint32, _I32, _I32, _VP, _VP]),
     "psdr_camera_rays": (C.c_int, [C.POINTER(PsdrCamera), C.c_uint32, _VP, _VP, _VP, _VP]),
     "psdr_camera_matrices": (C.c_int, [C.POINTER(PsdrCamera), _VP, _VP, _VP, _VP, _VP]),

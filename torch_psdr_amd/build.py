@@ -14,7 +14,7 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libpsdr_hip.so"
-SOURCES = ["capi.hip", "scene.hip", "render.hip", "adjoint.hip"]
+SOURCES = ["capi.hip", "scene.hip", "render.hip", "adjoint.hip", "boundary.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -1,16 +1,571 @@
-// Path-replay adjoint of the interior estimator and edge-sampling boundary
-// terms (src/path_integrator.cc:80-353).
+// Path-replay adjoint of the interior estimator (trace2light + render_interior,
+// src/path_integrator.cc:80-186, 355-372) for renderD's backward pass.
+//
+// One thread per sample lane.  Inputs: dLoss/dImage, the per-lane hit cache of
+// the forward render (no ray is traced again), the scene.  A forward sweep
+// replays the lane's PCG32 stream and records, per depth, the PCG state and the
+// throughput T_d; a reverse sweep recomputes each depth's contribution c_d and
+// throughput factor t_d (L = sum_d T_d c_d, T_{d+1} = T_d t_d) and pushes
+//     adj(c_d) = g T_d,   adj(t_d) = A_{d+1} T_d,   A_d = g c_d + A_{d+1} t_d
+// through hand-written VJPs of the reference's expressions.  Exactly the
+// reference's detach points are honoured: pdfs and MIS weights are constants,
+// hit barycentrics are constants (points move with the surface), emission,
+// albedo / roughness texels, normals, tangents, points, squared distances and
+// jacobians (A / detach(A)) are differentiable.
+//
+// Gradients: vertices and UV-texels by global fp32 atomics; light emission and
+// constant textures (one address hit by every lane) by LDS accumulation per
+// block, flushed with one global atomic per entry.
 #include <hip/hip_runtime.h>
 
 #include "render.h"
+#include "shading.h"
 
 namespace psdr {
 
-int render_backward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& cam, const float* dL_dI,
-                    float* grad_v, float* grad_t, float* grad_e, hipStream_t st) {
-  (void)s; (void)cfg; (void)cam; (void)dL_dI; (void)grad_v; (void)grad_t; (void)grad_e; (void)st;
-  set_error(PSDR_ERR_STATE, "render_backward: not built yet");
-  return PSDR_ERR_STATE;
+namespace {
+
+constexpr int ADJ_THREADS = 256;
+constexpr int HOT_MAX = 1024;  // LDS accumulators for emission + constant textures
+
+struct AdjParams {
+  SceneView sv;
+  CameraK cam;
+  int D;
+  uint32_t nL, nB;
+  int elv;
+  uint32_t spp;
+  int Kmax;
+  int64_t N;
+  const int* ctri;
+  const float* cu;
+  const float* cv;
+  const float* dLdI;
+  float* grad_v;
+  float* grad_t;
+  float* grad_e;
+  int hot_n;  // 3*n_lights + 4*n_objects if it fits in HOT_MAX, else 0
+};
+
+__device__ __forceinline__ f3 vjp_normalize(f3 x, f3 gy) {
+  const float r = 1.0f / sqrtf(sqnorm(x));
+  const f3 y = x * r;
+  return (gy - y * dot(y, gy)) * r;
+}
+
+__device__ __forceinline__ void acc_hot(float* hot, int hot_n, float* global, int slot, int gidx, float v) {
+  if (v == 0.f) return;
+  if (hot_n > 0) atomicAdd(hot + slot, v);
+  else atomicAdd(global + gidx, v);
+}
+
+struct VertexAdj {
+  f3 p, n, t;  // adjoints of the point, shading normal, tangent (after the front flip)
+  float area;  // adjoint of the triangle area (through the jacobian A / detach(A))
+};
+__device__ __forceinline__ VertexAdj zero_adj() { return VertexAdj{splat(0.f), splat(0.f), splat(0.f), 0.f}; }
+__device__ __forceinline__ bool nonzero(const VertexAdj& a) {
+  return (a.p.x != 0.f || a.p.y != 0.f || a.p.z != 0.f) || (a.n.x != 0.f || a.n.y != 0.f || a.n.z != 0.f) ||
+         (a.t.x != 0.f || a.t.y != 0.f || a.t.z != 0.f) || a.area != 0.f;
+}
+
+// VJP of a triangle record (src/mesh.cc:38-67) and of the barycentric point
+// p = (P0 + u (P1-P0)) + v (P2-P0) (src/scene.cc:130-133) into vertex gradients.
+// `flip` = -1 when the record's normal/tangent were negated to face the ray.
+__device__ void tri_vjp(const AdjParams& P, int tri, float u, float v, float flip, const VertexAdj& g) {
+  const SceneView& sv = P.sv;
+  const int obj = sv.tri_obj[tri];
+  if (!(sv.objs[obj].grad_mask & PSDR_GRAD_VERTICES)) return;
+  const float4 g0 = sv.tri_geom[5 * tri], g1 = sv.tri_geom[5 * tri + 1], g2 = sv.tri_geom[5 * tri + 2];
+  const f3 P0 = xyz4(g0), P1 = xyz4(g1), P2 = xyz4(g2);
+  f3 gP0 = g.p * ((1.0f - u) - v);
+  f3 gP1 = g.p * u;
+  f3 gP2 = g.p * v;
+  const f3 a = P2 - P1, b = P0 - P2, c = P1 - P0;
+  f3 ga = splat(0.f), gb = splat(0.f), gc = splat(0.f);
+  if (g.n.x != 0.f || g.n.y != 0.f || g.n.z != 0.f) {
+    const f3 gr = vjp_normalize(cross(c, b), g.n * flip);
+    gc = gc + cross(b, gr);
+    gb = gb + cross(gr, c);
+  }
+  if (g.t.x != 0.f || g.t.y != 0.f || g.t.z != 0.f) {
+    const float* uv = sv.tri_uv + 6 * tri;
+    const float duv1y = uv[3] - uv[1], duv2y = uv[5] - uv[1];
+    const f3 gr = vjp_normalize(duv2y * c + duv1y * b, g.t * flip);
+    gc = gc + duv2y * gr;
+    gb = gb + duv1y * gr;
+  }
+  if (g.area != 0.f) {
+    const float la = norm(a), lb = norm(b), lc = norm(c);
+    const float p = ((la + lb) + lc) / 2.0f;
+    const float q1 = p - la, q2 = p - lb, q3 = p - lc;
+    const float Q = ((p * q1) * q2) * q3;
+    if (Q > 0.f) {
+      const float dAdQ = 0.5f / sqrtf(Q);
+      const float s = g.area * dAdQ * 0.5f;
+      const float qqq = q1 * q2 * q3;
+      const float dla = qqq - p * q2 * q3 + p * q1 * q3 + p * q1 * q2;
+      const float dlb = qqq + p * q2 * q3 - p * q1 * q3 + p * q1 * q2;
+      const float dlc = qqq + p * q2 * q3 + p * q1 * q3 - p * q1 * q2;
+      ga = ga + a * (s * dla / la);
+      gb = gb + b * (s * dlb / lb);
+      gc = gc + c * (s * dlc / lc);
+    }
+  }
+  gP2 = gP2 + ga;
+  gP1 = gP1 - ga;
+  gP0 = gP0 + gb;
+  gP2 = gP2 - gb;
+  gP1 = gP1 + gc;
+  gP0 = gP0 - gc;
+  const int* f = sv.faces + 3 * tri;
+  const f3 gs[3] = {gP0, gP1, gP2};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    float* dst = P.grad_v + 3 * static_cast<int64_t>(f[k]);
+    if (gs[k].x != 0.f) atomicAdd(dst, gs[k].x);
+    if (gs[k].y != 0.f) atomicAdd(dst + 1, gs[k].y);
+    if (gs[k].z != 0.f) atomicAdd(dst + 2, gs[k].z);
+  }
+}
+
+// flip sign of a reconstructed intersection: its.n = flip * record normal
+__device__ __forceinline__ float flip_of(const SceneView& sv, const Its& it) {
+  const float4 g3 = sv.tri_geom[5 * it.tri + 3];
+  return (it.n.x == g3.x && it.n.y == g3.y && it.n.z == g3.z) ? 1.f : -1.f;
+}
+
+// VJP of Material::eval at (its, L) w.r.t. texels, its.n, L and its.wi.
+struct BsdfAdj {
+  f3 N, L, V;
+};
+__device__ BsdfAdj bsdf_vjp(const AdjParams& P, float* hot, const Its& it, f3 L, f3 g) {
+  const SceneView& sv = P.sv;
+  const ObjectK& ob = sv.objs[it.obj];
+  BsdfAdj r{splat(0.f), splat(0.f), splat(0.f)};
+  const int co = tex3_offset(ob.alb_off, ob.alb_h, ob.alb_w, it.uv);
+  f3 gcol;
+  if (ob.mat == PSDR_MAT_DIFFUSE_BSDF) {
+    const f3 color = tex3(sv.texels, co);
+    const int so = tex1_offset(ob.rough_off, ob.rough_h, ob.rough_w, it.uv);
+    const float sigma = sv.texels[so];
+    const float kk = (3.0f * REAL_PI - 4.0f) / 6.0f;
+    const float div = 1.0f / (REAL_PI + kk * sigma);
+    const float a = div, b = sigma * div;
+    const float nl_raw = dot(it.n, L), nv_raw = dot(it.n, it.wi);
+    const float NdotL = nl_raw < 0.f ? 0.f : nl_raw;
+    const float NdotV = nv_raw < 0.f ? 0.f : nv_raw;
+    const float VdotL = dot(it.wi, L);
+    float t = VdotL - NdotL * NdotV;
+    const bool tpos = t > 0.f;
+    const bool pickL = NdotL >= NdotV;
+    if (tpos) t = (pickL ? NdotL : NdotV) + REAL_EPS;
+    gcol = g * (a + b * t);
+    const float s = (g.x * color.x + g.y * color.y) + g.z * color.z;
+    const float ga = s, gb = s * t, gt = s * b;
+    if (ob.grad_mask & PSDR_GRAD_ROUGHNESS) {
+      const float dadsig = -kk * div * div;
+      const float gsig = ga * dadsig + gb * (div + sigma * dadsig);
+      if (ob.rough_h == 0) acc_hot(hot, P.hot_n, P.grad_t, 3 * sv.n_lights + 4 * it.obj + 3, so, gsig);
+      else atomicAdd(P.grad_t + so, gsig);
+    }
+    float gNL = 0.f, gNV = 0.f, gVL = 0.f;
+    if (tpos) {
+      if (pickL) gNL = gt;
+      else gNV = gt;
+    } else {
+      gVL = gt;
+      gNL = -gt * NdotV;
+      gNV = -gt * NdotL;
+    }
+    if (nl_raw > 0.f) {
+      r.N = r.N + L * gNL;
+      r.L = r.L + it.n * gNL;
+    }
+    if (nv_raw > 0.f) {
+      r.N = r.N + it.wi * gNV;
+      r.V = r.V + it.n * gNV;
+    }
+    r.V = r.V + L * gVL;
+    r.L = r.L + it.wi * gVL;
+  } else {
+    gcol = g * REAL_INV_PI;
+  }
+  if (ob.grad_mask & PSDR_GRAD_ALBEDO) {
+    if (ob.alb_h == 0) {
+      acc_hot(hot, P.hot_n, P.grad_t, 3 * sv.n_lights + 4 * it.obj + 0, co + 0, gcol.x);
+      acc_hot(hot, P.hot_n, P.grad_t, 3 * sv.n_lights + 4 * it.obj + 1, co + 1, gcol.y);
+      acc_hot(hot, P.hot_n, P.grad_t, 3 * sv.n_lights + 4 * it.obj + 2, co + 2, gcol.z);
+    } else {
+      atomicAdd(P.grad_t + co, gcol.x);
+      atomicAdd(P.grad_t + co + 1, gcol.y);
+      atomicAdd(P.grad_t + co + 2, gcol.z);
+    }
+  }
+  return r;
+}
+
+__device__ __forceinline__ void emit_vjp(const AdjParams& P, float* hot, int l, f3 g) {
+  const int obj = P.sv.lights[l].obj;
+  if (!(P.sv.objs[obj].grad_mask & PSDR_GRAD_EMIT)) return;
+  acc_hot(hot, P.hot_n, P.grad_e, 3 * l, 3 * l, g.x);
+  acc_hot(hot, P.hot_n, P.grad_e, 3 * l + 1, 3 * l + 1, g.y);
+  acc_hot(hot, P.hot_n, P.grad_e, 3 * l + 2, 3 * l + 2, g.z);
+}
+
+__device__ __forceinline__ f3 light_emit(const SceneView& sv, int l) {
+  return f3{sv.light_emit[3 * l], sv.light_emit[3 * l + 1], sv.light_emit[3 * l + 2]};
+}
+
+struct DepthOut {
+  f3 c;       // sum of weighted NEE + BSDF-MIS values at this depth
+  f3 t;       // extension throughput factor
+  bool ext;   // extension succeeded (the path continues)
+  int ntri;   // next vertex hit
+  float nu, nv;
+  VertexAdj cur;   // adjoint of this vertex (point, normal, tangent)
+  f3 gO;           // adjoint of this vertex's ray origin (through wi)
+  VertexAdj next;  // adjoint of the next vertex (point, normal, area)
+};
+
+__device__ __forceinline__ void draw_cached(const AdjParams& P, int64_t slot, int64_t lane, int* t, float* u,
+                                            float* v) {
+  const int64_t i = slot * P.N + lane;
+  *t = P.ctri[i];
+  *u = P.cu[i];
+  *v = P.cv[i];
+}
+
+// One depth of trace2light at vertex `its` (state: rng at the depth's first
+// draw).  With grad, adjoints adj_c / adj_t are pushed through this depth.
+__device__ DepthOut depth_step(const AdjParams& P, float* hot, const Its& its, Rng rng, int d, int64_t lane,
+                               bool grad, f3 adj_c, f3 adj_t) {
+  const SceneView& sv = P.sv;
+  DepthOut out;
+  out.c = splat(0.f);
+  out.t = splat(0.f);
+  out.ext = false;
+  out.ntri = -1;
+  out.nu = out.nv = 0.f;
+  out.cur = zero_adj();
+  out.next = zero_adj();
+  out.gO = splat(0.f);
+  const int64_t cslot = 1 + static_cast<int64_t>(d) * P.Kmax;
+  int k = 0;
+  f3 gV = splat(0.f);  // adjoint of its.wi
+  // ---- next-event estimation
+  for (uint32_t i = 0; i < P.nL; ++i, ++k) {
+    float ru = rng.next();
+    const float rv = rng.next();
+    float pdf_choose;
+    const int obj = discrete_sample(sv.obj_pmf, sv.obj_cmf, sv.n_objects, &ru, &pdf_choose);
+    const int l = sv.objs[obj].light;
+    if (l < 0) continue;
+    const LightPoint lp = sample_light_point(sv, l, ru, rv);
+    const float pdf_light = lp.pdf * pdf_choose;
+    const f3 dvec = lp.p - its.p;
+    const f3 dir = normalize(dvec);
+    int ht;
+    float hu, hv;
+    draw_cached(P, cslot + k, lane, &ht, &hu, &hv);
+    const Its itl = reconstruct(sv, ht, hu, hv, its.p, true);
+    if (!(itl.valid && itl.is_light && sqnorm(itl.p - lp.p) < REAL_EPS)) continue;
+    const f3 emit = light_emit(sv, l);
+    const f3 bsdf = mat_eval(sv, its, dir);
+    const float tmp = dot(itl.n, -dir) / itl.d2;
+    const float nd = dot(its.n, dir);
+    const float G = nd * tmp;
+    const float Jl = itl.area / itl.area;
+    const f3 value = (((emit * G) * bsdf) * Jl) / pdf_light;
+    float weight = 1.0f / static_cast<float>(P.nL);
+    if (P.nB > 0) weight = weight * mis_weight(pdf_light, mat_pdf(its, dir) * tmp, P.nL, P.nB);
+    out.c = out.c + value * weight;
+    if (!grad) continue;
+    const f3 av = adj_c * (weight / pdf_light);
+    emit_vjp(P, hot, l, av * bsdf * (G * Jl));
+    const f3 gbsdf = av * emit * (G * Jl);
+    const float gG = dot(av, emit * bsdf) * Jl;
+    const float gJ = dot(av, emit * bsdf) * G;
+    const BsdfAdj ba = bsdf_vjp(P, hot, its, dir, gbsdf);
+    out.cur.n = out.cur.n + ba.N + dir * (gG * tmp);
+    gV = gV + ba.V;
+    f3 gdir = ba.L + its.n * (gG * tmp);
+    const float gtmp = gG * nd;
+    VertexAdj lh = zero_adj();
+    lh.n = (-dir) * (gtmp / itl.d2);
+    gdir = gdir + (-itl.n) * (gtmp / itl.d2);
+    const float gd2 = -gtmp * tmp / itl.d2;
+    const f3 gpl = (itl.p - its.p) * (2.0f * gd2);
+    lh.p = gpl;
+    out.cur.p = out.cur.p - gpl;
+    lh.area = gJ / itl.area;
+    const f3 gdv = vjp_normalize(dvec, gdir);
+    out.cur.p = out.cur.p - gdv;
+    VertexAdj ls = zero_adj();
+    ls.p = gdv;
+    tri_vjp(P, itl.tri, hu, hv, flip_of(sv, itl), lh);
+    tri_vjp(P, lp.tri, lp.bu, lp.bv, 1.f, ls);
+  }
+  // ---- BSDF sampling with MIS
+  for (uint32_t i = 0; i < P.nB; ++i, ++k) {
+    const float ru = rng.next();
+    const float rv = rng.next();
+    const Sampled smp = mat_sample(sv, its, ru, rv);
+    int ht;
+    float hu, hv;
+    draw_cached(P, cslot + k, lane, &ht, &hu, &hv);
+    const Its itl = reconstruct(sv, ht, hu, hv, its.p, true);
+    if (!(itl.valid && itl.is_light)) continue;
+    const int l = itl.light;
+    const f3 emit = light_emit(sv, l);
+    const float tmp = dot(itl.n, -smp.wo) / itl.d2;
+    const float nd = dot(its.n, smp.wo);
+    const float G = nd * tmp;
+    const float pdf_bsdf = smp.pdf * tmp;
+    const float Jl = itl.area / itl.area;
+    const f3 value = (((emit * G) * smp.bsdf) * Jl) / pdf_bsdf;
+    float weight = 1.0f / static_cast<float>(P.nB);
+    if (P.nL > 0) {
+      const float pdf_light = (sv.light_power[l] / *sv.lights_total) * (1.0f / sv.light_area[l]);
+      weight = weight * mis_weight(pdf_bsdf, pdf_light, P.nB, P.nL);
+    }
+    out.c = out.c + value * weight;
+    if (!grad) continue;
+    const f3 av = adj_c * (weight / pdf_bsdf);
+    emit_vjp(P, hot, l, av * smp.bsdf * (G * Jl));
+    const f3 gbsdf = av * emit * (G * Jl);
+    const float gG = dot(av, emit * smp.bsdf) * Jl;
+    const float gJ = dot(av, emit * smp.bsdf) * G;
+    const BsdfAdj ba = bsdf_vjp(P, hot, its, smp.wo, gbsdf);
+    out.cur.n = out.cur.n + ba.N + smp.wo * (gG * tmp);
+    gV = gV + ba.V;
+    f3 gwo = ba.L + its.n * (gG * tmp);
+    const float gtmp = gG * nd;
+    VertexAdj lh = zero_adj();
+    lh.n = (-smp.wo) * (gtmp / itl.d2);
+    gwo = gwo + (-itl.n) * (gtmp / itl.d2);
+    const float gd2 = -gtmp * tmp / itl.d2;
+    const f3 gpl = (itl.p - its.p) * (2.0f * gd2);
+    lh.p = gpl;
+    out.cur.p = out.cur.p - gpl;
+    lh.area = gJ / itl.area;
+    // wo = normalize(u T + v B + cos N), B = cross(N, T)
+    const f3 B = cross(its.n, its.t);
+    const f3 wu = (smp.u * its.t + smp.v * B) + smp.cos_t * its.n;
+    const f3 gwu = vjp_normalize(wu, gwo);
+    const f3 gB = gwu * smp.v;
+    out.cur.t = out.cur.t + gwu * smp.u + cross(gB, its.n);
+    out.cur.n = out.cur.n + gwu * smp.cos_t + cross(its.t, gB);
+    tri_vjp(P, itl.tri, hu, hv, flip_of(sv, itl), lh);
+  }
+  // ---- extension
+  {
+    const float ru = rng.next();
+    const float rv = rng.next();
+    if (d < P.D - 1) {
+      const Sampled smp = mat_sample(sv, its, ru, rv);
+      int ht;
+      float hu, hv;
+      draw_cached(P, cslot + k, lane, &ht, &hu, &hv);
+      const Its itn = reconstruct(sv, ht, hu, hv, its.p, true);
+      if (smp.valid && itn.valid) {
+        out.ext = true;
+        out.ntri = ht;
+        out.nu = hu;
+        out.nv = hv;
+        const float tmp = dot(itn.n, -smp.wo) / itn.d2;
+        const float nd = dot(its.n, smp.wo);
+        const float G = nd * tmp;
+        const float pdf = smp.pdf * tmp;
+        const float Jn = itn.area / itn.area;
+        out.t = ((smp.bsdf * Jn) * G) / pdf;
+        if (grad) {
+          const f3 at = adj_t / pdf;
+          const f3 gbsdf = at * (Jn * G);
+          const float gJ = dot(at, smp.bsdf) * G;
+          const float gG = dot(at, smp.bsdf) * Jn;
+          const BsdfAdj ba = bsdf_vjp(P, hot, its, smp.wo, gbsdf);
+          out.cur.n = out.cur.n + ba.N + smp.wo * (gG * tmp);
+          gV = gV + ba.V;
+          f3 gwo = ba.L + its.n * (gG * tmp);
+          const float gtmp = gG * nd;
+          out.next.n = (-smp.wo) * (gtmp / itn.d2);
+          gwo = gwo + (-itn.n) * (gtmp / itn.d2);
+          const float gd2 = -gtmp * tmp / itn.d2;
+          const f3 gpn = (itn.p - its.p) * (2.0f * gd2);
+          out.next.p = gpn;
+          out.cur.p = out.cur.p - gpn;
+          out.next.area = gJ / itn.area;
+          const f3 B = cross(its.n, its.t);
+          const f3 wu = (smp.u * its.t + smp.v * B) + smp.cos_t * its.n;
+          const f3 gwu = vjp_normalize(wu, gwo);
+          const f3 gB = gwu * smp.v;
+          out.cur.t = out.cur.t + gwu * smp.u + cross(gB, its.n);
+          out.cur.n = out.cur.n + gwu * smp.cos_t + cross(its.t, gB);
+        }
+      }
+    }
+  }
+  if (grad && (gV.x != 0.f || gV.y != 0.f || gV.z != 0.f)) {
+    // wi = -normalize(p - O); p - O = -wi * sqrt(d2)
+    const f3 wu = its.wi * (-sqrtf(its.d2));
+    const f3 gw = vjp_normalize(wu, gV * -1.0f);
+    out.cur.p = out.cur.p + gw;
+    out.gO = out.gO - gw;
+  }
+  return out;
+}
+
+template <int DMAX>
+__global__ __launch_bounds__(ADJ_THREADS) void k_adjoint(AdjParams P) {
+  __shared__ float hot[HOT_MAX];
+  for (int i = threadIdx.x; i < P.hot_n; i += blockDim.x) hot[i] = 0.f;
+  __syncthreads();
+  const SceneView& sv = P.sv;
+  for (int64_t lane = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; lane < P.N;
+       lane += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t pix = P.spp > 1 ? lane / P.spp : lane;
+    f3 g = f3{P.dLdI[3 * pix], P.dLdI[3 * pix + 1], P.dLdI[3 * pix + 2]};
+    if (P.spp > 1) g = g / static_cast<float>(P.spp);
+    if (g.x == 0.f && g.y == 0.f && g.z == 0.f) continue;
+    Rng rng = lane_rng(static_cast<uint32_t>(lane));
+    rng.next();
+    rng.next();
+    int t0;
+    float u0, v0;
+    draw_cached(P, 0, lane, &t0, &u0, &v0);
+    Its its = reconstruct(sv, t0, u0, v0, P.cam.pos, true);
+    if (!its.valid) continue;
+    // forward sweep: per-depth PCG state, throughput and vertex
+    uint64_t rs[DMAX];
+    f3 Ts[DMAX];
+    int vt[DMAX + 1];
+    float vu[DMAX + 1], vv[DMAX + 1];
+    vt[0] = t0;
+    vu[0] = u0;
+    vv[0] = v0;
+    f3 T = splat(1.f), L = splat(0.f);
+    int n = 0;            // depths with NEE/BSDF work
+    bool light0 = false;  // camera vertex on a light with enable_light_visable
+    Its cur = its;
+    for (int d = 0; d < P.D && d < DMAX; ++d) {
+      if (cur.is_light) {
+        if (P.elv && d == 0) {
+          L = L + T * light_emit(sv, cur.light);
+          light0 = true;
+        }
+        break;
+      }
+      rs[d] = rng.state;
+      Ts[d] = T;
+      n = d + 1;
+      const DepthOut o = depth_step(P, nullptr, cur, rng, d, lane, false, splat(0.f), splat(0.f));
+      L = L + T * o.c;
+      // advance the stream past this depth's draws
+      for (uint32_t q = 0; q < 2 * (P.nL + P.nB + 1); ++q) rng.next_u32();
+      if (!o.ext) break;
+      T = T * o.t;
+      vt[d + 1] = o.ntri;
+      vu[d + 1] = o.nu;
+      vv[d + 1] = o.nv;
+      cur = reconstruct(sv, o.ntri, o.nu, o.nv, cur.p, true);
+    }
+    if (!isfinite(sqnorm(L))) continue;
+    // enable_light_visable: L += T0 * emit at a light camera vertex
+    if (light0) emit_vjp(P, hot, cur.light, g);
+    // reverse sweep.  pend = adjoint of vertex d+1 gathered so far (its own
+    // depth's terms and, through its wi, vertex d+2's); pendO = adjoint of
+    // vertex d's point from vertex d+1's wi.
+    f3 A = splat(0.f);  // adjoint of T_{d+1}
+    VertexAdj pend = zero_adj();
+    f3 pendO = splat(0.f);
+    for (int d = n - 1; d >= 0; --d) {
+      f3 O = P.cam.pos;
+      if (d > 0) O = reconstruct(sv, vt[d - 1], vu[d - 1], vv[d - 1], P.cam.pos, true).p;
+      const Its vx = reconstruct(sv, vt[d], vu[d], vv[d], O, true);
+      Rng r{rs[d], rng.inc};
+      const DepthOut o = depth_step(P, hot, vx, r, d, lane, true, g * Ts[d], A * Ts[d]);
+      if (o.ext) {
+        VertexAdj nx = pend;
+        nx.p = nx.p + o.next.p;
+        nx.n = nx.n + o.next.n;
+        nx.area = nx.area + o.next.area;
+        if (nonzero(nx)) {
+          const Its nv = reconstruct(sv, vt[d + 1], vu[d + 1], vv[d + 1], vx.p, true);
+          tri_vjp(P, vt[d + 1], vu[d + 1], vv[d + 1], flip_of(sv, nv), nx);
+        }
+      }
+      pend = o.cur;
+      pend.p = pend.p + pendO;
+      pendO = o.gO;
+      A = g * o.c + A * o.t;
+    }
+    // camera vertex: its own adjoints plus render_interior's L * J0 (J0 = A / detach(A))
+    pend.area = pend.area + dot(g, L) / its.area;
+    tri_vjp(P, vt[0], vu[0], vv[0], flip_of(sv, its), pend);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < P.hot_n; i += blockDim.x) {
+    const float v = hot[i];
+    if (v == 0.f) continue;
+    const int nl3 = 3 * P.sv.n_lights;
+    if (i < nl3) {
+      atomicAdd(P.grad_e + i, v);
+    } else {
+      const int o = (i - nl3) / 4, c = (i - nl3) % 4;
+      const ObjectK& ob = P.sv.objs[o];
+      if (c < 3) atomicAdd(P.grad_t + ob.alb_off + c, v);
+      else atomicAdd(P.grad_t + ob.rough_off, v);
+    }
+  }
+}
+
+inline int blocks(int64_t n, int t) { return static_cast<int>((n + t - 1) / t); }
+
+}  // namespace
+
+int render_backward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& camd, const float* dL_dI,
+                    float* grad_v, float* grad_t, float* grad_e, int terms, hipStream_t st) {
+  if (terms & PSDR_TERM_INTERIOR) {
+    const int rc = interior_backward(s, cfg, camd, dL_dI, grad_v, grad_t, grad_e, st);
+    if (rc) return rc;
+  }
+  if (terms & (PSDR_TERM_PRIMARY | PSDR_TERM_DIRECT)) return boundary_backward(s, cfg, camd, dL_dI, grad_v, terms, st);
+  return PSDR_OK;
+}
+
+int interior_backward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& camd, const float* dL_dI,
+                      float* grad_v, float* grad_t, float* grad_e, hipStream_t st) {
+  AdjParams P;
+  P.sv = s->view();
+  P.cam = make_camera(camd);
+  P.D = static_cast<int>(cfg.max_bounce);
+  P.nL = cfg.mis_light_samples;
+  P.nB = cfg.mis_bsdf_samples;
+  P.elv = cfg.enable_light_visable ? 1 : 0;
+  P.spp = cfg.spp_interior;
+  P.Kmax = s->last_K;
+  P.N = static_cast<int64_t>(s->last_N);
+  const int64_t slots = 1 + static_cast<int64_t>(s->last_D) * s->last_K;
+  P.ctri = s->hit_cache.as<int>();
+  P.cu = reinterpret_cast<const float*>(P.ctri + slots * P.N);
+  P.cv = P.cu + slots * P.N;
+  P.dLdI = dL_dI;
+  // scratch for absent gradient buffers
+  if (s->stats.ensure(4ull * (3 * static_cast<size_t>(s->n_vertices) + s->n_texels + 3 * s->n_lights + 64)))
+    return PSDR_ERR_NOMEM;
+  float* sink = s->stats.as<float>();
+  P.grad_v = grad_v ? grad_v : sink;
+  P.grad_t = grad_t ? grad_t : sink;
+  P.grad_e = grad_e ? grad_e : sink;
+  const int hot = 3 * s->n_lights + 4 * s->n_objects;
+  P.hot_n = hot <= HOT_MAX ? hot : 0;
+  const int grid = std::max(1, std::min(blocks(P.N, ADJ_THREADS), 256 * 16));
+  if (P.D <= 4) hipLaunchKernelGGL(k_adjoint<4>, dim3(grid), dim3(ADJ_THREADS), 0, st, P);
+  else if (P.D <= 8) hipLaunchKernelGGL(k_adjoint<8>, dim3(grid), dim3(ADJ_THREADS), 0, st, P);
+  else hipLaunchKernelGGL(k_adjoint<64>, dim3(grid), dim3(ADJ_THREADS), 0, st, P);
+  PSDR_HIP_CHECK(hipGetLastError());
+  return PSDR_OK;
 }
 
 }  // namespace psdr

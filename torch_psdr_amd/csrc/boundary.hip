@@ -1,0 +1,521 @@
+// Edge-sampling boundary terms of renderD (src/path_integrator.cc:11-78, 188-353).
+//
+// Primary (silhouette) edges, per lane of Sampler(spp_primary_edge*H*W):
+//   k_prim_setup   1 draw, hypercube cell pick (10k cells), edge point, camera
+//                  projection, rays at d -/+ 1e-5 * edge normal
+//   trace          both rays
+//   k_prim_check   both hits valid and one of them on the edge point
+//   run_depths x2  trace2light from each hit (B = 1, no light visibility),
+//                  the second continuing the first one's PCG stream
+//   k_prim_finish  preprocess: hypercube mass |dL / pdf|; render: the VJP of
+//                  [(x_e - detach x_e) . n] dL / detach(pdf) into the edge's vertices
+// Direct (shadow) edges, per lane of Sampler(spp_secondary_edge*H*W):
+//   k_direct       3 draws, 3-D hypercube pick (10k x 6 x 6 cells), edge and
+//                  light points, 3 closest hits (toward / away from the light,
+//                  camera visibility), boundary_term with its hit_triangle
+//                  re-parameterisation; preprocess accumulates |value|, render
+//                  pushes the VJP through hit_triangle into the edge's vertices.
+// The hypercube preprocessing passes reuse one Sampler(cells) stream across
+// rounds, exactly as the reference does.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "render.h"
+#include "shading.h"
+#include "traverse.h"
+#include "wavefront.h"
+
+namespace psdr {
+
+namespace {
+
+constexpr int BT = 256;
+constexpr int BSTACK = 16;
+
+inline int blocks(int64_t n, int t) { return static_cast<int>((n + t - 1) / t); }
+
+struct EdgeRec {
+  f3 p0, p1, out, nl, nr;
+  float len;
+  int v0, v1, obj;
+};
+__device__ __forceinline__ EdgeRec load_edge(const SceneView& sv, int e) {
+  const float* g = sv.edge_geom + 16 * e;
+  EdgeRec E;
+  E.p0 = f3{g[0], g[1], g[2]};
+  E.p1 = f3{g[3], g[4], g[5]};
+  E.out = f3{g[6], g[7], g[8]};
+  E.nl = f3{g[9], g[10], g[11]};
+  E.nr = f3{g[12], g[13], g[14]};
+  E.len = g[15];
+  E.v0 = sv.edge_topo[6 * e];
+  E.v1 = sv.edge_topo[6 * e + 1];
+  E.obj = sv.tri_obj[sv.edge_topo[6 * e + 2]];
+  return E;
+}
+
+__device__ __forceinline__ void scatter_edge(const SceneView& sv, float* grad_v, const EdgeRec& E, float t, f3 g) {
+  if (!(sv.objs[E.obj].grad_mask & PSDR_GRAD_VERTICES)) return;
+  const f3 g0 = g * (1.0f - t), g1 = g * t;
+  float* d0 = grad_v + 3 * static_cast<int64_t>(E.v0);
+  float* d1 = grad_v + 3 * static_cast<int64_t>(E.v1);
+  atomicAdd(d0, g0.x);
+  atomicAdd(d0 + 1, g0.y);
+  atomicAdd(d0 + 2, g0.z);
+  atomicAdd(d1, g1.x);
+  atomicAdd(d1 + 1, g1.y);
+  atomicAdd(d1 + 2, g1.z);
+}
+
+// ---------------------------------------------------------------- primary
+struct PrimRecs {
+  int* edge;
+  float* t;
+  float* pdf_ep;
+  float* pdf_cube;
+  int* pix;  // -1: invalid
+  float* n;  // [3][cap]
+  int cap;
+};
+PrimRecs carve_prim(DevBuf& b, int cap) {
+  PrimRecs r;
+  char* p = static_cast<char*>(b.p);
+  r.cap = cap;
+  r.edge = reinterpret_cast<int*>(p);
+  p += 4ull * cap;
+  r.t = reinterpret_cast<float*>(p);
+  p += 4ull * cap;
+  r.pdf_ep = reinterpret_cast<float*>(p);
+  p += 4ull * cap;
+  r.pdf_cube = reinterpret_cast<float*>(p);
+  p += 4ull * cap;
+  r.pix = reinterpret_cast<int*>(p);
+  p += 4ull * cap;
+  r.n = reinterpret_cast<float*>(p);
+  return r;
+}
+constexpr size_t PRIM_BYTES_PER = 4 * 5 + 12;
+
+// PRE: preprocess round (cell = lane, state persists); else the render pass.
+template <bool PRE>
+__global__ void k_prim_setup(SceneView sv, CameraK cam, int n, uint64_t* __restrict__ state, int res,
+                             const float* __restrict__ cpmf, const float* __restrict__ ccmf, PrimRecs rec,
+                             RaysView rays, QueueView q0, int* __restrict__ count0) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) *count0 = n;
+  if (i >= n) return;
+  Rng rng;
+  float s, pdf_cube = 1.f;
+  const float unit = 1.0f / static_cast<float>(res);
+  if (PRE) {
+    rng.state = state[i];
+    rng.inc = lane_inc(static_cast<uint32_t>(i));
+    s = (static_cast<float>(i) + rng.next()) * unit;
+  } else {
+    rng = lane_rng(static_cast<uint32_t>(i));
+    s = rng.next();
+    const int c = discrete_sample(cpmf, ccmf, res, &s, &pdf_cube);
+    s = (s + static_cast<float>(c)) * unit;
+  }
+  q0.lane[i] = static_cast<uint32_t>(i);
+  q0.rng[i] = rng.state;
+  // Scene::sample_edge_point (src/scene.cc:189-206)
+  float u = s, pdf_e;
+  const int e = discrete_sample(sv.edge_pmf, sv.edge_cmf, sv.n_edges, &u, &pdf_e);
+  const EdgeRec E = load_edge(sv, e);
+  const f3 ep = E.p0 + u * (E.p1 - E.p0);
+  int px, py;
+  const bool valid = camera_project(cam, ep, &px, &py);
+  const f3 d = normalize(ep - cam.pos);
+  const f3 nn = cross(d, normalize(E.p1 - E.p0));
+  f3 da = d - nn * REAL_EPS, db = d + nn * REAL_EPS;
+  if (!valid) da = db = splat(__builtin_nanf(""));
+  rays.o[3 * i] = cam.pos.x; rays.o[3 * i + 1] = cam.pos.y; rays.o[3 * i + 2] = cam.pos.z;
+  rays.d[3 * i] = da.x; rays.d[3 * i + 1] = da.y; rays.d[3 * i + 2] = da.z;
+  const int j = n + i;
+  rays.o[3 * j] = cam.pos.x; rays.o[3 * j + 1] = cam.pos.y; rays.o[3 * j + 2] = cam.pos.z;
+  rays.d[3 * j] = db.x; rays.d[3 * j + 1] = db.y; rays.d[3 * j + 2] = db.z;
+  rec.edge[i] = e;
+  rec.t[i] = u;
+  rec.pdf_ep[i] = pdf_e / E.len;
+  rec.pdf_cube[i] = pdf_cube;
+  rec.pix[i] = valid ? py * cam.W + px : -1;
+  rec.n[i] = nn.x;
+  rec.n[rec.cap + i] = nn.y;
+  rec.n[2 * rec.cap + i] = nn.z;
+}
+
+// both hits valid and one of them within 1e-4 of the edge point (src/path_integrator.cc:227-230)
+__global__ void k_prim_check(SceneView sv, CameraK cam, int n, PrimRecs rec, RaysView rays) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || rec.pix[i] < 0) return;
+  const EdgeRec E = load_edge(sv, rec.edge[i]);
+  const f3 ep = E.p0 + rec.t[i] * (E.p1 - E.p0);
+  const Its a = reconstruct(sv, rays.tri[i], rays.u[i], rays.v[i], cam.pos, true);
+  const Its b = reconstruct(sv, rays.tri[n + i], rays.u[n + i], rays.v[n + i], cam.pos, true);
+  const bool ok = a.valid && b.valid && (norm(a.p - ep) < 10.0f * REAL_EPS || norm(b.p - ep) < 10.0f * REAL_EPS);
+  if (!ok) {
+    rays.tri[i] = -1;
+    rays.tri[n + i] = -1;
+    rec.pix[i] = -1;
+  }
+}
+
+__global__ void k_prim_requeue(int n, QueueView q0, const uint64_t* __restrict__ rng_a, int* __restrict__ count0) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) *count0 = n;
+  if (i >= n) return;
+  q0.lane[i] = static_cast<uint32_t>(i);
+  q0.rng[i] = rng_a[i];
+}
+
+template <bool PRE>
+__global__ void k_prim_finish(SceneView sv, CameraK cam, int n, PrimRecs rec, const float* __restrict__ LA,
+                              const float* __restrict__ LB, const uint64_t* __restrict__ rng_b,
+                              uint64_t* __restrict__ state, float* __restrict__ mass, const float* __restrict__ dLdI,
+                              uint32_t spp, float* __restrict__ grad_v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (PRE) state[i] = rng_b[i];
+  const int pix = rec.pix[i];
+  if (pix < 0) return;
+  const f3 dL = f3{LA[3 * i], LA[3 * i + 1], LA[3 * i + 2]} - f3{LB[3 * i], LB[3 * i + 1], LB[3 * i + 2]};
+  const float pdf_ep = rec.pdf_ep[i];
+  if (PRE) {
+    const f3 value = dL / pdf_ep;
+    if (isfinite(sqnorm(value))) mass[i] = mass[i] + norm(value);
+    return;
+  }
+  // value = [(x_e - detach x_e) . n] dL / detach(pdf_ep), zero in value
+  const f3 value = (dL * 0.0f) / pdf_ep;
+  if (!isfinite(sqnorm(value))) return;
+  if (!isfinite(sqnorm(value / rec.pdf_cube[i]))) return;
+  f3 g = f3{dLdI[3 * pix], dLdI[3 * pix + 1], dLdI[3 * pix + 2]};
+  if (spp > 1) g = g / static_cast<float>(spp);
+  g = g / rec.pdf_cube[i];
+  const float gx = dot(g, dL) / pdf_ep;
+  if (gx == 0.f) return;
+  const f3 nn = f3{rec.n[i], rec.n[rec.cap + i], rec.n[2 * rec.cap + i]};
+  const EdgeRec E = load_edge(sv, rec.edge[i]);
+  scatter_edge(sv, grad_v, E, rec.t[i], nn * gx);
+}
+
+// ---------------------------------------------------------------- direct
+__device__ __forceinline__ f3 vjp_norm(f3 x, f3 gy) {
+  const float r = 1.0f / sqrtf(sqnorm(x));
+  const f3 y = x * r;
+  return (gy - y * dot(y, gy)) * r;
+}
+
+struct DirectOut {
+  bool valid;
+  int pix;
+  f3 value;  // preprocess value (non-differentiable form)
+};
+
+// eval_direct_boundary (src/path_integrator.cc:287-329) + boundary_term
+// (src/path_integrator.cc:31-78).  Render pass (!PRE): the adjoint of the value
+// is dLdI[pixel] * gscale and is pushed into the edge's vertices.
+template <bool PRE>
+__device__ DirectOut eval_direct(const SceneView& sv, const CameraK& cam, float sx, float sy, float sz, int* stk,
+                                 int* spill, int spill_stride, const float* dLdI, float gscale, float* grad_v) {
+  DirectOut r;
+  r.valid = false;
+  r.pix = -1;
+  r.value = splat(0.f);
+  if (sv.n_edges == 0) return r;
+  float ue = sx, pdf_e;
+  const int e = discrete_sample(sv.edge_pmf, sv.edge_cmf, sv.n_edges, &ue, &pdf_e);
+  const EdgeRec E = load_edge(sv, e);
+  const f3 ep = E.p0 + ue * (E.p1 - E.p0);
+  const float pdf_ep = pdf_e / E.len;
+  float y = sy, pdf_choose;
+  const int obj = discrete_sample(sv.obj_pmf, sv.obj_cmf, sv.n_objects, &y, &pdf_choose);
+  const int l = sv.objs[obj].light;
+  if (l < 0) return r;
+  const LightPoint lp = sample_light_point(sv, l, y, sz);
+  const float pdf_light = lp.pdf * pdf_choose;
+  const f3 dir = normalize(ep - lp.p);
+  bool valid = (dot(E.nl, dir) * dot(E.nr, dir) < 0.f) && norm(E.nl - E.nr) > REAL_EPS;
+  if (!valid) return r;
+  HitRec h = closest_hit<BSTACK>(sv.node, sv.leaf, sv.n_tris, ep, -dir, stk, spill, spill_stride);
+  const Its s0 = reconstruct(sv, h.tri, h.u, h.v, ep, true);
+  valid = s0.valid && s0.is_light && sqnorm(s0.p - lp.p) < REAL_EPS;
+  if (!valid) return r;
+  h = closest_hit<BSTACK>(sv.node, sv.leaf, sv.n_tris, ep, dir, stk, spill, spill_stride);
+  const Its d0 = reconstruct(sv, h.tri, h.u, h.v, ep, true);
+  valid = d0.valid && !d0.is_light;
+  if (!valid) return r;
+  int px, py;
+  if (!camera_project(cam, d0.p, &px, &py)) return r;
+  const f3 cd = normalize(d0.p - cam.pos);
+  h = closest_hit<BSTACK>(sv.node, sv.leaf, sv.n_tris, cam.pos, cd, stk, spill, spill_stride);
+  const Its i2 = reconstruct(sv, h.tri, h.u, h.v, cam.pos, true);
+  if (!(i2.valid && !i2.is_light && sqnorm(i2.p - d0.p) < REAL_EPS)) return r;
+  const float pdf_xw = ((s0.d2 * pdf_ep) * pdf_light) / dot(s0.n, dir);
+  // boundary_term
+  const f3 xs = s0.p, xd = d0.p, xb = ep, ns = s0.n, nd = d0.n;
+  const f3 db = normalize(xb - xs);
+  f3 ev = normalize(E.p1 - E.p0);
+  if (dot(cross(db, ev), E.out) < 0.f) ev = -ev;
+  const f3 n_plane = cross(db, ev);
+  const f3 edge_proj = cross(nd, n_plane);
+  const float cos_s = fabsf(dot(ns, db));
+  const float cos_b = dot(ev, db);
+  const float cos_d = dot(edge_proj, db);
+  const float sin_b = safe_sqrt(1.0f - cos_b * cos_b);
+  const float sin_d = safe_sqrt(1.0f - cos_d * cos_d);
+  const float len_sb = norm(xb - xs);
+  const float len_sd = norm(xd - xs);
+  const float Jb = ((len_sd * len_sb) * sin_b) / (sin_d * cos_s);
+  const float dG = (cos_s * dot(nd, -db)) / (len_sd * len_sd);
+  const int ls = s0.light;
+  const f3 Ts = f3{sv.light_emit[3 * ls], sv.light_emit[3 * ls + 1], sv.light_emit[3 * ls + 2]};
+  const f3 Td = mat_eval(sv, d0, -cd);
+  r.pix = py * cam.W + px;
+  if (PRE) {
+    r.value = ((splat(dG) * Jb) / pdf_xw * Ts) * Td;
+    r.valid = isfinite(sqnorm(r.value));
+    return r;
+  }
+  const f3 nep = cross(nd, edge_proj);
+  const f3 fbk = nep * dG;  // value = (x_hit - detach x_hit) * fbk * Jb / pdf_xw * Ts * Td
+  const f3 v0 = (((fbk * 0.0f) * Jb) / pdf_xw * Ts) * Td;
+  r.valid = isfinite(sqnorm(v0));
+  if (!r.valid) return r;
+  const f3 g = f3{dLdI[3 * r.pix], dLdI[3 * r.pix + 1], dLdI[3 * r.pix + 2]} * gscale;
+  const f3 K = (((g * fbk) * Jb) / pdf_xw * Ts) * Td;  // adjoint of x_hit
+  if (K.x == 0.f && K.y == 0.f && K.z == 0.f) return r;
+  // hit_triangle(xs, db, xd, edge_proj, nep) (src/path_integrator.cc:11-29)
+  const f3 E1 = edge_proj, E2 = nep;
+  const f3 T = xs - xd;
+  const f3 P = cross(db, E2);
+  const f3 Q = cross(T, E1);
+  const float pe = dot(P, E1);
+  const float cl = fminf(fmaxf(pe, REAL_EPS), 1.0f - REAL_EPS);
+  const float inv = 1.0f / cl;
+  const float gu = dot(K, E1), gv = dot(K, E2);
+  f3 gP = T * (gu * inv);
+  const float ginv = gu * dot(P, T) + gv * dot(Q, db);
+  f3 gD = Q * (gv * inv);
+  if (!(pe < REAL_EPS) && !(pe > 1.0f - REAL_EPS)) gP = gP + E1 * (-ginv * inv * inv);
+  gD = gD + cross(E2, gP);
+  const f3 gxb = vjp_norm(xb - xs, gD);
+  scatter_edge(sv, grad_v, E, ue, gxb);
+  return r;
+}
+
+template <bool PRE>
+__global__ __launch_bounds__(BT) void k_direct(SceneView sv, CameraK cam, int n, uint64_t* __restrict__ state,
+                                               int r0, int r1, int r2, const float* __restrict__ cpmf,
+                                               const float* __restrict__ ccmf, float* __restrict__ mass,
+                                               const float* __restrict__ dLdI, uint32_t spp,
+                                               float* __restrict__ grad_v, int* __restrict__ spill) {
+  __shared__ int stk[BSTACK * BT];
+  const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int stride = gridDim.x * blockDim.x;
+  const int cells = r0 * r1 * r2;
+  const float u0 = 1.0f / static_cast<float>(r0), u1 = 1.0f / static_cast<float>(r1),
+              u2 = 1.0f / static_cast<float>(r2);
+  for (int i = gtid; i < n; i += stride) {
+    Rng rng;
+    float s[3];
+    float pdf_cube = 1.f;
+    if (PRE) {
+      rng.state = state[i];
+      rng.inc = lane_inc(static_cast<uint32_t>(i));
+      for (int k = 0; k < 3; ++k) s[k] = rng.next();
+      state[i] = rng.state;
+      const int c0 = i / (r1 * r2), c1 = (i / r2) % r1, c2 = i % r2;
+      s[0] = (static_cast<float>(c0) + s[0]) * u0;
+      s[1] = (static_cast<float>(c1) + s[1]) * u1;
+      s[2] = (static_cast<float>(c2) + s[2]) * u2;
+    } else {
+      rng = lane_rng(static_cast<uint32_t>(i));
+      for (int k = 0; k < 3; ++k) s[k] = rng.next();
+      const int c = discrete_sample(cpmf, ccmf, cells, &s[2], &pdf_cube);
+      const int c0 = c / (r1 * r2), c1 = (c / r2) % r1, c2 = c % r2;
+      s[0] = (s[0] + static_cast<float>(c0)) * u0;
+      s[1] = (s[1] + static_cast<float>(c1)) * u1;
+      s[2] = (s[2] + static_cast<float>(c2)) * u2;
+    }
+    if (PRE) {
+      DirectOut o = eval_direct<true>(sv, cam, s[0], s[1], s[2], stk, spill + gtid, stride, nullptr, 0.f, nullptr);
+      if (o.valid) {
+        f3 v = o.value;
+        v.x = v.x < REAL_EPS ? REAL_EPS : v.x;
+        v.y = v.y < REAL_EPS ? REAL_EPS : v.y;
+        v.z = v.z < REAL_EPS ? REAL_EPS : v.z;
+        mass[i] = mass[i] + norm(v);
+      }
+    } else {
+      float gscale = 1.0f / pdf_cube;
+      if (spp > 1) gscale = gscale / static_cast<float>(spp);
+      eval_direct<false>(sv, cam, s[0], s[1], s[2], stk, spill + gtid, stride, dLdI, gscale, grad_v);
+    }
+  }
+}
+
+__global__ void k_fill_f32(float* p, int n, float v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void k_seed_states(uint64_t* st, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) st[i] = lane_rng(static_cast<uint32_t>(i)).state;
+}
+
+int g_cus_b = 0;
+int cus() {
+  if (!g_cus_b) {
+    int dev = 0;
+    hipGetDevice(&dev);
+    hipDeviceProp_t p;
+    g_cus_b = (hipGetDeviceProperties(&p, dev) == hipSuccess && p.multiProcessorCount > 0) ? p.multiProcessorCount
+                                                                                         : 256;
+  }
+  return g_cus_b;
+}
+
+}  // namespace
+
+// One primary pass over n lanes (PRE: preprocess round over n = cells).
+static int primary_pass(psdr_scene* s, const psdr_config& cfg, const CameraK& cam, bool pre, int n, int res,
+                        const float* cpmf, const float* ccmf, float* mass, const float* dLdI, float* grad_v,
+                        hipStream_t st) {
+  const SceneView sv = s->view();
+  const uint32_t nL = cfg.mis_light_samples;
+  const int Kmax = static_cast<int>(nL + 1 + 1);
+  const int64_t ray_cap = static_cast<int64_t>(n) * Kmax;
+  if (s->b_rec.ensure(PRIM_BYTES_PER * n) || s->b_rays.ensure(24ull * 2 * n) || s->b_hits.ensure(12ull * 2 * n) ||
+      s->b_q0.ensure(QUEUE_BYTES_PER * n) || s->b_LA.ensure(12ull * n) || s->b_LB.ensure(12ull * n) ||
+      s->b_rngA.ensure(8ull * n) || s->b_rngB.ensure(8ull * n) || s->q_a.ensure(QUEUE_BYTES_PER * n) ||
+      s->q_b.ensure(QUEUE_BYTES_PER * n) || s->rays.ensure(24ull * ray_cap) || s->hits.ensure(12ull * ray_cap) ||
+      s->rays2.ensure(24ull * ray_cap) || s->hits2.ensure(12ull * ray_cap) || s->b_counters.ensure(4 * 256))
+    return PSDR_ERR_NOMEM;
+  PrimRecs rec = carve_prim(s->b_rec, n);
+  RaysView r0 = carve_rays(s->b_rays, s->b_hits, 2ll * n);
+  const QueueView q0 = carve_queue(s->b_q0, n);
+  const QueueView qs[2] = {carve_queue(s->q_a, n), carve_queue(s->q_b, n)};
+  const RaysView rs[2] = {carve_rays(s->rays, s->hits, ray_cap), carve_rays(s->rays2, s->hits2, ray_cap)};
+  int* cnt = s->b_counters.as<int>();
+  int rc = zero_ints(cnt, 256, st);
+  if (rc) return rc;
+  uint64_t* state = s->b_state.as<uint64_t>();
+  if (pre)
+    hipLaunchKernelGGL(k_prim_setup<true>, dim3(blocks(n, BT)), dim3(BT), 0, st, sv, cam, n, state, res, cpmf, ccmf,
+                       rec, r0, q0, cnt);
+  else
+    hipLaunchKernelGGL(k_prim_setup<false>, dim3(blocks(n, BT)), dim3(BT), 0, st, sv, cam, n, state, res, cpmf, ccmf,
+                       rec, r0, q0, cnt);
+  PSDR_HIP_CHECK(hipGetLastError());
+  const int tgrid = trace_grid(2ll * n);
+  if (s->spill.ensure(std::max(s->spill.bytes, spill_bytes(tgrid)))) return PSDR_ERR_NOMEM;
+  rc = trace_rays(sv, r0.o, r0.d, 2 * n, nullptr, r0.tri, nullptr, r0.u, r0.v, s->spill.as<int>(), tgrid, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_prim_check, dim3(blocks(n, BT)), dim3(BT), 0, st, sv, cam, n, rec, r0);
+  WaveArgs wa;
+  wa.cam = cam;
+  wa.D = static_cast<int>(cfg.max_bounce);
+  wa.nL = nL;
+  wa.nB = 1;
+  wa.elv = 0;
+  wa.ad_jac = 0;
+  wa.keep = 0;
+  wa.cache = CacheView{nullptr, nullptr, nullptr, n};
+  wa.lane_L = s->b_LA.as<float>();
+  wa.lane_rng_out = s->b_rngA.as<uint64_t>();
+  wa.counters = cnt;
+  rc = run_depths(s, wa, q0, r0, qs, rs, n, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_prim_requeue, dim3(blocks(n, BT)), dim3(BT), 0, st, n, q0, s->b_rngA.as<uint64_t>(),
+                     cnt + 128);
+  wa.lane_L = s->b_LB.as<float>();
+  wa.lane_rng_out = s->b_rngB.as<uint64_t>();
+  wa.counters = cnt + 128;
+  rc = run_depths(s, wa, q0, offset_rays(r0, n), qs, rs, n, st);
+  if (rc) return rc;
+  if (pre)
+    hipLaunchKernelGGL(k_prim_finish<true>, dim3(blocks(n, BT)), dim3(BT), 0, st, sv, cam, n, rec,
+                       s->b_LA.as<float>(), s->b_LB.as<float>(), s->b_rngB.as<uint64_t>(), state, mass, dLdI,
+                       cfg.spp_primary_edge, grad_v);
+  else
+    hipLaunchKernelGGL(k_prim_finish<false>, dim3(blocks(n, BT)), dim3(BT), 0, st, sv, cam, n, rec,
+                       s->b_LA.as<float>(), s->b_LB.as<float>(), s->b_rngB.as<uint64_t>(), state, mass, dLdI,
+                       cfg.spp_primary_edge, grad_v);
+  PSDR_HIP_CHECK(hipGetLastError());
+  return PSDR_OK;
+}
+
+int boundary_backward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& camd, const float* dL_dI,
+                      float* grad_v, int terms, hipStream_t st) {
+  if (s->n_edges == 0) return PSDR_OK;
+  const CameraK cam = make_camera(camd);
+  const SceneView sv = s->view();
+  const int64_t HW = static_cast<int64_t>(cam.H) * cam.W;
+  if (!grad_v) {
+    if (s->stats.ensure(4ull * 3 * std::max(s->n_vertices, 1))) return PSDR_ERR_NOMEM;
+    grad_v = s->stats.as<float>();
+  }
+  if (terms & PSDR_TERM_PRIMARY) {
+    const int R = static_cast<int>(cfg.primary_edge_hypercube_resolution);
+    const int64_t n64 = static_cast<int64_t>(cfg.spp_primary_edge) * HW;
+    if (n64 > 0x7fffffff / 8) {
+      set_error(PSDR_ERR_ARG, "primary boundary: too many samples");
+      return PSDR_ERR_ARG;
+    }
+    const int n = static_cast<int>(n64);
+    if (s->b_state.ensure(8ull * std::max(R, 1)) || s->hc_mass.ensure(4ull * R) || s->hc_pmf.ensure(4ull * R) ||
+        s->hc_cmf.ensure(4ull * R) || s->b_tmp.ensure(4ull * (2 * (R / 256 + 2) + 16)))
+      return PSDR_ERR_NOMEM;
+    hipLaunchKernelGGL(k_seed_states, dim3(blocks(R, BT)), dim3(BT), 0, st, s->b_state.as<uint64_t>(), R);
+    hipLaunchKernelGGL(k_fill_f32, dim3(blocks(R, BT)), dim3(BT), 0, st, s->hc_mass.as<float>(), R, 0.f);
+    for (uint32_t r = 0; r < cfg.primary_edge_preprocess_rounds; ++r) {
+      const int rc = primary_pass(s, cfg, cam, true, R, R, nullptr, nullptr, s->hc_mass.as<float>(), nullptr,
+                                  nullptr, st);
+      if (rc) return rc;
+    }
+    int rc = sampler_tables(s->hc_mass.as<float>(), R, s->hc_pmf.as<float>(), s->hc_cmf.as<float>(), nullptr,
+                            s->b_tmp.as<float>(), st);
+    if (rc) return rc;
+    rc = primary_pass(s, cfg, cam, false, n, R, s->hc_pmf.as<float>(), s->hc_cmf.as<float>(), nullptr, dL_dI, grad_v,
+                      st);
+    if (rc) return rc;
+  }
+  if (terms & PSDR_TERM_DIRECT) {
+    const int r0 = static_cast<int>(cfg.secondary_edge_hypercube_resolution[0]);
+    const int r1 = static_cast<int>(cfg.secondary_edge_hypercube_resolution[1]);
+    const int r2 = static_cast<int>(cfg.secondary_edge_hypercube_resolution[2]);
+    const int64_t cells64 = static_cast<int64_t>(r0) * r1 * r2;
+    const int64_t n64 = static_cast<int64_t>(cfg.spp_secondary_edge) * HW;
+    if (cells64 > 0x7fffffff / 8 || n64 > 0x7fffffff / 8) {
+      set_error(PSDR_ERR_ARG, "direct boundary: too many cells or samples");
+      return PSDR_ERR_ARG;
+    }
+    const int cells = static_cast<int>(cells64), n = static_cast<int>(n64);
+    if (s->b_state.ensure(8ull * cells) || s->hc_mass.ensure(4ull * cells) || s->hc_pmf.ensure(4ull * cells) ||
+        s->hc_cmf.ensure(4ull * cells) || s->b_tmp.ensure(4ull * (2 * (cells / 256 + 2) + 16)))
+      return PSDR_ERR_NOMEM;
+    const int grid = std::max(1, std::min(blocks(std::max(cells, n), BT), cus() * 8));
+    if (s->spill.ensure(std::max(s->spill.bytes, static_cast<size_t>(grid) * BT * (STACK_TOTAL - BSTACK) * 4)))
+      return PSDR_ERR_NOMEM;
+    hipLaunchKernelGGL(k_seed_states, dim3(blocks(cells, BT)), dim3(BT), 0, st, s->b_state.as<uint64_t>(), cells);
+    hipLaunchKernelGGL(k_fill_f32, dim3(blocks(cells, BT)), dim3(BT), 0, st, s->hc_mass.as<float>(), cells, 0.f);
+    for (uint32_t r = 0; r < cfg.secondary_edge_preprocess_rounds; ++r) {
+      hipLaunchKernelGGL(k_direct<true>, dim3(grid), dim3(BT), 0, st, sv, cam, cells, s->b_state.as<uint64_t>(), r0,
+                         r1, r2, nullptr, nullptr, s->hc_mass.as<float>(), nullptr, 0u, nullptr, s->spill.as<int>());
+      PSDR_HIP_CHECK(hipGetLastError());
+    }
+    int rc = sampler_tables(s->hc_mass.as<float>(), cells, s->hc_pmf.as<float>(), s->hc_cmf.as<float>(), nullptr,
+                            s->b_tmp.as<float>(), st);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_direct<false>, dim3(grid), dim3(BT), 0, st, sv, cam, n, s->b_state.as<uint64_t>(), r0, r1,
+                       r2, s->hc_pmf.as<float>(), s->hc_cmf.as<float>(), nullptr, dL_dI, cfg.spp_secondary_edge,
+                       grad_v, s->spill.as<int>());
+    PSDR_HIP_CHECK(hipGetLastError());
+  }
+  return PSDR_OK;
+}
+
+}  // namespace psdr

@@ -433,7 +433,9 @@ int psdr_scene_destroy(psdr_scene* s) {
                     &s->counters,    &s->sort_keys,   &s->sort_vals,     &s->sort_hist,     &s->bounds,
                     &s->lane_L,      &s->q_a,         &s->q_b,           &s->rays,          &s->hits,
                     &s->rays2,       &s->hits2,       &s->spill,         &s->hit_cache,     &s->counters_r,
-                    &s->hc_mass,     &s->hc_pmf,      &s->hc_cmf,        &s->stats};
+                    &s->hc_mass,     &s->hc_pmf,      &s->hc_cmf,        &s->stats,         &s->b_rec,
+                    &s->b_rays,      &s->b_hits,      &s->b_q0,          &s->b_LA,          &s->b_LB,
+                    &s->b_rngA,      &s->b_rngB,      &s->b_state,       &s->b_tmp,         &s->b_counters};
   hipDeviceSynchronize();
   for (DevBuf* b : bufs) b->release();
   delete s;
@@ -482,7 +484,7 @@ int psdr_render_forward(psdr_scene* s, const psdr_config* cfg, const psdr_camera
 }
 
 int psdr_render_backward(psdr_scene* s, const psdr_config* cfg, const psdr_camera* cam, const float* dL_dI,
-                         float* grad_vertices, float* grad_texels, float* grad_emit, void* stream) {
+                         float* grad_vertices, float* grad_texels, float* grad_emit, int32_t terms, void* stream) {
   if (!s || !dL_dI || !cfg_ok(cfg) || !cam_ok(cam)) {
     set_error(PSDR_ERR_ARG, "render_backward: bad argument");
     return PSDR_ERR_ARG;
@@ -492,7 +494,11 @@ int psdr_render_backward(psdr_scene* s, const psdr_config* cfg, const psdr_camer
     set_error(PSDR_ERR_STATE, "render_backward: no matching forward render with keep=1");
     return PSDR_ERR_STATE;
   }
-  return render_backward(s, *cfg, *cam, dL_dI, grad_vertices, grad_texels, grad_emit, as_stream(stream));
+  if (terms & ~PSDR_TERM_ALL) {
+    set_error(PSDR_ERR_ARG, "render_backward: bad terms mask");
+    return PSDR_ERR_ARG;
+  }
+  return render_backward(s, *cfg, *cam, dL_dI, grad_vertices, grad_texels, grad_emit, terms, as_stream(stream));
 }
 
 int psdr_pcg_lane_draws(uint32_t lane0, int32_t n_lanes, int32_t n_draws, uint32_t* out, void* stream) {

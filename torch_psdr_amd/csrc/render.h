@@ -13,5 +13,9 @@ int camera_rays(const CameraK& cam, uint32_t spp, float* o, float* d, int* pix, 
 int render_forward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& cam, float* out, int keep,
                    hipStream_t st);
 int render_backward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& cam, const float* dL_dI,
-                    float* grad_v, float* grad_t, float* grad_e, hipStream_t st);
+                    float* grad_v, float* grad_t, float* grad_e, int terms, hipStream_t st);
+int interior_backward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& cam, const float* dL_dI,
+                      float* grad_v, float* grad_t, float* grad_e, hipStream_t st);
+int boundary_backward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& cam, const float* dL_dI,
+                      float* grad_v, int terms, hipStream_t st);
 }  // namespace psdr

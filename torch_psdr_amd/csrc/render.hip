@@ -23,12 +23,15 @@
 #include "render.h"
 #include "shading.h"
 #include "traverse.h"
+#include "wavefront.h"
 
 namespace psdr {
 
 namespace {
 
 constexpr int TRACE_THREADS = 256;
+// render counters: [d] paths entering shade(d), [64 + d] rays traced at depth d
+constexpr int COUNTERS = 128;
 constexpr int STACK_LDS = 16;
 
 __global__ void k_lane_draws(uint32_t lane0, int n_lanes, int n_draws, uint32_t* out) {
@@ -72,34 +75,6 @@ __global__ __launch_bounds__(TRACE_THREADS) void k_trace(SceneView sv, const flo
     ov[i] = h.v;
   }
 }
-
-struct QueueView {
-  uint32_t* lane;
-  uint64_t* rng;
-  float* T;  // [3][cap]
-  float* L;  // [3][cap]
-  int* tri;
-  float* u;
-  float* v;
-  float* O;  // [3][cap]
-  float* J0;
-  int cap;
-};
-
-struct RaysView {
-  float* o;  // [3*cap] xyz interleaved
-  float* d;
-  int* tri;
-  float* u;
-  float* v;
-};
-
-struct CacheView {
-  int* tri;  // [slots][N]
-  float* u;
-  float* v;
-  int64_t N;
-};
 
 struct ShadeParams {
   SceneView sv;
@@ -377,6 +352,8 @@ int num_cus() {
   return g_cus;
 }
 
+}  // namespace
+
 QueueView carve_queue(DevBuf& b, int cap) {
   QueueView q;
   char* p = static_cast<char*>(b.p);
@@ -400,7 +377,6 @@ QueueView carve_queue(DevBuf& b, int cap) {
   q.O = reinterpret_cast<float*>(p);
   return q;
 }
-constexpr size_t QUEUE_BYTES_PER = 8 + 4 * 5 + 12 * 3;
 
 RaysView carve_rays(DevBuf& rb, DevBuf& hb, int64_t cap) {
   RaysView r;
@@ -412,7 +388,21 @@ RaysView carve_rays(DevBuf& rb, DevBuf& hb, int64_t cap) {
   return r;
 }
 
-}  // namespace
+RaysView offset_rays(const RaysView& r, int64_t off) {
+  RaysView o;
+  o.o = r.o + 3 * off;
+  o.d = r.d + 3 * off;
+  o.tri = r.tri + off;
+  o.u = r.u + off;
+  o.v = r.v + off;
+  return o;
+}
+
+int zero_ints(int* p, int n, hipStream_t st) {
+  hipLaunchKernelGGL(k_zero_i32, dim3(blocks(n, 256)), dim3(256), 0, st, p, n);
+  PSDR_HIP_CHECK(hipGetLastError());
+  return PSDR_OK;
+}
 
 int trace_rays(const SceneView& sv, const float* ro, const float* rd, int n, const int* count_dev, int* tri,
                float* t, float* u, float* v, int* spill, int grid, hipStream_t st) {
@@ -443,11 +433,52 @@ int camera_rays(const CameraK& cam, uint32_t spp, float* o, float* d, int* pix, 
   return PSDR_OK;
 }
 
+int run_depths(psdr_scene* s, const WaveArgs& a, const QueueView& q0, const RaysView& r0, const QueueView* q_s,
+               const RaysView* r_s, int n, hipStream_t st) {
+  const SceneView sv = s->view();
+  const int Kmax = static_cast<int>(a.nL + a.nB + 1);
+  const int64_t ray_cap = static_cast<int64_t>(n) * Kmax;
+  const int tgrid = trace_grid(ray_cap);
+  if (s->spill.ensure(std::max(s->spill.bytes, spill_bytes(tgrid)))) return PSDR_ERR_NOMEM;
+  const int sgrid = std::max(1, std::min(blocks(n, 256), num_cus() * 8));
+  QueueView qin = q0;
+  RaysView rin = r0;
+  for (int d = 0; d <= a.D; ++d) {
+    const QueueView qout = q_s[d & 1];
+    const RaysView rout = r_s[d & 1];
+    ShadeParams P;
+    P.sv = sv;
+    P.cam = a.cam;
+    P.depth = d;
+    P.D = a.D;
+    P.nL = a.nL;
+    P.nB = a.nB;
+    P.elv = a.elv;
+    P.ad_jac = a.ad_jac;
+    P.K_in = d == 0 ? 1 : static_cast<int>(a.nL + a.nB) + ((d - 1 < a.D - 1) ? 1 : 0);
+    P.K_out = d < a.D ? static_cast<int>(a.nL + a.nB) + ((d < a.D - 1) ? 1 : 0) : 0;
+    P.Kmax = Kmax;
+    P.keep = a.keep;
+    hipLaunchKernelGGL(k_shade, dim3(sgrid), dim3(256), 0, st, P, qin, a.counters + d, rin, qout, a.counters + d + 1,
+                       rout, a.cache, a.lane_L, a.lane_rng_out);
+    PSDR_HIP_CHECK(hipGetLastError());
+    if (d < a.D && P.K_out > 0) {
+      hipLaunchKernelGGL(k_ray_count, dim3(1), dim3(1), 0, st, a.counters + d + 1, P.K_out, a.counters + 64 + d);
+      const int rc = trace_rays(sv, rout.o, rout.d, static_cast<int>(ray_cap), a.counters + 64 + d, rout.tri, nullptr,
+                                rout.u, rout.v, s->spill.as<int>(), tgrid, st);
+      if (rc) return rc;
+    }
+    qin = qout;
+    rin = rout;
+  }
+  return PSDR_OK;
+}
+
 int render_forward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& camd, float* out, int keep,
                    hipStream_t st) {
   const CameraK cam = make_camera(camd);
   const int64_t N64 = static_cast<int64_t>(cfg.spp_interior) * cam.H * cam.W;
-  if (N64 <= 0 || N64 > 0x7fffffff / 4) {
+  if (N64 <= 0 || N64 > 0x7fffffff / 4 || cfg.max_bounce > 60) {
     set_error(PSDR_ERR_ARG, "render: spp*H*W out of range");
     return PSDR_ERR_ARG;
   }
@@ -458,7 +489,7 @@ int render_forward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& cam
   const int64_t ray_cap = static_cast<int64_t>(N) * std::max(Kmax, 1);
   // buffers
   if (s->lane_L.ensure(12ull * N) || s->q_a.ensure(QUEUE_BYTES_PER * N) || s->q_b.ensure(QUEUE_BYTES_PER * N) ||
-      s->rays.ensure(24ull * ray_cap) || s->hits.ensure(12ull * ray_cap) || s->counters_r.ensure(256))
+      s->rays.ensure(24ull * ray_cap) || s->hits.ensure(12ull * ray_cap) || s->counters_r.ensure(4 * COUNTERS))
     return PSDR_ERR_NOMEM;
   const int tgrid = trace_grid(ray_cap);
   if (s->spill.ensure(spill_bytes(tgrid))) return PSDR_ERR_NOMEM;
@@ -475,7 +506,7 @@ int render_forward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& cam
   // second ray set: the shade kernel reads the hits of the previous depth while
   // writing the next depth's rays; rays and hits are double-buffered by halves.
   int* counters = s->counters_r.as<int>();
-  hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, counters, 64);
+  hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(COUNTERS), 0, st, counters, COUNTERS);
   const SceneView sv = s->view();
   float* lane_L = s->lane_L.as<float>();
 
@@ -486,41 +517,24 @@ int render_forward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& cam
   int rc = trace_rays(sv, rays_a.o, rays_a.d, N, counters + 0, rays_a.tri, nullptr, rays_a.u, rays_a.v,
                       s->spill.as<int>(), tgrid, st);
   if (rc) return rc;
-  // ray buffers ping-pong: a depth's shade reads rays_in (hits) and writes rays_out
   if (s->rays2.ensure(24ull * ray_cap) || s->hits2.ensure(12ull * ray_cap)) return PSDR_ERR_NOMEM;
   RaysView rays_b = carve_rays(s->rays2, s->hits2, ray_cap);
-  QueueView* qin = &qa;
-  QueueView* qout = &qb;
-  RaysView* rin = &rays_a;
-  RaysView* rout = &rays_b;
-  const int sgrid = std::max(1, std::min(blocks(N, 256), num_cus() * 8));
-  for (int d = 0; d <= D; ++d) {
-    ShadeParams P;
-    P.sv = sv;
-    P.cam = cam;
-    P.depth = d;
-    P.D = D;
-    P.nL = nL;
-    P.nB = nB;
-    P.elv = cfg.enable_light_visable ? 1 : 0;
-    P.ad_jac = keep ? 1 : 0;
-    P.K_in = d == 0 ? 1 : static_cast<int>(nL + nB) + ((d - 1 < D - 1) ? 1 : 0);
-    P.K_out = d < D ? static_cast<int>(nL + nB) + ((d < D - 1) ? 1 : 0) : 0;
-    P.Kmax = Kmax;
-    P.keep = keep;
-    // counters: [d] = paths entering shade(d); [d+1] = survivors; [64 + d] = rays out
-    hipLaunchKernelGGL(k_shade, dim3(sgrid), dim3(256), 0, st, P, *qin, counters + d, *rin, *qout, counters + d + 1,
-                       *rout, cache, lane_L, static_cast<uint64_t*>(nullptr));
-    PSDR_HIP_CHECK(hipGetLastError());
-    if (d < D && P.K_out > 0) {
-      hipLaunchKernelGGL(k_ray_count, dim3(1), dim3(1), 0, st, counters + d + 1, P.K_out, counters + 64 + d);
-      rc = trace_rays(sv, rout->o, rout->d, static_cast<int>(ray_cap), counters + 64 + d, rout->tri, nullptr,
-                      rout->u, rout->v, s->spill.as<int>(), tgrid, st);
-      if (rc) return rc;
-    }
-    std::swap(qin, qout);
-    std::swap(rin, rout);
-  }
+  WaveArgs wa;
+  wa.cam = cam;
+  wa.D = D;
+  wa.nL = nL;
+  wa.nB = nB;
+  wa.elv = cfg.enable_light_visable ? 1 : 0;
+  wa.ad_jac = keep ? 1 : 0;
+  wa.keep = keep;
+  wa.cache = cache;
+  wa.lane_L = lane_L;
+  wa.lane_rng_out = nullptr;
+  wa.counters = counters;
+  const QueueView qs[2] = {qb, qa};
+  const RaysView rs[2] = {rays_b, rays_a};
+  rc = run_depths(s, wa, qa, rays_a, qs, rs, N, st);
+  if (rc) return rc;
   const int P_ = cam.H * cam.W;
   hipLaunchKernelGGL(k_film, dim3(blocks(P_, 256)), dim3(256), 0, st, lane_L, P_, cfg.spp_interior, cfg.n_pass, out);
   PSDR_HIP_CHECK(hipGetLastError());

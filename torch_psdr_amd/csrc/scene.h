@@ -109,6 +109,8 @@ struct psdr_scene {
   // renderer scratch
   psdr::DevBuf lane_L, q_a, q_b, rays, hits, rays2, hits2, spill, hit_cache, counters_r, hc_mass, hc_pmf,
       hc_cmf, stats;
+  // boundary estimators
+  psdr::DevBuf b_rec, b_rays, b_hits, b_q0, b_LA, b_LB, b_rngA, b_rngB, b_state, b_tmp, b_counters;
   int light_tri_total = 0;
   // last forward (for backward / stats)
   uint64_t last_N = 0;
@@ -124,6 +126,8 @@ namespace psdr {
 // scene.hip
 int scene_derive(psdr_scene* s, hipStream_t st);
 int build_lbvh(psdr_scene* s, hipStream_t st);
+// DiscreteSampler tables of n weights (single segment); tmp >= 2*(n/256+2) floats
+int sampler_tables(const float* w, int n, float* pmf, float* cmf, float* total, float* tmp, hipStream_t st);
 // camera (host, capi.cpp)
 CameraK make_camera(const psdr_camera& c);
 }  // namespace psdr

@@ -91,11 +91,11 @@ __global__ void k_edge_geom(const float* __restrict__ V, const int* __restrict__
 // One block per segment [seg_begin, seg_begin + seg_count) of w.  tmp holds two
 // floats per chunk per segment (at 2*seg_begin/SCAN_CHUNK + ...).
 __global__ void k_sampler(const float* __restrict__ w, const int* __restrict__ seg_begin,
-                          const int* __restrict__ seg_count, float* __restrict__ pmf, float* __restrict__ cmf,
-                          float* __restrict__ totals, float* __restrict__ tmp) {
+                          const int* __restrict__ seg_count, int n_static, float* __restrict__ pmf,
+                          float* __restrict__ cmf, float* __restrict__ totals, float* __restrict__ tmp) {
   const int sgi = blockIdx.x;
   const int b = seg_begin ? seg_begin[sgi] : 0;
-  const int n = seg_count[sgi];
+  const int n = seg_count ? seg_count[sgi] : n_static;
   const int nchunk = (n + SCAN_CHUNK - 1) / SCAN_CHUNK;
   float* csum = tmp + 2 * ((b + SCAN_CHUNK - 1) / SCAN_CHUNK + sgi);
   float* coff = csum + nchunk;
@@ -440,7 +440,14 @@ inline int blocks(long long n, int t) { return static_cast<int>((n + t - 1) / t)
 static int launch_sampler(const float* w, const int* seg_begin, const int* seg_count, int nseg, float* pmf,
                           float* cmf, float* totals, float* tmp, hipStream_t st) {
   if (nseg <= 0) return PSDR_OK;
-  hipLaunchKernelGGL(k_sampler, dim3(nseg), dim3(256), 0, st, w, seg_begin, seg_count, pmf, cmf, totals, tmp);
+  hipLaunchKernelGGL(k_sampler, dim3(nseg), dim3(256), 0, st, w, seg_begin, seg_count, 0, pmf, cmf, totals, tmp);
+  PSDR_HIP_CHECK(hipGetLastError());
+  return PSDR_OK;
+}
+
+int sampler_tables(const float* w, int n, float* pmf, float* cmf, float* total, float* tmp, hipStream_t st) {
+  if (n <= 0) return PSDR_OK;
+  hipLaunchKernelGGL(k_sampler, dim3(1), dim3(256), 0, st, w, nullptr, nullptr, n, pmf, cmf, total, tmp);
   PSDR_HIP_CHECK(hipGetLastError());
   return PSDR_OK;
 }

@@ -1,0 +1,62 @@
+// Wavefront queues shared by the forward render and the boundary estimators.
+#pragma once
+
+#include "scene.h"
+
+namespace psdr {
+
+// Path queue (SoA, capacity `cap`): one entry per live path.
+struct QueueView {
+  uint32_t* lane;
+  uint64_t* rng;  // PCG state at the first draw of the pending depth
+  float* T;       // [3][cap] throughput
+  float* L;       // [3][cap] radiance so far
+  int* tri;       // current vertex hit
+  float* u;
+  float* v;
+  float* O;  // [3][cap] origin of the ray that produced the current vertex
+  float* J0;
+  int cap;
+};
+constexpr size_t QUEUE_BYTES_PER = 8 + 4 * 5 + 12 * 3;
+
+// Ray queue (xyz interleaved origins/directions) and the traced hits.
+struct RaysView {
+  float* o;
+  float* d;
+  int* tri;
+  float* u;
+  float* v;
+};
+
+struct CacheView {
+  int* tri;  // [slots][N]
+  float* u;
+  float* v;
+  int64_t N;
+};
+
+QueueView carve_queue(DevBuf& b, int cap);
+RaysView carve_rays(DevBuf& rb, DevBuf& hb, int64_t cap);
+RaysView offset_rays(const RaysView& r, int64_t off);
+
+// trace2light (src/path_integrator.cc:80-186) over a queue of paths whose first
+// vertex is the hit of a ray from the camera position (rays `r0`, queue `q0`
+// with *count0 entries).  Writes radiance (times J0) per lane into lane_L and,
+// if lane_rng_out is set, the final PCG state per lane.  Uses q_s[2], r_s[2] as
+// ping-pong scratch (capacities >= n paths and n*Kmax rays).
+struct WaveArgs {
+  CameraK cam;
+  int D;
+  uint32_t nL, nB;
+  int elv, ad_jac, keep;
+  CacheView cache;
+  float* lane_L;
+  uint64_t* lane_rng_out;
+  int* counters;  // 128 ints, zeroed by the callee except counters[0] = *count0
+};
+int run_depths(psdr_scene* s, const WaveArgs& a, const QueueView& q0, const RaysView& r0, const QueueView* q_s,
+               const RaysView* r_s, int n, hipStream_t st);
+int zero_ints(int* p, int n, hipStream_t st);
+
+}  // namespace psdr

@@ -150,7 +150,7 @@ def render_backward(token: int, cam: List[float], cfg: List[int], grad: torch.Te
     c, k = make_config(cfg), make_camera(cam)
     g = grad.contiguous().to(torch.float32)
     N.check(ns.lib.psdr_render_backward(ns.handle, N.C.byref(c), N.C.byref(k), g.data_ptr(), gv.data_ptr(),
-                                        gt.data_ptr(), ge.data_ptr(), N.current_stream()))
+                                        gt.data_ptr(), ge.data_ptr(), N.PSDR_TERM_ALL, N.current_stream()))
     return [gv, gt, ge[: emit.shape[0]].clone()]
 
 

@@ -128,3 +128,34 @@ def perturbed_sphere(level: int, seed: int, sigma: float = 0.02) -> np.ndarray:
     rng = np.random.default_rng(seed)
     r = 1.0 + sigma * rng.standard_normal(V.shape[0])
     return (V.astype(np.float64) * r[:, None]).astype(np.float32)
+
+
+def c2_parts(dr, device, res=512, spp=16, spp_edge=4, level=5, tex_res=512):
+    """C2: perturbed level-5 icosphere (seed 3) with a 512^2 texture (seed 1),
+    target = unperturbed mesh with texture seed 2; gradients w.r.t. the sphere's
+    vertices and texels."""
+    V0, F = icosphere(level)
+    Vp = perturbed_sphere(level, seed=3)
+    uv = torch.tensor(sphere_uv(V0), device=device)
+    faces = torch.tensor(F, device=device)
+    Vg, Fg = quad(10.0, -1.0)
+    Vl, Fl = quad(2.0, 4.0, facing_down=True)
+    ground_v = torch.tensor(Vg, device=device)
+    ground_f = torch.tensor(Fg, device=device)
+    ground_c = torch.tensor([0.5] * 3, device=device)
+    light_v = torch.tensor(Vl, device=device)
+    light_f = torch.tensor(Fl, device=device)
+    emit = torch.tensor([10.0] * 3, device=device)
+    cam = camera(dr, device, res, res)
+    integ = dr.PathIntegrator(spp_interior=spp, max_bounce=3, spp_primary_edge=spp_edge,
+                              spp_secondary_edge=spp_edge)
+
+    def scene(vertices, tex):
+        sphere = dr.Mesh(vertices, uv, faces, faces, dr.DiffuseMaterial(tex))
+        ground = dr.Mesh(ground_v, None, ground_f, None, dr.DiffuseMaterial(ground_c))
+        light = dr.AreaLight(light_v, light_f, emit)
+        return dr.Scene([cam], [sphere, ground], [light])
+
+    return dict(V0=torch.tensor(V0, device=device), Vp=torch.tensor(Vp, device=device),
+                tex_opt=texture(1, tex_res, device), tex_target=texture(2, tex_res, device), scene=scene,
+                integrator=integ, res=res, spp=spp, spp_edge=spp_edge, n_tris=int(F.shape[0]) + 4)
