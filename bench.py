@@ -37,8 +37,19 @@ def main():
     ap.add_argument("--res", type=int, default=512)
     ap.add_argument("--spp", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample-res", type=int, default=64)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU-baseline sample time")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a 1 GiB device-to-device copy (scales rocprof FETCH/WRITE_SIZE)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        x = torch.ones(1 << 28, device="cuda:0")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": x.numel() * 4, "copy_bytes_written": y.numel() * 4, "launches": 3}))
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU sharding is design-only)", file=sys.stderr)
@@ -93,7 +104,7 @@ def main():
     # dominant kernel: closest-hit traversal, timed live on a batch of camera rays
     roof = trace_roofline(torch, N, ops, parts, integ)
 
-    cpu = None if args.no_cpu_baseline else cpu_baseline(parts, args.cpu_sample_res, spp)
+    cpu = None if args.no_cpu_baseline else cpu_baseline(parts, spp, args.cpu_seconds)
     out = {
         "metric": "fwd+bwd ms/iter & Mrays/s at 512^2 16spp vs plain GPU path, equal grad error",
         "value": round(mrays, 3),
@@ -153,10 +164,11 @@ def trace_roofline(torch, N, ops, parts, integ):
             "launch_ms": round(sec * 1e3, 4), "grays_per_s": round(n / sec / 1e9, 3)}
 
 
-def cpu_baseline(parts, sample_res, spp):
+def cpu_baseline(parts, spp, target_s):
     """The CPU oracle (a port of the reference algorithm) timed on this host on a
-    bounded sample: the same scene at sample_res^2 with the same spp, renderD
-    forward + forward-mode derivative (interior + both boundary terms)."""
+    bounded sample: the same scene at a reduced resolution with the same spp,
+    renderD forward + forward-mode derivative (interior + both boundary terms);
+    the resolution is chosen from a 64^2 trial so the sample takes ~target_s."""
     try:
         import numpy as np
         import torch
@@ -180,11 +192,18 @@ def cpu_baseline(parts, sample_res, spp):
         rng = np.random.default_rng(0)
         osc = O.Scene(flat, d_vertices=rng.normal(size=tuple(flat.vertices.shape)).astype(np.float32),
                       d_texels=rng.normal(size=flat.texels.numel()).astype(np.float32))
-        cam = [0.0, 1.0, 5.0, 0, 0, 0, 0, 1, 0, 3.14159265 / 4, sample_res, sample_res]
         cfg = [1, spp, 0, 4, 4, 3, 1, 1, 1, 1, 10000, 10000, 6, 6]
-        t = time.perf_counter()
-        O.render(osc, cfg, cam, mode=1, boundary=True)
-        sec = time.perf_counter() - t
+
+        def run(r):
+            cam = [0.0, 1.0, 5.0, 0, 0, 0, 0, 1, 0, 3.14159265 / 4, r, r]
+            t = time.perf_counter()
+            O.render(osc, cfg, cam, mode=1, boundary=True)
+            return time.perf_counter() - t
+
+        t64 = run(64)
+        sample_res = int(min(512, max(64, 64 * (target_s / max(t64, 1e-3)) ** 0.5)))
+        sample_res -= sample_res % 8
+        sec = run(sample_res)
         n_int = spp * sample_res * sample_res
         n_e = 4 * sample_res * sample_res
         rays = 10 * n_int + 23 * n_e

@@ -857,7 +857,9 @@ Camera make_camera(const oracle_camera* c) {
 }
 
 // ---------------------------------------------------------------------------
-// LBVH restatement: 30-bit Morton codes of triangle box centres, stable sort,
+// LBVH restatement: 30-bit Morton codes of triangle box centres with bit 30 set
+// for triangles whose largest padded extent is <= 1/8 of the centre-bounds extent,
+// stable sort,
 // Karras (2012) hierarchy over keys (morton << 32 | sorted position).
 // ---------------------------------------------------------------------------
 inline uint32_t expand_bits(uint32_t v) {
@@ -1059,7 +1061,14 @@ int oracle_lbvh(const float* tri9, int n, uint32_t* morton, int* order, int* chi
       const float x = ext > 0.f ? (cen[3 * f + a] - cmin[a]) / ext : 0.5f;
       q[a] = static_cast<uint32_t>(std::fmin(std::fmax(x * 1024.0f, 0.0f), 1023.0f));
     }
-    morton[f] = (expand_bits(q[0]) << 2) | (expand_bits(q[1]) << 1) | expand_bits(q[2]);
+    uint32_t m = (expand_bits(q[0]) << 2) | (expand_bits(q[1]) << 1) | expand_bits(q[2]);
+    float cmx = 0.f, emx = 0.f;
+    for (int a = 0; a < 3; ++a) {
+      emx = std::fmax(emx, cmax[a] - cmin[a]);
+      cmx = std::fmax(cmx, hi[3 * f + a] - lo[3 * f + a]);
+    }
+    if (!(cmx > 0.125f * emx)) m |= 1u << 30;  // small triangles sort after the large ones
+    morton[f] = m;
   }
   std::vector<int> ord(n);
   std::iota(ord.begin(), ord.end(), 0);

@@ -224,32 +224,45 @@ inline Pcg32 lane_rng(uint32_t lane) {
 
 // ---------------------------------------------------------------------------
 // Deterministic fp32 reductions: the project fixes the summation order of
-// enoki::hsum / enoki::psum as a chunked left-to-right scan (chunks of 256,
-// then a left-to-right scan of chunk totals).  The GPU restates the same order.
+// enoki::hsum / enoki::psum as three sequential levels: chunks of 256 elements
+// summed left to right; groups of 64 chunk totals scanned left to right; group
+// totals scanned left to right.  A chunk's offset is (group offset + offset of
+// the chunk inside its group); an element's prefix is (chunk offset + running
+// sum inside the chunk).  The GPU restates the same order.
 // ---------------------------------------------------------------------------
 constexpr int SCAN_CHUNK = 256;
+constexpr int SCAN_GROUP = 64;
 
 inline void chunked_inclusive_scan(const float* x, int n, float* out, float* total) {
   const int nchunk = (n + SCAN_CHUNK - 1) / SCAN_CHUNK;
-  std::vector<float> csum(nchunk, 0.f);
+  const int ngroup = (nchunk + SCAN_GROUP - 1) / SCAN_GROUP;
+  std::vector<float> csum(nchunk, 0.f), inner(nchunk, 0.f), gsum(ngroup, 0.f), goff(ngroup, 0.f);
   for (int c = 0; c < nchunk; ++c) {
     float s = 0.f;
     for (int i = c * SCAN_CHUNK; i < std::min(n, (c + 1) * SCAN_CHUNK); ++i) s = s + x[i];
     csum[c] = s;
   }
+  for (int g = 0; g < ngroup; ++g) {
+    float run = 0.f;
+    for (int c = g * SCAN_GROUP; c < std::min(nchunk, (g + 1) * SCAN_GROUP); ++c) {
+      inner[c] = run;
+      run = run + csum[c];
+    }
+    gsum[g] = run;
+  }
   float run = 0.f;
-  std::vector<float> coff(nchunk, 0.f);
-  for (int c = 0; c < nchunk; ++c) {
-    coff[c] = run;
-    run = run + csum[c];
+  for (int g = 0; g < ngroup; ++g) {
+    goff[g] = run;
+    run = run + gsum[g];
   }
   if (total) *total = run;
   if (out) {
     for (int c = 0; c < nchunk; ++c) {
+      const float off = goff[c / SCAN_GROUP] + inner[c];
       float s = 0.f;
       for (int i = c * SCAN_CHUNK; i < std::min(n, (c + 1) * SCAN_CHUNK); ++i) {
         s = s + x[i];
-        out[i] = coff[c] + s;
+        out[i] = off + s;
       }
     }
   }

@@ -322,19 +322,47 @@ __global__ __launch_bounds__(256) void k_shade(ShadeParams P, QueueView qi, cons
   }
 }
 
-// Film::add + n_pass averaging (src/camera.h:75-87, src/path_integrator.cc:385-400)
-__global__ void k_film(const float* __restrict__ lane_L, int P_, uint32_t spp, uint32_t n_pass,
-                       float* __restrict__ out) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= P_) return;
-  for (int c = 0; c < 3; ++c) {
-    float s = 0.f;
-    for (uint32_t k = 0; k < spp; ++k) s = s + lane_L[3 * (static_cast<int64_t>(p) * spp + k) + c];
-    if (spp > 1) s = s / static_cast<float>(spp);
-    float acc = 0.f;
-    for (uint32_t n = 0; n < n_pass; ++n) acc = acc + s;
-    if (n_pass > 1) acc = acc / static_cast<float>(n_pass);
-    out[3 * p + c] = acc;
+// Film::add + n_pass averaging (src/camera.h:75-87, src/path_integrator.cc:385-400).
+// A block stages a tile of whole pixels' lane radiance through LDS (coalesced
+// loads), then each (pixel, channel) thread sums its spp lanes in lane order.
+constexpr int FILM_LANES = 1024;
+__global__ __launch_bounds__(256) void k_film(const float* __restrict__ lane_L, int P_, uint32_t spp,
+                                              uint32_t n_pass, float* __restrict__ out) {
+  __shared__ float tile[3 * FILM_LANES];
+  const int ppb = spp <= FILM_LANES ? FILM_LANES / static_cast<int>(spp) : 1;  // pixels per block
+  const int p0 = blockIdx.x * ppb;
+  const int np = min(ppb, P_ - p0);
+  if (np <= 0) return;
+  const int tid = threadIdx.x;
+  const int pc = tid / 3, c = tid % 3;  // this thread's (pixel, channel) within the tile
+  float s = 0.f;
+  for (int64_t l0 = 0; l0 < static_cast<int64_t>(np) * spp; l0 += FILM_LANES) {
+    const int64_t base = static_cast<int64_t>(p0) * spp + l0;
+    const int nl = static_cast<int>(min(static_cast<int64_t>(FILM_LANES), static_cast<int64_t>(np) * spp - l0));
+    __syncthreads();
+    for (int i = tid; i < 3 * nl; i += blockDim.x) tile[i] = lane_L[3 * base + i];
+    __syncthreads();
+    if (spp <= FILM_LANES) {
+      for (int q = pc; q < np; q += blockDim.x / 3) {
+        if (tid >= 3 * (blockDim.x / 3)) break;
+        float acc = 0.f;
+        for (uint32_t k = 0; k < spp; ++k) acc = acc + tile[3 * (q * static_cast<int>(spp) + static_cast<int>(k)) + c];
+        if (spp > 1) acc = acc / static_cast<float>(spp);
+        float fin = 0.f;
+        for (uint32_t n = 0; n < n_pass; ++n) fin = fin + acc;
+        if (n_pass > 1) fin = fin / static_cast<float>(n_pass);
+        out[3 * (p0 + q) + c] = fin;
+      }
+    } else if (tid < 3) {
+      for (int k = 0; k < nl; ++k) s = s + tile[3 * k + c];
+    }
+  }
+  if (spp > FILM_LANES && tid < 3) {
+    float acc = s / static_cast<float>(spp);
+    float fin = 0.f;
+    for (uint32_t n = 0; n < n_pass; ++n) fin = fin + acc;
+    if (n_pass > 1) fin = fin / static_cast<float>(n_pass);
+    out[3 * p0 + c] = fin;
   }
 }
 
@@ -536,7 +564,9 @@ int render_forward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& cam
   rc = run_depths(s, wa, qa, rays_a, qs, rs, N, st);
   if (rc) return rc;
   const int P_ = cam.H * cam.W;
-  hipLaunchKernelGGL(k_film, dim3(blocks(P_, 256)), dim3(256), 0, st, lane_L, P_, cfg.spp_interior, cfg.n_pass, out);
+  const int ppb = cfg.spp_interior <= static_cast<uint32_t>(FILM_LANES) ? FILM_LANES / static_cast<int>(cfg.spp_interior)
+                                                                        : 1;
+  hipLaunchKernelGGL(k_film, dim3(blocks(P_, ppb)), dim3(256), 0, st, lane_L, P_, cfg.spp_interior, cfg.n_pass, out);
   PSDR_HIP_CHECK(hipGetLastError());
   s->last_N = N;
   s->last_K = Kmax;

@@ -3,11 +3,12 @@
 //  * k_tri_prep       : per-triangle record (src/mesh.cc:5-67).
 //  * k_edge_geom      : per-edge geometry (src/mesh.cc:97-204) over a canonical
 //                       (i<j)-sorted edge list built once at scene creation.
-//  * k_sampler        : DiscreteSampler tables (src/sampler.h:86-96) with the
-//                       project's fixed fp32 order (chunks of 256 summed left to
-//                       right, then chunk totals left to right).
+//  * k_sampler / k_chunk_* : DiscreteSampler tables (src/sampler.h:86-96) with
+//                       the project's fixed fp32 order (chunks of 256, groups of
+//                       64 chunk totals, group totals; each level left to right).
 //  * LBVH (replaces the OptiX GAS, src/scene_optix.cc:187-238): 30-bit Morton
-//    codes of padded triangle-box centres, a stable LSD radix sort (LDS digit
+//    codes of padded triangle-box centres (bit 30 set for triangles whose largest
+//    padded extent is at most 1/8 of the centre-bounds extent), a stable LSD radix sort (LDS digit
 //    histograms + wave-ballot multisplit ranking), the Karras (2012) hierarchy
 //    over keys (morton << 32 | sorted position), and an atomic bottom-up refit.
 //    The integer output is deterministic and matches the CPU restatement exactly.
@@ -109,7 +110,11 @@ __global__ void k_sampler(const float* __restrict__ w, const int* __restrict__ s
   __syncthreads();
   if (threadIdx.x == 0) {
     float run = 0.f;
-    for (int c = 0; c < nchunk; ++c) run = run + csum[c];
+    for (int g0 = 0; g0 < nchunk; g0 += 64) {
+      float gr = 0.f;
+      for (int c = g0; c < min(nchunk, g0 + 64); ++c) gr = gr + csum[c];
+      run = run + gr;
+    }
     s_total = run;
     if (totals) totals[sgi] = run;
   }
@@ -128,9 +133,13 @@ __global__ void k_sampler(const float* __restrict__ w, const int* __restrict__ s
   __syncthreads();
   if (threadIdx.x == 0) {
     float run = 0.f;
-    for (int c = 0; c < nchunk; ++c) {
-      coff[c] = run;
-      run = run + csum[c];
+    for (int g0 = 0; g0 < nchunk; g0 += 64) {
+      float gr = 0.f;
+      for (int c = g0; c < min(nchunk, g0 + 64); ++c) {
+        coff[c] = run + gr;
+        gr = gr + csum[c];
+      }
+      run = run + gr;
     }
   }
   __syncthreads();
@@ -143,6 +152,86 @@ __global__ void k_sampler(const float* __restrict__ w, const int* __restrict__ s
       cmf[b + i] = off + s;
     }
   }
+}
+
+// Three-level fixed order for long segments: chunks of 256 (sequential), groups
+// of 64 chunk totals (sequential), group totals (sequential).  Chunks are staged
+// through LDS with coalesced loads; one thread per chunk adds in order.
+constexpr int SCAN_GROUP = 64;
+constexpr int CHUNKS_PER_BLOCK = 4;
+// csum[c] = sum of chunk c of (w / *div if div else w); writes pmf when div
+__global__ __launch_bounds__(256) void k_chunk_sums(const float* __restrict__ w, const float* __restrict__ div,
+                                                     int n, float* __restrict__ csum, float* __restrict__ pmf_out) {
+  __shared__ float tile[CHUNKS_PER_BLOCK * SCAN_CHUNK];
+  const int c0 = blockIdx.x * CHUNKS_PER_BLOCK;
+  const int base = c0 * SCAN_CHUNK;
+  const int m = min(n - base, CHUNKS_PER_BLOCK * SCAN_CHUNK);
+  if (m <= 0) return;
+  const float total = div ? *div : 1.f;
+  for (int i = threadIdx.x; i < m; i += blockDim.x) {
+    float v = w[base + i];
+    if (div) {
+      v = v / total;
+      pmf_out[base + i] = v;
+    }
+    tile[i] = v;
+  }
+  __syncthreads();
+  const int c = threadIdx.x;
+  if (c < CHUNKS_PER_BLOCK && c * SCAN_CHUNK < m) {
+    const int e = min(m, (c + 1) * SCAN_CHUNK);
+    float s = 0.f;
+    for (int i = c * SCAN_CHUNK; i < e; ++i) s = s + tile[i];
+    csum[c0 + c] = s;
+  }
+}
+// per group: exclusive scan of its chunk totals in place, group total -> gsum
+__global__ void k_group_scan(float* __restrict__ csum, int nchunk, float* __restrict__ gsum) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ngroup = (nchunk + SCAN_GROUP - 1) / SCAN_GROUP;
+  if (g >= ngroup) return;
+  float run = 0.f;
+  const int e = min(nchunk, (g + 1) * SCAN_GROUP);
+  for (int c = g * SCAN_GROUP; c < e; ++c) {
+    const float x = csum[c];
+    csum[c] = run;
+    run = run + x;
+  }
+  gsum[g] = run;
+}
+// exclusive scan of group totals in place; total -> *total
+__global__ void k_top_scan(float* __restrict__ gsum, int ngroup, float* __restrict__ total) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  float run = 0.f;
+  for (int g = 0; g < ngroup; ++g) {
+    const float x = gsum[g];
+    gsum[g] = run;
+    run = run + x;
+  }
+  *total = run;
+}
+__global__ __launch_bounds__(256) void k_chunk_cmf(const float* __restrict__ pmf, int n,
+                                                    const float* __restrict__ inner, const float* __restrict__ goff,
+                                                    float* __restrict__ cmf) {
+  __shared__ float tile[CHUNKS_PER_BLOCK * SCAN_CHUNK];
+  const int c0 = blockIdx.x * CHUNKS_PER_BLOCK;
+  const int base = c0 * SCAN_CHUNK;
+  const int m = min(n - base, CHUNKS_PER_BLOCK * SCAN_CHUNK);
+  if (m <= 0) return;
+  for (int i = threadIdx.x; i < m; i += blockDim.x) tile[i] = pmf[base + i];
+  __syncthreads();
+  const int c = threadIdx.x;
+  if (c < CHUNKS_PER_BLOCK && c * SCAN_CHUNK < m) {
+    const float off = goff[(c0 + c) / SCAN_GROUP] + inner[c0 + c];
+    const int e = min(m, (c + 1) * SCAN_CHUNK);
+    float s = 0.f;
+    for (int i = c * SCAN_CHUNK; i < e; ++i) {
+      s = s + tile[i];
+      tile[i] = off + s;
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < m; i += blockDim.x) cmf[base + i] = tile[i];
 }
 
 // light-triangle areas gathered contiguously per light, then per-light sampler.
@@ -255,7 +344,18 @@ __global__ void k_morton(const float4* __restrict__ geom, int n, const float* __
     const float x = ext > 0.f ? (c[k] - bounds[k]) / ext : 0.5f;
     q[k] = static_cast<uint32_t>(fminf(fmaxf(x * 1024.0f, 0.0f), 1023.0f));
   }
-  const uint32_t m = (expand_bits(q[0]) << 2) | (expand_bits(q[1]) << 1) | expand_bits(q[2]);
+  uint32_t m = (expand_bits(q[0]) << 2) | (expand_bits(q[1]) << 1) | expand_bits(q[2]);
+  // size partition: small triangles get bit 30, so the root separates the few
+  // large ones (ground planes, lights) from the rest instead of letting their
+  // boxes inflate every ancestor of a deep leaf
+  float lo[3], hi[3], cmax = 0.f, emax = 0.f;
+  tri_box(geom, f, lo, hi);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    emax = fmaxf(emax, bounds[3 + k] - bounds[k]);
+    cmax = fmaxf(cmax, hi[k] - lo[k]);
+  }
+  if (!(cmax > 0.125f * emax)) m |= 1u << 30;
   morton[f] = m;
   keys[f] = m;
   vals[f] = static_cast<uint32_t>(f);
@@ -447,8 +547,28 @@ static int launch_sampler(const float* w, const int* seg_begin, const int* seg_c
 
 int sampler_tables(const float* w, int n, float* pmf, float* cmf, float* total, float* tmp, hipStream_t st) {
   if (n <= 0) return PSDR_OK;
-  hipLaunchKernelGGL(k_sampler, dim3(1), dim3(256), 0, st, w, nullptr, nullptr, n, pmf, cmf, total, tmp);
+  const int nchunk = (n + SCAN_CHUNK - 1) / SCAN_CHUNK;
+  if (nchunk <= 8) {
+    hipLaunchKernelGGL(k_sampler, dim3(1), dim3(256), 0, st, w, nullptr, nullptr, n, pmf, cmf, total, tmp);
+    PSDR_HIP_CHECK(hipGetLastError());
+    return PSDR_OK;
+  }
+  // tmp: [inner nchunk][gsum ngroup][tot 2]
+  const int ngroup = (nchunk + SCAN_GROUP - 1) / SCAN_GROUP;
+  float* inner = tmp;
+  float* gsum = tmp + nchunk;
+  float* tot = gsum + ngroup;
+  const int nb = blocks(nchunk, CHUNKS_PER_BLOCK);
+  const int ng = blocks(ngroup, 64);
+  hipLaunchKernelGGL(k_chunk_sums, dim3(nb), dim3(256), 0, st, w, nullptr, n, inner, nullptr);
+  hipLaunchKernelGGL(k_group_scan, dim3(ng), dim3(64), 0, st, inner, nchunk, gsum);
+  hipLaunchKernelGGL(k_top_scan, dim3(1), dim3(64), 0, st, gsum, ngroup, tot);
+  hipLaunchKernelGGL(k_chunk_sums, dim3(nb), dim3(256), 0, st, w, tot, n, inner, pmf);
+  hipLaunchKernelGGL(k_group_scan, dim3(ng), dim3(64), 0, st, inner, nchunk, gsum);
+  hipLaunchKernelGGL(k_top_scan, dim3(1), dim3(64), 0, st, gsum, ngroup, tot + 1);
+  hipLaunchKernelGGL(k_chunk_cmf, dim3(nb), dim3(256), 0, st, pmf, n, inner, gsum, cmf);
   PSDR_HIP_CHECK(hipGetLastError());
+  if (total) PSDR_HIP_CHECK(hipMemcpyAsync(total, tot, 4, hipMemcpyDeviceToDevice, st));
   return PSDR_OK;
 }
 
@@ -492,9 +612,7 @@ int scene_derive(psdr_scene* s, hipStream_t st) {
     hipLaunchKernelGGL(k_edge_geom, dim3(blocks(ne, 256)), dim3(256), 0, st, s->vertices, s->edge_topo.as<int>(),
                        ne, s->edge_geom.as<float>(), wbuf);
     PSDR_HIP_CHECK(hipGetLastError());
-    int* cnt = segi + 2 * nl + 1;  // holds ne
-    int rc = launch_sampler(wbuf, nullptr, cnt, 1, s->edge_pmf.as<float>(), s->edge_cmf.as<float>(), nullptr, tmp,
-                            st);
+    int rc = sampler_tables(wbuf, ne, s->edge_pmf.as<float>(), s->edge_cmf.as<float>(), nullptr, tmp, st);
     if (rc) return rc;
   }
   return build_lbvh(s, st);

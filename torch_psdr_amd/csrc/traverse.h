@@ -26,8 +26,8 @@ struct HitRec {
   float t, u, v;
 };
 
-__device__ __forceinline__ float slab_entry(float4 lo_hi_a, float4 lo_hi_b, float4 lo_hi_c, int which, f3 o, f3 inv,
-                                            float tbest) {
+__device__ __forceinline__ float slab_entry(float4 lo_hi_a, float4 lo_hi_b, float4 lo_hi_c, int which, f3 oi,
+                                            f3 inv, float tbest) {
   float lx, ly, lz, hx, hy, hz;
   if (which == 0) {
     lx = lo_hi_a.x; ly = lo_hi_a.y; lz = lo_hi_a.z;
@@ -36,12 +36,14 @@ __device__ __forceinline__ float slab_entry(float4 lo_hi_a, float4 lo_hi_b, floa
     lx = lo_hi_b.z; ly = lo_hi_b.w; lz = lo_hi_c.x;
     hx = lo_hi_c.y; hy = lo_hi_c.z; hz = lo_hi_c.w;
   }
-  const float t0x = (lx - o.x) * inv.x, t1x = (hx - o.x) * inv.x;
-  const float t0y = (ly - o.y) * inv.y, t1y = (hy - o.y) * inv.y;
-  const float t0z = (lz - o.z) * inv.z, t1z = (hz - o.z) * inv.z;
+  // t = (b - o) * inv as fma(b, inv, -o * inv); boxes are padded and the test is
+  // widened, so this need not round like the exact form
+  const float t0x = __builtin_fmaf(lx, inv.x, oi.x), t1x = __builtin_fmaf(hx, inv.x, oi.x);
+  const float t0y = __builtin_fmaf(ly, inv.y, oi.y), t1y = __builtin_fmaf(hy, inv.y, oi.y);
+  const float t0z = __builtin_fmaf(lz, inv.z, oi.z), t1z = __builtin_fmaf(hz, inv.z, oi.z);
   const float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), 0.f));
   const float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
-  const bool hit = tn <= tf * 1.0000004f && tn <= tbest * 1.0000004f;
+  const bool hit = tn <= tf * 1.000001f && tn <= tbest * 1.000001f;
   return hit ? tn : F_INF;
 }
 
@@ -73,6 +75,7 @@ __device__ __forceinline__ HitRec closest_hit(const float4* __restrict__ node, c
     return h;
   }
   const f3 inv = f3{1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+  const f3 oi = f3{-(o.x * inv.x), -(o.y * inv.y), -(o.z * inv.z)};
   const int lane = threadIdx.x, nthr = blockDim.x;
   int sp = 0;
   int ni = 0;
@@ -81,8 +84,8 @@ __device__ __forceinline__ HitRec closest_hit(const float4* __restrict__ node, c
     if (static_cast<unsigned>(ni) >= static_cast<unsigned>(n_internal)) break;  // malformed tree guard
     const float4 a = node[4 * ni], b = node[4 * ni + 1], c = node[4 * ni + 2], e = node[4 * ni + 3];
     int cl = __float_as_int(e.x), cr = __float_as_int(e.y);
-    float tl = slab_entry(a, b, c, 0, o, inv, h.t);
-    float tr = slab_entry(a, b, c, 1, o, inv, h.t);
+    float tl = slab_entry(a, b, c, 0, oi, inv, h.t);
+    float tr = slab_entry(a, b, c, 1, oi, inv, h.t);
     if (tl != F_INF && cl < 0) {
       if ((cl & 0x7fffffff) < n_tris) leaf_test(leaf, cl & 0x7fffffff, o, d, h);
       tl = F_INF;
