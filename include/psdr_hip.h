@@ -197,6 +197,15 @@ int psdr_last_lane_radiance(const psdr_scene* scene, float* out, void* stream);
  * traced; counts needs max_bounce + 2 entries.  Synchronises the stream. */
 int psdr_last_render_stats(const psdr_scene* scene, int64_t* counts, int32_t n_counts, int64_t* rays_traced);
 
+/* Live timing of the closest-hit traversal kernel (no reference counterpart;
+ * used to measure it inside a timed region).  enable = 1 synchronises the
+ * device, clears the totals and starts recording a HIP event pair around every
+ * traversal launch on its stream (at most 4096 launches).  enable = 0 stops,
+ * synchronises and returns the summed launch durations in milliseconds, the
+ * number of launches and their algorithmic bytes (24 B ray read + 12 B, or 16 B
+ * with t, hit written per ray).  Process-wide; not for concurrent callers. */
+int psdr_trace_timing(int32_t enable, double* total_ms, int64_t* launches, int64_t* bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -111,6 +111,7 @@ _SIGS = {
     "psdr_scene_bvh": (C.c_int, [_VP, _VP, _VP, _VP, _VP, _VP, _VP]),
     "psdr_last_lane_radiance": (C.c_int, [_VP, _VP, _VP]),
     "psdr_last_render_stats": (C.c_int, [_VP, C.POINTER(C.c_int64), _I32, C.POINTER(C.c_int64)]),
+    "psdr_trace_timing": (C.c_int, [_I32, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
 }
 
 EXPORTED = tuple(_SIGS)

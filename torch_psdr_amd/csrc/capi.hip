@@ -547,6 +547,10 @@ int psdr_scene_trace(psdr_scene* s, const float* ro, const float* rd, int32_t n,
   return trace_rays(s->view(), ro, rd, n, nullptr, tri, t, u, v, s->spill.as<int>(), grid, as_stream(stream));
 }
 
+int psdr_trace_timing(int32_t enable, double* total_ms, int64_t* launches, int64_t* bytes) {
+  return trace_timing(enable, total_ms, launches, bytes);
+}
+
 int psdr_scene_derived(const psdr_scene* s, float* tri7, int32_t* edge_topo, float* edge_geom, void* stream) {
   if (!s) {
     set_error(PSDR_ERR_ARG, "scene is NULL");

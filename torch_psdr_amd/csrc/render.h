@@ -7,6 +7,7 @@ namespace psdr {
 int trace_rays(const SceneView& sv, const float* ro, const float* rd, int n, const int* count_dev, int* tri,
                float* t, float* u, float* v, int* spill, int grid, hipStream_t st);
 int trace_grid(int64_t max_rays);
+int trace_timing(int enable, double* total_ms, int64_t* launches, int64_t* bytes);
 size_t spill_bytes(int grid);
 int lane_draws(uint32_t lane0, int n_lanes, int n_draws, uint32_t* out, hipStream_t st);
 int camera_rays(const CameraK& cam, uint32_t spp, float* o, float* d, int* pix, hipStream_t st);

@@ -19,6 +19,8 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
+#include <vector>
 
 #include "render.h"
 #include "shading.h"
@@ -60,9 +62,13 @@ __global__ __launch_bounds__(TRACE_THREADS) void k_trace(SceneView sv, const flo
                                                          const float* __restrict__ rd, int n,
                                                          const int* __restrict__ count, int* __restrict__ otri,
                                                          float* __restrict__ ot, float* __restrict__ ou,
-                                                         float* __restrict__ ov, int* __restrict__ spill) {
+                                                         float* __restrict__ ov, int* __restrict__ spill,
+                                                         unsigned long long* __restrict__ timed_bytes) {
   __shared__ int stack[STACK_LDS * TRACE_THREADS];
   const int cnt = count ? *count : n;
+  // algorithmic bytes of this launch: ray read (24 B) + tri, u, v (+ t) written
+  if (timed_bytes && blockIdx.x == 0 && threadIdx.x == 0)
+    atomicAdd(timed_bytes, static_cast<unsigned long long>(cnt) * (ot ? 40ull : 36ull));
   const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
   const int stride = gridDim.x * blockDim.x;
   for (int i = gtid; i < cnt; i += stride) {
@@ -432,10 +438,78 @@ int zero_ints(int* p, int n, hipStream_t st) {
   return PSDR_OK;
 }
 
+namespace {
+// Optional live timing of every k_trace launch (HIP events on the launch
+// stream) plus the launches' algorithmic bytes counted on the device.
+struct TraceTimer {
+  std::mutex mu;
+  bool on = false;
+  std::vector<hipEvent_t> ev;  // (start, end) pairs
+  size_t used = 0;             // pairs recorded since the last start
+  int64_t dropped = 0;         // launches beyond the event pool
+  unsigned long long* bytes = nullptr;
+};
+TraceTimer& trace_timer() {
+  static TraceTimer t;
+  return t;
+}
+constexpr size_t TIMER_PAIRS = 4096;
+}  // namespace
+
 int trace_rays(const SceneView& sv, const float* ro, const float* rd, int n, const int* count_dev, int* tri,
                float* t, float* u, float* v, int* spill, int grid, hipStream_t st) {
-  hipLaunchKernelGGL(k_trace, dim3(grid), dim3(TRACE_THREADS), 0, st, sv, ro, rd, n, count_dev, tri, t, u, v, spill);
+  TraceTimer& tt = trace_timer();
+  std::lock_guard<std::mutex> lock(tt.mu);
+  const bool timed = tt.on && tt.used < TIMER_PAIRS;
+  if (tt.on && !timed) ++tt.dropped;
+  if (timed) PSDR_HIP_CHECK(hipEventRecord(tt.ev[2 * tt.used], st));
+  hipLaunchKernelGGL(k_trace, dim3(grid), dim3(TRACE_THREADS), 0, st, sv, ro, rd, n, count_dev, tri, t, u, v, spill,
+                     timed ? tt.bytes : nullptr);
   PSDR_HIP_CHECK(hipGetLastError());
+  if (timed) {
+    PSDR_HIP_CHECK(hipEventRecord(tt.ev[2 * tt.used + 1], st));
+    ++tt.used;
+  }
+  return PSDR_OK;
+}
+
+int trace_timing(int enable, double* total_ms, int64_t* launches, int64_t* bytes) {
+  TraceTimer& tt = trace_timer();
+  std::lock_guard<std::mutex> lock(tt.mu);
+  if (enable) {
+    if (tt.ev.empty()) {
+      tt.ev.resize(2 * TIMER_PAIRS);
+      for (auto& e : tt.ev) PSDR_HIP_CHECK(hipEventCreate(&e));
+      PSDR_HIP_CHECK(hipMalloc(&tt.bytes, sizeof(unsigned long long)));
+    }
+    PSDR_HIP_CHECK(hipDeviceSynchronize());
+    PSDR_HIP_CHECK(hipMemset(tt.bytes, 0, sizeof(unsigned long long)));
+    tt.used = 0;
+    tt.dropped = 0;
+    tt.on = true;
+    return PSDR_OK;
+  }
+  if (!tt.on) {
+    set_error(PSDR_ERR_STATE, "trace_timing: timing was not started");
+    return PSDR_ERR_STATE;
+  }
+  tt.on = false;
+  PSDR_HIP_CHECK(hipDeviceSynchronize());
+  double ms = 0.0;
+  for (size_t i = 0; i < tt.used; ++i) {
+    float e = 0.f;
+    PSDR_HIP_CHECK(hipEventElapsedTime(&e, tt.ev[2 * i], tt.ev[2 * i + 1]));
+    ms += e;
+  }
+  unsigned long long b = 0;
+  PSDR_HIP_CHECK(hipMemcpy(&b, tt.bytes, sizeof(b), hipMemcpyDeviceToHost));
+  if (tt.dropped) {
+    set_error(PSDR_ERR_STATE, "trace_timing: more launches than the event pool holds");
+    return PSDR_ERR_STATE;
+  }
+  if (total_ms) *total_ms = ms;
+  if (launches) *launches = static_cast<int64_t>(tt.used);
+  if (bytes) *bytes = static_cast<int64_t>(b);
   return PSDR_OK;
 }
 
