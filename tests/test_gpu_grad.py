@@ -114,3 +114,18 @@ def test_renderD_autograd_end_to_end(gpu_device, oracle):
     # renderC and renderD agree in value
     img_c = integ.renderC(scene)[0]
     np.testing.assert_allclose(img.detach().cpu().numpy(), img_c.cpu().numpy(), rtol=0, atol=0)
+
+
+def test_all_terms_equal_sum_of_terms(gpu_device):
+    """PSDR_TERM_ALL runs the three terms side by side on forked streams; its
+    gradients must equal the sum of the terms computed one call each."""
+    scene, integ = _setup(gpu_device, dict(level=2, tex_res=8), 32,
+                          dict(spp_interior=2, max_bounce=2, spp_primary_edge=2, spp_secondary_edge=2))
+    g = torch.randn((32, 32, 3), generator=torch.Generator().manual_seed(5)).to(gpu_device)
+    parts = [_gpu_vjp(scene, integ, g, t) for t in (N.PSDR_TERM_INTERIOR, N.PSDR_TERM_PRIMARY, N.PSDR_TERM_DIRECT)]
+    for rep in range(2):
+        tot = _gpu_vjp(scene, integ, g, N.PSDR_TERM_ALL)
+        for b in range(3):
+            ref = sum(p[b] for p in parts)
+            scale = max(np.abs(ref).max(), 1e-12)
+            assert np.abs(tot[b] - ref).max() <= 1e-4 * scale, (rep, b, np.abs(tot[b] - ref).max(), scale)

@@ -524,13 +524,52 @@ inline int blocks(int64_t n, int t) { return static_cast<int>((n + t - 1) / t); 
 
 }  // namespace
 
+// The interior, primary-boundary and direct-boundary terms share only their
+// (atomically accumulated) gradient outputs and read-only scene data, so they
+// run side by side: the boundary terms on two auxiliary streams forked from
+// and joined back into the caller's stream.  Every scratch buffer a term uses
+// is its own; the gradient sink for an absent vertex gradient is sized before
+// the fork.
 int render_backward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& camd, const float* dL_dI,
                     float* grad_v, float* grad_t, float* grad_e, int terms, hipStream_t st) {
-  if (terms & PSDR_TERM_INTERIOR) {
-    const int rc = interior_backward(s, cfg, camd, dL_dI, grad_v, grad_t, grad_e, st);
-    if (rc) return rc;
+  const bool ti = (terms & PSDR_TERM_INTERIOR) != 0;
+  const bool tp = (terms & PSDR_TERM_PRIMARY) != 0 && s->n_edges > 0;
+  const bool td = (terms & PSDR_TERM_DIRECT) != 0 && s->n_edges > 0;
+  if (s->stats.ensure(4ull * (3 * static_cast<size_t>(s->n_vertices) + s->n_texels + 3 * s->n_lights + 64)))
+    return PSDR_ERR_NOMEM;
+  float* gv = grad_v ? grad_v : s->stats.as<float>();
+  if (static_cast<int>(ti) + static_cast<int>(tp) + static_cast<int>(td) <= 1) {
+    if (ti) return interior_backward(s, cfg, camd, dL_dI, grad_v, grad_t, grad_e, st);
+    if (tp) return boundary_primary(s, cfg, camd, dL_dI, gv, st);
+    if (td) return boundary_direct(s, cfg, camd, dL_dI, gv, st);
+    return PSDR_OK;
   }
-  if (terms & (PSDR_TERM_PRIMARY | PSDR_TERM_DIRECT)) return boundary_backward(s, cfg, camd, dL_dI, grad_v, terms, st);
+  int rc = s->aux.ensure();
+  if (rc) return rc;
+  PSDR_HIP_CHECK(hipEventRecord(s->aux.fork, st));
+  hipStream_t sp = s->aux.st[0], sd = s->aux.st[1];
+  PSDR_HIP_CHECK(hipStreamWaitEvent(sp, s->aux.fork, 0));
+  PSDR_HIP_CHECK(hipStreamWaitEvent(sd, s->aux.fork, 0));
+  // launch order by host enqueue cost: the single interior launch first, so the
+  // device is busy while the host enqueues the boundary chains (the primary
+  // chain, with the most launches, last)
+  int rci = ti ? interior_backward(s, cfg, camd, dL_dI, grad_v, grad_t, grad_e, st) : PSDR_OK;
+  int rcd = td ? boundary_direct(s, cfg, camd, dL_dI, gv, sd) : PSDR_OK;
+  int rcp = tp ? boundary_primary(s, cfg, camd, dL_dI, gv, sp) : PSDR_OK;
+  // join even after an error so no work is left running unordered
+  const hipError_t e0 = hipEventRecord(s->aux.join[0], sp);
+  const hipError_t e1 = hipEventRecord(s->aux.join[1], sd);
+  const hipError_t e2 = hipStreamWaitEvent(st, s->aux.join[0], 0);
+  const hipError_t e3 = hipStreamWaitEvent(st, s->aux.join[1], 0);
+  if (rci) return rci;
+  if (rcp) return rcp;
+  if (rcd) return rcd;
+  for (hipError_t e : {e0, e1, e2, e3}) {
+    if (e != hipSuccess) {
+      set_error(PSDR_ERR_HIP, std::string("backward stream join: ") + hipGetErrorString(e));
+      return PSDR_ERR_HIP;
+    }
+  }
   return PSDR_OK;
 }
 

@@ -2,7 +2,11 @@
 //
 // Primary (silhouette) edges, per lane of Sampler(spp_primary_edge*H*W):
 //   k_prim_setup   1 draw, hypercube cell pick (10k cells), edge point, camera
-//                  projection, rays at d -/+ 1e-5 * edge normal
+//                  projection; sort key = Morton code of the pixel
+//   sort           render pass only: lanes are processed in pixel-Morton order
+//                  (results stay per lane; only the order of the queue changes,
+//                  so neighbouring lanes of a wave trace neighbouring rays)
+//   k_prim_rays    rays at d -/+ 1e-5 * edge normal, queue in processing order
 //   trace          both rays
 //   k_prim_check   both hits valid and one of them on the edge point
 //   run_depths x2  trace2light from each hit (B = 1, no light visibility),
@@ -98,12 +102,20 @@ PrimRecs carve_prim(DevBuf& b, int cap) {
 constexpr size_t PRIM_BYTES_PER = 4 * 5 + 12;
 
 // PRE: preprocess round (cell = lane, state persists); else the render pass.
+__device__ __forceinline__ uint32_t spread_bits16(uint32_t x) {
+  x &= 0xffffu;
+  x = (x | (x << 8)) & 0x00ff00ffu;
+  x = (x | (x << 4)) & 0x0f0f0f0fu;
+  x = (x | (x << 2)) & 0x33333333u;
+  x = (x | (x << 1)) & 0x55555555u;
+  return x;
+}
+
 template <bool PRE>
 __global__ void k_prim_setup(SceneView sv, CameraK cam, int n, uint64_t* __restrict__ state, int res,
                              const float* __restrict__ cpmf, const float* __restrict__ ccmf, PrimRecs rec,
-                             RaysView rays, QueueView q0, int* __restrict__ count0) {
+                             uint64_t* __restrict__ rng0, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) *count0 = n;
   if (i >= n) return;
   Rng rng;
   float s, pdf_cube = 1.f;
@@ -118,8 +130,7 @@ __global__ void k_prim_setup(SceneView sv, CameraK cam, int n, uint64_t* __restr
     const int c = discrete_sample(cpmf, ccmf, res, &s, &pdf_cube);
     s = (s + static_cast<float>(c)) * unit;
   }
-  q0.lane[i] = static_cast<uint32_t>(i);
-  q0.rng[i] = rng.state;
+  rng0[i] = rng.state;
   // Scene::sample_edge_point (src/scene.cc:189-206)
   float u = s, pdf_e;
   const int e = discrete_sample(sv.edge_pmf, sv.edge_cmf, sv.n_edges, &u, &pdf_e);
@@ -129,13 +140,11 @@ __global__ void k_prim_setup(SceneView sv, CameraK cam, int n, uint64_t* __restr
   const bool valid = camera_project(cam, ep, &px, &py);
   const f3 d = normalize(ep - cam.pos);
   const f3 nn = cross(d, normalize(E.p1 - E.p0));
-  f3 da = d - nn * REAL_EPS, db = d + nn * REAL_EPS;
-  if (!valid) da = db = splat(__builtin_nanf(""));
-  rays.o[3 * i] = cam.pos.x; rays.o[3 * i + 1] = cam.pos.y; rays.o[3 * i + 2] = cam.pos.z;
-  rays.d[3 * i] = da.x; rays.d[3 * i + 1] = da.y; rays.d[3 * i + 2] = da.z;
-  const int j = n + i;
-  rays.o[3 * j] = cam.pos.x; rays.o[3 * j + 1] = cam.pos.y; rays.o[3 * j + 2] = cam.pos.z;
-  rays.d[3 * j] = db.x; rays.d[3 * j + 1] = db.y; rays.d[3 * j + 2] = db.z;
+  if (keys) {
+    keys[i] = valid ? (spread_bits16(static_cast<uint32_t>(px)) | (spread_bits16(static_cast<uint32_t>(py)) << 1))
+                    : 0xffffffffu;
+    vals[i] = static_cast<uint32_t>(i);
+  }
   rec.edge[i] = e;
   rec.t[i] = u;
   rec.pdf_ep[i] = pdf_e / E.len;
@@ -146,28 +155,54 @@ __global__ void k_prim_setup(SceneView sv, CameraK cam, int n, uint64_t* __restr
   rec.n[2 * rec.cap + i] = nn.z;
 }
 
-// both hits valid and one of them within 1e-4 of the edge point (src/path_integrator.cc:227-230)
-__global__ void k_prim_check(SceneView sv, CameraK cam, int n, PrimRecs rec, RaysView rays) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n || rec.pix[i] < 0) return;
+// Position p of the processing order handles lane order[p] (order = null: p):
+// its two rays go to p and n + p and its queue entry to p.
+__global__ void k_prim_rays(SceneView sv, CameraK cam, int n, const uint32_t* __restrict__ order, PrimRecs rec,
+                            const uint64_t* __restrict__ rng0, RaysView rays, QueueView q0,
+                            int* __restrict__ count0) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p == 0) *count0 = n;
+  if (p >= n) return;
+  const int i = order ? static_cast<int>(order[p]) : p;
+  q0.lane[p] = static_cast<uint32_t>(i);
+  q0.rng[p] = rng0[i];
+  // the same expressions as k_prim_setup, so the rays are bit-identical
   const EdgeRec E = load_edge(sv, rec.edge[i]);
   const f3 ep = E.p0 + rec.t[i] * (E.p1 - E.p0);
-  const Its a = reconstruct(sv, rays.tri[i], rays.u[i], rays.v[i], cam.pos, true);
-  const Its b = reconstruct(sv, rays.tri[n + i], rays.u[n + i], rays.v[n + i], cam.pos, true);
+  const f3 d = normalize(ep - cam.pos);
+  const f3 nn = f3{rec.n[i], rec.n[rec.cap + i], rec.n[2 * rec.cap + i]};
+  f3 da = d - nn * REAL_EPS, db = d + nn * REAL_EPS;
+  if (rec.pix[i] < 0) da = db = splat(__builtin_nanf(""));
+  rays.o[3 * p] = cam.pos.x; rays.o[3 * p + 1] = cam.pos.y; rays.o[3 * p + 2] = cam.pos.z;
+  rays.d[3 * p] = da.x; rays.d[3 * p + 1] = da.y; rays.d[3 * p + 2] = da.z;
+  const int j = n + p;
+  rays.o[3 * j] = cam.pos.x; rays.o[3 * j + 1] = cam.pos.y; rays.o[3 * j + 2] = cam.pos.z;
+  rays.d[3 * j] = db.x; rays.d[3 * j + 1] = db.y; rays.d[3 * j + 2] = db.z;
+}
+
+// both hits valid and one of them within 1e-4 of the edge point (src/path_integrator.cc:227-230)
+__global__ void k_prim_check(SceneView sv, CameraK cam, int n, QueueView q0, PrimRecs rec, RaysView rays) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int i = static_cast<int>(q0.lane[p]);
+  if (rec.pix[i] < 0) return;
+  const EdgeRec E = load_edge(sv, rec.edge[i]);
+  const f3 ep = E.p0 + rec.t[i] * (E.p1 - E.p0);
+  const Its a = reconstruct(sv, rays.tri[p], rays.u[p], rays.v[p], cam.pos, true);
+  const Its b = reconstruct(sv, rays.tri[n + p], rays.u[n + p], rays.v[n + p], cam.pos, true);
   const bool ok = a.valid && b.valid && (norm(a.p - ep) < 10.0f * REAL_EPS || norm(b.p - ep) < 10.0f * REAL_EPS);
   if (!ok) {
-    rays.tri[i] = -1;
-    rays.tri[n + i] = -1;
+    rays.tri[p] = -1;
+    rays.tri[n + p] = -1;
     rec.pix[i] = -1;
   }
 }
 
 __global__ void k_prim_requeue(int n, QueueView q0, const uint64_t* __restrict__ rng_a, int* __restrict__ count0) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) *count0 = n;
-  if (i >= n) return;
-  q0.lane[i] = static_cast<uint32_t>(i);
-  q0.rng[i] = rng_a[i];
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p == 0) *count0 = n;
+  if (p >= n) return;
+  q0.rng[p] = rng_a[q0.lane[p]];
 }
 
 template <bool PRE>
@@ -311,14 +346,16 @@ __global__ __launch_bounds__(BT) void k_direct(SceneView sv, CameraK cam, int n,
                                                int r0, int r1, int r2, const float* __restrict__ cpmf,
                                                const float* __restrict__ ccmf, float* __restrict__ mass,
                                                const float* __restrict__ dLdI, uint32_t spp,
-                                               float* __restrict__ grad_v, int* __restrict__ spill) {
+                                               float* __restrict__ grad_v, int* __restrict__ spill,
+                                               const uint32_t* __restrict__ order) {
   __shared__ int stk[BSTACK * BT];
   const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
   const int stride = gridDim.x * blockDim.x;
   const int cells = r0 * r1 * r2;
   const float u0 = 1.0f / static_cast<float>(r0), u1 = 1.0f / static_cast<float>(r1),
               u2 = 1.0f / static_cast<float>(r2);
-  for (int i = gtid; i < n; i += stride) {
+  for (int p = gtid; p < n; p += stride) {
+    const int i = order ? static_cast<int>(order[p]) : p;
     Rng rng;
     float s[3];
     float pdf_cube = 1.f;
@@ -355,6 +392,20 @@ __global__ __launch_bounds__(BT) void k_direct(SceneView sv, CameraK cam, int n,
       eval_direct<false>(sv, cam, s[0], s[1], s[2], stk, spill + gtid, stride, dLdI, gscale, grad_v);
     }
   }
+}
+
+// Sort key of a direct-boundary lane: its hypercube cell (edge position major),
+// drawn exactly as k_direct<false> draws it.
+__global__ void k_direct_keys(int n, int cells, const float* __restrict__ cpmf, const float* __restrict__ ccmf,
+                              uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  Rng rng = lane_rng(static_cast<uint32_t>(i));
+  rng.next();
+  rng.next();
+  float s2 = rng.next(), pdf;
+  keys[i] = static_cast<uint32_t>(discrete_sample(cpmf, ccmf, cells, &s2, &pdf));
+  vals[i] = static_cast<uint32_t>(i);
 }
 
 __global__ void k_fill_f32(float* p, int n, float v) {
@@ -403,18 +454,39 @@ static int primary_pass(psdr_scene* s, const psdr_config& cfg, const CameraK& ca
   int rc = zero_ints(cnt, 256, st);
   if (rc) return rc;
   uint64_t* state = s->b_state.as<uint64_t>();
+  // the lane's PCG state after the setup draw; pass B's output overwrites it later
+  uint64_t* rng0 = s->b_rngB.as<uint64_t>();
+  // render pass: process lanes in pixel-Morton order (the preprocess lanes are
+  // hypercube cells, already in edge order)
+  const uint32_t* order = nullptr;
+  uint32_t *k0 = nullptr, *v0 = nullptr;
+  if (!pre) {
+    if (s->b_keys.ensure(8ull * n) || s->b_vals.ensure(8ull * n) || s->b_hist.ensure(4 * radix_hist_words(n)))
+      return PSDR_ERR_NOMEM;
+    k0 = s->b_keys.as<uint32_t>();
+    v0 = s->b_vals.as<uint32_t>();
+  }
   if (pre)
     hipLaunchKernelGGL(k_prim_setup<true>, dim3(blocks(n, BT)), dim3(BT), 0, st, sv, cam, n, state, res, cpmf, ccmf,
-                       rec, r0, q0, cnt);
+                       rec, rng0, nullptr, nullptr);
   else
     hipLaunchKernelGGL(k_prim_setup<false>, dim3(blocks(n, BT)), dim3(BT), 0, st, sv, cam, n, state, res, cpmf, ccmf,
-                       rec, r0, q0, cnt);
+                       rec, rng0, k0, v0);
+  PSDR_HIP_CHECK(hipGetLastError());
+  if (!pre) {
+    // keys are 2 x 16-bit Morton codes; a 2 x 11-bit code needs 3 byte passes
+    const int passes = (cam.W <= 2048 && cam.H <= 2048) ? 3 : 4;
+    rc = radix_sort_pairs(k0, v0, k0 + n, v0 + n, s->b_hist.as<uint32_t>(), n, passes, st);
+    if (rc) return rc;
+    order = (passes & 1) ? v0 + n : v0;
+  }
+  hipLaunchKernelGGL(k_prim_rays, dim3(blocks(n, BT)), dim3(BT), 0, st, sv, cam, n, order, rec, rng0, r0, q0, cnt);
   PSDR_HIP_CHECK(hipGetLastError());
   const int tgrid = trace_grid(2ll * n);
   if (s->spill.ensure(std::max(s->spill.bytes, spill_bytes(tgrid)))) return PSDR_ERR_NOMEM;
-  rc = trace_rays(sv, r0.o, r0.d, 2 * n, nullptr, r0.tri, nullptr, r0.u, r0.v, s->spill.as<int>(), tgrid, st);
+  rc = trace_rays(sv, r0.o, r0.d, 2 * n, nullptr, 1, 0, r0.tri, nullptr, r0.u, r0.v, s->spill.as<int>(), tgrid, st);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_prim_check, dim3(blocks(n, BT)), dim3(BT), 0, st, sv, cam, n, rec, r0);
+  hipLaunchKernelGGL(k_prim_check, dim3(blocks(n, BT)), dim3(BT), 0, st, sv, cam, n, q0, rec, r0);
   WaveArgs wa;
   wa.cam = cam;
   wa.D = static_cast<int>(cfg.max_bounce);
@@ -448,73 +520,83 @@ static int primary_pass(psdr_scene* s, const psdr_config& cfg, const CameraK& ca
   return PSDR_OK;
 }
 
-int boundary_backward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& camd, const float* dL_dI,
-                      float* grad_v, int terms, hipStream_t st) {
+int boundary_primary(psdr_scene* s, const psdr_config& cfg, const psdr_camera& camd, const float* dL_dI,
+                     float* grad_v, hipStream_t st) {
+  if (s->n_edges == 0) return PSDR_OK;
+  const CameraK cam = make_camera(camd);
+  const int64_t HW = static_cast<int64_t>(cam.H) * cam.W;
+  const int R = static_cast<int>(cfg.primary_edge_hypercube_resolution);
+  const int64_t n64 = static_cast<int64_t>(cfg.spp_primary_edge) * HW;
+  if (n64 > 0x7fffffff / 8) {
+    set_error(PSDR_ERR_ARG, "primary boundary: too many samples");
+    return PSDR_ERR_ARG;
+  }
+  const int n = static_cast<int>(n64);
+  if (s->b_state.ensure(8ull * std::max(R, 1)) || s->hc_mass.ensure(4ull * R) || s->hc_pmf.ensure(4ull * R) ||
+      s->hc_cmf.ensure(4ull * R) || s->b_tmp.ensure(4ull * (2 * (R / 256 + 2) + 16)))
+    return PSDR_ERR_NOMEM;
+  hipLaunchKernelGGL(k_seed_states, dim3(blocks(R, BT)), dim3(BT), 0, st, s->b_state.as<uint64_t>(), R);
+  hipLaunchKernelGGL(k_fill_f32, dim3(blocks(R, BT)), dim3(BT), 0, st, s->hc_mass.as<float>(), R, 0.f);
+  for (uint32_t r = 0; r < cfg.primary_edge_preprocess_rounds; ++r) {
+    const int rc = primary_pass(s, cfg, cam, true, R, R, nullptr, nullptr, s->hc_mass.as<float>(), nullptr, nullptr,
+                                st);
+    if (rc) return rc;
+  }
+  int rc = sampler_tables(s->hc_mass.as<float>(), R, s->hc_pmf.as<float>(), s->hc_cmf.as<float>(), nullptr,
+                          s->b_tmp.as<float>(), st);
+  if (rc) return rc;
+  return primary_pass(s, cfg, cam, false, n, R, s->hc_pmf.as<float>(), s->hc_cmf.as<float>(), nullptr, dL_dI, grad_v,
+                      st);
+}
+
+int boundary_direct(psdr_scene* s, const psdr_config& cfg, const psdr_camera& camd, const float* dL_dI,
+                    float* grad_v, hipStream_t st) {
   if (s->n_edges == 0) return PSDR_OK;
   const CameraK cam = make_camera(camd);
   const SceneView sv = s->view();
   const int64_t HW = static_cast<int64_t>(cam.H) * cam.W;
-  if (!grad_v) {
-    if (s->stats.ensure(4ull * 3 * std::max(s->n_vertices, 1))) return PSDR_ERR_NOMEM;
-    grad_v = s->stats.as<float>();
+  const int r0 = static_cast<int>(cfg.secondary_edge_hypercube_resolution[0]);
+  const int r1 = static_cast<int>(cfg.secondary_edge_hypercube_resolution[1]);
+  const int r2 = static_cast<int>(cfg.secondary_edge_hypercube_resolution[2]);
+  const int64_t cells64 = static_cast<int64_t>(r0) * r1 * r2;
+  const int64_t n64 = static_cast<int64_t>(cfg.spp_secondary_edge) * HW;
+  if (cells64 > 0x7fffffff / 8 || n64 > 0x7fffffff / 8) {
+    set_error(PSDR_ERR_ARG, "direct boundary: too many cells or samples");
+    return PSDR_ERR_ARG;
   }
-  if (terms & PSDR_TERM_PRIMARY) {
-    const int R = static_cast<int>(cfg.primary_edge_hypercube_resolution);
-    const int64_t n64 = static_cast<int64_t>(cfg.spp_primary_edge) * HW;
-    if (n64 > 0x7fffffff / 8) {
-      set_error(PSDR_ERR_ARG, "primary boundary: too many samples");
-      return PSDR_ERR_ARG;
-    }
-    const int n = static_cast<int>(n64);
-    if (s->b_state.ensure(8ull * std::max(R, 1)) || s->hc_mass.ensure(4ull * R) || s->hc_pmf.ensure(4ull * R) ||
-        s->hc_cmf.ensure(4ull * R) || s->b_tmp.ensure(4ull * (2 * (R / 256 + 2) + 16)))
-      return PSDR_ERR_NOMEM;
-    hipLaunchKernelGGL(k_seed_states, dim3(blocks(R, BT)), dim3(BT), 0, st, s->b_state.as<uint64_t>(), R);
-    hipLaunchKernelGGL(k_fill_f32, dim3(blocks(R, BT)), dim3(BT), 0, st, s->hc_mass.as<float>(), R, 0.f);
-    for (uint32_t r = 0; r < cfg.primary_edge_preprocess_rounds; ++r) {
-      const int rc = primary_pass(s, cfg, cam, true, R, R, nullptr, nullptr, s->hc_mass.as<float>(), nullptr,
-                                  nullptr, st);
-      if (rc) return rc;
-    }
-    int rc = sampler_tables(s->hc_mass.as<float>(), R, s->hc_pmf.as<float>(), s->hc_cmf.as<float>(), nullptr,
-                            s->b_tmp.as<float>(), st);
-    if (rc) return rc;
-    rc = primary_pass(s, cfg, cam, false, n, R, s->hc_pmf.as<float>(), s->hc_cmf.as<float>(), nullptr, dL_dI, grad_v,
-                      st);
-    if (rc) return rc;
-  }
-  if (terms & PSDR_TERM_DIRECT) {
-    const int r0 = static_cast<int>(cfg.secondary_edge_hypercube_resolution[0]);
-    const int r1 = static_cast<int>(cfg.secondary_edge_hypercube_resolution[1]);
-    const int r2 = static_cast<int>(cfg.secondary_edge_hypercube_resolution[2]);
-    const int64_t cells64 = static_cast<int64_t>(r0) * r1 * r2;
-    const int64_t n64 = static_cast<int64_t>(cfg.spp_secondary_edge) * HW;
-    if (cells64 > 0x7fffffff / 8 || n64 > 0x7fffffff / 8) {
-      set_error(PSDR_ERR_ARG, "direct boundary: too many cells or samples");
-      return PSDR_ERR_ARG;
-    }
-    const int cells = static_cast<int>(cells64), n = static_cast<int>(n64);
-    if (s->b_state.ensure(8ull * cells) || s->hc_mass.ensure(4ull * cells) || s->hc_pmf.ensure(4ull * cells) ||
-        s->hc_cmf.ensure(4ull * cells) || s->b_tmp.ensure(4ull * (2 * (cells / 256 + 2) + 16)))
-      return PSDR_ERR_NOMEM;
-    const int grid = std::max(1, std::min(blocks(std::max(cells, n), BT), cus() * 8));
-    if (s->spill.ensure(std::max(s->spill.bytes, static_cast<size_t>(grid) * BT * (STACK_TOTAL - BSTACK) * 4)))
-      return PSDR_ERR_NOMEM;
-    hipLaunchKernelGGL(k_seed_states, dim3(blocks(cells, BT)), dim3(BT), 0, st, s->b_state.as<uint64_t>(), cells);
-    hipLaunchKernelGGL(k_fill_f32, dim3(blocks(cells, BT)), dim3(BT), 0, st, s->hc_mass.as<float>(), cells, 0.f);
-    for (uint32_t r = 0; r < cfg.secondary_edge_preprocess_rounds; ++r) {
-      hipLaunchKernelGGL(k_direct<true>, dim3(grid), dim3(BT), 0, st, sv, cam, cells, s->b_state.as<uint64_t>(), r0,
-                         r1, r2, nullptr, nullptr, s->hc_mass.as<float>(), nullptr, 0u, nullptr, s->spill.as<int>());
-      PSDR_HIP_CHECK(hipGetLastError());
-    }
-    int rc = sampler_tables(s->hc_mass.as<float>(), cells, s->hc_pmf.as<float>(), s->hc_cmf.as<float>(), nullptr,
-                            s->b_tmp.as<float>(), st);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_direct<false>, dim3(grid), dim3(BT), 0, st, sv, cam, n, s->b_state.as<uint64_t>(), r0, r1,
-                       r2, s->hc_pmf.as<float>(), s->hc_cmf.as<float>(), nullptr, dL_dI, cfg.spp_secondary_edge,
-                       grad_v, s->spill.as<int>());
+  const int cells = static_cast<int>(cells64), n = static_cast<int>(n64);
+  const int grid = std::max(1, std::min(blocks(std::max(cells, n), BT), cus() * 8));
+  if (s->d_state.ensure(8ull * cells) || s->d_mass.ensure(4ull * cells) || s->d_pmf.ensure(4ull * cells) ||
+      s->d_cmf.ensure(4ull * cells) || s->d_tmp.ensure(4ull * (2 * (cells / 256 + 2) + 16)) ||
+      s->d_spill.ensure(static_cast<size_t>(grid) * BT * (STACK_TOTAL - BSTACK) * 4) ||
+      s->d_keys.ensure(8ull * n) || s->d_vals.ensure(8ull * n) || s->d_hist.ensure(4 * radix_hist_words(n)))
+    return PSDR_ERR_NOMEM;
+  uint64_t* state = s->d_state.as<uint64_t>();
+  hipLaunchKernelGGL(k_seed_states, dim3(blocks(cells, BT)), dim3(BT), 0, st, state, cells);
+  hipLaunchKernelGGL(k_fill_f32, dim3(blocks(cells, BT)), dim3(BT), 0, st, s->d_mass.as<float>(), cells, 0.f);
+  for (uint32_t r = 0; r < cfg.secondary_edge_preprocess_rounds; ++r) {
+    hipLaunchKernelGGL(k_direct<true>, dim3(grid), dim3(BT), 0, st, sv, cam, cells, state, r0, r1, r2, nullptr, nullptr,
+                       s->d_mass.as<float>(), nullptr, 0u, nullptr, s->d_spill.as<int>(), nullptr);
     PSDR_HIP_CHECK(hipGetLastError());
   }
+  int rc = sampler_tables(s->d_mass.as<float>(), cells, s->d_pmf.as<float>(), s->d_cmf.as<float>(), nullptr,
+                          s->d_tmp.as<float>(), st);
+  if (rc) return rc;
+  // process the lanes in cell order (edge position major): neighbouring lanes
+  // of a wave then trace from neighbouring edge points
+  uint32_t* k0 = s->d_keys.as<uint32_t>();
+  uint32_t* v0 = s->d_vals.as<uint32_t>();
+  hipLaunchKernelGGL(k_direct_keys, dim3(blocks(n, BT)), dim3(BT), 0, st, n, cells, s->d_pmf.as<float>(),
+                     s->d_cmf.as<float>(), k0, v0);
+  PSDR_HIP_CHECK(hipGetLastError());
+  const int passes = cells <= (1 << 24) ? 3 : 4;
+  rc = radix_sort_pairs(k0, v0, k0 + n, v0 + n, s->d_hist.as<uint32_t>(), n, passes, st);
+  if (rc) return rc;
+  const uint32_t* order = (passes & 1) ? v0 + n : v0;
+  hipLaunchKernelGGL(k_direct<false>, dim3(grid), dim3(BT), 0, st, sv, cam, n, state, r0, r1, r2,
+                     s->d_pmf.as<float>(), s->d_cmf.as<float>(), nullptr, dL_dI, cfg.spp_secondary_edge, grad_v,
+                     s->d_spill.as<int>(), order);
+  PSDR_HIP_CHECK(hipGetLastError());
   return PSDR_OK;
 }
 

@@ -41,6 +41,24 @@ void DevBuf::release() {
   bytes = 0;
 }
 
+AuxStreams::~AuxStreams() {
+  for (int i = 0; i < 2; ++i) {
+    if (st[i]) hipStreamDestroy(st[i]);
+    if (join[i]) hipEventDestroy(join[i]);
+  }
+  if (fork) hipEventDestroy(fork);
+}
+
+int AuxStreams::ensure() {
+  if (fork) return PSDR_OK;
+  for (int i = 0; i < 2; ++i) {
+    PSDR_HIP_CHECK(hipStreamCreateWithFlags(&st[i], hipStreamNonBlocking));
+    PSDR_HIP_CHECK(hipEventCreateWithFlags(&join[i], hipEventDisableTiming));
+  }
+  PSDR_HIP_CHECK(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+  return PSDR_OK;
+}
+
 // ---------------------------------------------------------------------------
 // Camera matrices (src/camera.h:7-17, 33-53, 108-124; near/far src/psdr.cc:27-28)
 // in fp32; row-major m[r][c]; products summed k = 0..3 left to right.
@@ -369,7 +387,7 @@ static int scene_create_impl(const psdr_scene_desc* d, psdr_scene* s, hipStream_
   PSDR_TRY(s->counters.ensure(4 * n1));
   PSDR_TRY(s->sort_keys.ensure(8 * n1));
   PSDR_TRY(s->sort_vals.ensure(8 * n1));
-  PSDR_TRY(s->sort_hist.ensure(4ull * 256 * ((n1 + 2047) / 2048 + 1)));
+  PSDR_TRY(s->sort_hist.ensure(4ull * radix_hist_words(n1)));
   PSDR_TRY(s->bounds.ensure(4ull * (8 + 6 * 1024)));
   PSDR_TRY(scene_set_buffers(s, d));
   PSDR_TRY(scene_derive(s, st));
@@ -425,20 +443,8 @@ int psdr_scene_update(psdr_scene* s, const psdr_scene_desc* desc, void* stream) 
 
 int psdr_scene_destroy(psdr_scene* s) {
   if (!s) return PSDR_OK;
-  DevBuf* bufs[] = {&s->faces,       &s->tri_uv,      &s->tri_obj,       &s->objs,          &s->lights,
-                    &s->edge_topo,   &s->tri_geom,    &s->edge_geom,     &s->edge_pmf,      &s->edge_cmf,
-                    &s->light_tri_pmf, &s->light_tri_cmf, &s->light_power, &s->light_area, &s->lights_total,
-                    &s->obj_pmf,     &s->obj_cmf,     &s->scan_tmp,      &s->morton,        &s->order,
-                    &s->child,       &s->parent,      &s->box,           &s->leaf,          &s->node,
-                    &s->counters,    &s->sort_keys,   &s->sort_vals,     &s->sort_hist,     &s->bounds,
-                    &s->lane_L,      &s->q_a,         &s->q_b,           &s->rays,          &s->hits,
-                    &s->rays2,       &s->hits2,       &s->spill,         &s->hit_cache,     &s->counters_r,
-                    &s->hc_mass,     &s->hc_pmf,      &s->hc_cmf,        &s->stats,         &s->b_rec,
-                    &s->b_rays,      &s->b_hits,      &s->b_q0,          &s->b_LA,          &s->b_LB,
-                    &s->b_rngA,      &s->b_rngB,      &s->b_state,       &s->b_tmp,         &s->b_counters};
   hipDeviceSynchronize();
-  for (DevBuf* b : bufs) b->release();
-  delete s;
+  delete s;  // buffers, streams and events are released by their owners
   return PSDR_OK;
 }
 
@@ -544,7 +550,7 @@ int psdr_scene_trace(psdr_scene* s, const float* ro, const float* rd, int32_t n,
   if (n == 0) return PSDR_OK;
   const int grid = trace_grid(n);
   PSDR_TRY(s->spill.ensure(std::max(s->spill.bytes, spill_bytes(grid))));
-  return trace_rays(s->view(), ro, rd, n, nullptr, tri, t, u, v, s->spill.as<int>(), grid, as_stream(stream));
+  return trace_rays(s->view(), ro, rd, n, nullptr, 1, 0, tri, t, u, v, s->spill.as<int>(), grid, as_stream(stream));
 }
 
 int psdr_trace_timing(int32_t enable, double* total_ms, int64_t* launches, int64_t* bytes) {
@@ -635,8 +641,11 @@ int psdr_last_render_stats(const psdr_scene* s, int64_t* counts, int32_t n_count
   PSDR_HIP_CHECK(hipMemcpy(c.data(), s->counters_r.p, 4 * 128, hipMemcpyDeviceToHost));
   for (int i = 0; i < n_counts && i < 64; ++i) counts[i] = c[i];
   if (rays_traced) {
+    // camera rays + per depth d < D: alive paths x (light + BSDF samples, + 1
+    // extension ray except at the last depth)
+    const int64_t nLB = static_cast<int64_t>(s->last_cfg.mis_light_samples) + s->last_cfg.mis_bsdf_samples;
     int64_t r = c[0];
-    for (int d = 0; d < s->last_D; ++d) r += c[64 + d];
+    for (int d = 0; d < s->last_D; ++d) r += static_cast<int64_t>(c[d + 1]) * (nLB + (d < s->last_D - 1 ? 1 : 0));
     *rays_traced = r;
   }
   return PSDR_OK;

@@ -4,8 +4,8 @@
 #include "scene.h"
 
 namespace psdr {
-int trace_rays(const SceneView& sv, const float* ro, const float* rd, int n, const int* count_dev, int* tri,
-               float* t, float* u, float* v, int* spill, int grid, hipStream_t st);
+int trace_rays(const SceneView& sv, const float* ro, const float* rd, int n, const int* count_dev, int K, int seg,
+               int* tri, float* t, float* u, float* v, int* spill, int grid, hipStream_t st);
 int trace_grid(int64_t max_rays);
 int trace_timing(int enable, double* total_ms, int64_t* launches, int64_t* bytes);
 size_t spill_bytes(int grid);
@@ -17,6 +17,8 @@ int render_backward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& ca
                     float* grad_v, float* grad_t, float* grad_e, int terms, hipStream_t st);
 int interior_backward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& cam, const float* dL_dI,
                       float* grad_v, float* grad_t, float* grad_e, hipStream_t st);
-int boundary_backward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& cam, const float* dL_dI,
-                      float* grad_v, int terms, hipStream_t st);
+int boundary_primary(psdr_scene* s, const psdr_config& cfg, const psdr_camera& cam, const float* dL_dI,
+                     float* grad_v, hipStream_t st);
+int boundary_direct(psdr_scene* s, const psdr_config& cfg, const psdr_camera& cam, const float* dL_dI,
+                    float* grad_v, hipStream_t st);
 }  // namespace psdr

@@ -32,7 +32,7 @@ namespace psdr {
 namespace {
 
 constexpr int TRACE_THREADS = 256;
-// render counters: [d] paths entering shade(d), [64 + d] rays traced at depth d
+// render counters: [d] paths entering shade(d)
 constexpr int COUNTERS = 128;
 constexpr int STACK_LDS = 16;
 
@@ -60,18 +60,27 @@ __global__ void k_camera_rays(CameraK cam, uint32_t spp, int64_t N, float* o, fl
 // AoS (ro[3*i+k]); outputs per ray.
 __global__ __launch_bounds__(TRACE_THREADS) void k_trace(SceneView sv, const float* __restrict__ ro,
                                                          const float* __restrict__ rd, int n,
-                                                         const int* __restrict__ count, int* __restrict__ otri,
+                                                         const int* __restrict__ count, int K, int seg,
+                                                         int* __restrict__ otri,
                                                          float* __restrict__ ot, float* __restrict__ ou,
                                                          float* __restrict__ ov, int* __restrict__ spill,
                                                          unsigned long long* __restrict__ timed_bytes) {
   __shared__ int stack[STACK_LDS * TRACE_THREADS];
-  const int cnt = count ? *count : n;
+  // rays are K segments of `per` rays each, segment k starting at k * seg
+  // (one segment per ray kind, so a wave traces rays of one kind)
+  const int per = count ? *count : n;
+  const int cnt = per * K;
   // algorithmic bytes of this launch: ray read (24 B) + tri, u, v (+ t) written
   if (timed_bytes && blockIdx.x == 0 && threadIdx.x == 0)
     atomicAdd(timed_bytes, static_cast<unsigned long long>(cnt) * (ot ? 40ull : 36ull));
   const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
   const int stride = gridDim.x * blockDim.x;
-  for (int i = gtid; i < cnt; i += stride) {
+  for (int ii = gtid; ii < cnt; ii += stride) {
+    int i = ii;
+    if (K > 1) {
+      const int k = ii / per;
+      i = k * seg + (ii - k * per);
+    }
     const f3 o = f3{ro[3 * i], ro[3 * i + 1], ro[3 * i + 2]};
     const f3 d = f3{rd[3 * i], rd[3 * i + 1], rd[3 * i + 2]};
     const HitRec h = closest_hit<STACK_LDS>(sv.node, sv.leaf, sv.n_tris, o, d, stack, spill + gtid, stride);
@@ -109,8 +118,6 @@ __global__ void k_zero_i32(int* p, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;
 }
-
-__global__ void k_ray_count(const int* paths, int k, int* rays) { *rays = *paths * k; }
 
 __global__ void k_cam(CameraK cam, uint32_t spp, int N, QueueView q, RaysView rays, int* count_out) {
   const int lane = blockIdx.x * blockDim.x + threadIdx.x;
@@ -167,15 +174,15 @@ __global__ __launch_bounds__(256) void k_shade(ShadeParams P, QueueView qi, cons
         J0 = qi.J0[j];
         O = load3(qi.O, qi.cap, j);
         const Its ip = reconstruct(sv, qi.tri[j], qi.u[j], qi.v[j], O, true);
-        const int rb = j * P.K_in;
+        const int rs = qi.cap;  // ray segment stride: ray k of queue entry j is at k * rs + j
         const int64_t cslot = 1 + static_cast<int64_t>(d - 1) * P.Kmax;
         int k = 0;
         // next-event estimation (src/path_integrator.cc:103-131)
         for (uint32_t i = 0; i < P.nL; ++i, ++k) {
           float ru = rng.next();
           const float rv = rng.next();
-          const int ht = rin.tri[rb + k];
-          const float hu = rin.u[rb + k], hv = rin.v[rb + k];
+          const int ht = rin.tri[k * rs + j];
+          const float hu = rin.u[k * rs + j], hv = rin.v[k * rs + j];
           if (P.keep) {
             const int64_t ci = (cslot + k) * cache.N + lane;
             cache.tri[ci] = ht;
@@ -207,8 +214,8 @@ __global__ __launch_bounds__(256) void k_shade(ShadeParams P, QueueView qi, cons
         for (uint32_t i = 0; i < P.nB; ++i, ++k) {
           const float ru = rng.next();
           const float rv = rng.next();
-          const int ht = rin.tri[rb + k];
-          const float hu = rin.u[rb + k], hv = rin.v[rb + k];
+          const int ht = rin.tri[k * rs + j];
+          const float hu = rin.u[k * rs + j], hv = rin.v[k * rs + j];
           if (P.keep) {
             const int64_t ci = (cslot + k) * cache.N + lane;
             cache.tri[ci] = ht;
@@ -238,8 +245,8 @@ __global__ __launch_bounds__(256) void k_shade(ShadeParams P, QueueView qi, cons
           const float ru = rng.next();
           const float rv = rng.next();
           if (d - 1 < P.D - 1) {
-            const int ht = rin.tri[rb + k];
-            const float hu = rin.u[rb + k], hv = rin.v[rb + k];
+            const int ht = rin.tri[k * rs + j];
+            const float hu = rin.u[k * rs + j], hv = rin.v[k * rs + j];
             if (P.keep) {
               const int64_t ci = (cslot + k) * cache.N + lane;
               cache.tri[ci] = ht;
@@ -279,7 +286,7 @@ __global__ __launch_bounds__(256) void k_shade(ShadeParams P, QueueView qi, cons
     const int slot = wave_alloc(alive, count_out);
     if (have && alive) {
       const uint64_t rng_start = rng.state;
-      const int rb = slot * P.K_out;
+      const int rs = qo.cap;
       int k = 0;
       for (uint32_t i = 0; i < P.nL; ++i, ++k) {
         float ru = rng.next();
@@ -292,20 +299,20 @@ __global__ __launch_bounds__(256) void k_shade(ShadeParams P, QueueView qi, cons
           const LightPoint lp = sample_light_point(sv, l, ru, rv);
           dir = normalize(lp.p - its.p);
         }
-        put_ray(rout, rb + k, its.p, dir);
+        put_ray(rout, k * rs + slot, its.p, dir);
       }
       for (uint32_t i = 0; i < P.nB; ++i, ++k) {
         const float ru = rng.next();
         const float rv = rng.next();
         const Sampled smp = mat_sample(sv, its, ru, rv);
-        put_ray(rout, rb + k, its.p, smp.wo);
+        put_ray(rout, k * rs + slot, its.p, smp.wo);
       }
       {
         const float ru = rng.next();
         const float rv = rng.next();
         if (d < P.D - 1) {
           const Sampled smp = mat_sample(sv, its, ru, rv);
-          put_ray(rout, rb + k, its.p, smp.wo);
+          put_ray(rout, k * rs + slot, its.p, smp.wo);
         }
       }
       qo.lane[slot] = lane;
@@ -456,15 +463,15 @@ TraceTimer& trace_timer() {
 constexpr size_t TIMER_PAIRS = 4096;
 }  // namespace
 
-int trace_rays(const SceneView& sv, const float* ro, const float* rd, int n, const int* count_dev, int* tri,
-               float* t, float* u, float* v, int* spill, int grid, hipStream_t st) {
+int trace_rays(const SceneView& sv, const float* ro, const float* rd, int n, const int* count_dev, int K, int seg,
+               int* tri, float* t, float* u, float* v, int* spill, int grid, hipStream_t st) {
   TraceTimer& tt = trace_timer();
   std::lock_guard<std::mutex> lock(tt.mu);
   const bool timed = tt.on && tt.used < TIMER_PAIRS;
   if (tt.on && !timed) ++tt.dropped;
   if (timed) PSDR_HIP_CHECK(hipEventRecord(tt.ev[2 * tt.used], st));
-  hipLaunchKernelGGL(k_trace, dim3(grid), dim3(TRACE_THREADS), 0, st, sv, ro, rd, n, count_dev, tri, t, u, v, spill,
-                     timed ? tt.bytes : nullptr);
+  hipLaunchKernelGGL(k_trace, dim3(grid), dim3(TRACE_THREADS), 0, st, sv, ro, rd, n, count_dev, K, seg, tri, t, u, v,
+                     spill, timed ? tt.bytes : nullptr);
   PSDR_HIP_CHECK(hipGetLastError());
   if (timed) {
     PSDR_HIP_CHECK(hipEventRecord(tt.ev[2 * tt.used + 1], st));
@@ -565,9 +572,8 @@ int run_depths(psdr_scene* s, const WaveArgs& a, const QueueView& q0, const Rays
                        rout, a.cache, a.lane_L, a.lane_rng_out);
     PSDR_HIP_CHECK(hipGetLastError());
     if (d < a.D && P.K_out > 0) {
-      hipLaunchKernelGGL(k_ray_count, dim3(1), dim3(1), 0, st, a.counters + d + 1, P.K_out, a.counters + 64 + d);
-      const int rc = trace_rays(sv, rout.o, rout.d, static_cast<int>(ray_cap), a.counters + 64 + d, rout.tri, nullptr,
-                                rout.u, rout.v, s->spill.as<int>(), tgrid, st);
+      const int rc = trace_rays(sv, rout.o, rout.d, n, a.counters + d + 1, P.K_out, n, rout.tri, nullptr, rout.u,
+                                rout.v, s->spill.as<int>(), tgrid, st);
       if (rc) return rc;
     }
     qin = qout;
@@ -616,7 +622,7 @@ int render_forward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& cam
   hipLaunchKernelGGL(k_cam, dim3(blocks(N, 256)), dim3(256), 0, st, cam, cfg.spp_interior, N, qa, rays_a,
                      counters + 0);
   PSDR_HIP_CHECK(hipGetLastError());
-  int rc = trace_rays(sv, rays_a.o, rays_a.d, N, counters + 0, rays_a.tri, nullptr, rays_a.u, rays_a.v,
+  int rc = trace_rays(sv, rays_a.o, rays_a.d, N, counters + 0, 1, N, rays_a.tri, nullptr, rays_a.u, rays_a.v,
                       s->spill.as<int>(), tgrid, st);
   if (rc) return rc;
   if (s->rays2.ensure(24ull * ray_cap) || s->hits2.ensure(12ull * ray_cap)) return PSDR_ERR_NOMEM;

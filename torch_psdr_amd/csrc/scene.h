@@ -38,9 +38,25 @@ void set_error(int code, const std::string& msg);
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
   int ensure(size_t n);
   void release();
   template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+// Streams and events for running independent gradient terms side by side
+// (created on first use, default priority).
+struct AuxStreams {
+  hipStream_t st[2] = {nullptr, nullptr};
+  hipEvent_t fork = nullptr, join[2] = {nullptr, nullptr};
+  AuxStreams() = default;
+  AuxStreams(const AuxStreams&) = delete;
+  AuxStreams& operator=(const AuxStreams&) = delete;
+  ~AuxStreams();
+  int ensure();
 };
 
 // Object record as seen by kernels.
@@ -110,7 +126,11 @@ struct psdr_scene {
   psdr::DevBuf lane_L, q_a, q_b, rays, hits, rays2, hits2, spill, hit_cache, counters_r, hc_mass, hc_pmf,
       hc_cmf, stats;
   // boundary estimators
-  psdr::DevBuf b_rec, b_rays, b_hits, b_q0, b_LA, b_LB, b_rngA, b_rngB, b_state, b_tmp, b_counters;
+  psdr::DevBuf b_rec, b_rays, b_hits, b_q0, b_LA, b_LB, b_rngA, b_rngB, b_state, b_tmp, b_counters, b_keys, b_vals,
+      b_hist;
+  // direct-boundary scratch (separate from the primary pass's, so both can run at once)
+  psdr::DevBuf d_state, d_mass, d_pmf, d_cmf, d_tmp, d_spill, d_keys, d_vals, d_hist;
+  psdr::AuxStreams aux;
   int light_tri_total = 0;
   // last forward (for backward / stats)
   uint64_t last_N = 0;
@@ -125,6 +145,12 @@ struct psdr_scene {
 namespace psdr {
 // scene.hip
 int scene_derive(psdr_scene* s, hipStream_t st);
+// Stable LSD radix sort of (key, value) pairs over the low 8*passes key bits
+// (hist: radix_hist_words(n) words).  Buffers ping-pong per pass: the result is
+// in (k0, v0) after an even number of passes, in (k1, v1) after an odd one.
+size_t radix_hist_words(int n);
+int radix_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, uint32_t* hist, int n, int passes,
+                     hipStream_t st);
 int build_lbvh(psdr_scene* s, hipStream_t st);
 // DiscreteSampler tables of n weights (single segment); tmp >= 2*(n/256+2) floats
 int sampler_tables(const float* w, int n, float* pmf, float* cmf, float* total, float* tmp, hipStream_t st);

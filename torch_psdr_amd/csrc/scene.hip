@@ -618,6 +618,89 @@ int scene_derive(psdr_scene* s, hipStream_t st) {
   return build_lbvh(s, st);
 }
 
+// exclusive scan of m uint32 in three launches: per-block sums, a single-block
+// scan of those, per-block local scans plus the block offset
+constexpr int XS_THREADS = 256, XS_PER = 8, XS_BLOCK = XS_THREADS * XS_PER;
+
+__global__ __launch_bounds__(XS_THREADS) void k_xscan_sums(const uint32_t* __restrict__ a, int m,
+                                                           uint32_t* __restrict__ part) {
+  __shared__ uint32_t red[XS_THREADS / 64];
+  const int base = blockIdx.x * XS_BLOCK;
+  uint32_t sum = 0;
+  for (int j = 0; j < XS_PER; ++j) {
+    const int i = base + j * XS_THREADS + static_cast<int>(threadIdx.x);
+    if (i < m) sum += a[i];
+  }
+  for (int off = 32; off > 0; off >>= 1) sum += __shfl_down(sum, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = sum;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int w = 0; w < XS_THREADS / 64; ++w) t += red[w];
+    part[blockIdx.x] = t;
+  }
+}
+
+__global__ __launch_bounds__(XS_THREADS) void k_xscan_down(uint32_t* __restrict__ a, int m,
+                                                           const uint32_t* __restrict__ part) {
+  __shared__ uint32_t s[XS_THREADS];
+  const int base = blockIdx.x * XS_BLOCK + static_cast<int>(threadIdx.x) * XS_PER;
+  uint32_t v[XS_PER];
+  uint32_t sum = 0;
+  for (int j = 0; j < XS_PER; ++j) {
+    v[j] = base + j < m ? a[base + j] : 0u;
+    sum += v[j];
+  }
+  s[threadIdx.x] = sum;
+  __syncthreads();
+  for (int off = 1; off < XS_THREADS; off <<= 1) {
+    const uint32_t x = threadIdx.x >= static_cast<unsigned>(off) ? s[threadIdx.x - off] : 0u;
+    __syncthreads();
+    s[threadIdx.x] += x;
+    __syncthreads();
+  }
+  uint32_t run = part[blockIdx.x] + s[threadIdx.x] - sum;
+  for (int j = 0; j < XS_PER; ++j) {
+    if (base + j < m) a[base + j] = run;
+    run += v[j];
+  }
+}
+
+size_t radix_hist_words(int n) {
+  const size_t m = 256ull * std::max(blocks(n, SORT_TILE), 1);
+  return m + (m + XS_BLOCK - 1) / XS_BLOCK;
+}
+
+int exclusive_scan_u32(uint32_t* a, int m, uint32_t* part, hipStream_t st) {
+  const int nb = blocks(m, XS_BLOCK);
+  if (nb <= 1) {
+    hipLaunchKernelGGL(k_scan_u32, dim3(1), dim3(1024), 0, st, a, m);
+  } else {
+    hipLaunchKernelGGL(k_xscan_sums, dim3(nb), dim3(XS_THREADS), 0, st, a, m, part);
+    hipLaunchKernelGGL(k_scan_u32, dim3(1), dim3(1024), 0, st, part, nb);
+    hipLaunchKernelGGL(k_xscan_down, dim3(nb), dim3(XS_THREADS), 0, st, a, m, part);
+  }
+  PSDR_HIP_CHECK(hipGetLastError());
+  return PSDR_OK;
+}
+
+int radix_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, uint32_t* hist, int n, int passes,
+                     hipStream_t st) {
+  if (n <= 0) return PSDR_OK;
+  const int tiles = blocks(n, SORT_TILE);
+  for (int pass = 0; pass < passes; ++pass) {
+    const int shift = 8 * pass;
+    hipLaunchKernelGGL(k_radix_hist, dim3(tiles), dim3(SORT_THREADS), 0, st, k0, n, shift, hist);
+    const int rc = exclusive_scan_u32(hist, 256 * tiles, hist + 256 * tiles, st);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(SORT_THREADS), 0, st, k0, v0, k1, v1, n, shift, hist);
+    PSDR_HIP_CHECK(hipGetLastError());
+    std::swap(k0, k1);
+    std::swap(v0, v1);
+  }
+  return PSDR_OK;
+}
+
 int build_lbvh(psdr_scene* s, hipStream_t st) {
   const int n = s->n_tris;
   if (n < 2) {
@@ -639,18 +722,8 @@ int build_lbvh(psdr_scene* s, hipStream_t st) {
   hipLaunchKernelGGL(k_morton, dim3(blocks(n, 256)), dim3(256), 0, st, s->tri_geom.as<float4>(), n,
                      s->bounds.as<float>(), s->morton.as<uint32_t>(), k0, v0);
   PSDR_HIP_CHECK(hipGetLastError());
-  const int tiles = blocks(n, SORT_TILE);
-  for (int pass = 0; pass < 4; ++pass) {
-    const int shift = 8 * pass;
-    hipLaunchKernelGGL(k_radix_hist, dim3(tiles), dim3(SORT_THREADS), 0, st, k0, n, shift,
-                       s->sort_hist.as<uint32_t>());
-    hipLaunchKernelGGL(k_scan_u32, dim3(1), dim3(1024), 0, st, s->sort_hist.as<uint32_t>(), 256 * tiles);
-    hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(SORT_THREADS), 0, st, k0, v0, k1, v1, n, shift,
-                       s->sort_hist.as<uint32_t>());
-    PSDR_HIP_CHECK(hipGetLastError());
-    std::swap(k0, k1);
-    std::swap(v0, v1);
-  }
+  const int rc = radix_sort_pairs(k0, v0, k1, v1, s->sort_hist.as<uint32_t>(), n, 4, st);
+  if (rc) return rc;
   // after an even number of passes the sorted data is back in k0/v0 (= buffer start)
   PSDR_HIP_CHECK(hipMemcpyAsync(s->order.p, v0, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, st));
   hipLaunchKernelGGL(k_karras, dim3(blocks(n - 1, 256)), dim3(256), 0, st, k0, n, s->child.as<int>(),
