@@ -38,6 +38,8 @@ def main():
     ap.add_argument("--res", type=int, default=512)
     ap.add_argument("--spp", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--plain-steps", type=int, default=3,
+                    help="steps of the reference-shaped baseline path to time (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU-baseline sample time")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device-to-device copy (scales rocprof FETCH/WRITE_SIZE)")
@@ -107,6 +109,10 @@ def main():
     mrays = rays / (ms * 1e-3) / 1e6
 
     roof = trace_roofline(tr_ms.value, tr_launch.value, tr_bytes.value, default_workload=(res, spp) == (512, 16))
+    plain = None
+    if args.plain_steps > 0:
+        plain = plain_baseline(torch, ops, step, verts, tex, args.plain_steps, rays)
+        plain["speedup"] = round(plain["ms_per_step"] / ms, 2)
     step_bytes = step_algorithmic_bytes(N_int, N_e, spp, D, L, B)
     cpu = None if args.no_cpu_baseline else cpu_baseline(parts, spp, args.cpu_seconds)
     out = {
@@ -124,14 +130,39 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"vertex+texture gradient step, icosphere L5 (20480 tris) + ground + light, "
                                f"{res}x{res}, {spp} spp, depth 3, edge spp {spe}",
-                   "rays_per_step": rays, "loss": float(loss.detach().cpu())},
+                   "rays_per_step": rays, "loss": float(loss.detach().cpu()),
+                   "grad_abs_max": {"vertices": float(verts.grad.abs().max()), "texels": float(tex.grad.abs().max())}},
         "roofline": roof,
         "step_roofline": {"bound": "hbm", "achieved": round(step_bytes / (ms * 1e-3) / 1e9, 2), "peak": HBM_PEAK_GBS,
                           "unit": "GB/s", "frac": round(step_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
                           "algorithmic_bytes": step_bytes},
         "cpu_baseline": cpu,
+        "plain_gpu_path": plain,
     }
     print(json.dumps(out))
+
+
+def plain_baseline(torch, ops, step, verts, tex, n_steps, rays):
+    """The same step in the reference's launch shape (full-width launches without
+    compaction, one synchronised trace launch per ray kind, boundary samples in
+    lane order, gradient terms one after another).  Both shapes compute the same
+    estimator from the same samples, so their gradient error is equal; the
+    largest gradient difference (accumulation order only) is reported."""
+    step()
+    torch.cuda.synchronize()
+    g_fused = (verts.grad.clone(), tex.grad.clone())
+    with ops.launch_shape("plain"):
+        step()  # warm-up, also the comparison step
+        torch.cuda.synchronize()
+        g_plain = (verts.grad.clone(), tex.grad.clone())
+        t0 = time.perf_counter()
+        for _ in range(n_steps):
+            step()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / n_steps
+    diff = max(float((a - b).abs().max() / b.abs().max().clamp_min(1e-30)) for a, b in zip(g_plain, g_fused))
+    return {"ms_per_step": round(ms, 3), "value": round(rays / (ms * 1e-3) / 1e6, 3), "unit": "Mrays/s",
+            "steps": n_steps, "grad_max_rel_diff_vs_fused": diff}
 
 
 # HBM bytes per k_trace launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this

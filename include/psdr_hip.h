@@ -152,6 +152,20 @@ int psdr_render_forward(psdr_scene* scene, const psdr_config* cfg, const psdr_ca
 int psdr_render_backward(psdr_scene* scene, const psdr_config* cfg, const psdr_camera* cam, const float* dL_dI,
                          float* grad_vertices, float* grad_texels, float* grad_emit, int32_t terms, void* stream);
 
+/* The same two entry points in the reference's launch shape, kept as the
+ * measured baseline of the fused path (no reference counterpart): every depth
+ * runs over all N lanes without compaction, each ray kind (light, BSDF,
+ * extension sample) is traced by its own full-width launch followed by a host
+ * synchronisation, as SceneOptix::hit synchronises after every launch
+ * (src/scene_optix.cc:275), boundary samples are processed in lane order and the
+ * gradient terms run one after another on the caller's stream.  Results equal
+ * those of psdr_render_forward / psdr_render_backward. */
+int psdr_render_forward_plain(psdr_scene* scene, const psdr_config* cfg, const psdr_camera* cam, float* out_rgb,
+                              int32_t keep, void* stream);
+int psdr_render_backward_plain(psdr_scene* scene, const psdr_config* cfg, const psdr_camera* cam,
+                               const float* dL_dI, float* grad_vertices, float* grad_texels, float* grad_emit,
+                               int32_t terms, void* stream);
+
 /* ---- component entry points (tests, benchmarks) --------------------------- */
 /* First n_draws PCG32 outputs of lanes [lane0, lane0+n_lanes) of Sampler
  * (src/sampler.h:11-17); out is device [n_lanes*n_draws]. */

@@ -102,6 +102,9 @@ _SIGS = {
     "psdr_render_forward": (C.c_int, [_VP, C.POINTER(PsdrConfig), C.POINTER(PsdrCamera), _VP, _I32, _VP]),
     "psdr_render_backward": (C.c_int, [_VP, C.POINTER(PsdrConfig), C.POINTER(PsdrCamera), _VP, _VP, _VP, _VP, _I32,
                                        _VP]),
+    "psdr_render_forward_plain": (C.c_int, [_VP, C.POINTER(PsdrConfig), C.POINTER(PsdrCamera), _VP, _I32, _VP]),
+    "psdr_render_backward_plain": (C.c_int, [_VP, C.POINTER(PsdrConfig), C.POINTER(PsdrCamera), _VP, _VP, _VP, _VP,
+                                             _I32, _VP]),
     "psdr_pcg_lane_draws": (C.c_int, [C.c_uint32, _I32, _I32, _VP, _VP]),
     "psdr_camera_rays": (C.c_int, [C.POINTER(PsdrCamera), C.c_uint32, _VP, _VP, _VP, _VP]),
     "psdr_camera_matrices": (C.c_int, [C.POINTER(PsdrCamera), _VP, _VP, _VP, _VP, _VP]),

@@ -538,11 +538,11 @@ int render_backward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& ca
   if (s->stats.ensure(4ull * (3 * static_cast<size_t>(s->n_vertices) + s->n_texels + 3 * s->n_lights + 64)))
     return PSDR_ERR_NOMEM;
   float* gv = grad_v ? grad_v : s->stats.as<float>();
-  if (static_cast<int>(ti) + static_cast<int>(tp) + static_cast<int>(td) <= 1) {
-    if (ti) return interior_backward(s, cfg, camd, dL_dI, grad_v, grad_t, grad_e, st);
-    if (tp) return boundary_primary(s, cfg, camd, dL_dI, gv, st);
-    if (td) return boundary_direct(s, cfg, camd, dL_dI, gv, st);
-    return PSDR_OK;
+  if (s->plain || static_cast<int>(ti) + static_cast<int>(tp) + static_cast<int>(td) <= 1) {
+    int rc = ti ? interior_backward(s, cfg, camd, dL_dI, grad_v, grad_t, grad_e, st) : PSDR_OK;
+    if (!rc && tp) rc = boundary_primary(s, cfg, camd, dL_dI, gv, st);
+    if (!rc && td) rc = boundary_direct(s, cfg, camd, dL_dI, gv, st);
+    return rc;
   }
   int rc = s->aux.ensure();
   if (rc) return rc;

@@ -460,7 +460,8 @@ static int primary_pass(psdr_scene* s, const psdr_config& cfg, const CameraK& ca
   // hypercube cells, already in edge order)
   const uint32_t* order = nullptr;
   uint32_t *k0 = nullptr, *v0 = nullptr;
-  if (!pre) {
+  const bool sorted = !pre && !s->plain;
+  if (sorted) {
     if (s->b_keys.ensure(8ull * n) || s->b_vals.ensure(8ull * n) || s->b_hist.ensure(4 * radix_hist_words(n)))
       return PSDR_ERR_NOMEM;
     k0 = s->b_keys.as<uint32_t>();
@@ -473,7 +474,7 @@ static int primary_pass(psdr_scene* s, const psdr_config& cfg, const CameraK& ca
     hipLaunchKernelGGL(k_prim_setup<false>, dim3(blocks(n, BT)), dim3(BT), 0, st, sv, cam, n, state, res, cpmf, ccmf,
                        rec, rng0, k0, v0);
   PSDR_HIP_CHECK(hipGetLastError());
-  if (!pre) {
+  if (sorted) {
     // keys are 2 x 16-bit Morton codes; a 2 x 11-bit code needs 3 byte passes
     const int passes = (cam.W <= 2048 && cam.H <= 2048) ? 3 : 4;
     rc = radix_sort_pairs(k0, v0, k0 + n, v0 + n, s->b_hist.as<uint32_t>(), n, passes, st);
@@ -486,6 +487,7 @@ static int primary_pass(psdr_scene* s, const psdr_config& cfg, const CameraK& ca
   if (s->spill.ensure(std::max(s->spill.bytes, spill_bytes(tgrid)))) return PSDR_ERR_NOMEM;
   rc = trace_rays(sv, r0.o, r0.d, 2 * n, nullptr, 1, 0, r0.tri, nullptr, r0.u, r0.v, s->spill.as<int>(), tgrid, st);
   if (rc) return rc;
+  if (s->plain) PSDR_HIP_CHECK(hipStreamSynchronize(st));
   hipLaunchKernelGGL(k_prim_check, dim3(blocks(n, BT)), dim3(BT), 0, st, sv, cam, n, q0, rec, r0);
   WaveArgs wa;
   wa.cam = cam;
@@ -499,6 +501,7 @@ static int primary_pass(psdr_scene* s, const psdr_config& cfg, const CameraK& ca
   wa.lane_L = s->b_LA.as<float>();
   wa.lane_rng_out = s->b_rngA.as<uint64_t>();
   wa.counters = cnt;
+  wa.plain = s->plain ? 1 : 0;
   rc = run_depths(s, wa, q0, r0, qs, rs, n, st);
   if (rc) return rc;
   hipLaunchKernelGGL(k_prim_requeue, dim3(blocks(n, BT)), dim3(BT), 0, st, n, q0, s->b_rngA.as<uint64_t>(),
@@ -584,15 +587,18 @@ int boundary_direct(psdr_scene* s, const psdr_config& cfg, const psdr_camera& ca
   if (rc) return rc;
   // process the lanes in cell order (edge position major): neighbouring lanes
   // of a wave then trace from neighbouring edge points
-  uint32_t* k0 = s->d_keys.as<uint32_t>();
-  uint32_t* v0 = s->d_vals.as<uint32_t>();
-  hipLaunchKernelGGL(k_direct_keys, dim3(blocks(n, BT)), dim3(BT), 0, st, n, cells, s->d_pmf.as<float>(),
-                     s->d_cmf.as<float>(), k0, v0);
-  PSDR_HIP_CHECK(hipGetLastError());
-  const int passes = cells <= (1 << 24) ? 3 : 4;
-  rc = radix_sort_pairs(k0, v0, k0 + n, v0 + n, s->d_hist.as<uint32_t>(), n, passes, st);
-  if (rc) return rc;
-  const uint32_t* order = (passes & 1) ? v0 + n : v0;
+  const uint32_t* order = nullptr;
+  if (!s->plain) {
+    uint32_t* k0 = s->d_keys.as<uint32_t>();
+    uint32_t* v0 = s->d_vals.as<uint32_t>();
+    hipLaunchKernelGGL(k_direct_keys, dim3(blocks(n, BT)), dim3(BT), 0, st, n, cells, s->d_pmf.as<float>(),
+                       s->d_cmf.as<float>(), k0, v0);
+    PSDR_HIP_CHECK(hipGetLastError());
+    const int passes = cells <= (1 << 24) ? 3 : 4;
+    rc = radix_sort_pairs(k0, v0, k0 + n, v0 + n, s->d_hist.as<uint32_t>(), n, passes, st);
+    if (rc) return rc;
+    order = (passes & 1) ? v0 + n : v0;
+  }
   hipLaunchKernelGGL(k_direct<false>, dim3(grid), dim3(BT), 0, st, sv, cam, n, state, r0, r1, r2,
                      s->d_pmf.as<float>(), s->d_cmf.as<float>(), nullptr, dL_dI, cfg.spp_secondary_edge, grad_v,
                      s->d_spill.as<int>(), order);

@@ -287,22 +287,47 @@ SceneView psdr_scene::view() const {
 
 static hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
 
-static int scene_set_buffers(psdr_scene* s, const psdr_scene_desc* d) {
+static int upload(DevBuf& b, const void* src, size_t bytes, hipStream_t st) {
+  PSDR_TRY(b.ensure(bytes));
+  if (bytes) PSDR_HIP_CHECK(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, st));
+  return PSDR_OK;
+}
+
+static ObjectK object_k(const psdr_object& b) {
+  return ObjectK{b.material, b.light,   b.albedo_offset, b.albedo_h,  b.albedo_w, b.rough_offset,
+                 b.rough_h,  b.rough_w, b.tri_begin,     b.tri_count, b.grad_mask};
+}
+
+// Points the scene at the caller's vertex / texel / emission buffers.  On an
+// update the object table may change in everything but its triangle ranges and
+// light slots (texture offsets and sizes, material, which parameters take
+// gradients); a changed table is uploaded again.
+static int scene_set_buffers(psdr_scene* s, const psdr_scene_desc* d, hipStream_t st) {
   if (d->n_tris != s->n_tris || d->n_vertices != s->n_vertices || d->n_objects != s->n_objects ||
       d->n_lights != s->n_lights) {
     set_error(PSDR_ERR_ARG, "scene update: counts differ from the scene at creation");
     return PSDR_ERR_ARG;
   }
+  bool changed = false;
+  for (int o = 0; o < d->n_objects; ++o) {
+    const psdr_object& a = s->objects[o];
+    const psdr_object& b = d->objects[o];
+    if (a.tri_begin != b.tri_begin || a.tri_count != b.tri_count || a.light != b.light) {
+      set_error(PSDR_ERR_ARG, "scene update: object " + std::to_string(o) + " changed its triangles or light slot");
+      return PSDR_ERR_ARG;
+    }
+    changed = changed || std::memcmp(&a, &b, sizeof(psdr_object)) != 0;
+  }
+  if (changed) {
+    s->objects.assign(d->objects, d->objects + d->n_objects);
+    s->objs_host.resize(s->objects.size());
+    for (size_t o = 0; o < s->objects.size(); ++o) s->objs_host[o] = object_k(s->objects[o]);
+    PSDR_TRY(upload(s->objs, s->objs_host.data(), sizeof(ObjectK) * s->objs_host.size(), st));
+  }
   s->vertices = d->vertices;
   s->texels = d->texels;
   s->light_emit = d->light_emit;
   s->n_texels = d->n_texels;
-  return PSDR_OK;
-}
-
-static int upload(DevBuf& b, const void* src, size_t bytes, hipStream_t st) {
-  PSDR_TRY(b.ensure(bytes));
-  if (bytes) PSDR_HIP_CHECK(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, st));
   return PSDR_OK;
 }
 
@@ -337,12 +362,8 @@ static int scene_create_impl(const psdr_scene_desc* d, psdr_scene* s, hipStream_
   std::vector<int> topo = edge_topology(hfaces, nt);
   s->n_edges = static_cast<int>(topo.size() / 6);
   // objects / lights
-  std::vector<ObjectK> ok(d->n_objects);
-  for (int o = 0; o < d->n_objects; ++o) {
-    const psdr_object& b = d->objects[o];
-    ok[o] = ObjectK{b.material,     b.light,  b.albedo_offset, b.albedo_h,  b.albedo_w,  b.rough_offset,
-                    b.rough_h,      b.rough_w, b.tri_begin,    b.tri_count, b.grad_mask};
-  }
+  s->objs_host.resize(d->n_objects);
+  for (int o = 0; o < d->n_objects; ++o) s->objs_host[o] = object_k(d->objects[o]);
   std::vector<LightK> lk(d->n_lights);
   int off = 0;
   for (int l = 0; l < d->n_lights; ++l) {
@@ -361,7 +382,7 @@ static int scene_create_impl(const psdr_scene_desc* d, psdr_scene* s, hipStream_
   const size_t tmp_floats = 2 * (wmax / 256 + 2 + static_cast<size_t>(nl) + 2) + 16;
   PSDR_TRY(s->scan_tmp.ensure(4 * (hdr.size() + wmax + tmp_floats)));
   PSDR_HIP_CHECK(hipMemcpyAsync(s->scan_tmp.p, hdr.data(), 4 * hdr.size(), hipMemcpyHostToDevice, st));
-  PSDR_TRY(upload(s->objs, ok.data(), sizeof(ObjectK) * ok.size(), st));
+  PSDR_TRY(upload(s->objs, s->objs_host.data(), sizeof(ObjectK) * s->objs_host.size(), st));
   PSDR_TRY(upload(s->lights, lk.data(), sizeof(LightK) * lk.size(), st));
   PSDR_TRY(upload(s->edge_topo, topo.data(), 4 * topo.size(), st));
   // derived buffers
@@ -388,8 +409,11 @@ static int scene_create_impl(const psdr_scene_desc* d, psdr_scene* s, hipStream_
   PSDR_TRY(s->sort_keys.ensure(8 * n1));
   PSDR_TRY(s->sort_vals.ensure(8 * n1));
   PSDR_TRY(s->sort_hist.ensure(4ull * radix_hist_words(n1)));
+  PSDR_TRY(s->ranges.ensure(8ull * n1));
+  // leaf boxes [n], internal-node boxes [n-1], 64-leaf chunk boxes, 6 floats each
+  PSDR_TRY(s->bvh_boxes.ensure(24ull * (2ull * n1 + n1 / 64 + 1)));
   PSDR_TRY(s->bounds.ensure(4ull * (8 + 6 * 1024)));
-  PSDR_TRY(scene_set_buffers(s, d));
+  PSDR_TRY(scene_set_buffers(s, d, st));
   PSDR_TRY(scene_derive(s, st));
   PSDR_HIP_CHECK(hipStreamSynchronize(st));
   return PSDR_OK;
@@ -436,7 +460,7 @@ int psdr_scene_update(psdr_scene* s, const psdr_scene_desc* desc, void* stream) 
     return PSDR_ERR_ARG;
   }
   PSDR_TRY(check_desc(desc));
-  PSDR_TRY(scene_set_buffers(s, desc));
+  PSDR_TRY(scene_set_buffers(s, desc, as_stream(stream)));
   s->have_cache = false;
   return scene_derive(s, as_stream(stream));
 }
@@ -505,6 +529,35 @@ int psdr_render_backward(psdr_scene* s, const psdr_config* cfg, const psdr_camer
     return PSDR_ERR_ARG;
   }
   return render_backward(s, *cfg, *cam, dL_dI, grad_vertices, grad_texels, grad_emit, terms, as_stream(stream));
+}
+
+namespace {
+struct PlainScope {  // sets psdr_scene::plain for one call
+  psdr_scene* s;
+  explicit PlainScope(psdr_scene* sc) : s(sc) { s->plain = true; }
+  ~PlainScope() { s->plain = false; }
+};
+}  // namespace
+
+int psdr_render_forward_plain(psdr_scene* s, const psdr_config* cfg, const psdr_camera* cam, float* out_rgb,
+                              int32_t keep, void* stream) {
+  if (!s) {
+    set_error(PSDR_ERR_ARG, "render_forward_plain: bad argument");
+    return PSDR_ERR_ARG;
+  }
+  PlainScope scope(s);
+  return psdr_render_forward(s, cfg, cam, out_rgb, keep, stream);
+}
+
+int psdr_render_backward_plain(psdr_scene* s, const psdr_config* cfg, const psdr_camera* cam, const float* dL_dI,
+                               float* grad_vertices, float* grad_texels, float* grad_emit, int32_t terms,
+                               void* stream) {
+  if (!s) {
+    set_error(PSDR_ERR_ARG, "render_backward_plain: bad argument");
+    return PSDR_ERR_ARG;
+  }
+  PlainScope scope(s);
+  return psdr_render_backward(s, cfg, cam, dL_dI, grad_vertices, grad_texels, grad_emit, terms, stream);
 }
 
 int psdr_pcg_lane_draws(uint32_t lane0, int32_t n_lanes, int32_t n_draws, uint32_t* out, void* stream) {

@@ -101,7 +101,11 @@ struct ShadeParams {
   int ad_jac;  // renderD semantics: jacobi = A / detach(A)
   int K_in, K_out, Kmax;
   int keep;
+  int plain;  // reference launch shape: no compaction, queue entry j stays at slot j
 };
+
+// plain launch shape: a finished path keeps its queue slot with this bit set
+constexpr uint32_t LANE_DEAD = 0x80000000u;
 
 __device__ __forceinline__ void store3(float* a, int cap, int j, f3 v) {
   a[j] = v.x;
@@ -142,8 +146,9 @@ __global__ __launch_bounds__(256) void k_shade(ShadeParams P, QueueView qi, cons
   const int d = P.depth;
   for (int base = blockIdx.x * blockDim.x; base < cnt; base += gridDim.x * blockDim.x) {
     const int j = base + threadIdx.x;
-    const bool have = j < cnt;
+    bool have = j < cnt;
     bool alive = false;
+    bool dead_in = false;  // plain shape: slot of a path that finished earlier
     uint32_t lane = 0;
     Rng rng{0, 0};
     f3 T = splat(1.f), L = splat(0.f);
@@ -151,6 +156,10 @@ __global__ __launch_bounds__(256) void k_shade(ShadeParams P, QueueView qi, cons
     Its its;
     float cu = 0.f, cv = 0.f;
     f3 O = P.cam.pos;
+    if (have && P.plain && (qi.lane[j] & LANE_DEAD)) {
+      dead_in = true;
+      have = false;
+    }
     if (have) {
       lane = qi.lane[j];
       rng.state = qi.rng[j];
@@ -283,7 +292,18 @@ __global__ __launch_bounds__(256) void k_shade(ShadeParams P, QueueView qi, cons
         alive = false;
       }
     }
-    const int slot = wave_alloc(alive, count_out);
+    int slot = j;
+    if (P.plain) {
+      if (j == 0) *count_out = cnt;
+      if (dead_in || (have && !alive)) {
+        // keep the slot, marked finished, with rays that miss at once
+        qo.lane[j] = (have ? lane : qi.lane[j]) | LANE_DEAD;
+        const f3 nan3 = splat(__builtin_nanf(""));
+        for (int k = 0; k < P.K_out; ++k) put_ray(rout, k * qo.cap + j, nan3, nan3);
+      }
+    } else {
+      slot = wave_alloc(alive, count_out);
+    }
     if (have && alive) {
       const uint64_t rng_start = rng.state;
       const int rs = qo.cap;
@@ -568,13 +588,26 @@ int run_depths(psdr_scene* s, const WaveArgs& a, const QueueView& q0, const Rays
     P.K_out = d < a.D ? static_cast<int>(a.nL + a.nB) + ((d < a.D - 1) ? 1 : 0) : 0;
     P.Kmax = Kmax;
     P.keep = a.keep;
+    P.plain = a.plain;
     hipLaunchKernelGGL(k_shade, dim3(sgrid), dim3(256), 0, st, P, qin, a.counters + d, rin, qout, a.counters + d + 1,
                        rout, a.cache, a.lane_L, a.lane_rng_out);
     PSDR_HIP_CHECK(hipGetLastError());
     if (d < a.D && P.K_out > 0) {
-      const int rc = trace_rays(sv, rout.o, rout.d, n, a.counters + d + 1, P.K_out, n, rout.tri, nullptr, rout.u,
-                                rout.v, s->spill.as<int>(), tgrid, st);
-      if (rc) return rc;
+      if (a.plain) {
+        // one full-width launch per ray kind, each followed by a host sync
+        const int g1 = trace_grid(n);
+        for (int k = 0; k < P.K_out; ++k) {
+          const int64_t o = static_cast<int64_t>(k) * n;
+          const int rc = trace_rays(sv, rout.o + 3 * o, rout.d + 3 * o, n, nullptr, 1, 0, rout.tri + o, nullptr,
+                                    rout.u + o, rout.v + o, s->spill.as<int>(), g1, st);
+          if (rc) return rc;
+          PSDR_HIP_CHECK(hipStreamSynchronize(st));
+        }
+      } else {
+        const int rc = trace_rays(sv, rout.o, rout.d, n, a.counters + d + 1, P.K_out, n, rout.tri, nullptr, rout.u,
+                                  rout.v, s->spill.as<int>(), tgrid, st);
+        if (rc) return rc;
+      }
     }
     qin = qout;
     rin = rout;
@@ -625,6 +658,7 @@ int render_forward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& cam
   int rc = trace_rays(sv, rays_a.o, rays_a.d, N, counters + 0, 1, N, rays_a.tri, nullptr, rays_a.u, rays_a.v,
                       s->spill.as<int>(), tgrid, st);
   if (rc) return rc;
+  if (s->plain) PSDR_HIP_CHECK(hipStreamSynchronize(st));
   if (s->rays2.ensure(24ull * ray_cap) || s->hits2.ensure(12ull * ray_cap)) return PSDR_ERR_NOMEM;
   RaysView rays_b = carve_rays(s->rays2, s->hits2, ray_cap);
   WaveArgs wa;
@@ -639,6 +673,7 @@ int render_forward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& cam
   wa.lane_L = lane_L;
   wa.lane_rng_out = nullptr;
   wa.counters = counters;
+  wa.plain = s->plain ? 1 : 0;
   const QueueView qs[2] = {qb, qa};
   const RaysView rs[2] = {rays_b, rays_a};
   rc = run_depths(s, wa, qa, rays_a, qs, rs, N, st);

@@ -110,6 +110,7 @@ struct SceneView {
 struct psdr_scene {
   int n_vertices = 0, n_tris = 0, n_objects = 0, n_lights = 0, n_texels = 0, n_edges = 0;
   std::vector<psdr_object> objects;
+  std::vector<psdr::ObjectK> objs_host;  // source of the last object-table upload
   std::vector<int> light_obj;
   // caller-owned device buffers of the last create/update
   const float* vertices = nullptr;
@@ -121,7 +122,8 @@ struct psdr_scene {
   psdr::DevBuf tri_geom, edge_geom, edge_pmf, edge_cmf, light_tri_pmf, light_tri_cmf, light_power, light_area,
       lights_total, obj_pmf, obj_cmf, scan_tmp;
   // LBVH
-  psdr::DevBuf morton, order, child, parent, box, leaf, node, counters, sort_keys, sort_vals, sort_hist, bounds;
+  psdr::DevBuf morton, order, child, parent, box, leaf, node, counters, sort_keys, sort_vals, sort_hist, bounds,
+      ranges, bvh_boxes;
   // renderer scratch
   psdr::DevBuf lane_L, q_a, q_b, rays, hits, rays2, hits2, spill, hit_cache, counters_r, hc_mass, hc_pmf,
       hc_cmf, stats;
@@ -131,6 +133,11 @@ struct psdr_scene {
   // direct-boundary scratch (separate from the primary pass's, so both can run at once)
   psdr::DevBuf d_state, d_mass, d_pmf, d_cmf, d_tmp, d_spill, d_keys, d_vals, d_hist;
   psdr::AuxStreams aux;
+  // Reference launch shape, set for the duration of a *_plain call: every
+  // depth's kernels run over all lanes (no compaction), each ray kind is traced
+  // by its own launch followed by a host sync, boundary samples are processed in
+  // lane order and the gradient terms run one after another.
+  bool plain = false;
   int light_tri_total = 0;
   // last forward (for backward / stats)
   uint64_t last_N = 0;

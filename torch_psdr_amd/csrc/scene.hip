@@ -449,7 +449,8 @@ __device__ __forceinline__ int kdelta(const uint32_t* ms, int n, int i, int j) {
   return __clzll(static_cast<long long>(a ^ b));
 }
 
-__global__ void k_karras(const uint32_t* __restrict__ ms, int n, int* __restrict__ child, int* __restrict__ parent) {
+__global__ void k_karras(const uint32_t* __restrict__ ms, int n, int* __restrict__ child, int* __restrict__ parent,
+                         int2* __restrict__ range) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n - 1) return;
   const int d = (kdelta(ms, n, i, i + 1) - kdelta(ms, n, i, i - 1)) >= 0 ? 1 : -1;
@@ -472,17 +473,27 @@ __global__ void k_karras(const uint32_t* __restrict__ ms, int n, int* __restrict
   const int right = (max(i, j) == g + 1) ? ((g + 1) | static_cast<int>(0x80000000u)) : g + 1;
   child[2 * i] = left;
   child[2 * i + 1] = right;
+  range[i] = make_int2(min(i, j), max(i, j));
   parent[left < 0 ? (n - 1 + (left & 0x7fffffff)) : left] = i;
   parent[right < 0 ? (n - 1 + (right & 0x7fffffff)) : right] = i;
   if (i == 0) parent[0] = -1;
 }
 
-__global__ void k_leaf(const float4* __restrict__ geom, const uint32_t* __restrict__ order, int n,
-                       float4* __restrict__ leaf, int* __restrict__ counters) {
+// Leaf records in sorted order, the sorted order itself and each leaf's padded box.
+__global__ void k_leaf(const float4* __restrict__ geom, const uint32_t* __restrict__ sorted, int n,
+                       float4* __restrict__ leaf, uint32_t* __restrict__ order, float* __restrict__ lbox) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < n - 1) counters[k] = 0;
   if (k >= n) return;
-  const int f = order ? static_cast<int>(order[k]) : k;
+  const int f = sorted ? static_cast<int>(sorted[k]) : k;
+  if (order) order[k] = static_cast<uint32_t>(f);
+  if (lbox) {
+    float lo[3], hi[3];
+    tri_box(geom, f, lo, hi);
+    for (int a = 0; a < 3; ++a) {
+      lbox[6 * k + a] = lo[a];
+      lbox[6 * k + 3 + a] = hi[a];
+    }
+  }
   const float4 a = geom[5 * f], b = geom[5 * f + 1], c = geom[5 * f + 2];
   const f3 p0 = xyz(a), p1 = xyz(b), p2 = xyz(c);
   const f3 e1 = p1 - p0, e2 = p2 - p0;
@@ -491,41 +502,79 @@ __global__ void k_leaf(const float4* __restrict__ geom, const uint32_t* __restri
   leaf[3 * k + 2] = make_float4(e2.x, e2.y, e2.z, 0.f);
 }
 
-__global__ void k_refit(const float4* __restrict__ geom, const uint32_t* __restrict__ order, int n,
-                        const int* __restrict__ child, const int* __restrict__ parent, float* box,
-                        int* counters) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  float lo[3], hi[3];
-  tri_box(geom, static_cast<int>(order[k]), lo, hi);
-  int c = k | static_cast<int>(0x80000000u);
-  int p = parent[n - 1 + k];
-  while (p >= 0) {
-    const int slot = (child[2 * p] == c) ? 0 : 1;
-    float* b = box + 12 * p + 6 * slot;
-    for (int a = 0; a < 3; ++a) {
-      __hip_atomic_store(b + a, lo[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(b + 3 + a, hi[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    __threadfence();
-    const int old = atomicAdd(counters + p, 1);
-    if (old == 0) return;
-    __threadfence();
-    const float* sb = box + 12 * p + 6 * (1 - slot);
-    for (int a = 0; a < 3; ++a) {
-      lo[a] = fminf(lo[a], __hip_atomic_load(sb + a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-      hi[a] = fmaxf(hi[a], __hip_atomic_load(sb + 3 + a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-    }
-    c = p;
-    p = parent[p];
+// Boxes of internal nodes without a bottom-up pass: internal node i covers
+// the sorted leaves range[i] = [first, last], and its box is the union of those
+// leaves' boxes.  min/max are exact, so any grouping gives the same bits.  A
+// wave reduces a node's range from 64-leaf chunk boxes plus the partial chunks
+// at its ends; no inter-workgroup communication is needed.
+__device__ __forceinline__ void box_acc(float lo[3], float hi[3], const float* b) {
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = fminf(lo[a], b[a]);
+    hi[a] = fmaxf(hi[a], b[3 + a]);
   }
 }
+__device__ __forceinline__ void wave_box_reduce(float lo[3], float hi[3]) {
+  for (int m = 32; m > 0; m >>= 1)
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = fminf(lo[a], __shfl_xor(lo[a], m, 64));
+      hi[a] = fmaxf(hi[a], __shfl_xor(hi[a], m, 64));
+    }
+}
 
-__global__ void k_pack_nodes(const float* __restrict__ box, const int* __restrict__ child, int n,
+// one wave per 64-leaf chunk
+__global__ __launch_bounds__(256) void k_chunk_boxes(const float* __restrict__ lbox, int n, float* __restrict__ cbox) {
+  const int c = static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int lane = static_cast<int>(threadIdx.x & 63);
+  if (c * 64 >= n) return;  // whole wave leaves together
+  float lo[3] = {F_INF, F_INF, F_INF}, hi[3] = {-F_INF, -F_INF, -F_INF};
+  const int k = c * 64 + lane;
+  if (k < n) box_acc(lo, hi, lbox + 6 * k);
+  wave_box_reduce(lo, hi);
+  if (lane == 0)
+    for (int a = 0; a < 3; ++a) {
+      cbox[6 * c + a] = lo[a];
+      cbox[6 * c + 3 + a] = hi[a];
+    }
+}
+
+// one wave per internal node
+__global__ __launch_bounds__(256) void k_node_boxes(const float* __restrict__ lbox, const float* __restrict__ cbox,
+                                                    const int2* __restrict__ range, int n, float* __restrict__ nbox) {
+  const int i = static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int lane = static_cast<int>(threadIdx.x & 63);
+  if (i >= n - 1) return;  // whole wave leaves together
+  const int2 r = range[i];
+  const int first = r.x, last = r.y;
+  float lo[3] = {F_INF, F_INF, F_INF}, hi[3] = {-F_INF, -F_INF, -F_INF};
+  const int c0 = (first + 63) >> 6, c1 = (last + 1) >> 6;  // full chunks [c0, c1)
+  if (c0 >= c1) {
+    for (int k = first + lane; k <= last; k += 64) box_acc(lo, hi, lbox + 6 * k);
+  } else {
+    for (int k = first + lane; k < 64 * c0; k += 64) box_acc(lo, hi, lbox + 6 * k);
+    for (int c = c0 + lane; c < c1; c += 64) box_acc(lo, hi, cbox + 6 * c);
+    for (int k = 64 * c1 + lane; k <= last; k += 64) box_acc(lo, hi, lbox + 6 * k);
+  }
+  wave_box_reduce(lo, hi);
+  if (lane == 0)
+    for (int a = 0; a < 3; ++a) {
+      nbox[6 * i + a] = lo[a];
+      nbox[6 * i + 3 + a] = hi[a];
+    }
+}
+
+// node i = {left box, right box, child ids}; box keeps the same 12 floats
+__global__ void k_pack_nodes(const float* __restrict__ lbox, const float* __restrict__ nbox,
+                             const int* __restrict__ child, int n, float* __restrict__ box,
                              float4* __restrict__ node) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n - 1) return;
-  const float* b = box + 12 * i;
+  float b[12];
+  for (int s = 0; s < 2; ++s) {
+    const int c = child[2 * i + s];
+    const float* src = c < 0 ? lbox + 6 * (c & 0x7fffffff) : nbox + 6 * c;
+    for (int a = 0; a < 6; ++a) b[6 * s + a] = src[a];
+  }
+  for (int a = 0; a < 12; ++a) box[12 * i + a] = b[a];
   node[4 * i] = make_float4(b[0], b[1], b[2], b[3]);
   node[4 * i + 1] = make_float4(b[4], b[5], b[6], b[7]);
   node[4 * i + 2] = make_float4(b[8], b[9], b[10], b[11]);
@@ -673,7 +722,7 @@ size_t radix_hist_words(int n) {
 
 int exclusive_scan_u32(uint32_t* a, int m, uint32_t* part, hipStream_t st) {
   const int nb = blocks(m, XS_BLOCK);
-  if (nb <= 1) {
+  if (nb <= 16) {  // up to 32k words: one block is cheaper than three launches
     hipLaunchKernelGGL(k_scan_u32, dim3(1), dim3(1024), 0, st, a, m);
   } else {
     hipLaunchKernelGGL(k_xscan_sums, dim3(nb), dim3(XS_THREADS), 0, st, a, m, part);
@@ -706,7 +755,7 @@ int build_lbvh(psdr_scene* s, hipStream_t st) {
   if (n < 2) {
     if (n == 1) {
       hipLaunchKernelGGL(k_leaf, dim3(1), dim3(64), 0, st, s->tri_geom.as<float4>(), nullptr, 1,
-                         s->leaf.as<float4>(), s->counters.as<int>());
+                         s->leaf.as<float4>(), nullptr, nullptr);
       PSDR_HIP_CHECK(hipGetLastError());
     }
     return PSDR_OK;
@@ -725,16 +774,19 @@ int build_lbvh(psdr_scene* s, hipStream_t st) {
   const int rc = radix_sort_pairs(k0, v0, k1, v1, s->sort_hist.as<uint32_t>(), n, 4, st);
   if (rc) return rc;
   // after an even number of passes the sorted data is back in k0/v0 (= buffer start)
-  PSDR_HIP_CHECK(hipMemcpyAsync(s->order.p, v0, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, st));
+  float* lbox = s->bvh_boxes.as<float>();
+  float* nbox = lbox + 6ull * n;
+  float* cbox = nbox + 6ull * (n - 1);
+  int2* range = s->ranges.as<int2>();
   hipLaunchKernelGGL(k_karras, dim3(blocks(n - 1, 256)), dim3(256), 0, st, k0, n, s->child.as<int>(),
-                     s->parent.as<int>());
-  hipLaunchKernelGGL(k_leaf, dim3(blocks(n, 256)), dim3(256), 0, st, s->tri_geom.as<float4>(),
-                     s->order.as<uint32_t>(), n, s->leaf.as<float4>(), s->counters.as<int>());
-  hipLaunchKernelGGL(k_refit, dim3(blocks(n, 256)), dim3(256), 0, st, s->tri_geom.as<float4>(),
-                     s->order.as<uint32_t>(), n, s->child.as<int>(), s->parent.as<int>(), s->box.as<float>(),
-                     s->counters.as<int>());
-  hipLaunchKernelGGL(k_pack_nodes, dim3(blocks(n - 1, 256)), dim3(256), 0, st, s->box.as<float>(),
-                     s->child.as<int>(), n, s->node.as<float4>());
+                     s->parent.as<int>(), range);
+  hipLaunchKernelGGL(k_leaf, dim3(blocks(n, 256)), dim3(256), 0, st, s->tri_geom.as<float4>(), v0, n,
+                     s->leaf.as<float4>(), s->order.as<uint32_t>(), lbox);
+  const int chunks = (n + 63) / 64;
+  hipLaunchKernelGGL(k_chunk_boxes, dim3(blocks(64ll * chunks, 256)), dim3(256), 0, st, lbox, n, cbox);
+  hipLaunchKernelGGL(k_node_boxes, dim3(blocks(64ll * (n - 1), 256)), dim3(256), 0, st, lbox, cbox, range, n, nbox);
+  hipLaunchKernelGGL(k_pack_nodes, dim3(blocks(n - 1, 256)), dim3(256), 0, st, lbox, nbox, s->child.as<int>(), n,
+                     s->box.as<float>(), s->node.as<float4>());
   PSDR_HIP_CHECK(hipGetLastError());
   return PSDR_OK;
 }

@@ -27,6 +27,35 @@ _ids = itertools.count(1)
 _entries: Dict[int, Tuple["NativeScene", FlatScene]] = {}
 _by_topo: Dict[tuple, "NativeScene"] = {}
 _MAX_CACHED = 8
+# "fused" (default) or "plain": the reference's launch shape, used as the
+# measured baseline (psdr_render_forward_plain / psdr_render_backward_plain).
+# Process-wide, because autograd runs the backward on its own thread.
+_shape = ["fused"]
+
+
+class launch_shape:
+    """Context manager selecting the native launch shape of renders (and their
+    backward passes) issued inside it."""
+
+    def __init__(self, name: str):
+        if name not in ("fused", "plain"):
+            raise ValueError(f"unknown launch shape {name!r}")
+        self.name = name
+
+    def __enter__(self):
+        with _lock:
+            self.prev = _shape[0]
+            _shape[0] = self.name
+        return self
+
+    def __exit__(self, *exc):
+        with _lock:
+            _shape[0] = self.prev
+        return False
+
+
+def _plain() -> bool:
+    return _shape[0] == "plain"
 
 
 class NativeScene:
@@ -119,8 +148,8 @@ def forward_native(ns: NativeScene, flat: FlatScene, cam: Sequence[float], cfg: 
     H, W = int(cam[10]), int(cam[11])
     out = torch.empty((H, W, 3), dtype=torch.float32, device=flat.vertices.device)
     c, k = make_config(cfg), make_camera(cam)
-    N.check(ns.lib.psdr_render_forward(ns.handle, N.C.byref(c), N.C.byref(k), out.data_ptr(), int(keep),
-                                       N.current_stream()))
+    fn = ns.lib.psdr_render_forward_plain if _plain() else ns.lib.psdr_render_forward
+    N.check(fn(ns.handle, N.C.byref(c), N.C.byref(k), out.data_ptr(), int(keep), N.current_stream()))
     ns.last_forward = (tuple(cam), tuple(cfg), bool(keep))
     return out
 
@@ -149,8 +178,9 @@ def render_backward(token: int, cam: List[float], cfg: List[int], grad: torch.Te
     ge = torch.zeros((max(int(emit.shape[0]), 1), 3), dtype=torch.float32, device=emit.device)
     c, k = make_config(cfg), make_camera(cam)
     g = grad.contiguous().to(torch.float32)
-    N.check(ns.lib.psdr_render_backward(ns.handle, N.C.byref(c), N.C.byref(k), g.data_ptr(), gv.data_ptr(),
-                                        gt.data_ptr(), ge.data_ptr(), N.PSDR_TERM_ALL, N.current_stream()))
+    fn = ns.lib.psdr_render_backward_plain if _plain() else ns.lib.psdr_render_backward
+    N.check(fn(ns.handle, N.C.byref(c), N.C.byref(k), g.data_ptr(), gv.data_ptr(), gt.data_ptr(), ge.data_ptr(),
+               N.PSDR_TERM_ALL, N.current_stream()))
     return [gv, gt, ge[: emit.shape[0]].clone()]
 
 
