@@ -18,6 +18,7 @@
 // block, flushed with one global atomic per entry.
 #include <hip/hip_runtime.h>
 
+#include "grad_cache.h"
 #include "render.h"
 #include "shading.h"
 
@@ -27,6 +28,12 @@ namespace {
 
 constexpr int ADJ_THREADS = 256;
 constexpr int HOT_MAX = 1024;  // LDS accumulators for emission + constant textures
+// LDS aggregation of vertex (array 0) and RGB-texel (array 1) gradients of one
+// workgroup's contiguous lane range (only k_adjoint uses these arrays)
+constexpr int ADJ_SLOTS = 1024;
+__shared__ int adj_keys[ADJ_SLOTS];
+__shared__ float adj_vals[3 * ADJ_SLOTS];
+__device__ __forceinline__ LdsGradCache<ADJ_SLOTS> adj_cache() { return LdsGradCache<ADJ_SLOTS>{adj_keys, adj_vals}; }
 
 struct AdjParams {
   SceneView sv;
@@ -120,13 +127,9 @@ __device__ void tri_vjp(const AdjParams& P, int tri, float u, float v, float fli
   gP0 = gP0 - gc;
   const int* f = sv.faces + 3 * tri;
   const f3 gs[3] = {gP0, gP1, gP2};
+  LdsGradCache<ADJ_SLOTS> cache = adj_cache();
 #pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    float* dst = P.grad_v + 3 * static_cast<int64_t>(f[k]);
-    if (gs[k].x != 0.f) atomicAdd(dst, gs[k].x);
-    if (gs[k].y != 0.f) atomicAdd(dst + 1, gs[k].y);
-    if (gs[k].z != 0.f) atomicAdd(dst + 2, gs[k].z);
-  }
+  for (int k = 0; k < 3; ++k) cache.add(P.grad_v, 0, 3 * static_cast<int64_t>(f[k]), gs[k]);
 }
 
 // flip sign of a reconstructed intersection: its.n = flip * record normal
@@ -197,9 +200,7 @@ __device__ BsdfAdj bsdf_vjp(const AdjParams& P, float* hot, const Its& it, f3 L,
       acc_hot(hot, P.hot_n, P.grad_t, 3 * sv.n_lights + 4 * it.obj + 1, co + 1, gcol.y);
       acc_hot(hot, P.hot_n, P.grad_t, 3 * sv.n_lights + 4 * it.obj + 2, co + 2, gcol.z);
     } else {
-      atomicAdd(P.grad_t + co, gcol.x);
-      atomicAdd(P.grad_t + co + 1, gcol.y);
-      atomicAdd(P.grad_t + co + 2, gcol.z);
+      adj_cache().add(P.grad_t, 1, co, gcol);
     }
   }
   return r;
@@ -420,10 +421,14 @@ template <int DMAX>
 __global__ __launch_bounds__(ADJ_THREADS) void k_adjoint(AdjParams P) {
   __shared__ float hot[HOT_MAX];
   for (int i = threadIdx.x; i < P.hot_n; i += blockDim.x) hot[i] = 0.f;
+  adj_cache().init();
   __syncthreads();
   const SceneView& sv = P.sv;
-  for (int64_t lane = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; lane < P.N;
-       lane += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+  // each workgroup replays a contiguous lane range: neighbouring pixels share
+  // triangles and texels, so their scatters aggregate in LDS
+  const int64_t per = ((P.N + gridDim.x - 1) / gridDim.x + ADJ_THREADS - 1) / ADJ_THREADS * ADJ_THREADS;
+  const int64_t lane_end = min(P.N, static_cast<int64_t>(blockIdx.x) * per + per);
+  for (int64_t lane = static_cast<int64_t>(blockIdx.x) * per + threadIdx.x; lane < lane_end; lane += ADJ_THREADS) {
     const int64_t pix = P.spp > 1 ? lane / P.spp : lane;
     f3 g = f3{P.dLdI[3 * pix], P.dLdI[3 * pix + 1], P.dLdI[3 * pix + 2]};
     if (P.spp > 1) g = g / static_cast<float>(P.spp);
@@ -505,6 +510,7 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adjoint(AdjParams P) {
     tri_vjp(P, vt[0], vu[0], vv[0], flip_of(sv, its), pend);
   }
   __syncthreads();
+  adj_cache().flush(P.grad_v, P.grad_t);
   for (int i = threadIdx.x; i < P.hot_n; i += blockDim.x) {
     const float v = hot[i];
     if (v == 0.f) continue;

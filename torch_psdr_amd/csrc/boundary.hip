@@ -25,6 +25,7 @@
 
 #include <algorithm>
 
+#include "grad_cache.h"
 #include "render.h"
 #include "shading.h"
 #include "traverse.h"
@@ -59,17 +60,16 @@ __device__ __forceinline__ EdgeRec load_edge(const SceneView& sv, int e) {
   return E;
 }
 
-__device__ __forceinline__ void scatter_edge(const SceneView& sv, float* grad_v, const EdgeRec& E, float t, f3 g) {
+// boundary samples aggregate their edge-vertex gradients per workgroup
+constexpr int EDGE_SLOTS = 512;
+using EdgeCache = LdsGradCache<EDGE_SLOTS>;
+
+__device__ __forceinline__ void scatter_edge(const SceneView& sv, float* grad_v, const EdgeRec& E, float t, f3 g,
+                                             EdgeCache* gc) {
   if (!(sv.objs[E.obj].grad_mask & PSDR_GRAD_VERTICES)) return;
   const f3 g0 = g * (1.0f - t), g1 = g * t;
-  float* d0 = grad_v + 3 * static_cast<int64_t>(E.v0);
-  float* d1 = grad_v + 3 * static_cast<int64_t>(E.v1);
-  atomicAdd(d0, g0.x);
-  atomicAdd(d0 + 1, g0.y);
-  atomicAdd(d0 + 2, g0.z);
-  atomicAdd(d1, g1.x);
-  atomicAdd(d1 + 1, g1.y);
-  atomicAdd(d1 + 2, g1.z);
+  gc->add(grad_v, 0, 3 * static_cast<int64_t>(E.v0), g0);
+  gc->add(grad_v, 0, 3 * static_cast<int64_t>(E.v1), g1);
 }
 
 // ---------------------------------------------------------------- primary
@@ -206,12 +206,9 @@ __global__ void k_prim_requeue(int n, QueueView q0, const uint64_t* __restrict__
 }
 
 template <bool PRE>
-__global__ void k_prim_finish(SceneView sv, CameraK cam, int n, PrimRecs rec, const float* __restrict__ LA,
-                              const float* __restrict__ LB, const uint64_t* __restrict__ rng_b,
-                              uint64_t* __restrict__ state, float* __restrict__ mass, const float* __restrict__ dLdI,
-                              uint32_t spp, float* __restrict__ grad_v) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+__device__ __forceinline__ void prim_finish_one(const SceneView& sv, int i, PrimRecs& rec, const float* LA,
+                                                const float* LB, const uint64_t* rng_b, uint64_t* state, float* mass,
+                                                const float* dLdI, uint32_t spp, float* grad_v, EdgeCache* gc) {
   if (PRE) state[i] = rng_b[i];
   const int pix = rec.pix[i];
   if (pix < 0) return;
@@ -233,7 +230,32 @@ __global__ void k_prim_finish(SceneView sv, CameraK cam, int n, PrimRecs rec, co
   if (gx == 0.f) return;
   const f3 nn = f3{rec.n[i], rec.n[rec.cap + i], rec.n[2 * rec.cap + i]};
   const EdgeRec E = load_edge(sv, rec.edge[i]);
-  scatter_edge(sv, grad_v, E, rec.t[i], nn * gx);
+  scatter_edge(sv, grad_v, E, rec.t[i], nn * gx, gc);
+}
+
+// Position p handles lane order[p] (the processing order of the pass), so a
+// workgroup's samples are neighbours and share edges.
+template <bool PRE>
+__global__ __launch_bounds__(BT) void k_prim_finish(SceneView sv, CameraK cam, int n, const uint32_t* __restrict__ order,
+                                                    PrimRecs rec, const float* __restrict__ LA,
+                                                    const float* __restrict__ LB, const uint64_t* __restrict__ rng_b,
+                                                    uint64_t* __restrict__ state, float* __restrict__ mass,
+                                                    const float* __restrict__ dLdI, uint32_t spp,
+                                                    float* __restrict__ grad_v) {
+  __shared__ int gk[PRE ? 1 : EDGE_SLOTS];
+  __shared__ float gvals[PRE ? 1 : 3 * EDGE_SLOTS];
+  EdgeCache gc{gk, gvals};
+  if (!PRE) {
+    gc.init();
+    __syncthreads();
+  }
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n)
+    prim_finish_one<PRE>(sv, static_cast<int>(order[p]), rec, LA, LB, rng_b, state, mass, dLdI, spp, grad_v, &gc);
+  if (!PRE) {
+    __syncthreads();
+    gc.flush(grad_v, grad_v);
+  }
 }
 
 // ---------------------------------------------------------------- direct
@@ -254,7 +276,8 @@ struct DirectOut {
 // is dLdI[pixel] * gscale and is pushed into the edge's vertices.
 template <bool PRE>
 __device__ DirectOut eval_direct(const SceneView& sv, const CameraK& cam, float sx, float sy, float sz, int* stk,
-                                 int* spill, int spill_stride, const float* dLdI, float gscale, float* grad_v) {
+                                 int* spill, int spill_stride, const float* dLdI, float gscale, float* grad_v,
+                                 EdgeCache* gc) {
   DirectOut r;
   r.valid = false;
   r.pix = -1;
@@ -337,7 +360,7 @@ __device__ DirectOut eval_direct(const SceneView& sv, const CameraK& cam, float 
   if (!(pe < REAL_EPS) && !(pe > 1.0f - REAL_EPS)) gP = gP + E1 * (-ginv * inv * inv);
   gD = gD + cross(E2, gP);
   const f3 gxb = vjp_norm(xb - xs, gD);
-  scatter_edge(sv, grad_v, E, ue, gxb);
+  scatter_edge(sv, grad_v, E, ue, gxb, gc);
   return r;
 }
 
@@ -349,12 +372,23 @@ __global__ __launch_bounds__(BT) void k_direct(SceneView sv, CameraK cam, int n,
                                                float* __restrict__ grad_v, int* __restrict__ spill,
                                                const uint32_t* __restrict__ order) {
   __shared__ int stk[BSTACK * BT];
+  __shared__ int gk[PRE ? 1 : EDGE_SLOTS];
+  __shared__ float gvals[PRE ? 1 : 3 * EDGE_SLOTS];
+  EdgeCache gc{gk, gvals};
+  if (!PRE) {
+    gc.init();
+    __syncthreads();
+  }
   const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
   const int stride = gridDim.x * blockDim.x;
   const int cells = r0 * r1 * r2;
   const float u0 = 1.0f / static_cast<float>(r0), u1 = 1.0f / static_cast<float>(r1),
               u2 = 1.0f / static_cast<float>(r2);
-  for (int p = gtid; p < n; p += stride) {
+  // each workgroup takes a contiguous range of positions (neighbouring samples
+  // in the processing order share edges, so their gradients aggregate in LDS)
+  const int per = ((n + static_cast<int>(gridDim.x) - 1) / static_cast<int>(gridDim.x) + BT - 1) / BT * BT;
+  const int p_end = min(n, static_cast<int>(blockIdx.x) * per + per);
+  for (int p = static_cast<int>(blockIdx.x) * per + static_cast<int>(threadIdx.x); p < p_end; p += BT) {
     const int i = order ? static_cast<int>(order[p]) : p;
     Rng rng;
     float s[3];
@@ -378,7 +412,8 @@ __global__ __launch_bounds__(BT) void k_direct(SceneView sv, CameraK cam, int n,
       s[2] = (s[2] + static_cast<float>(c2)) * u2;
     }
     if (PRE) {
-      DirectOut o = eval_direct<true>(sv, cam, s[0], s[1], s[2], stk, spill + gtid, stride, nullptr, 0.f, nullptr);
+      DirectOut o =
+          eval_direct<true>(sv, cam, s[0], s[1], s[2], stk, spill + gtid, stride, nullptr, 0.f, nullptr, nullptr);
       if (o.valid) {
         f3 v = o.value;
         v.x = v.x < REAL_EPS ? REAL_EPS : v.x;
@@ -389,8 +424,12 @@ __global__ __launch_bounds__(BT) void k_direct(SceneView sv, CameraK cam, int n,
     } else {
       float gscale = 1.0f / pdf_cube;
       if (spp > 1) gscale = gscale / static_cast<float>(spp);
-      eval_direct<false>(sv, cam, s[0], s[1], s[2], stk, spill + gtid, stride, dLdI, gscale, grad_v);
+      eval_direct<false>(sv, cam, s[0], s[1], s[2], stk, spill + gtid, stride, dLdI, gscale, grad_v, &gc);
     }
+  }
+  if (!PRE) {
+    __syncthreads();
+    gc.flush(grad_v, grad_v);
   }
 }
 
@@ -512,11 +551,11 @@ static int primary_pass(psdr_scene* s, const psdr_config& cfg, const CameraK& ca
   rc = run_depths(s, wa, q0, offset_rays(r0, n), qs, rs, n, st);
   if (rc) return rc;
   if (pre)
-    hipLaunchKernelGGL(k_prim_finish<true>, dim3(blocks(n, BT)), dim3(BT), 0, st, sv, cam, n, rec,
+    hipLaunchKernelGGL(k_prim_finish<true>, dim3(blocks(n, BT)), dim3(BT), 0, st, sv, cam, n, q0.lane, rec,
                        s->b_LA.as<float>(), s->b_LB.as<float>(), s->b_rngB.as<uint64_t>(), state, mass, dLdI,
                        cfg.spp_primary_edge, grad_v);
   else
-    hipLaunchKernelGGL(k_prim_finish<false>, dim3(blocks(n, BT)), dim3(BT), 0, st, sv, cam, n, rec,
+    hipLaunchKernelGGL(k_prim_finish<false>, dim3(blocks(n, BT)), dim3(BT), 0, st, sv, cam, n, q0.lane, rec,
                        s->b_LA.as<float>(), s->b_LB.as<float>(), s->b_rngB.as<uint64_t>(), state, mass, dLdI,
                        cfg.spp_primary_edge, grad_v);
   PSDR_HIP_CHECK(hipGetLastError());
