@@ -115,6 +115,11 @@ def set_threads(n: int):
     load().oracle_set_threads(int(n))
 
 
+def set_fd_measure(on: bool):
+    """Boundary samples in the raster measure of a pixel mean (for finite-difference checks)."""
+    load().oracle_set_fd_measure(int(bool(on)))
+
+
 def pcg_kat(initstate: int, initseq: int, n: int) -> np.ndarray:
     out = np.zeros(n, dtype=np.uint32)
     load().oracle_pcg_kat(C.c_uint64(initstate), C.c_uint64(initseq), n, _p(out))

@@ -574,6 +574,12 @@ struct BoundaryOut {
   bool valid = false;
 };
 
+// Test-only switch: rescale each boundary sample from the reference's
+// normalisation (world measure, samples summed per pixel and divided by the
+// edge spp) to the raster measure of a pixel mean (divided by all spp*H*W
+// samples), so the boundary terms can be compared with finite differences.
+bool g_fd_measure = false;
+
 // eval_primary_boundary (src/path_integrator.cc:202-251). The camera is not a
 // differentiated input here, so the camera-origin term is zero.
 template <class F>
@@ -604,6 +610,8 @@ V3<F> eval_primary(const Scene<F>& sc, const Scene<float>& scv, const Camera& ca
   if (ad) {
     const F xn = dot(ep - lift<F>(epv), lift<F>(n));
     value = (lift<F>(dL) * xn) / F(detach(pdf_ep));
+    if (g_fd_measure && valid)
+      value = value * F(static_cast<float>(cam.raster_jacobian(epv, d) / (static_cast<double>(cam.H) * cam.W)));
   } else {
     value = lift<F>(dL) / pdf_ep;
   }
@@ -651,7 +659,13 @@ V3<F> eval_direct(const Scene<F>& sc, const Camera& cam, float sx, float sy, flo
   const V3<float> Ts = detach(sc.lights[is0.light].emit);
   Its<F> id0c = id0;
   const V3<float> Td = detach(sc.eval(id0c, lift<F>(-cd)));
-  const V3<F> value = (Tb * lift<F>(Ts)) * lift<F>(Td);
+  V3<F> value = (Tb * lift<F>(Ts)) * lift<F>(Td);
+  if (g_fd_measure && ad) {
+    // receiver area -> area perpendicular to the camera ray -> raster
+    const double cosr = std::fabs(static_cast<double>(dot(detach(id0.n), cd)));
+    value = value * F(static_cast<float>(cam.raster_jacobian(xd, cd) * cosr /
+                                         (static_cast<double>(cam.H) * cam.W)));
+  }
   valid = valid && std::isfinite(val(squared_norm(value)));
   bo->valid = valid;
   return value;
@@ -1135,6 +1149,8 @@ int oracle_lbvh(const float* tri9, int n, uint32_t* morton, int* order, int* chi
 // forward-mode tangent given in the scene's d_* arrays; the tangent image is
 // split into interior / primary-boundary / direct-boundary parts (any may be
 // NULL to skip that estimator).  lane_L (optional): per-lane interior radiance.
+void oracle_set_fd_measure(int on) { g_fd_measure = on != 0; }
+
 int oracle_render(const oracle_scene* s, const oracle_config* cfg, const oracle_camera* c, int mode, float* img,
                   float* d_int, float* d_pri, float* d_dir, float* lane_L) {
   const Camera cam = make_camera(c);

@@ -435,6 +435,38 @@ struct Camera {
     *px = x;
     *py = y;
   }
+  // Raster area per unit world area of a small patch at x perpendicular to the
+  // unit direction n (finite differences of the projection in double).  Not part
+  // of the reference: it converts a boundary sample's world-measure value into
+  // the raster measure of a pixel mean for the finite-difference checks.
+  double raster_jacobian(const V3<float>& x, const V3<float>& n) const {
+    auto proj = [&](double X, double Y, double Z, double* rx, double* ry) {
+      double c[4], r[4];
+      for (int i = 0; i < 4; ++i)
+        c[i] = w2c.m[i][0] * X + w2c.m[i][1] * Y + w2c.m[i][2] * Z + w2c.m[i][3];
+      for (int i = 0; i < 3; ++i) c[i] /= c[3];
+      for (int i = 0; i < 4; ++i) r[i] = c2r.m[i][0] * c[0] + c2r.m[i][1] * c[1] + c2r.m[i][2] * c[2] + c2r.m[i][3];
+      *rx = r[0] / r[3];
+      *ry = r[1] / r[3];
+    };
+    // orthonormal a, b perpendicular to n
+    double nx = n.x, ny = n.y, nz = n.z;
+    const double nl = std::sqrt(nx * nx + ny * ny + nz * nz);
+    nx /= nl; ny /= nl; nz /= nl;
+    double ax = 0, ay = 0, az = 0;
+    if (std::fabs(nx) < 0.9) ax = 1; else ay = 1;
+    const double d = ax * nx + ay * ny + az * nz;
+    ax -= d * nx; ay -= d * ny; az -= d * nz;
+    const double al = std::sqrt(ax * ax + ay * ay + az * az);
+    ax /= al; ay /= al; az /= al;
+    const double bx = ny * az - nz * ay, by = nz * ax - nx * az, bz = nx * ay - ny * ax;
+    const double h = 1e-4;
+    double r0x, r0y, rax, ray, rbx, rby;
+    proj(x.x, x.y, x.z, &r0x, &r0y);
+    proj(x.x + h * ax, x.y + h * ay, x.z + h * az, &rax, &ray);
+    proj(x.x + h * bx, x.y + h * by, x.z + h * bz, &rbx, &rby);
+    return std::fabs((rax - r0x) * (rby - r0y) - (ray - r0y) * (rbx - r0x)) / (h * h);
+  }
   // gen_camera_ray_with_point_ (src/camera.h:260-276)
   bool ray_with_point(const V3<float>& pt, int* px, int* py) const {
     const V3<float> pc = xform_pos(w2c, pt);

@@ -372,3 +372,53 @@ def test_vertex_jvp_matches_lambert_derivative(oracle):
     assert okp.sum() > 100
     rel = np.abs(got[okp].mean() - want_pix[okp].mean()) / np.abs(want_pix[okp].mean())
     assert rel < 0.02, rel
+
+
+def _flat_quad_scene(quad_vertices):
+    """A front-facing unit quad (albedo 0.8) before a back wall (albedo 0.3),
+    lit by a tiny bright light in front of the quad, so the quad's shadow falls
+    behind it and its radiance is almost constant: the only visible effect of
+    moving the quad is its silhouette moving over the wall."""
+    import torch_psdr_amd as dr
+    from torch_psdr_amd.flat import flatten
+    cpu = torch.device("cpu")
+    Fq = torch.tensor([[0, 1, 2], [0, 2, 3]], dtype=torch.int32)
+    Vw = torch.tensor([[-10, -10, -3], [10, -10, -3], [10, 10, -3], [-10, 10, -3]], dtype=torch.float32)
+    h = 0.05
+    Vl = torch.tensor([[-h, -h, 20], [h, -h, 20], [h, h, 20], [-h, h, 20]], dtype=torch.float32)
+    quad = dr.Mesh(torch.tensor(quad_vertices), None, Fq, None, dr.DiffuseMaterial(torch.tensor([0.8, 0.8, 0.8])))
+    wall = dr.Mesh(Vw, None, Fq, None, dr.DiffuseMaterial(torch.tensor([0.3, 0.3, 0.3])))
+    light = dr.AreaLight(Vl, torch.tensor([[0, 2, 1], [0, 3, 2]], dtype=torch.int32), torch.tensor([3e5, 3e5, 3e5]))
+    return flatten([quad, wall], [light], cpu)
+
+
+@pytest.mark.parametrize("axis", [0, 1])
+def test_primary_boundary_matches_finite_differences(oracle, axis):
+    """The primary-edge estimator (src/path_integrator.cc:188-251), once each
+    sample is taken from the reference's normalisation (world-measure edge
+    integral, samples summed per pixel over spp_edge, hypercube cell pmf used as
+    a density) to the raster measure of a pixel mean (x raster Jacobian at the
+    edge point / (H*W*cells)), equals the central difference of the rendered
+    image for a translated quad.  This pins edge sampling, the hypercube
+    sampler, the edge-point projection, both side hits and the radiance
+    difference; the normalisation itself is the reference's."""
+    res, spp, spe, eps = 64, 64, 16, 0.01
+    Vq = np.array([[-0.5, -0.5, 0], [0.5, -0.5, 0], [0.5, 0.5, 0], [-0.5, 0.5, 0]], np.float32)
+    flat = _flat_quad_scene(Vq)
+    w = np.zeros(tuple(flat.vertices.shape), np.float32)
+    w[:4, axis] = 1.0
+    cam = [0.0, 0.0, 5.0, 0, 0, 0, 0, 1, 0, math.pi / 4, res, res]
+    cfg = [1, spp, 0, spe, spe, 1, 1, 1, 1, 1, 10000, 10000, 6, 6]
+    ramp = np.linspace(-1, 1, res, dtype=np.float32)
+    g = np.tile(ramp[None, :, None], (res, 1, 3)) if axis == 0 else np.tile(ramp[:, None, None], (1, res, 3))
+    oracle.set_fd_measure(True)
+    try:
+        _, d_int, d_pri, d_dir = oracle.render(oracle.Scene(flat, d_vertices=w), cfg, cam, mode=1, boundary=True)
+    finally:
+        oracle.set_fd_measure(False)
+    jvp = float((g * (d_int + d_pri / 10000.0 + d_dir / 360000.0)).sum())
+    plus = oracle.render(oracle.Scene(_flat_quad_scene(Vq + eps * w[:4])), cfg, cam)
+    minus = oracle.render(oracle.Scene(_flat_quad_scene(Vq - eps * w[:4])), cfg, cam)
+    fd = float((g * (plus - minus)).sum() / (2 * eps))
+    assert abs(fd) > 100.0
+    assert abs(jvp - fd) <= 0.03 * abs(fd), (jvp, fd)
