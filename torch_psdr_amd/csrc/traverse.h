@@ -47,10 +47,37 @@ __device__ __forceinline__ float slab_entry(float4 lo_hi_a, float4 lo_hi_b, floa
   return hit ? tn : F_INF;
 }
 
+// Division-free rejection ahead of ray_tri: with s = sign(det), the exact
+// test's u = un / det, v = vn / det, t = tn / det (each rounded twice) can only
+// pass if s*un, s*vn >= 0 (up to values too small to round away from zero),
+// s*(un + vn) <= |det|, |det| * tmin <= s*tn and s*tn <= |det| * tbest.  The
+// margins (2^-60 relative near zero, 2^-18 elsewhere) exceed every rounding
+// error of both forms (for |det| < 2^126, i.e. coordinates far below 2^63), so a
+// triangle rejected here is rejected by ray_tri too; NaN compares false, leaving
+// such cases to ray_tri.
+PSDR_HD bool tri_maybe_hit(f3 o, f3 d, f3 p0, f3 e1, f3 e2, float tbest) {
+  const f3 pv = cross(d, e2);
+  const float det = dot(e1, pv);
+  const float adet = fabsf(det);
+  const float sg = det < 0.f ? -1.f : 1.f;
+  const f3 tv = o - p0;
+  const float su = dot(tv, pv) * sg;
+  const float tiny = adet * 0x1p-60f;
+  if (su < -tiny || su > adet * (1.0f + 0x1p-18f)) return false;
+  const f3 qv = cross(tv, e1);
+  const float sv = dot(d, qv) * sg;
+  if (sv < -tiny || su + sv > adet * (1.0f + 0x1p-18f)) return false;
+  const float st = dot(e2, qv) * sg;
+  if (st < adet * (RAY_TMIN * (1.0f - 0x1p-18f)) || st > adet * (tbest * (1.0f + 0x1p-18f))) return false;
+  return true;
+}
+
 __device__ __forceinline__ void leaf_test(const float4* __restrict__ leaf, int k, f3 o, f3 d, HitRec& h) {
   const float4 a = leaf[3 * k], b = leaf[3 * k + 1], c = leaf[3 * k + 2];
   float t, u, v;
-  if (ray_tri(o, d, f3{a.x, a.y, a.z}, f3{b.x, b.y, b.z}, f3{c.x, c.y, c.z}, &t, &u, &v)) {
+  const f3 p0{a.x, a.y, a.z}, e1{b.x, b.y, b.z}, e2{c.x, c.y, c.z};
+  if (!tri_maybe_hit(o, d, p0, e1, e2, h.t)) return;
+  if (ray_tri(o, d, p0, e1, e2, &t, &u, &v)) {
     const int id = __float_as_int(a.w);
     if (t < h.t || (t == h.t && id < h.tri)) {
       h.t = t;
