@@ -297,7 +297,9 @@ __device__ DirectOut eval_direct(const SceneView& sv, const CameraK& cam, float 
   const f3 dir = normalize(ep - lp.p);
   bool valid = (dot(E.nl, dir) * dot(E.nr, dir) < 0.f) && norm(E.nl - E.nr) > REAL_EPS;
   if (!valid) return r;
-  HitRec h = closest_hit<BSTACK>(sv.node, sv.leaf, sv.n_tris, ep, -dir, stk, spill, spill_stride);
+  const float tl = norm(ep - lp.p);
+  HitRec h = closest_hit<BSTACK>(sv.node, sv.leaf, sv.n_tris, ep, -dir, stk, spill, spill_stride, vis_tmax(tl),
+                                 vis_tocc(tl));
   const Its s0 = reconstruct(sv, h.tri, h.u, h.v, ep, true);
   valid = s0.valid && s0.is_light && sqnorm(s0.p - lp.p) < REAL_EPS;
   if (!valid) return r;
@@ -308,7 +310,9 @@ __device__ DirectOut eval_direct(const SceneView& sv, const CameraK& cam, float 
   int px, py;
   if (!camera_project(cam, d0.p, &px, &py)) return r;
   const f3 cd = normalize(d0.p - cam.pos);
-  h = closest_hit<BSTACK>(sv.node, sv.leaf, sv.n_tris, cam.pos, cd, stk, spill, spill_stride);
+  const float tc = norm(d0.p - cam.pos);
+  h = closest_hit<BSTACK>(sv.node, sv.leaf, sv.n_tris, cam.pos, cd, stk, spill, spill_stride, vis_tmax(tc),
+                          vis_tocc(tc));
   const Its i2 = reconstruct(sv, h.tri, h.u, h.v, cam.pos, true);
   if (!(i2.valid && !i2.is_light && sqnorm(i2.p - d0.p) < REAL_EPS)) return r;
   const float pdf_xw = ((s0.d2 * pdf_ep) * pdf_light) / dot(s0.n, dir);
@@ -478,11 +482,11 @@ static int primary_pass(psdr_scene* s, const psdr_config& cfg, const CameraK& ca
   const uint32_t nL = cfg.mis_light_samples;
   const int Kmax = static_cast<int>(nL + 1 + 1);
   const int64_t ray_cap = static_cast<int64_t>(n) * Kmax;
-  if (s->b_rec.ensure(PRIM_BYTES_PER * n) || s->b_rays.ensure(24ull * 2 * n) || s->b_hits.ensure(12ull * 2 * n) ||
+  if (s->b_rec.ensure(PRIM_BYTES_PER * n) || s->b_rays.ensure(RAY_BYTES_PER * 2 * n) || s->b_hits.ensure(12ull * 2 * n) ||
       s->b_q0.ensure(QUEUE_BYTES_PER * n) || s->b_LA.ensure(12ull * n) || s->b_LB.ensure(12ull * n) ||
       s->b_rngA.ensure(8ull * n) || s->b_rngB.ensure(8ull * n) || s->q_a.ensure(QUEUE_BYTES_PER * n) ||
-      s->q_b.ensure(QUEUE_BYTES_PER * n) || s->rays.ensure(24ull * ray_cap) || s->hits.ensure(12ull * ray_cap) ||
-      s->rays2.ensure(24ull * ray_cap) || s->hits2.ensure(12ull * ray_cap) || s->b_counters.ensure(4 * 256))
+      s->q_b.ensure(QUEUE_BYTES_PER * n) || s->rays.ensure(RAY_BYTES_PER * ray_cap) || s->hits.ensure(12ull * ray_cap) ||
+      s->rays2.ensure(RAY_BYTES_PER * ray_cap) || s->hits2.ensure(12ull * ray_cap) || s->b_counters.ensure(4 * 256))
     return PSDR_ERR_NOMEM;
   PrimRecs rec = carve_prim(s->b_rec, n);
   RaysView r0 = carve_rays(s->b_rays, s->b_hits, 2ll * n);
@@ -524,7 +528,8 @@ static int primary_pass(psdr_scene* s, const psdr_config& cfg, const CameraK& ca
   PSDR_HIP_CHECK(hipGetLastError());
   const int tgrid = trace_grid(2ll * n);
   if (s->spill.ensure(std::max(s->spill.bytes, spill_bytes(tgrid)))) return PSDR_ERR_NOMEM;
-  rc = trace_rays(sv, r0.o, r0.d, 2 * n, nullptr, 1, 0, r0.tri, nullptr, r0.u, r0.v, s->spill.as<int>(), tgrid, st);
+  rc = trace_rays(sv, r0.o, r0.d, 2 * n, nullptr, 1, 0, nullptr, 0, r0.tri, nullptr, r0.u, r0.v, s->spill.as<int>(),
+                  tgrid, st);
   if (rc) return rc;
   if (s->plain) PSDR_HIP_CHECK(hipStreamSynchronize(st));
   hipLaunchKernelGGL(k_prim_check, dim3(blocks(n, BT)), dim3(BT), 0, st, sv, cam, n, q0, rec, r0);

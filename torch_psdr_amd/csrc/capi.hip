@@ -603,7 +603,8 @@ int psdr_scene_trace(psdr_scene* s, const float* ro, const float* rd, int32_t n,
   if (n == 0) return PSDR_OK;
   const int grid = trace_grid(n);
   PSDR_TRY(s->spill.ensure(std::max(s->spill.bytes, spill_bytes(grid))));
-  return trace_rays(s->view(), ro, rd, n, nullptr, 1, 0, tri, t, u, v, s->spill.as<int>(), grid, as_stream(stream));
+  return trace_rays(s->view(), ro, rd, n, nullptr, 1, 0, nullptr, 0, tri, t, u, v, s->spill.as<int>(), grid,
+                    as_stream(stream));
 }
 
 int psdr_trace_timing(int32_t enable, double* total_ms, int64_t* launches, int64_t* bytes) {

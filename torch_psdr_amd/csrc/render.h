@@ -5,7 +5,9 @@
 
 namespace psdr {
 int trace_rays(const SceneView& sv, const float* ro, const float* rd, int n, const int* count_dev, int K, int seg,
-               int* tri, float* t, float* u, float* v, int* spill, int grid, hipStream_t st);
+               const float* tl, int nvis, int* tri, float* t, float* u, float* v, int* spill, int grid,
+               hipStream_t st);
+constexpr unsigned long long RAY_BYTES_PER = 28;  // ray buffers: origin, direction, light distance
 int trace_grid(int64_t max_rays);
 int trace_timing(int enable, double* total_ms, int64_t* launches, int64_t* bytes);
 size_t spill_bytes(int grid);

@@ -61,6 +61,7 @@ __global__ void k_camera_rays(CameraK cam, uint32_t spp, int64_t N, float* o, fl
 __global__ __launch_bounds__(TRACE_THREADS) void k_trace(SceneView sv, const float* __restrict__ ro,
                                                          const float* __restrict__ rd, int n,
                                                          const int* __restrict__ count, int K, int seg,
+                                                         const float* __restrict__ rtl, int nvis,
                                                          int* __restrict__ otri,
                                                          float* __restrict__ ot, float* __restrict__ ou,
                                                          float* __restrict__ ov, int* __restrict__ spill,
@@ -76,14 +77,22 @@ __global__ __launch_bounds__(TRACE_THREADS) void k_trace(SceneView sv, const flo
   const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
   const int stride = gridDim.x * blockDim.x;
   for (int ii = gtid; ii < cnt; ii += stride) {
-    int i = ii;
+    int i = ii, k = 0;
     if (K > 1) {
-      const int k = ii / per;
+      k = ii / per;
       i = k * seg + (ii - k * per);
     }
     const f3 o = f3{ro[3 * i], ro[3 * i + 1], ro[3 * i + 2]};
     const f3 d = f3{rd[3 * i], rd[3 * i + 1], rd[3 * i + 2]};
-    const HitRec h = closest_hit<STACK_LDS>(sv.node, sv.leaf, sv.n_tris, o, d, stack, spill + gtid, stride);
+    // segments [0, nvis) are light-sample rays: visibility queries (see closest_hit)
+    float tmax = F_INF, tocc = -F_INF;
+    if (k < nvis) {
+      const float t = rtl[i];
+      tmax = vis_tmax(t);
+      tocc = vis_tocc(t);
+    }
+    const HitRec h =
+        closest_hit<STACK_LDS>(sv.node, sv.leaf, sv.n_tris, o, d, stack, spill + gtid, stride, tmax, tocc);
     otri[i] = h.tri;
     if (ot) ot[i] = h.tri >= 0 ? h.t : -1.f;
     ou[i] = h.u;
@@ -315,11 +324,14 @@ __global__ __launch_bounds__(256) void k_shade(ShadeParams P, QueueView qi, cons
         const int obj = discrete_sample(sv.obj_pmf, sv.obj_cmf, sv.n_objects, &ru, &pdf_choose);
         const int l = sv.objs[obj].light;
         f3 dir = splat(__builtin_nanf(""));
+        float tl = F_INF;
         if (l >= 0) {
           const LightPoint lp = sample_light_point(sv, l, ru, rv);
           dir = normalize(lp.p - its.p);
+          tl = norm(lp.p - its.p);
         }
         put_ray(rout, k * rs + slot, its.p, dir);
+        rout.tl[k * rs + slot] = tl;
       }
       for (uint32_t i = 0; i < P.nB; ++i, ++k) {
         const float ru = rng.next();
@@ -443,6 +455,7 @@ RaysView carve_rays(DevBuf& rb, DevBuf& hb, int64_t cap) {
   RaysView r;
   r.o = rb.as<float>();
   r.d = r.o + 3 * cap;
+  r.tl = r.d + 3 * cap;
   r.tri = hb.as<int>();
   r.u = reinterpret_cast<float*>(r.tri + cap);
   r.v = r.u + cap;
@@ -453,6 +466,7 @@ RaysView offset_rays(const RaysView& r, int64_t off) {
   RaysView o;
   o.o = r.o + 3 * off;
   o.d = r.d + 3 * off;
+  o.tl = r.tl + off;
   o.tri = r.tri + off;
   o.u = r.u + off;
   o.v = r.v + off;
@@ -484,14 +498,15 @@ constexpr size_t TIMER_PAIRS = 4096;
 }  // namespace
 
 int trace_rays(const SceneView& sv, const float* ro, const float* rd, int n, const int* count_dev, int K, int seg,
-               int* tri, float* t, float* u, float* v, int* spill, int grid, hipStream_t st) {
+               const float* tl, int nvis, int* tri, float* t, float* u, float* v, int* spill, int grid,
+               hipStream_t st) {
   TraceTimer& tt = trace_timer();
   std::lock_guard<std::mutex> lock(tt.mu);
   const bool timed = tt.on && tt.used < TIMER_PAIRS;
   if (tt.on && !timed) ++tt.dropped;
   if (timed) PSDR_HIP_CHECK(hipEventRecord(tt.ev[2 * tt.used], st));
-  hipLaunchKernelGGL(k_trace, dim3(grid), dim3(TRACE_THREADS), 0, st, sv, ro, rd, n, count_dev, K, seg, tri, t, u, v,
-                     spill, timed ? tt.bytes : nullptr);
+  hipLaunchKernelGGL(k_trace, dim3(grid), dim3(TRACE_THREADS), 0, st, sv, ro, rd, n, count_dev, K, seg, tl, nvis, tri,
+                     t, u, v, spill, timed ? tt.bytes : nullptr);
   PSDR_HIP_CHECK(hipGetLastError());
   if (timed) {
     PSDR_HIP_CHECK(hipEventRecord(tt.ev[2 * tt.used + 1], st));
@@ -598,14 +613,16 @@ int run_depths(psdr_scene* s, const WaveArgs& a, const QueueView& q0, const Rays
         const int g1 = trace_grid(n);
         for (int k = 0; k < P.K_out; ++k) {
           const int64_t o = static_cast<int64_t>(k) * n;
-          const int rc = trace_rays(sv, rout.o + 3 * o, rout.d + 3 * o, n, nullptr, 1, 0, rout.tri + o, nullptr,
-                                    rout.u + o, rout.v + o, s->spill.as<int>(), g1, st);
+          const int rc = trace_rays(sv, rout.o + 3 * o, rout.d + 3 * o, n, nullptr, 1, 0, rout.tl + o,
+                                    k < static_cast<int>(a.nL) ? 1 : 0, rout.tri + o, nullptr, rout.u + o,
+                                    rout.v + o, s->spill.as<int>(), g1, st);
           if (rc) return rc;
           PSDR_HIP_CHECK(hipStreamSynchronize(st));
         }
       } else {
-        const int rc = trace_rays(sv, rout.o, rout.d, n, a.counters + d + 1, P.K_out, n, rout.tri, nullptr, rout.u,
-                                  rout.v, s->spill.as<int>(), tgrid, st);
+        const int rc = trace_rays(sv, rout.o, rout.d, n, a.counters + d + 1, P.K_out, n, rout.tl,
+                                  static_cast<int>(a.nL), rout.tri, nullptr, rout.u, rout.v, s->spill.as<int>(), tgrid,
+                                  st);
         if (rc) return rc;
       }
     }
@@ -630,7 +647,7 @@ int render_forward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& cam
   const int64_t ray_cap = static_cast<int64_t>(N) * std::max(Kmax, 1);
   // buffers
   if (s->lane_L.ensure(12ull * N) || s->q_a.ensure(QUEUE_BYTES_PER * N) || s->q_b.ensure(QUEUE_BYTES_PER * N) ||
-      s->rays.ensure(24ull * ray_cap) || s->hits.ensure(12ull * ray_cap) || s->counters_r.ensure(4 * COUNTERS))
+      s->rays.ensure(RAY_BYTES_PER * ray_cap) || s->hits.ensure(12ull * ray_cap) || s->counters_r.ensure(4 * COUNTERS))
     return PSDR_ERR_NOMEM;
   const int tgrid = trace_grid(ray_cap);
   if (s->spill.ensure(spill_bytes(tgrid))) return PSDR_ERR_NOMEM;
@@ -655,11 +672,11 @@ int render_forward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& cam
   hipLaunchKernelGGL(k_cam, dim3(blocks(N, 256)), dim3(256), 0, st, cam, cfg.spp_interior, N, qa, rays_a,
                      counters + 0);
   PSDR_HIP_CHECK(hipGetLastError());
-  int rc = trace_rays(sv, rays_a.o, rays_a.d, N, counters + 0, 1, N, rays_a.tri, nullptr, rays_a.u, rays_a.v,
-                      s->spill.as<int>(), tgrid, st);
+  int rc = trace_rays(sv, rays_a.o, rays_a.d, N, counters + 0, 1, N, nullptr, 0, rays_a.tri, nullptr, rays_a.u,
+                      rays_a.v, s->spill.as<int>(), tgrid, st);
   if (rc) return rc;
   if (s->plain) PSDR_HIP_CHECK(hipStreamSynchronize(st));
-  if (s->rays2.ensure(24ull * ray_cap) || s->hits2.ensure(12ull * ray_cap)) return PSDR_ERR_NOMEM;
+  if (s->rays2.ensure(RAY_BYTES_PER * ray_cap) || s->hits2.ensure(12ull * ray_cap)) return PSDR_ERR_NOMEM;
   RaysView rays_b = carve_rays(s->rays2, s->hits2, ray_cap);
   WaveArgs wa;
   wa.cam = cam;

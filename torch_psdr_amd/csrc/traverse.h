@@ -91,11 +91,17 @@ __device__ __forceinline__ void leaf_test(const float4* __restrict__ leaf, int k
 // stack_lds: this block's LDS stack base ([STACK_LDS][blockDim]); spill: this
 // thread's global overflow area with stride `spill_stride` between entries.
 template <int STACK_LDS>
+// tmax / tocc: for a visibility query (is the closest hit the target point,
+// within the reference's squared-distance tolerance REAL_EPS?) hits beyond
+// tmax cannot matter, and a hit before tocc already decides "not visible", so
+// traversal stops there and returns that hit instead of the closest one.
+// Defaults (inf, -inf) give the plain closest hit.
 __device__ __forceinline__ HitRec closest_hit(const float4* __restrict__ node, const float4* __restrict__ leaf,
                                               int n_tris, f3 o, f3 d, int* stack_lds, int* spill,
-                                              int spill_stride) {
+                                              int spill_stride, float tmax = F_INF, float tocc = -F_INF) {
   HitRec h{-1, F_INF, -1.f, -1.f};
   if (!(d.x == d.x) || n_tris <= 0) return h;
+  h.t = tmax;
   if (n_tris == 1) {
     leaf_test(leaf, 0, o, d, h);
     if (h.tri < 0) h.t = F_INF;
@@ -121,6 +127,7 @@ __device__ __forceinline__ HitRec closest_hit(const float4* __restrict__ node, c
       if ((cr & 0x7fffffff) < n_tris) leaf_test(leaf, cr & 0x7fffffff, o, d, h);
       tr = F_INF;
     }
+    if (h.t < tocc && h.tri >= 0) break;  // occluded before the target
     const bool hl = tl != F_INF, hr = tr != F_INF;
     if (hl && hr) {
       int nearc = cl, farc = cr;
@@ -142,7 +149,14 @@ __device__ __forceinline__ HitRec closest_hit(const float4* __restrict__ node, c
       ni = (sp < STACK_LDS) ? stack_lds[sp * nthr + lane] : spill[(sp - STACK_LDS) * spill_stride];
     }
   }
+  if (h.tri < 0) h.t = F_INF;
   return h;
 }
+
+// Window of a visibility query toward a target at distance t: a hit at least
+// 4e-3 (plus 1e-5 relative, for the direction's rounding) nearer or farther
+// than the target lies farther from it than sqrt(REAL_EPS) = 3.2e-3.
+__device__ __forceinline__ float vis_tmax(float t) { return t + (4e-3f + 1e-5f * t); }
+__device__ __forceinline__ float vis_tocc(float t) { return t - (4e-3f + 1e-5f * t); }
 
 }  // namespace psdr

@@ -24,6 +24,7 @@ constexpr size_t QUEUE_BYTES_PER = 8 + 4 * 5 + 12 * 3;
 struct RaysView {
   float* o;
   float* d;
+  float* tl;  // light-sample rays: distance to the sampled light point
   int* tri;
   float* u;
   float* v;
