@@ -18,6 +18,8 @@
 // block, flushed with one global atomic per entry.
 #include <hip/hip_runtime.h>
 
+#include <cstring>
+
 #include "grad_cache.h"
 #include "render.h"
 #include "shading.h"
@@ -532,51 +534,81 @@ inline int blocks(int64_t n, int t) { return static_cast<int>((n + t - 1) / t); 
 
 // The interior, primary-boundary and direct-boundary terms share only their
 // (atomically accumulated) gradient outputs and read-only scene data, so they
-// run side by side: the boundary terms on two auxiliary streams forked from
-// and joined back into the caller's stream.  Every scratch buffer a term uses
-// is its own; the gradient sink for an absent vertex gradient is sized before
-// the fork.
-int render_backward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& camd, const float* dL_dI,
-                    float* grad_v, float* grad_t, float* grad_e, int terms, hipStream_t st) {
-  const bool ti = (terms & PSDR_TERM_INTERIOR) != 0;
-  const bool tp = (terms & PSDR_TERM_PRIMARY) != 0 && s->n_edges > 0;
-  const bool td = (terms & PSDR_TERM_DIRECT) != 0 && s->n_edges > 0;
-  if (s->stats.ensure(4ull * (3 * static_cast<size_t>(s->n_vertices) + s->n_texels + 3 * s->n_lights + 64)))
-    return PSDR_ERR_NOMEM;
-  float* gv = grad_v ? grad_v : s->stats.as<float>();
-  if (s->plain || static_cast<int>(ti) + static_cast<int>(tp) + static_cast<int>(td) <= 1) {
-    int rc = ti ? interior_backward(s, cfg, camd, dL_dI, grad_v, grad_t, grad_e, st) : PSDR_OK;
-    if (!rc && tp) rc = boundary_primary(s, cfg, camd, dL_dI, gv, st);
-    if (!rc && td) rc = boundary_direct(s, cfg, camd, dL_dI, gv, st);
-    return rc;
+// run side by side: the boundary terms on two auxiliary streams forked from and
+// joined back into the caller's stream, each with its own scratch buffers.  The
+// parts of the boundary terms that do not depend on dL/dI are normally issued by
+// the forward render (boundary_prepare_async) and overlap it; the backward then
+// only forms the dL/dI-weighted scatters.
+
+int aux_join(psdr_scene* s, hipStream_t st) {
+  if (!s->aux.fork) return PSDR_OK;
+  for (int i = 0; i < 2; ++i) {
+    PSDR_HIP_CHECK(hipEventRecord(s->aux.join[i], s->aux.st[i]));
+    PSDR_HIP_CHECK(hipStreamWaitEvent(st, s->aux.join[i], 0));
   }
+  return PSDR_OK;
+}
+
+static int aux_fork(psdr_scene* s, hipStream_t st) {
   int rc = s->aux.ensure();
   if (rc) return rc;
   PSDR_HIP_CHECK(hipEventRecord(s->aux.fork, st));
-  hipStream_t sp = s->aux.st[0], sd = s->aux.st[1];
-  PSDR_HIP_CHECK(hipStreamWaitEvent(sp, s->aux.fork, 0));
-  PSDR_HIP_CHECK(hipStreamWaitEvent(sd, s->aux.fork, 0));
-  // launch order by host enqueue cost: the single interior launch first, so the
-  // device is busy while the host enqueues the boundary chains (the primary
-  // chain, with the most launches, last)
+  for (int i = 0; i < 2; ++i) PSDR_HIP_CHECK(hipStreamWaitEvent(s->aux.st[i], s->aux.fork, 0));
+  return PSDR_OK;
+}
+
+int boundary_prepare_async(psdr_scene* s, const psdr_config& cfg, const psdr_camera& cam) {
+  s->bprep = false;
+  if (s->n_edges == 0 || s->plain || !s->aux.fork) return PSDR_OK;
+  for (int i = 0; i < 2; ++i) PSDR_HIP_CHECK(hipStreamWaitEvent(s->aux.st[i], s->aux.fork, 0));
+  int rc = PSDR_OK;
+  // direct first: its prepare is a handful of large launches
+  if (cfg.spp_secondary_edge > 0) rc = boundary_direct_prepare(s, cfg, cam, s->aux.st[1]);
+  if (!rc && cfg.spp_primary_edge > 0) rc = boundary_primary_prepare(s, cfg, cam, s->aux.st[0]);
+  if (rc) return rc;
+  s->bprep = true;
+  s->bprep_cfg = cfg;
+  s->bprep_cam = cam;
+  return PSDR_OK;
+}
+
+int render_backward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& camd, const float* dL_dI,
+                    float* grad_v, float* grad_t, float* grad_e, int terms, hipStream_t st) {
+  const bool ti = (terms & PSDR_TERM_INTERIOR) != 0;
+  const bool tp = (terms & PSDR_TERM_PRIMARY) != 0 && s->n_edges > 0 && cfg.spp_primary_edge > 0;
+  const bool td = (terms & PSDR_TERM_DIRECT) != 0 && s->n_edges > 0 && cfg.spp_secondary_edge > 0;
+  if (s->stats.ensure(4ull * (3 * static_cast<size_t>(s->n_vertices) + s->n_texels + 3 * s->n_lights + 64)))
+    return PSDR_ERR_NOMEM;
+  float* gv = grad_v ? grad_v : s->stats.as<float>();
+  if (s->plain) {
+    int rc = ti ? interior_backward(s, cfg, camd, dL_dI, grad_v, grad_t, grad_e, st) : PSDR_OK;
+    if (!rc && tp) rc = boundary_primary_prepare(s, cfg, camd, st);
+    if (!rc && tp) rc = boundary_primary_finish(s, cfg, camd, dL_dI, gv, st);
+    if (!rc && td) rc = boundary_direct_prepare(s, cfg, camd, st);
+    if (!rc && td) rc = boundary_direct_finish(s, cfg, camd, dL_dI, gv, st);
+    return rc;
+  }
+  const bool prepared = s->bprep && std::memcmp(&s->bprep_cfg, &cfg, sizeof(cfg)) == 0 &&
+                        std::memcmp(&s->bprep_cam, &camd, sizeof(camd)) == 0;
+  int rc = (tp || td) ? aux_fork(s, st) : PSDR_OK;
+  if (rc) return rc;
+  // launch order by host enqueue cost: the single interior launch first
   int rci = ti ? interior_backward(s, cfg, camd, dL_dI, grad_v, grad_t, grad_e, st) : PSDR_OK;
-  int rcd = td ? boundary_direct(s, cfg, camd, dL_dI, gv, sd) : PSDR_OK;
-  int rcp = tp ? boundary_primary(s, cfg, camd, dL_dI, gv, sp) : PSDR_OK;
+  int rcd = PSDR_OK, rcp = PSDR_OK;
+  if (td) {
+    if (!prepared) rcd = boundary_direct_prepare(s, cfg, camd, s->aux.st[1]);
+    if (!rcd) rcd = boundary_direct_finish(s, cfg, camd, dL_dI, gv, s->aux.st[1]);
+  }
+  if (tp) {
+    if (!prepared) rcp = boundary_primary_prepare(s, cfg, camd, s->aux.st[0]);
+    if (!rcp) rcp = boundary_primary_finish(s, cfg, camd, dL_dI, gv, s->aux.st[0]);
+  }
   // join even after an error so no work is left running unordered
-  const hipError_t e0 = hipEventRecord(s->aux.join[0], sp);
-  const hipError_t e1 = hipEventRecord(s->aux.join[1], sd);
-  const hipError_t e2 = hipStreamWaitEvent(st, s->aux.join[0], 0);
-  const hipError_t e3 = hipStreamWaitEvent(st, s->aux.join[1], 0);
+  const int rcj = (tp || td) ? aux_join(s, st) : PSDR_OK;
   if (rci) return rci;
   if (rcp) return rcp;
   if (rcd) return rcd;
-  for (hipError_t e : {e0, e1, e2, e3}) {
-    if (e != hipSuccess) {
-      set_error(PSDR_ERR_HIP, std::string("backward stream join: ") + hipGetErrorString(e));
-      return PSDR_ERR_HIP;
-    }
-  }
-  return PSDR_OK;
+  return rcj;
 }
 
 int interior_backward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& camd, const float* dL_dI,

@@ -276,8 +276,7 @@ struct DirectOut {
 // is dLdI[pixel] * gscale and is pushed into the edge's vertices.
 template <bool PRE>
 __device__ DirectOut eval_direct(const SceneView& sv, const CameraK& cam, float sx, float sy, float sz, int* stk,
-                                 int* spill, int spill_stride, const float* dLdI, float gscale, float* grad_v,
-                                 EdgeCache* gc) {
+                                 int* spill, int spill_stride, float gscale, float* rec) {
   DirectOut r;
   r.valid = false;
   r.pix = -1;
@@ -346,9 +345,9 @@ __device__ DirectOut eval_direct(const SceneView& sv, const CameraK& cam, float 
   const f3 v0 = (((fbk * 0.0f) * Jb) / pdf_xw * Ts) * Td;
   r.valid = isfinite(sqnorm(v0));
   if (!r.valid) return r;
-  const f3 g = f3{dLdI[3 * r.pix], dLdI[3 * r.pix + 1], dLdI[3 * r.pix + 2]} * gscale;
-  const f3 K = (((g * fbk) * Jb) / pdf_xw * Ts) * Td;  // adjoint of x_hit
-  if (K.x == 0.f && K.y == 0.f && K.z == 0.f) return r;
+  // The vertex adjoint is linear in the pixel adjoint g: gxb = M g.  Column c of
+  // M is the VJP below for g = e_c (gscale folded in), so the pass that knows
+  // dL/dI only forms M g and scatters it.
   // hit_triangle(xs, db, xd, edge_proj, nep) (src/path_integrator.cc:11-29)
   const f3 E1 = edge_proj, E2 = nep;
   const f3 T = xs - xd;
@@ -357,32 +356,39 @@ __device__ DirectOut eval_direct(const SceneView& sv, const CameraK& cam, float 
   const float pe = dot(P, E1);
   const float cl = fminf(fmaxf(pe, REAL_EPS), 1.0f - REAL_EPS);
   const float inv = 1.0f / cl;
-  const float gu = dot(K, E1), gv = dot(K, E2);
-  f3 gP = T * (gu * inv);
-  const float ginv = gu * dot(P, T) + gv * dot(Q, db);
-  f3 gD = Q * (gv * inv);
-  if (!(pe < REAL_EPS) && !(pe > 1.0f - REAL_EPS)) gP = gP + E1 * (-ginv * inv * inv);
-  gD = gD + cross(E2, gP);
-  const f3 gxb = vjp_norm(xb - xs, gD);
-  scatter_edge(sv, grad_v, E, ue, gxb, gc);
+  const bool unclamped = !(pe < REAL_EPS) && !(pe > 1.0f - REAL_EPS);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const f3 g = f3{c == 0 ? gscale : 0.f, c == 1 ? gscale : 0.f, c == 2 ? gscale : 0.f};
+    const f3 K = (((g * fbk) * Jb) / pdf_xw * Ts) * Td;  // adjoint of x_hit
+    const float gu = dot(K, E1), gv = dot(K, E2);
+    f3 gP = T * (gu * inv);
+    const float ginv = gu * dot(P, T) + gv * dot(Q, db);
+    f3 gD = Q * (gv * inv);
+    if (unclamped) gP = gP + E1 * (-ginv * inv * inv);
+    gD = gD + cross(E2, gP);
+    const f3 gxb = vjp_norm(xb - xs, gD);
+    rec[3 * c] = gxb.x;
+    rec[3 * c + 1] = gxb.y;
+    rec[3 * c + 2] = gxb.z;
+  }
+  rec[9] = __int_as_float(r.pix);
+  rec[10] = __int_as_float(e);
+  rec[11] = ue;
   return r;
 }
 
+// PRE: preprocess round over the cells (accumulates |value| into mass).
+// Render pass: position p evaluates lane order[p] and writes its 12-float
+// record (the 3x3 map from the pixel adjoint to the edge-point adjoint, pixel,
+// edge, edge parameter; pixel -1 when the sample is invalid).
 template <bool PRE>
 __global__ __launch_bounds__(BT) void k_direct(SceneView sv, CameraK cam, int n, uint64_t* __restrict__ state,
                                                int r0, int r1, int r2, const float* __restrict__ cpmf,
                                                const float* __restrict__ ccmf, float* __restrict__ mass,
-                                               const float* __restrict__ dLdI, uint32_t spp,
-                                               float* __restrict__ grad_v, int* __restrict__ spill,
+                                               uint32_t spp, float* __restrict__ drec, int* __restrict__ spill,
                                                const uint32_t* __restrict__ order) {
   __shared__ int stk[BSTACK * BT];
-  __shared__ int gk[PRE ? 1 : EDGE_SLOTS];
-  __shared__ float gvals[PRE ? 1 : 3 * EDGE_SLOTS];
-  EdgeCache gc{gk, gvals};
-  if (!PRE) {
-    gc.init();
-    __syncthreads();
-  }
   const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
   const int stride = gridDim.x * blockDim.x;
   const int cells = r0 * r1 * r2;
@@ -416,8 +422,7 @@ __global__ __launch_bounds__(BT) void k_direct(SceneView sv, CameraK cam, int n,
       s[2] = (s[2] + static_cast<float>(c2)) * u2;
     }
     if (PRE) {
-      DirectOut o =
-          eval_direct<true>(sv, cam, s[0], s[1], s[2], stk, spill + gtid, stride, nullptr, 0.f, nullptr, nullptr);
+      DirectOut o = eval_direct<true>(sv, cam, s[0], s[1], s[2], stk, spill + gtid, stride, 0.f, nullptr);
       if (o.valid) {
         f3 v = o.value;
         v.x = v.x < REAL_EPS ? REAL_EPS : v.x;
@@ -428,13 +433,39 @@ __global__ __launch_bounds__(BT) void k_direct(SceneView sv, CameraK cam, int n,
     } else {
       float gscale = 1.0f / pdf_cube;
       if (spp > 1) gscale = gscale / static_cast<float>(spp);
-      eval_direct<false>(sv, cam, s[0], s[1], s[2], stk, spill + gtid, stride, dLdI, gscale, grad_v, &gc);
+      float* rec = drec + 12ll * p;
+      rec[9] = __int_as_float(-1);
+      eval_direct<false>(sv, cam, s[0], s[1], s[2], stk, spill + gtid, stride, gscale, rec);
     }
   }
-  if (!PRE) {
-    __syncthreads();
-    gc.flush(grad_v, grad_v);
+}
+
+// Direct-boundary gradients from the records: g = dL/dI at the sample's pixel,
+// edge-point adjoint M g, split onto the edge's two vertices (aggregated per
+// workgroup; positions are in cell order, so neighbours share edges).
+__global__ __launch_bounds__(BT) void k_direct_scatter(SceneView sv, int n, const float* __restrict__ drec,
+                                                       const float* __restrict__ dLdI, float* __restrict__ grad_v) {
+  __shared__ int gk[EDGE_SLOTS];
+  __shared__ float gvals[3 * EDGE_SLOTS];
+  EdgeCache gc{gk, gvals};
+  gc.init();
+  __syncthreads();
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) {
+    const float* rec = drec + 12ll * p;
+    const int pix = __float_as_int(rec[9]);
+    if (pix >= 0) {
+      const f3 g = f3{dLdI[3 * pix], dLdI[3 * pix + 1], dLdI[3 * pix + 2]};
+      const f3 gxb = (f3{rec[0], rec[1], rec[2]} * g.x + f3{rec[3], rec[4], rec[5]} * g.y) +
+                     f3{rec[6], rec[7], rec[8]} * g.z;
+      if (gxb.x != 0.f || gxb.y != 0.f || gxb.z != 0.f) {
+        const EdgeRec E = load_edge(sv, __float_as_int(rec[10]));
+        scatter_edge(sv, grad_v, E, rec[11], gxb, &gc);
+      }
+    }
   }
+  __syncthreads();
+  gc.flush(grad_v, grad_v);
 }
 
 // Sort key of a direct-boundary lane: its hypercube cell (edge position major),
@@ -474,25 +505,26 @@ int cus() {
 
 }  // namespace
 
-// One primary pass over n lanes (PRE: preprocess round over n = cells).
+// One primary pass over n lanes up to both sides' radiance (PRE: a preprocess
+// round over n = cells, which also accumulates the cell masses).
 static int primary_pass(psdr_scene* s, const psdr_config& cfg, const CameraK& cam, bool pre, int n, int res,
-                        const float* cpmf, const float* ccmf, float* mass, const float* dLdI, float* grad_v,
-                        hipStream_t st) {
+                        const float* cpmf, const float* ccmf, float* mass, hipStream_t st) {
   const SceneView sv = s->view();
   const uint32_t nL = cfg.mis_light_samples;
   const int Kmax = static_cast<int>(nL + 1 + 1);
   const int64_t ray_cap = static_cast<int64_t>(n) * Kmax;
   if (s->b_rec.ensure(PRIM_BYTES_PER * n) || s->b_rays.ensure(RAY_BYTES_PER * 2 * n) || s->b_hits.ensure(12ull * 2 * n) ||
       s->b_q0.ensure(QUEUE_BYTES_PER * n) || s->b_LA.ensure(12ull * n) || s->b_LB.ensure(12ull * n) ||
-      s->b_rngA.ensure(8ull * n) || s->b_rngB.ensure(8ull * n) || s->q_a.ensure(QUEUE_BYTES_PER * n) ||
-      s->q_b.ensure(QUEUE_BYTES_PER * n) || s->rays.ensure(RAY_BYTES_PER * ray_cap) || s->hits.ensure(12ull * ray_cap) ||
-      s->rays2.ensure(RAY_BYTES_PER * ray_cap) || s->hits2.ensure(12ull * ray_cap) || s->b_counters.ensure(4 * 256))
+      s->b_rngA.ensure(8ull * n) || s->b_rngB.ensure(8ull * n) || s->p_qa.ensure(QUEUE_BYTES_PER * n) ||
+      s->p_qb.ensure(QUEUE_BYTES_PER * n) || s->p_rays.ensure(RAY_BYTES_PER * ray_cap) ||
+      s->p_hits.ensure(12ull * ray_cap) || s->p_rays2.ensure(RAY_BYTES_PER * ray_cap) ||
+      s->p_hits2.ensure(12ull * ray_cap) || s->b_counters.ensure(4 * 256))
     return PSDR_ERR_NOMEM;
   PrimRecs rec = carve_prim(s->b_rec, n);
   RaysView r0 = carve_rays(s->b_rays, s->b_hits, 2ll * n);
   const QueueView q0 = carve_queue(s->b_q0, n);
-  const QueueView qs[2] = {carve_queue(s->q_a, n), carve_queue(s->q_b, n)};
-  const RaysView rs[2] = {carve_rays(s->rays, s->hits, ray_cap), carve_rays(s->rays2, s->hits2, ray_cap)};
+  const QueueView qs[2] = {carve_queue(s->p_qa, n), carve_queue(s->p_qb, n)};
+  const RaysView rs[2] = {carve_rays(s->p_rays, s->p_hits, ray_cap), carve_rays(s->p_rays2, s->p_hits2, ray_cap)};
   int* cnt = s->b_counters.as<int>();
   int rc = zero_ints(cnt, 256, st);
   if (rc) return rc;
@@ -527,8 +559,8 @@ static int primary_pass(psdr_scene* s, const psdr_config& cfg, const CameraK& ca
   hipLaunchKernelGGL(k_prim_rays, dim3(blocks(n, BT)), dim3(BT), 0, st, sv, cam, n, order, rec, rng0, r0, q0, cnt);
   PSDR_HIP_CHECK(hipGetLastError());
   const int tgrid = trace_grid(2ll * n);
-  if (s->spill.ensure(std::max(s->spill.bytes, spill_bytes(tgrid)))) return PSDR_ERR_NOMEM;
-  rc = trace_rays(sv, r0.o, r0.d, 2 * n, nullptr, 1, 0, nullptr, 0, r0.tri, nullptr, r0.u, r0.v, s->spill.as<int>(),
+  if (s->p_spill.ensure(std::max(s->p_spill.bytes, spill_bytes(tgrid)))) return PSDR_ERR_NOMEM;
+  rc = trace_rays(sv, r0.o, r0.d, 2 * n, nullptr, 1, 0, nullptr, 0, r0.tri, nullptr, r0.u, r0.v, s->p_spill.as<int>(),
                   tgrid, st);
   if (rc) return rc;
   if (s->plain) PSDR_HIP_CHECK(hipStreamSynchronize(st));
@@ -546,6 +578,7 @@ static int primary_pass(psdr_scene* s, const psdr_config& cfg, const CameraK& ca
   wa.lane_rng_out = s->b_rngA.as<uint64_t>();
   wa.counters = cnt;
   wa.plain = s->plain ? 1 : 0;
+  wa.spill = &s->p_spill;
   rc = run_depths(s, wa, q0, r0, qs, rs, n, st);
   if (rc) return rc;
   hipLaunchKernelGGL(k_prim_requeue, dim3(blocks(n, BT)), dim3(BT), 0, st, n, q0, s->b_rngA.as<uint64_t>(),
@@ -555,80 +588,96 @@ static int primary_pass(psdr_scene* s, const psdr_config& cfg, const CameraK& ca
   wa.counters = cnt + 128;
   rc = run_depths(s, wa, q0, offset_rays(r0, n), qs, rs, n, st);
   if (rc) return rc;
-  if (pre)
+  if (pre) {
     hipLaunchKernelGGL(k_prim_finish<true>, dim3(blocks(n, BT)), dim3(BT), 0, st, sv, cam, n, q0.lane, rec,
-                       s->b_LA.as<float>(), s->b_LB.as<float>(), s->b_rngB.as<uint64_t>(), state, mass, dLdI,
-                       cfg.spp_primary_edge, grad_v);
-  else
-    hipLaunchKernelGGL(k_prim_finish<false>, dim3(blocks(n, BT)), dim3(BT), 0, st, sv, cam, n, q0.lane, rec,
-                       s->b_LA.as<float>(), s->b_LB.as<float>(), s->b_rngB.as<uint64_t>(), state, mass, dLdI,
-                       cfg.spp_primary_edge, grad_v);
-  PSDR_HIP_CHECK(hipGetLastError());
+                       s->b_LA.as<float>(), s->b_LB.as<float>(), s->b_rngB.as<uint64_t>(), state, mass, nullptr,
+                       cfg.spp_primary_edge, nullptr);
+    PSDR_HIP_CHECK(hipGetLastError());
+  }
   return PSDR_OK;
 }
 
-int boundary_primary(psdr_scene* s, const psdr_config& cfg, const psdr_camera& camd, const float* dL_dI,
-                     float* grad_v, hipStream_t st) {
+// Number of primary / direct render-pass lanes (spp_edge * H * W), or -1 (error set).
+static int edge_lanes(uint32_t spp, const CameraK& cam, const char* what) {
+  const int64_t n64 = static_cast<int64_t>(spp) * cam.H * cam.W;
+  if (n64 > 0x7fffffff / 8) {
+    set_error(PSDR_ERR_ARG, std::string(what) + ": too many samples");
+    return -1;
+  }
+  return static_cast<int>(n64);
+}
+
+int boundary_primary_prepare(psdr_scene* s, const psdr_config& cfg, const psdr_camera& camd, hipStream_t st) {
   if (s->n_edges == 0) return PSDR_OK;
   const CameraK cam = make_camera(camd);
-  const int64_t HW = static_cast<int64_t>(cam.H) * cam.W;
   const int R = static_cast<int>(cfg.primary_edge_hypercube_resolution);
-  const int64_t n64 = static_cast<int64_t>(cfg.spp_primary_edge) * HW;
-  if (n64 > 0x7fffffff / 8) {
-    set_error(PSDR_ERR_ARG, "primary boundary: too many samples");
-    return PSDR_ERR_ARG;
-  }
-  const int n = static_cast<int>(n64);
+  const int n = edge_lanes(cfg.spp_primary_edge, cam, "primary boundary");
+  if (n < 0) return PSDR_ERR_ARG;
   if (s->b_state.ensure(8ull * std::max(R, 1)) || s->hc_mass.ensure(4ull * R) || s->hc_pmf.ensure(4ull * R) ||
       s->hc_cmf.ensure(4ull * R) || s->b_tmp.ensure(4ull * (2 * (R / 256 + 2) + 16)))
     return PSDR_ERR_NOMEM;
   hipLaunchKernelGGL(k_seed_states, dim3(blocks(R, BT)), dim3(BT), 0, st, s->b_state.as<uint64_t>(), R);
   hipLaunchKernelGGL(k_fill_f32, dim3(blocks(R, BT)), dim3(BT), 0, st, s->hc_mass.as<float>(), R, 0.f);
   for (uint32_t r = 0; r < cfg.primary_edge_preprocess_rounds; ++r) {
-    const int rc = primary_pass(s, cfg, cam, true, R, R, nullptr, nullptr, s->hc_mass.as<float>(), nullptr, nullptr,
-                                st);
+    const int rc = primary_pass(s, cfg, cam, true, R, R, nullptr, nullptr, s->hc_mass.as<float>(), st);
     if (rc) return rc;
   }
   int rc = sampler_tables(s->hc_mass.as<float>(), R, s->hc_pmf.as<float>(), s->hc_cmf.as<float>(), nullptr,
                           s->b_tmp.as<float>(), st);
   if (rc) return rc;
-  return primary_pass(s, cfg, cam, false, n, R, s->hc_pmf.as<float>(), s->hc_cmf.as<float>(), nullptr, dL_dI, grad_v,
-                      st);
+  return primary_pass(s, cfg, cam, false, n, R, s->hc_pmf.as<float>(), s->hc_cmf.as<float>(), nullptr, st);
 }
 
-int boundary_direct(psdr_scene* s, const psdr_config& cfg, const psdr_camera& camd, const float* dL_dI,
-                    float* grad_v, hipStream_t st) {
+int boundary_primary_finish(psdr_scene* s, const psdr_config& cfg, const psdr_camera& camd, const float* dL_dI,
+                            float* grad_v, hipStream_t st) {
+  if (s->n_edges == 0) return PSDR_OK;
+  const CameraK cam = make_camera(camd);
+  const int n = edge_lanes(cfg.spp_primary_edge, cam, "primary boundary");
+  if (n < 0) return PSDR_ERR_ARG;
+  if (n == 0) return PSDR_OK;
+  const PrimRecs rec = carve_prim(s->b_rec, n);
+  const QueueView q0 = carve_queue(s->b_q0, n);
+  hipLaunchKernelGGL(k_prim_finish<false>, dim3(blocks(n, BT)), dim3(BT), 0, st, s->view(), cam, n, q0.lane, rec,
+                     s->b_LA.as<float>(), s->b_LB.as<float>(), s->b_rngB.as<uint64_t>(), nullptr, nullptr, dL_dI,
+                     cfg.spp_primary_edge, grad_v);
+  PSDR_HIP_CHECK(hipGetLastError());
+  return PSDR_OK;
+}
+
+int boundary_direct_prepare(psdr_scene* s, const psdr_config& cfg, const psdr_camera& camd, hipStream_t st) {
   if (s->n_edges == 0) return PSDR_OK;
   const CameraK cam = make_camera(camd);
   const SceneView sv = s->view();
-  const int64_t HW = static_cast<int64_t>(cam.H) * cam.W;
   const int r0 = static_cast<int>(cfg.secondary_edge_hypercube_resolution[0]);
   const int r1 = static_cast<int>(cfg.secondary_edge_hypercube_resolution[1]);
   const int r2 = static_cast<int>(cfg.secondary_edge_hypercube_resolution[2]);
   const int64_t cells64 = static_cast<int64_t>(r0) * r1 * r2;
-  const int64_t n64 = static_cast<int64_t>(cfg.spp_secondary_edge) * HW;
-  if (cells64 > 0x7fffffff / 8 || n64 > 0x7fffffff / 8) {
-    set_error(PSDR_ERR_ARG, "direct boundary: too many cells or samples");
+  if (cells64 > 0x7fffffff / 8) {
+    set_error(PSDR_ERR_ARG, "direct boundary: too many cells");
     return PSDR_ERR_ARG;
   }
-  const int cells = static_cast<int>(cells64), n = static_cast<int>(n64);
+  const int n = edge_lanes(cfg.spp_secondary_edge, cam, "direct boundary");
+  if (n < 0) return PSDR_ERR_ARG;
+  const int cells = static_cast<int>(cells64);
   const int grid = std::max(1, std::min(blocks(std::max(cells, n), BT), cus() * 8));
   if (s->d_state.ensure(8ull * cells) || s->d_mass.ensure(4ull * cells) || s->d_pmf.ensure(4ull * cells) ||
       s->d_cmf.ensure(4ull * cells) || s->d_tmp.ensure(4ull * (2 * (cells / 256 + 2) + 16)) ||
       s->d_spill.ensure(static_cast<size_t>(grid) * BT * (STACK_TOTAL - BSTACK) * 4) ||
-      s->d_keys.ensure(8ull * n) || s->d_vals.ensure(8ull * n) || s->d_hist.ensure(4 * radix_hist_words(n)))
+      s->d_keys.ensure(8ull * n) || s->d_vals.ensure(8ull * n) || s->d_hist.ensure(4 * radix_hist_words(n)) ||
+      s->d_rec.ensure(48ull * std::max(n, 1)))
     return PSDR_ERR_NOMEM;
   uint64_t* state = s->d_state.as<uint64_t>();
   hipLaunchKernelGGL(k_seed_states, dim3(blocks(cells, BT)), dim3(BT), 0, st, state, cells);
   hipLaunchKernelGGL(k_fill_f32, dim3(blocks(cells, BT)), dim3(BT), 0, st, s->d_mass.as<float>(), cells, 0.f);
   for (uint32_t r = 0; r < cfg.secondary_edge_preprocess_rounds; ++r) {
     hipLaunchKernelGGL(k_direct<true>, dim3(grid), dim3(BT), 0, st, sv, cam, cells, state, r0, r1, r2, nullptr, nullptr,
-                       s->d_mass.as<float>(), nullptr, 0u, nullptr, s->d_spill.as<int>(), nullptr);
+                       s->d_mass.as<float>(), 0u, nullptr, s->d_spill.as<int>(), nullptr);
     PSDR_HIP_CHECK(hipGetLastError());
   }
   int rc = sampler_tables(s->d_mass.as<float>(), cells, s->d_pmf.as<float>(), s->d_cmf.as<float>(), nullptr,
                           s->d_tmp.as<float>(), st);
   if (rc) return rc;
+  if (n == 0) return PSDR_OK;
   // process the lanes in cell order (edge position major): neighbouring lanes
   // of a wave then trace from neighbouring edge points
   const uint32_t* order = nullptr;
@@ -644,8 +693,21 @@ int boundary_direct(psdr_scene* s, const psdr_config& cfg, const psdr_camera& ca
     order = (passes & 1) ? v0 + n : v0;
   }
   hipLaunchKernelGGL(k_direct<false>, dim3(grid), dim3(BT), 0, st, sv, cam, n, state, r0, r1, r2,
-                     s->d_pmf.as<float>(), s->d_cmf.as<float>(), nullptr, dL_dI, cfg.spp_secondary_edge, grad_v,
-                     s->d_spill.as<int>(), order);
+                     s->d_pmf.as<float>(), s->d_cmf.as<float>(), nullptr, cfg.spp_secondary_edge,
+                     s->d_rec.as<float>(), s->d_spill.as<int>(), order);
+  PSDR_HIP_CHECK(hipGetLastError());
+  return PSDR_OK;
+}
+
+int boundary_direct_finish(psdr_scene* s, const psdr_config& cfg, const psdr_camera& camd, const float* dL_dI,
+                           float* grad_v, hipStream_t st) {
+  if (s->n_edges == 0) return PSDR_OK;
+  const CameraK cam = make_camera(camd);
+  const int n = edge_lanes(cfg.spp_secondary_edge, cam, "direct boundary");
+  if (n < 0) return PSDR_ERR_ARG;
+  if (n == 0) return PSDR_OK;
+  hipLaunchKernelGGL(k_direct_scatter, dim3(blocks(n, BT)), dim3(BT), 0, st, s->view(), n, s->d_rec.as<float>(), dL_dI,
+                     grad_v);
   PSDR_HIP_CHECK(hipGetLastError());
   return PSDR_OK;
 }

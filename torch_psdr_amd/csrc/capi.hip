@@ -460,8 +460,11 @@ int psdr_scene_update(psdr_scene* s, const psdr_scene_desc* desc, void* stream) 
     return PSDR_ERR_ARG;
   }
   PSDR_TRY(check_desc(desc));
+  // a boundary prepass still running on the auxiliary streams reads the scene
+  PSDR_TRY(aux_join(s, as_stream(stream)));
   PSDR_TRY(scene_set_buffers(s, desc, as_stream(stream)));
   s->have_cache = false;
+  s->bprep = false;
   return scene_derive(s, as_stream(stream));
 }
 

@@ -19,8 +19,18 @@ int render_backward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& ca
                     float* grad_v, float* grad_t, float* grad_e, int terms, hipStream_t st);
 int interior_backward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& cam, const float* dL_dI,
                       float* grad_v, float* grad_t, float* grad_e, hipStream_t st);
-int boundary_primary(psdr_scene* s, const psdr_config& cfg, const psdr_camera& cam, const float* dL_dI,
-                     float* grad_v, hipStream_t st);
-int boundary_direct(psdr_scene* s, const psdr_config& cfg, const psdr_camera& cam, const float* dL_dI,
-                    float* grad_v, hipStream_t st);
+// Edge-sampling boundary terms in two parts: prepare (everything that does not
+// depend on dL/dI: hypercube preprocessing, sampler, samples, traversals) and
+// finish (the dL/dI-weighted scatter into vertex gradients).
+int boundary_primary_prepare(psdr_scene* s, const psdr_config& cfg, const psdr_camera& cam, hipStream_t st);
+int boundary_primary_finish(psdr_scene* s, const psdr_config& cfg, const psdr_camera& cam, const float* dL_dI,
+                            float* grad_v, hipStream_t st);
+int boundary_direct_prepare(psdr_scene* s, const psdr_config& cfg, const psdr_camera& cam, hipStream_t st);
+int boundary_direct_finish(psdr_scene* s, const psdr_config& cfg, const psdr_camera& cam, const float* dL_dI,
+                           float* grad_v, hipStream_t st);
+// Launches both prepare parts on the scene's auxiliary streams, each waiting
+// for the fork event the forward render recorded before its own work.
+int boundary_prepare_async(psdr_scene* s, const psdr_config& cfg, const psdr_camera& cam);
+// Makes `st` wait for everything issued on the auxiliary streams.
+int aux_join(psdr_scene* s, hipStream_t st);
 }  // namespace psdr

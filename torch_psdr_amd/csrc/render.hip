@@ -583,7 +583,8 @@ int run_depths(psdr_scene* s, const WaveArgs& a, const QueueView& q0, const Rays
   const int Kmax = static_cast<int>(a.nL + a.nB + 1);
   const int64_t ray_cap = static_cast<int64_t>(n) * Kmax;
   const int tgrid = trace_grid(ray_cap);
-  if (s->spill.ensure(std::max(s->spill.bytes, spill_bytes(tgrid)))) return PSDR_ERR_NOMEM;
+  DevBuf& spill = a.spill ? *a.spill : s->spill;
+  if (spill.ensure(std::max(spill.bytes, spill_bytes(tgrid)))) return PSDR_ERR_NOMEM;
   const int sgrid = std::max(1, std::min(blocks(n, 256), num_cus() * 8));
   QueueView qin = q0;
   RaysView rin = r0;
@@ -615,13 +616,13 @@ int run_depths(psdr_scene* s, const WaveArgs& a, const QueueView& q0, const Rays
           const int64_t o = static_cast<int64_t>(k) * n;
           const int rc = trace_rays(sv, rout.o + 3 * o, rout.d + 3 * o, n, nullptr, 1, 0, rout.tl + o,
                                     k < static_cast<int>(a.nL) ? 1 : 0, rout.tri + o, nullptr, rout.u + o,
-                                    rout.v + o, s->spill.as<int>(), g1, st);
+                                    rout.v + o, spill.as<int>(), g1, st);
           if (rc) return rc;
           PSDR_HIP_CHECK(hipStreamSynchronize(st));
         }
       } else {
         const int rc = trace_rays(sv, rout.o, rout.d, n, a.counters + d + 1, P.K_out, n, rout.tl,
-                                  static_cast<int>(a.nL), rout.tri, nullptr, rout.u, rout.v, s->spill.as<int>(), tgrid,
+                                  static_cast<int>(a.nL), rout.tri, nullptr, rout.u, rout.v, spill.as<int>(), tgrid,
                                   st);
         if (rc) return rc;
       }
@@ -645,6 +646,15 @@ int render_forward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& cam
   const uint32_t nL = cfg.mis_light_samples, nB = cfg.mis_bsdf_samples;
   const int Kmax = static_cast<int>(nL + nB + 1);
   const int64_t ray_cap = static_cast<int64_t>(N) * std::max(Kmax, 1);
+  // a previous boundary prepass reads buffers this render rewrites or the
+  // caller may change afterwards: order it before this render
+  int rcj = aux_join(s, st);
+  if (rcj) return rcj;
+  const bool prep = keep && !s->plain && s->n_edges > 0 && (cfg.spp_primary_edge > 0 || cfg.spp_secondary_edge > 0);
+  if (prep) {
+    if (s->aux.ensure()) return PSDR_ERR_HIP;
+    PSDR_HIP_CHECK(hipEventRecord(s->aux.fork, st));
+  }
   // buffers
   if (s->lane_L.ensure(12ull * N) || s->q_a.ensure(QUEUE_BYTES_PER * N) || s->q_b.ensure(QUEUE_BYTES_PER * N) ||
       s->rays.ensure(RAY_BYTES_PER * ray_cap) || s->hits.ensure(12ull * ray_cap) || s->counters_r.ensure(4 * COUNTERS))
@@ -700,6 +710,14 @@ int render_forward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& cam
                                                                         : 1;
   hipLaunchKernelGGL(k_film, dim3(blocks(P_, ppb)), dim3(256), 0, st, lane_L, P_, cfg.spp_interior, cfg.n_pass, out);
   PSDR_HIP_CHECK(hipGetLastError());
+  // renderD: the boundary terms' dL/dI-independent work runs on the auxiliary
+  // streams from the fork point recorded before this render, overlapping it
+  // and the caller's loss
+  s->bprep = false;
+  if (prep) {
+    rc = boundary_prepare_async(s, cfg, camd);
+    if (rc) return rc;
+  }
   s->last_N = N;
   s->last_K = Kmax;
   s->last_D = D;

@@ -130,14 +130,20 @@ struct psdr_scene {
   // boundary estimators
   psdr::DevBuf b_rec, b_rays, b_hits, b_q0, b_LA, b_LB, b_rngA, b_rngB, b_state, b_tmp, b_counters, b_keys, b_vals,
       b_hist;
+  // primary-boundary wavefront scratch (separate from the interior render's)
+  psdr::DevBuf p_qa, p_qb, p_rays, p_hits, p_rays2, p_hits2, p_spill;
   // direct-boundary scratch (separate from the primary pass's, so both can run at once)
-  psdr::DevBuf d_state, d_mass, d_pmf, d_cmf, d_tmp, d_spill, d_keys, d_vals, d_hist;
+  psdr::DevBuf d_state, d_mass, d_pmf, d_cmf, d_tmp, d_spill, d_keys, d_vals, d_hist, d_rec;
   psdr::AuxStreams aux;
   // Reference launch shape, set for the duration of a *_plain call: every
   // depth's kernels run over all lanes (no compaction), each ray kind is traced
   // by its own launch followed by a host sync, boundary samples are processed in
   // lane order and the gradient terms run one after another.
   bool plain = false;
+  // boundary prepass issued by the last forward render with keep != 0
+  bool bprep = false;
+  psdr_config bprep_cfg{};
+  psdr_camera bprep_cam{};
   int light_tri_total = 0;
   // last forward (for backward / stats)
   uint64_t last_N = 0;

@@ -56,6 +56,7 @@ struct WaveArgs {
   uint64_t* lane_rng_out;
   int* counters;  // 128 ints, zeroed by the callee except counters[0] = *count0
   int plain = 0;  // reference launch shape (see psdr_scene::plain)
+  DevBuf* spill = nullptr;  // traversal stack overflow area (grown as needed)
 };
 int run_depths(psdr_scene* s, const WaveArgs& a, const QueueView& q0, const RaysView& r0, const QueueView* q_s,
                const RaysView* r_s, int n, hipStream_t st);
