@@ -495,7 +495,7 @@ int g_cus_b = 0;
 int cus() {
   if (!g_cus_b) {
     int dev = 0;
-    hipGetDevice(&dev);
+    (void)hipGetDevice(&dev);
     hipDeviceProp_t p;
     g_cus_b = (hipGetDeviceProperties(&p, dev) == hipSuccess && p.multiProcessorCount > 0) ? p.multiProcessorCount
                                                                                          : 256;

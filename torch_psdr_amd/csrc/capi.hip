@@ -470,7 +470,7 @@ int psdr_scene_update(psdr_scene* s, const psdr_scene_desc* desc, void* stream) 
 
 int psdr_scene_destroy(psdr_scene* s) {
   if (!s) return PSDR_OK;
-  hipDeviceSynchronize();
+  (void)hipDeviceSynchronize();
   delete s;  // buffers, streams and events are released by their owners
   return PSDR_OK;
 }

@@ -417,7 +417,7 @@ int g_cus = 0;
 int num_cus() {
   if (g_cus == 0) {
     int dev = 0;
-    hipGetDevice(&dev);
+    (void)hipGetDevice(&dev);
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, dev) == hipSuccess) g_cus = prop.multiProcessorCount;
     if (g_cus <= 0) g_cus = 256;
