@@ -49,6 +49,10 @@ struct AdjParams {
   const int* ctri;
   const float* cu;
   const float* cv;
+  const float* cT;      // [D][3][N] throughput per depth (forward render)
+  const uint64_t* crs;  // [D][N] PCG state per depth
+  const float* cL;      // [3][N] path radiance
+  const int* cinfo;     // [N] depths | 256 light camera vertex | 512 radiance not finite
   const float* dLdI;
   float* grad_v;
   float* grad_t;
@@ -435,15 +439,18 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adjoint(AdjParams P) {
     f3 g = f3{P.dLdI[3 * pix], P.dLdI[3 * pix + 1], P.dLdI[3 * pix + 2]};
     if (P.spp > 1) g = g / static_cast<float>(P.spp);
     if (g.x == 0.f && g.y == 0.f && g.z == 0.f) continue;
-    Rng rng = lane_rng(static_cast<uint32_t>(lane));
-    rng.next();
-    rng.next();
+    const int info = P.cinfo[lane];
+    if (info & 512) continue;  // the forward dropped this lane's non-finite radiance
+    const Rng rng{0, lane_inc(static_cast<uint32_t>(lane))};
     int t0;
     float u0, v0;
     draw_cached(P, 0, lane, &t0, &u0, &v0);
-    Its its = reconstruct(sv, t0, u0, v0, P.cam.pos, true);
+    const Its its = reconstruct(sv, t0, u0, v0, P.cam.pos, true);
     if (!its.valid) continue;
-    // forward sweep: per-depth PCG state, throughput and vertex
+    // per-depth state recorded by the forward render: PCG state and throughput
+    // at each vertex, and the vertices themselves (the extension hits)
+    const int n = min(info & 255, DMAX);  // depths with light/BSDF work
+    const bool light0 = (info & 256) != 0;  // camera vertex on a light, enable_light_visable
     uint64_t rs[DMAX];
     f3 Ts[DMAX];
     int vt[DMAX + 1];
@@ -451,33 +458,14 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adjoint(AdjParams P) {
     vt[0] = t0;
     vu[0] = u0;
     vv[0] = v0;
-    f3 T = splat(1.f), L = splat(0.f);
-    int n = 0;            // depths with NEE/BSDF work
-    bool light0 = false;  // camera vertex on a light with enable_light_visable
-    Its cur = its;
-    for (int d = 0; d < P.D && d < DMAX; ++d) {
-      if (cur.is_light) {
-        if (P.elv && d == 0) {
-          L = L + T * light_emit(sv, cur.light);
-          light0 = true;
-        }
-        break;
-      }
-      rs[d] = rng.state;
-      Ts[d] = T;
-      n = d + 1;
-      const DepthOut o = depth_step(P, nullptr, cur, rng, d, lane, false, splat(0.f), splat(0.f));
-      L = L + T * o.c;
-      // advance the stream past this depth's draws
-      for (uint32_t q = 0; q < 2 * (P.nL + P.nB + 1); ++q) rng.next_u32();
-      if (!o.ext) break;
-      T = T * o.t;
-      vt[d + 1] = o.ntri;
-      vu[d + 1] = o.nu;
-      vv[d + 1] = o.nv;
-      cur = reconstruct(sv, o.ntri, o.nu, o.nv, cur.p, true);
+    for (int d = 0; d < n; ++d) {
+      rs[d] = P.crs[d * P.N + lane];
+      Ts[d] = f3{P.cT[(3ll * d) * P.N + lane], P.cT[(3ll * d + 1) * P.N + lane], P.cT[(3ll * d + 2) * P.N + lane]};
+      // extension hit of depth d (used only where depth d's extension is valid)
+      draw_cached(P, 1 + static_cast<int64_t>(d) * P.Kmax + P.nL + P.nB, lane, &vt[d + 1], &vu[d + 1], &vv[d + 1]);
     }
-    if (!isfinite(sqnorm(L))) continue;
+    const f3 L = f3{P.cL[lane], P.cL[P.N + lane], P.cL[2 * P.N + lane]};
+    const Its& cur = its;
     // enable_light_visable: L += T0 * emit at a light camera vertex
     if (light0) emit_vjp(P, hot, cur.light, g);
     // reverse sweep.  pend = adjoint of vertex d+1 gathered so far (its own
@@ -627,6 +615,11 @@ int interior_backward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& 
   P.ctri = s->hit_cache.as<int>();
   P.cu = reinterpret_cast<const float*>(P.ctri + slots * P.N);
   P.cv = P.cu + slots * P.N;
+  const int64_t Dr = std::max(s->last_D, 1);
+  P.crs = s->replay.as<uint64_t>();
+  P.cT = reinterpret_cast<const float*>(P.crs + Dr * P.N);
+  P.cL = P.cT + 3 * Dr * P.N;
+  P.cinfo = reinterpret_cast<const int*>(P.cL + 3 * P.N);
   P.dLdI = dL_dI;
   // scratch for absent gradient buffers
   if (s->stats.ensure(4ull * (3 * static_cast<size_t>(s->n_vertices) + s->n_texels + 3 * s->n_lights + 64)))

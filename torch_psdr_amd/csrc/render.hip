@@ -157,6 +157,7 @@ __global__ __launch_bounds__(256) void k_shade(ShadeParams P, QueueView qi, cons
     const int j = base + threadIdx.x;
     bool have = j < cnt;
     bool alive = false;
+    bool light0 = false;   // camera vertex on a light, counted with enable_light_visable
     bool dead_in = false;  // plain shape: slot of a path that finished earlier
     uint32_t lane = 0;
     Rng rng{0, 0};
@@ -294,6 +295,7 @@ __global__ __launch_bounds__(256) void k_shade(ShadeParams P, QueueView qi, cons
           if (P.elv && d == 0) {
             const int l = its.light;
             L = L + T * f3{sv.light_emit[3 * l], sv.light_emit[3 * l + 1], sv.light_emit[3 * l + 2]};
+            light0 = true;
           }
           alive = false;
         }
@@ -315,6 +317,13 @@ __global__ __launch_bounds__(256) void k_shade(ShadeParams P, QueueView qi, cons
     }
     if (have && alive) {
       const uint64_t rng_start = rng.state;
+      if (P.keep) {
+        const int64_t N = cache.N;
+        cache.T[(3ll * d) * N + lane] = T.x;
+        cache.T[(3ll * d + 1) * N + lane] = T.y;
+        cache.T[(3ll * d + 2) * N + lane] = T.z;
+        cache.rs[d * N + lane] = rng_start;
+      }
       const int rs = qo.cap;
       int k = 0;
       for (uint32_t i = 0; i < P.nL; ++i, ++k) {
@@ -357,7 +366,15 @@ __global__ __launch_bounds__(256) void k_shade(ShadeParams P, QueueView qi, cons
       qo.u[slot] = cu;
       qo.v[slot] = cv;
     } else if (have) {
-      if (!isfinite(sqnorm(L))) L = splat(0.f);
+      const bool finite = isfinite(sqnorm(L));
+      if (P.keep) {
+        const int64_t N = cache.N;
+        cache.L[lane] = L.x;
+        cache.L[N + lane] = L.y;
+        cache.L[2 * N + lane] = L.z;
+        cache.info[lane] = d | (light0 ? 256 : 0) | (finite ? 0 : 512);
+      }
+      if (!finite) L = splat(0.f);
       L = L * J0;
       lane_L[3 * static_cast<int64_t>(lane)] = L.x;
       lane_L[3 * static_cast<int64_t>(lane) + 1] = L.y;
@@ -662,12 +679,17 @@ int render_forward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& cam
   const int tgrid = trace_grid(ray_cap);
   if (s->spill.ensure(spill_bytes(tgrid))) return PSDR_ERR_NOMEM;
   const int64_t slots = 1 + static_cast<int64_t>(D) * Kmax;
-  CacheView cache{nullptr, nullptr, nullptr, N};
+  CacheView cache{nullptr, nullptr, nullptr, N, nullptr, nullptr, nullptr, nullptr};
   if (keep) {
-    if (s->hit_cache.ensure(12ull * slots * N)) return PSDR_ERR_NOMEM;
+    if (s->hit_cache.ensure(12ull * slots * N) || s->replay.ensure((20ull * std::max(D, 1) + 16ull) * N))
+      return PSDR_ERR_NOMEM;
     cache.tri = s->hit_cache.as<int>();
     cache.u = reinterpret_cast<float*>(cache.tri + slots * N);
     cache.v = cache.u + slots * N;
+    cache.rs = s->replay.as<uint64_t>();
+    cache.T = reinterpret_cast<float*>(cache.rs + static_cast<int64_t>(std::max(D, 1)) * N);
+    cache.L = cache.T + 3ll * std::max(D, 1) * N;
+    cache.info = reinterpret_cast<int*>(cache.L + 3ll * N);
   }
   QueueView qa = carve_queue(s->q_a, N), qb = carve_queue(s->q_b, N);
   RaysView rays_a = carve_rays(s->rays, s->hits, ray_cap);

@@ -125,6 +125,7 @@ struct psdr_scene {
   psdr::DevBuf morton, order, child, parent, box, leaf, node, counters, sort_keys, sort_vals, sort_hist, bounds,
       ranges, bvh_boxes;
   // renderer scratch
+  psdr::DevBuf replay;  // per-lane, per-depth state of the last forward with keep (path replay)
   psdr::DevBuf lane_L, q_a, q_b, rays, hits, rays2, hits2, spill, hit_cache, counters_r, hc_mass, hc_pmf,
       hc_cmf, stats;
   // boundary estimators

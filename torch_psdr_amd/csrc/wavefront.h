@@ -35,6 +35,11 @@ struct CacheView {
   float* u;
   float* v;
   int64_t N;
+  // per-depth replay state written by the forward render (keep != 0)
+  float* T;      // [D][3][N] throughput at the vertex of depth d
+  uint64_t* rs;  // [D][N] PCG state before depth d's draws
+  float* L;      // [3][N] path radiance before the area jacobian
+  int* info;     // [N] depths with light/BSDF work | 256 light camera vertex | 512 radiance not finite
 };
 
 QueueView carve_queue(DevBuf& b, int cap);
