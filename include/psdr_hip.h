@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define PSDR_ABI_VERSION 1
+#define PSDR_ABI_VERSION 2
 
 /* status codes */
 #define PSDR_OK 0
@@ -47,6 +47,8 @@ typedef struct psdr_object {
   int32_t rough_h, rough_w;
   int32_t tri_begin, tri_count; /* triangle range of this object */
   int32_t grad_mask;       /* PSDR_GRAD_* bits: which of this object's inputs need gradients */
+  int32_t normal_offset;   /* float offset of the 3-channel normal map (DiffuseBsdf), or -1 for none */
+  int32_t normal_h, normal_w;
 } psdr_object;
 
 /* grad_mask bits (mirror `requires_grad` of the wrapped tensors,
@@ -55,6 +57,7 @@ typedef struct psdr_object {
 #define PSDR_GRAD_ALBEDO 2
 #define PSDR_GRAD_ROUGHNESS 4
 #define PSDR_GRAD_EMIT 8
+#define PSDR_GRAD_NORMAL 16
 
 /* Scene description.  Device arrays: vertices, faces, tri_uv, tri_obj, texels,
  * light_emit.  Host arrays: objects, light_obj. */

@@ -23,6 +23,7 @@ class OracleScene(C.Structure):
         ("obj_light", C.c_void_p),
         ("n_texels", C.c_int), ("texels", C.c_void_p), ("d_texels", C.c_void_p),
         ("n_lights", C.c_int), ("light_emit", C.c_void_p), ("d_light_emit", C.c_void_p), ("light_obj", C.c_void_p),
+        ("obj_normal", C.c_void_p),
     ]
 
 
@@ -73,6 +74,8 @@ class Scene:
         self.obj_rough = np.array([[o["rough_offset"], o["rough_h"], o["rough_w"]] for o in objs],
                                   dtype=np.int32).reshape(-1)
         self.obj_light = np.array([o["light"] for o in objs], dtype=np.int32)
+        self.obj_normal = np.array([[o.get("normal_offset", -1), o.get("normal_h", 0), o.get("normal_w", 0)]
+                                    for o in objs], dtype=np.int32).reshape(-1)
         self.light_obj = np.array(flat.light_obj, dtype=np.int32)
         cv = lambda a: None if a is None else np.ascontiguousarray(np.asarray(a, dtype=np.float32).reshape(-1))
         self.d_vertices, self.d_texels, self.d_emit = cv(d_vertices), cv(d_texels), cv(d_emit)
@@ -89,6 +92,7 @@ class Scene:
         s.texels, s.d_texels = _a(self.texels), _a(self.d_texels)
         s.n_lights = self.light_obj.size
         s.light_emit, s.d_light_emit, s.light_obj = _a(self.emit), _a(self.d_emit), _a(self.light_obj)
+        s.obj_normal = _a(self.obj_normal)
         self.s = s
 
 

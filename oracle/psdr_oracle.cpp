@@ -35,6 +35,7 @@ struct oracle_scene {
   const float* light_emit;    // [n_lights*3]
   const float* d_light_emit;  // tangent or NULL
   const int* light_obj;       // [n_lights]
+  const int* obj_normal;      // [n_objects*3]: normal map offset (-1: none), h, w; or NULL
 };
 
 struct oracle_camera {
@@ -254,6 +255,10 @@ template <class F> struct Scene {
       objs[o].mat = s.obj_mat[o];
       objs[o].albedo = {s.obj_albedo[3 * o], s.obj_albedo[3 * o + 1], s.obj_albedo[3 * o + 2], 3};
       objs[o].rough = {s.obj_rough[3 * o], s.obj_rough[3 * o + 1], s.obj_rough[3 * o + 2], 1};
+      if (s.obj_normal && s.obj_normal[3 * o] >= 0 && objs[o].mat == MAT_DIFFUSE_BSDF) {
+        objs[o].normal = {s.obj_normal[3 * o], s.obj_normal[3 * o + 1], s.obj_normal[3 * o + 2], 3};
+        objs[o].has_normal = true;
+      }
       objs[o].light = s.obj_light[o];
       objs[o].tri_begin = s.n_tris;
       objs[o].tri_count = 0;
@@ -363,6 +368,17 @@ template <class F> struct Scene {
     V3<F> bsdf;
     bool valid = false;
   };
+  // DiffuseBsdf shading normal (src/diffuse_bsdf.cc:13-19, 61-68): the
+  // geometric normal, or with a normal map normalize(TBN (2 n - 1)) where TBN
+  // has the columns T, B = Ng x T, Ng (the matrix-vector product summed in that
+  // column order).
+  V3<F> shading_normal(const Object& ob, const Its<F>& it) const {
+    if (!ob.has_normal) return it.n;
+    const V3<F> tn = tex3(ob.normal, it.uv, texels);
+    const V3<F> v = tn * F(2.0f) - V3<F>(F(1.0f));
+    const V3<F> B = cross(it.n, it.t);
+    return normalize((it.t * v.x + B * v.y) + it.n * v.z);
+  }
   // DiffuseBsdf shading value (src/diffuse_bsdf.cc:42-75)
   V3<F> bsdf_diffuse_bsdf(const Object& ob, const Its<F>& it, const V3<F>& L) const {
     const V3<F> color = tex3(ob.albedo, it.uv, texels);
@@ -370,7 +386,7 @@ template <class F> struct Scene {
     const F div = F(1.0f) / (F(REAL_PI) + F((3.0f * REAL_PI - 4.0f) / 6.0f) * sigma);
     const F a = div;
     const F b = sigma * div;
-    const V3<F>& N = it.n;
+    const V3<F> N = shading_normal(ob, it);
     const V3<F>& Vw = it.wi;
     F NdotL = dot(N, L);
     if (val(NdotL) < 0.f) NdotL = F(0.f);
@@ -393,9 +409,9 @@ template <class F> struct Scene {
   Sampled sample(const Its<F>& it, float ru, float rv, bool valid) const {
     Sampled s;
     const Object& ob = objs[it.obj];
-    const V3<F>& N = it.n;
     const V3<F>& T = it.t;
-    const V3<F> B = cross(N, T);
+    const V3<F> B = cross(it.n, T);
+    const V3<F> N = ob.mat == MAT_DIFFUSE_BSDF ? shading_normal(ob, it) : it.n;
     const float cos_t = safe_sqrt(ru);
     const float sin_t = safe_sqrt(1.0f - ru);
     const float phi = rv * REAL_2PI;
@@ -407,7 +423,7 @@ template <class F> struct Scene {
     s.pdf = cos_t * REAL_INV_PI;
     s.valid = valid && it.valid;
     if (ob.mat == MAT_DIFFUSE_BSDF) {
-      s.valid = s.valid && val(dot(N, it.wi)) > 0.f;
+      s.valid = s.valid && val(dot(it.n, it.wi)) > 0.f;
       s.bsdf = bsdf_diffuse_bsdf(ob, it, s.wo);
     } else {
       s.bsdf = tex3(ob.albedo, it.uv, texels) * F(REAL_INV_PI);

@@ -538,6 +538,8 @@ struct Object {
   int mat = MAT_DIFFUSE;
   Texture albedo;
   Texture rough;
+  Texture normal;           // DiffuseBsdf normal map (has_normal)
+  bool has_normal = false;
   int light = -1;
   int tri_begin = 0, tri_count = 0;
 };

@@ -12,8 +12,9 @@ from torch_psdr_amd.flat import flatten
 
 
 def objects(device, level=1, tex_res=None, tex_seed=1, sphere_vertices=None, light_y=4.0, emit=10.0,
-            bsdf_rough=None, requires_grad=False):
-    """Sphere (+ optional UV texture), ground quad, quad light: (meshes, lights, leaves)."""
+            bsdf_rough=None, normal_res=None, requires_grad=False):
+    """Sphere (+ optional UV texture, DiffuseBsdf roughness, normal map), ground quad, quad
+    light: (meshes, lights, leaves)."""
     Vs, Fs = W.icosphere(level)
     if sphere_vertices is not None:
         Vs = sphere_vertices
@@ -23,6 +24,9 @@ def objects(device, level=1, tex_res=None, tex_seed=1, sphere_vertices=None, lig
     if tex_res is None:
         color = torch.tensor([0.5, 0.5, 0.5], device=device, requires_grad=requires_grad)
         uv, uvi = None, None
+        if normal_res is not None:
+            uv = torch.tensor(W.sphere_uv(W.icosphere(level)[0]), device=device)
+            uvi = fs
     else:
         g = torch.Generator().manual_seed(tex_seed)
         color = (0.2 + 0.6 * torch.rand((tex_res, tex_res, 3), generator=g)).to(device)
@@ -35,7 +39,15 @@ def objects(device, level=1, tex_res=None, tex_seed=1, sphere_vertices=None, lig
     else:
         rough = torch.tensor([bsdf_rough], device=device, requires_grad=requires_grad)
         leaves["rough"] = rough
-        mat = dr.DiffuseBsdfMaterial(color, rough)
+        normal = None
+        if normal_res is not None:
+            # tangent-space normals near +z: 2n-1 in [-0.3, 0.3]^2 x [0.6, 1]
+            gn = torch.Generator().manual_seed(tex_seed + 100)
+            r = torch.rand((normal_res, normal_res, 3), generator=gn)
+            normal = torch.stack([0.35 + 0.3 * r[..., 0], 0.35 + 0.3 * r[..., 1], 0.8 + 0.2 * r[..., 2]], -1)
+            normal = normal.to(device).requires_grad_(requires_grad)
+            leaves["normal"] = normal
+        mat = dr.DiffuseBsdfMaterial(color, rough, normal)
     sphere = dr.Mesh(vs, uv, fs, uvi, mat)
     Vg, Fg = W.quad(10.0, -1.0)
     gv = torch.tensor(Vg, device=device, requires_grad=requires_grad)

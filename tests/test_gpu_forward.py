@@ -153,6 +153,8 @@ CASES = [
     dict(name="lights-visible", kw=dict(level=2), res=24, cfg=dict(spp_interior=2, max_bounce=2,
                                                                   enable_light_visable=True)),
     dict(name="bsdf-rough", kw=dict(level=2, bsdf_rough=0.5), res=24, cfg=dict(spp_interior=2, max_bounce=3)),
+    dict(name="bsdf-normal-map", kw=dict(level=2, bsdf_rough=0.5, tex_res=8, normal_res=8), res=24,
+         cfg=dict(spp_interior=2, max_bounce=3)),
     dict(name="npass3", kw=dict(level=1), res=16, cfg=dict(spp_interior=3, max_bounce=2, n_pass=3)),
 ]
 

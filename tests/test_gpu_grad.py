@@ -74,6 +74,8 @@ INTERIOR_CASES = [
     dict(name="bsdf-rough", kw=dict(level=2, bsdf_rough=0.5, tex_res=8), res=16,
          cfg=dict(spp_interior=2, max_bounce=2)),
     dict(name="elv", kw=dict(level=1), res=16, cfg=dict(spp_interior=2, max_bounce=2, enable_light_visable=True)),
+    dict(name="normal-map", kw=dict(level=2, bsdf_rough=0.5, tex_res=8, normal_res=8), res=16,
+         cfg=dict(spp_interior=2, max_bounce=2)),
 ]
 
 

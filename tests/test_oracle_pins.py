@@ -422,3 +422,50 @@ def test_primary_boundary_matches_finite_differences(oracle, axis):
     fd = float((g * (plus - minus)).sum() / (2 * eps))
     assert abs(fd) > 100.0
     assert abs(jvp - fd) <= 0.03 * abs(fd), (jvp, fd)
+
+
+def test_flat_normal_map_equals_geometric_normal(oracle):
+    """A normal map whose every texel is (0.5, 0.5, 1) maps to TBN (0, 0, 1) = Ng,
+    so the image equals the one rendered without a normal map (to the rounding
+    of renormalising the unit normal)."""
+    cfg = [1, 4, 0, 1, 1, 3, 1, 1, 1, 1, 10000, 10000, 6, 6]
+    cp = H.cam_params(24)
+    from torch_psdr_amd.flat import flatten
+    meshes, lights, lv = H.objects(torch.device("cpu"), level=2, bsdf_rough=0.5, tex_res=8, normal_res=4)
+    with torch.no_grad():
+        lv["normal"].copy_(torch.tensor([0.5, 0.5, 1.0]).expand_as(lv["normal"]))
+    flat_mapped = flatten(meshes, lights, torch.device("cpu"))
+    meshes2, lights2, _ = H.objects(torch.device("cpu"), level=2, bsdf_rough=0.5, tex_res=8)
+    flat_plain = flatten(meshes2, lights2, torch.device("cpu"))
+    a = oracle.render(oracle.Scene(flat_mapped), cfg, cp)
+    b = oracle.render(oracle.Scene(flat_plain), cfg, cp)
+    assert np.abs(a).sum() > 0
+    np.testing.assert_allclose(a, b, rtol=1e-4, atol=1e-6)
+
+
+def test_normal_map_texel_jvp_matches_central_difference(oracle):
+    """Forward-mode derivative w.r.t. the normal-map texels equals the central
+    difference of the image.  Direct light only (one bounce, light samples only):
+    the map then enters through the BSDF value alone, as normalize(TBN (2n-1)).
+    (Through sampled directions the reference differentiates at fixed hit
+    barycentrics, which a difference quotient does not; that part is pinned by
+    the GPU-vs-oracle derivative identity instead.)"""
+    cfg = [1, 4, 0, 1, 1, 1, 1, 0, 1, 1, 10000, 10000, 6, 6]
+    cp = H.cam_params(24)
+    flat, leaves = H.cpu_flat(level=2, bsdf_rough=0.5, tex_res=8, normal_res=4)
+    objs = flat.objects
+    o = next(ob for ob in objs if ob.get("normal_offset", -1) >= 0)
+    off, n = o["normal_offset"], 3 * o["normal_h"] * o["normal_w"]
+    w = np.zeros(flat.texels.numel(), np.float32)
+    w[off:off + n] = np.random.default_rng(9).normal(size=n).astype(np.float32)
+    _, d_int, _, _ = oracle.render(oracle.Scene(flat, d_texels=w), cfg, cp, mode=1, boundary=False)
+    h = 1e-3
+    base = flat.texels.clone()
+    flat.texels = base + h * torch.from_numpy(w)
+    ip = oracle.render(oracle.Scene(flat), cfg, cp).astype(np.float64)
+    flat.texels = base - h * torch.from_numpy(w)
+    im = oracle.render(oracle.Scene(flat), cfg, cp).astype(np.float64)
+    flat.texels = base
+    fd = (ip - im) / (2 * h)
+    assert np.abs(d_int).max() > 1e-3
+    assert np.abs(fd - d_int).max() < 2e-3 * max(1.0, np.abs(fd).max())

@@ -19,13 +19,14 @@ PSDR_GRAD_VERTICES = 1
 PSDR_GRAD_ALBEDO = 2
 PSDR_GRAD_ROUGHNESS = 4
 PSDR_GRAD_EMIT = 8
+PSDR_GRAD_NORMAL = 16
 
 PSDR_TERM_INTERIOR = 1
 PSDR_TERM_PRIMARY = 2
 PSDR_TERM_DIRECT = 4
 PSDR_TERM_ALL = 7
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 
 class PsdrObject(C.Structure):
@@ -41,6 +42,9 @@ class PsdrObject(C.Structure):
         ("tri_begin", C.c_int32),
         ("tri_count", C.c_int32),
         ("grad_mask", C.c_int32),
+        ("normal_offset", C.c_int32),
+        ("normal_h", C.c_int32),
+        ("normal_w", C.c_int32),
     ]
 
 

@@ -3,14 +3,13 @@
 Same class names, constructor arguments, shapes, dtypes, defaults and
 assertions as the reference wrapper; `PathIntegrator.renderC` / `renderD`
 return a list of (H, W, 3) fp32 images, one per camera (row 0 = top), and
-`renderD`'s images are differentiable w.r.t. mesh / light vertices, albedo and
-roughness texels and light emission through torch autograd.
+`renderD`'s images are differentiable w.r.t. mesh / light vertices, albedo,
+roughness and normal-map texels and light emission through torch autograd.
 
 Differences, each deliberate:
   * 2-D (H, W) textures are accepted as 1-channel maps (the reference's `==`
     typo at src/python/torch_psdr.py:233 rejects them);
-  * OrthographicCamera, EnvLight, SH9Light and normal-mapped DiffuseBsdfMaterial
-    raise NotImplementedError (no workload of this build uses them);
+  * OrthographicCamera, EnvLight and SH9Light raise NotImplementedError;
   * camera parameters are constants: they get no gradient (a warning is given
     when one requires grad);
   * the stray `from turtle import forward` (src/python/torch_psdr.py:2) is gone.
@@ -148,12 +147,13 @@ class DiffuseBsdfMaterial(Material):
         assert color.tex_type == "Constant3f" or color.tex_type == "UV3f"
         assert roughness.tex_type == "Constant1f" or roughness.tex_type == "UV1f"
         assert normal is None or normal.tex_type == "Constant3f" or normal.tex_type == "UV3f"
-        if normal is not None:
-            raise NotImplementedError("normal maps are not implemented by torch_psdr_amd")
         self.color = color
         self.roughness = roughness
+        self.normal = normal
         self.merge_variables(*color.get_variables())
         self.merge_variables(*roughness.get_variables())
+        if normal is not None:
+            self.merge_variables(*normal.get_variables())
 
 
 class PrincipledMaterial(Material):

@@ -144,7 +144,7 @@ __device__ __forceinline__ float flip_of(const SceneView& sv, const Its& it) {
   return (it.n.x == g3.x && it.n.y == g3.y && it.n.z == g3.z) ? 1.f : -1.f;
 }
 
-// VJP of Material::eval at (its, L) w.r.t. texels, its.n, L and its.wi.
+// VJP of Material::eval at (its, L) w.r.t. texels, the shading normal, L and its.wi.
 struct BsdfAdj {
   f3 N, L, V;
 };
@@ -161,7 +161,8 @@ __device__ BsdfAdj bsdf_vjp(const AdjParams& P, float* hot, const Its& it, f3 L,
     const float kk = (3.0f * REAL_PI - 4.0f) / 6.0f;
     const float div = 1.0f / (REAL_PI + kk * sigma);
     const float a = div, b = sigma * div;
-    const float nl_raw = dot(it.n, L), nv_raw = dot(it.n, it.wi);
+    const f3 N = shading_normal(sv, ob, it);
+    const float nl_raw = dot(N, L), nv_raw = dot(N, it.wi);
     const float NdotL = nl_raw < 0.f ? 0.f : nl_raw;
     const float NdotV = nv_raw < 0.f ? 0.f : nv_raw;
     const float VdotL = dot(it.wi, L);
@@ -189,11 +190,11 @@ __device__ BsdfAdj bsdf_vjp(const AdjParams& P, float* hot, const Its& it, f3 L,
     }
     if (nl_raw > 0.f) {
       r.N = r.N + L * gNL;
-      r.L = r.L + it.n * gNL;
+      r.L = r.L + N * gNL;
     }
     if (nv_raw > 0.f) {
       r.N = r.N + it.wi * gNV;
-      r.V = r.V + it.n * gNV;
+      r.V = r.V + N * gNV;
     }
     r.V = r.V + L * gVL;
     r.L = r.L + it.wi * gVL;
@@ -210,6 +211,29 @@ __device__ BsdfAdj bsdf_vjp(const AdjParams& P, float* hot, const Its& it, f3 L,
     }
   }
   return r;
+}
+
+// Adjoint gN of the shading normal into the vertex adjoints: the geometric
+// normal itself, or through the normal map N = normalize(M), M = (T x + B y) +
+// Ng z, B = Ng x T, (x, y, z) = 2 n - 1 into T, Ng and the normal-map texel.
+__device__ void shading_normal_vjp(const AdjParams& P, const Its& it, f3 gN, VertexAdj& cur) {
+  const SceneView& sv = P.sv;
+  const ObjectK& ob = sv.objs[it.obj];
+  if (!(ob.mat == PSDR_MAT_DIFFUSE_BSDF && ob.nrm_off >= 0)) {
+    cur.n = cur.n + gN;
+    return;
+  }
+  const int o = tex3_offset(ob.nrm_off, ob.nrm_h, ob.nrm_w, it.uv);
+  const f3 tn = tex3(sv.texels, o);
+  const f3 v = tn * 2.0f - splat(1.0f);
+  const f3 B = cross(it.n, it.t);
+  const f3 M = (it.t * v.x + B * v.y) + it.n * v.z;
+  const f3 gM = vjp_normalize(M, gN);
+  const f3 gB = gM * v.y;
+  cur.t = cur.t + gM * v.x + cross(gB, it.n);
+  cur.n = cur.n + gM * v.z + cross(it.t, gB);
+  if (ob.grad_mask & PSDR_GRAD_NORMAL)
+    adj_cache().add(P.grad_t, 1, o, f3{dot(gM, it.t), dot(gM, B), dot(gM, it.n)} * 2.0f);
 }
 
 __device__ __forceinline__ void emit_vjp(const AdjParams& P, float* hot, int l, f3 g) {
@@ -294,7 +318,8 @@ __device__ DepthOut depth_step(const AdjParams& P, float* hot, const Its& its, R
     const float gG = dot(av, emit * bsdf) * Jl;
     const float gJ = dot(av, emit * bsdf) * G;
     const BsdfAdj ba = bsdf_vjp(P, hot, its, dir, gbsdf);
-    out.cur.n = out.cur.n + ba.N + dir * (gG * tmp);
+    shading_normal_vjp(P, its, ba.N, out.cur);
+    out.cur.n = out.cur.n + dir * (gG * tmp);
     gV = gV + ba.V;
     f3 gdir = ba.L + its.n * (gG * tmp);
     const float gtmp = gG * nd;
@@ -344,7 +369,8 @@ __device__ DepthOut depth_step(const AdjParams& P, float* hot, const Its& its, R
     const float gG = dot(av, emit * smp.bsdf) * Jl;
     const float gJ = dot(av, emit * smp.bsdf) * G;
     const BsdfAdj ba = bsdf_vjp(P, hot, its, smp.wo, gbsdf);
-    out.cur.n = out.cur.n + ba.N + smp.wo * (gG * tmp);
+    shading_normal_vjp(P, its, ba.N, out.cur);
+    out.cur.n = out.cur.n + smp.wo * (gG * tmp);
     gV = gV + ba.V;
     f3 gwo = ba.L + its.n * (gG * tmp);
     const float gtmp = gG * nd;
@@ -356,13 +382,14 @@ __device__ DepthOut depth_step(const AdjParams& P, float* hot, const Its& its, R
     lh.p = gpl;
     out.cur.p = out.cur.p - gpl;
     lh.area = gJ / itl.area;
-    // wo = normalize(u T + v B + cos N), B = cross(N, T)
+    // wo = normalize(u T + v B + cos N), B = cross(Ng, T), N the shading normal
     const f3 B = cross(its.n, its.t);
-    const f3 wu = (smp.u * its.t + smp.v * B) + smp.cos_t * its.n;
+    const f3 wu = (smp.u * its.t + smp.v * B) + smp.cos_t * shading_normal(sv, sv.objs[its.obj], its);
     const f3 gwu = vjp_normalize(wu, gwo);
     const f3 gB = gwu * smp.v;
     out.cur.t = out.cur.t + gwu * smp.u + cross(gB, its.n);
-    out.cur.n = out.cur.n + gwu * smp.cos_t + cross(its.t, gB);
+    shading_normal_vjp(P, its, gwu * smp.cos_t, out.cur);
+    out.cur.n = out.cur.n + cross(its.t, gB);
     tri_vjp(P, itl.tri, hu, hv, flip_of(sv, itl), lh);
   }
   // ---- extension
@@ -392,7 +419,8 @@ __device__ DepthOut depth_step(const AdjParams& P, float* hot, const Its& its, R
           const float gJ = dot(at, smp.bsdf) * G;
           const float gG = dot(at, smp.bsdf) * Jn;
           const BsdfAdj ba = bsdf_vjp(P, hot, its, smp.wo, gbsdf);
-          out.cur.n = out.cur.n + ba.N + smp.wo * (gG * tmp);
+          shading_normal_vjp(P, its, ba.N, out.cur);
+          out.cur.n = out.cur.n + smp.wo * (gG * tmp);
           gV = gV + ba.V;
           f3 gwo = ba.L + its.n * (gG * tmp);
           const float gtmp = gG * nd;
@@ -404,11 +432,12 @@ __device__ DepthOut depth_step(const AdjParams& P, float* hot, const Its& its, R
           out.cur.p = out.cur.p - gpn;
           out.next.area = gJ / itn.area;
           const f3 B = cross(its.n, its.t);
-          const f3 wu = (smp.u * its.t + smp.v * B) + smp.cos_t * its.n;
+          const f3 wu = (smp.u * its.t + smp.v * B) + smp.cos_t * shading_normal(sv, sv.objs[its.obj], its);
           const f3 gwu = vjp_normalize(wu, gwo);
           const f3 gB = gwu * smp.v;
           out.cur.t = out.cur.t + gwu * smp.u + cross(gB, its.n);
-          out.cur.n = out.cur.n + gwu * smp.cos_t + cross(its.t, gB);
+          shading_normal_vjp(P, its, gwu * smp.cos_t, out.cur);
+          out.cur.n = out.cur.n + cross(its.t, gB);
         }
       }
     }
@@ -550,9 +579,9 @@ int boundary_prepare_async(psdr_scene* s, const psdr_config& cfg, const psdr_cam
   if (s->n_edges == 0 || s->plain || !s->aux.fork) return PSDR_OK;
   for (int i = 0; i < 2; ++i) PSDR_HIP_CHECK(hipStreamWaitEvent(s->aux.st[i], s->aux.fork, 0));
   int rc = PSDR_OK;
-  // direct first: its prepare is a handful of large launches
-  if (cfg.spp_secondary_edge > 0) rc = boundary_direct_prepare(s, cfg, cam, s->aux.st[1]);
-  if (!rc && cfg.spp_primary_edge > 0) rc = boundary_primary_prepare(s, cfg, cam, s->aux.st[0]);
+  // the primary chain is the longest: enqueue it first
+  if (cfg.spp_primary_edge > 0) rc = boundary_primary_prepare(s, cfg, cam, s->aux.st[0]);
+  if (!rc && cfg.spp_secondary_edge > 0) rc = boundary_direct_prepare(s, cfg, cam, s->aux.st[1]);
   if (rc) return rc;
   s->bprep = true;
   s->bprep_cfg = cfg;

@@ -233,6 +233,12 @@ static int check_desc(const psdr_scene_desc* d) {
       set_error(PSDR_ERR_ARG, "object " + std::to_string(o) + ": roughness texture out of bounds");
       return PSDR_ERR_ARG;
     }
+    const int64_t nrm = ob.normal_h == 0 ? 3 : 3ll * ob.normal_h * ob.normal_w;
+    if (ob.material == PSDR_MAT_DIFFUSE_BSDF && ob.normal_offset != -1 &&
+        (ob.normal_offset < 0 || ob.normal_h < 0 || ob.normal_w < 0 || ob.normal_offset + nrm > d->n_texels)) {
+      set_error(PSDR_ERR_ARG, "object " + std::to_string(o) + ": normal map out of bounds");
+      return PSDR_ERR_ARG;
+    }
   }
   for (int l = 0; l < d->n_lights; ++l) {
     const int o = d->light_obj[l];
@@ -294,8 +300,10 @@ static int upload(DevBuf& b, const void* src, size_t bytes, hipStream_t st) {
 }
 
 static ObjectK object_k(const psdr_object& b) {
-  return ObjectK{b.material, b.light,   b.albedo_offset, b.albedo_h,  b.albedo_w, b.rough_offset,
-                 b.rough_h,  b.rough_w, b.tri_begin,     b.tri_count, b.grad_mask};
+  return ObjectK{b.material,  b.light,   b.albedo_offset, b.albedo_h,  b.albedo_w,
+                 b.rough_offset, b.rough_h, b.rough_w,     b.tri_begin, b.tri_count,
+                 b.grad_mask, b.material == PSDR_MAT_DIFFUSE_BSDF ? b.normal_offset : -1,
+                 b.normal_h,  b.normal_w};
 }
 
 // Points the scene at the caller's vertex / texel / emission buffers.  On an

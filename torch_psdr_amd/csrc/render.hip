@@ -668,9 +668,15 @@ int render_forward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& cam
   int rcj = aux_join(s, st);
   if (rcj) return rcj;
   const bool prep = keep && !s->plain && s->n_edges > 0 && (cfg.spp_primary_edge > 0 || cfg.spp_secondary_edge > 0);
+  s->bprep = false;
   if (prep) {
+    // renderD: the boundary terms' dL/dI-independent work goes to the
+    // auxiliary streams first (the longest chain, enqueued before this render
+    // so it starts at once) and overlaps this render and the caller's loss
     if (s->aux.ensure()) return PSDR_ERR_HIP;
     PSDR_HIP_CHECK(hipEventRecord(s->aux.fork, st));
+    const int rcp = boundary_prepare_async(s, cfg, camd);
+    if (rcp) return rcp;
   }
   // buffers
   if (s->lane_L.ensure(12ull * N) || s->q_a.ensure(QUEUE_BYTES_PER * N) || s->q_b.ensure(QUEUE_BYTES_PER * N) ||
@@ -732,14 +738,6 @@ int render_forward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& cam
                                                                         : 1;
   hipLaunchKernelGGL(k_film, dim3(blocks(P_, ppb)), dim3(256), 0, st, lane_L, P_, cfg.spp_interior, cfg.n_pass, out);
   PSDR_HIP_CHECK(hipGetLastError());
-  // renderD: the boundary terms' dL/dI-independent work runs on the auxiliary
-  // streams from the fork point recorded before this render, overlapping it
-  // and the caller's loss
-  s->bprep = false;
-  if (prep) {
-    rc = boundary_prepare_async(s, cfg, camd);
-    if (rc) return rc;
-  }
   s->last_N = N;
   s->last_K = Kmax;
   s->last_D = D;

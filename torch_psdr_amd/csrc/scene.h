@@ -66,6 +66,7 @@ struct ObjectK {
   int rough_off, rough_h, rough_w;
   int tri_begin, tri_count;
   int grad_mask;
+  int nrm_off, nrm_h, nrm_w;  // normal map (DiffuseBsdf), nrm_off < 0: none
 };
 
 // Light record as seen by kernels.

@@ -72,14 +72,28 @@ __device__ __forceinline__ int tex1_offset(int off, int h, int w, f2 uv) {
 }
 __device__ __forceinline__ f3 tex3(const float* tex, int o) { return f3{tex[o], tex[o + 1], tex[o + 2]}; }
 
-// DiffuseBsdfMaterial value (src/diffuse_bsdf.cc:42-75) for N = its.n
-__device__ __forceinline__ f3 diffuse_bsdf_value(f3 color, float sigma, const Its& it, f3 L) {
+// DiffuseBsdf shading normal (src/diffuse_bsdf.cc:13-19, 61-68): the
+// geometric normal, or with a normal map normalize(TBN (2 n - 1)) where TBN has
+// the columns T, B = Ng x T, Ng (the matrix-vector product summed in that order).
+__device__ __forceinline__ f3 normal_map_value(const SceneView& sv, const ObjectK& ob, const Its& it) {
+  const int o = tex3_offset(ob.nrm_off, ob.nrm_h, ob.nrm_w, it.uv);
+  const f3 tn = tex3(sv.texels, o);
+  const f3 v = tn * 2.0f - splat(1.0f);
+  const f3 B = cross(it.n, it.t);
+  return normalize((it.t * v.x + B * v.y) + it.n * v.z);
+}
+__device__ __forceinline__ f3 shading_normal(const SceneView& sv, const ObjectK& ob, const Its& it) {
+  return (ob.mat == PSDR_MAT_DIFFUSE_BSDF && ob.nrm_off >= 0) ? normal_map_value(sv, ob, it) : it.n;
+}
+
+// DiffuseBsdfMaterial value (src/diffuse_bsdf.cc:42-75) for shading normal N
+__device__ __forceinline__ f3 diffuse_bsdf_value(f3 color, float sigma, f3 N, const Its& it, f3 L) {
   const float div = 1.0f / (REAL_PI + ((3.0f * REAL_PI - 4.0f) / 6.0f) * sigma);
   const float a = div;
   const float b = sigma * div;
-  float NdotL = dot(it.n, L);
+  float NdotL = dot(N, L);
   if (NdotL < 0.f) NdotL = 0.f;
-  float NdotV = dot(it.n, it.wi);
+  float NdotV = dot(N, it.wi);
   if (NdotV < 0.f) NdotV = 0.f;
   const float VdotL = dot(it.wi, L);
   float t = VdotL - NdotL * NdotV;
@@ -93,7 +107,7 @@ __device__ __forceinline__ f3 mat_eval(const SceneView& sv, const Its& it, f3 wo
   const f3 color = tex3(sv.texels, tex3_offset(ob.alb_off, ob.alb_h, ob.alb_w, it.uv));
   if (ob.mat == PSDR_MAT_DIFFUSE_BSDF) {
     const float sigma = sv.texels[tex1_offset(ob.rough_off, ob.rough_h, ob.rough_w, it.uv)];
-    return diffuse_bsdf_value(color, sigma, it, wo);
+    return diffuse_bsdf_value(color, sigma, shading_normal(sv, ob, it), it, wo);
   }
   return color * REAL_INV_PI;
 }
@@ -111,6 +125,7 @@ __device__ __forceinline__ Sampled mat_sample(const SceneView& sv, const Its& it
   Sampled s;
   const ObjectK& ob = sv.objs[it.obj];
   const f3 B = cross(it.n, it.t);
+  const f3 N = shading_normal(sv, ob, it);
   const float cos_t = safe_sqrt(ru);
   const float sin_t = safe_sqrt(1.0f - ru);
   const float phi = rv * REAL_2PI;
@@ -118,7 +133,7 @@ __device__ __forceinline__ Sampled mat_sample(const SceneView& sv, const Its& it
   psdr_sincos(phi, &sp, &cp);
   const float u = cp * sin_t;
   const float v = sp * sin_t;
-  s.wo = normalize((u * it.t + v * B) + cos_t * it.n);
+  s.wo = normalize((u * it.t + v * B) + cos_t * N);
   s.pdf = cos_t * REAL_INV_PI;
   s.cos_t = cos_t;
   s.u = u;
@@ -128,7 +143,7 @@ __device__ __forceinline__ Sampled mat_sample(const SceneView& sv, const Its& it
   if (ob.mat == PSDR_MAT_DIFFUSE_BSDF) {
     s.valid = s.valid && dot(it.n, it.wi) > 0.f;
     const float sigma = sv.texels[tex1_offset(ob.rough_off, ob.rough_h, ob.rough_w, it.uv)];
-    s.bsdf = diffuse_bsdf_value(color, sigma, it, s.wo);
+    s.bsdf = diffuse_bsdf_value(color, sigma, N, it, s.wo);
   } else {
     s.bsdf = color * REAL_INV_PI;
   }

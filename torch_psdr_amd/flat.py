@@ -100,7 +100,8 @@ def flatten(meshes, lights, device) -> FlatScene:
             uvs.append(torch.tensor(_DEFAULT_UV, dtype=torch.float32, device=device).expand(nf, 6))
         tobj.append(torch.full((nf,), oi, dtype=torch.int32, device=device))
         rec = dict(material=N.PSDR_MAT_DIFFUSE, light=-1, albedo_offset=0, albedo_h=0, albedo_w=0,
-                   rough_offset=0, rough_h=0, rough_w=0, tri_begin=toff, tri_count=nf, grad_mask=0)
+                   rough_offset=0, rough_h=0, rough_w=0, tri_begin=toff, tri_count=nf, grad_mask=0,
+                   normal_offset=-1, normal_h=0, normal_w=0)
         if v.requires_grad:
             rec["grad_mask"] |= N.PSDR_GRAD_VERTICES
         if light is not None:
@@ -122,6 +123,11 @@ def flatten(meshes, lights, device) -> FlatScene:
                 rec.update(rough_offset=r_off, rough_h=r_h, rough_w=r_w)
                 if mat.roughness.data.requires_grad:
                     rec["grad_mask"] |= N.PSDR_GRAD_ROUGHNESS
+                if mat.normal is not None:
+                    n_off, n_h, n_w, tex_off = _tex_entry(mat.normal, tex_pieces, tex_off)
+                    rec.update(normal_offset=n_off, normal_h=n_h, normal_w=n_w)
+                    if mat.normal.data.requires_grad:
+                        rec["grad_mask"] |= N.PSDR_GRAD_NORMAL
         objects.append(rec)
         vranges.append((voff, voff + nv))
         key.append((f.data_ptr(), f._version, tuple(f.shape), nv,
