@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define PSDR_ABI_VERSION 2
+#define PSDR_ABI_VERSION 3
 
 /* status codes */
 #define PSDR_OK 0
@@ -95,16 +95,27 @@ typedef struct psdr_config {
   uint32_t secondary_edge_hypercube_resolution[3];
 } psdr_config;
 
-/* PerspectiveCamera(look_from, look_at, up, vfov[rad], height, width)
- * (src/psdr.cc:22-29, src/camera.h:208-277). */
+/* Camera kinds: PerspectiveCamera(look_from, look_at, up, vfov[rad], height,
+ * width) (src/psdr.cc:22-29, src/camera.h:208-277) and
+ * OrthographicCamera(look_from, look_at, up, view_width, height, width)
+ * (src/psdr.cc:30-37, src/camera.h:279-362). */
+#define PSDR_CAMERA_PERSPECTIVE 0
+#define PSDR_CAMERA_ORTHOGRAPHIC 1
+
 typedef struct psdr_camera {
   float look_from[3];
   float look_at[3];
   float up[3];
-  float vfov;
+  float vfov;        /* perspective: vertical field of view in radians */
   int32_t height;
   int32_t width;
+  int32_t kind;      /* PSDR_CAMERA_* */
+  float view_width;  /* orthographic: visible width in world units */
 } psdr_camera;
+
+/* Camera gradient layout (PSDR_CAMERA_GRAD floats): d/d look_from[3],
+ * look_at[3], up[3], then vfov (perspective) or view_width (orthographic). */
+#define PSDR_CAMERA_GRAD 10
 
 typedef struct psdr_scene psdr_scene; /* opaque */
 
@@ -145,15 +156,22 @@ int psdr_render_forward(psdr_scene* scene, const psdr_config* cfg, const psdr_ca
 #define PSDR_TERM_ALL 7
 
 /* Backward of renderD (Enoki's FloatD.backward, src/python/torch_psdr.py:47-72)
- * for the image of the last psdr_render_forward(keep=1) with the same camera and
- * config: accumulates dLoss/d(vertices), d(texels), d(light emit) given dL_dI
- * (height, width, 3).  The gradient buffers are ADDED to (zero them first).
- * Any gradient pointer may be NULL.  `terms` selects PSDR_TERM_* (renderD uses
- * PSDR_TERM_ALL).  Interior term by path replay; primary and direct-shadow
- * boundary terms by edge sampling with their hypercube preprocessing
- * (src/path_integrator.cc:188-353). */
+ * for the image of a psdr_render_forward(keep=1) with the same camera and
+ * config: accumulates dLoss/d(vertices), d(texels), d(light emit) and
+ * d(camera parameters) (PSDR_CAMERA_GRAD floats) given dL_dI (height, width, 3).
+ * The gradient buffers are ADDED to (zero them first).  Any gradient pointer
+ * may be NULL.  `terms` selects PSDR_TERM_* (renderD uses PSDR_TERM_ALL).
+ * Interior term by path replay; primary and direct-shadow boundary terms by
+ * edge sampling with their hypercube preprocessing
+ * (src/path_integrator.cc:188-353).  Camera parameters reach the image
+ * derivative only through the ray origin: the camera vertex's direction to the
+ * viewer (Scene::hit, src/scene.cc:134-136) and the primary boundary's origin
+ * term (src/path_integrator.cc:240-244).  The scene keeps the forward state of
+ * its last few distinct (camera, config) pairs, so the backward passes of a
+ * multi-camera renderD find theirs without rendering again. */
 int psdr_render_backward(psdr_scene* scene, const psdr_config* cfg, const psdr_camera* cam, const float* dL_dI,
-                         float* grad_vertices, float* grad_texels, float* grad_emit, int32_t terms, void* stream);
+                         float* grad_vertices, float* grad_texels, float* grad_emit, float* grad_camera,
+                         int32_t terms, void* stream);
 
 /* The same two entry points in the reference's launch shape, kept as the
  * measured baseline of the fused path (no reference counterpart): every depth
@@ -167,7 +185,7 @@ int psdr_render_forward_plain(psdr_scene* scene, const psdr_config* cfg, const p
                               int32_t keep, void* stream);
 int psdr_render_backward_plain(psdr_scene* scene, const psdr_config* cfg, const psdr_camera* cam,
                                const float* dL_dI, float* grad_vertices, float* grad_texels, float* grad_emit,
-                               int32_t terms, void* stream);
+                               float* grad_camera, int32_t terms, void* stream);
 
 /* ---- component entry points (tests, benchmarks) --------------------------- */
 /* First n_draws PCG32 outputs of lanes [lane0, lane0+n_lanes) of Sampler
@@ -181,6 +199,14 @@ int psdr_camera_rays(const psdr_camera* cam, uint32_t spp, float* o, float* d, i
 /* Host copies of the camera matrices (row-major 4x4): camera2world,
  * world2camera, camera2raster, raster2camera; and the camera position. */
 int psdr_camera_matrices(const psdr_camera* cam, float* c2w, float* w2c, float* c2r, float* r2c, float* pos);
+
+/* Host: the chain rule from camera-matrix adjoints to camera parameters that
+ * psdr_render_backward applies on the device (Enoki's reverse mode through
+ * look_at / perspective / orthographic and the matrix inverses,
+ * src/camera.h:7-53, 108-124).  matrix_adjoints (25 floats): dLoss/d of
+ * camera2world columns 0, 1, 3 (rows 0-2 each), world2camera rows 0, 1 and
+ * raster2camera rows 0, 1; grad (PSDR_CAMERA_GRAD floats) is added to. */
+int psdr_camera_param_vjp(const psdr_camera* cam, const float* matrix_adjoints, float* grad);
 
 /* Closest hit (src/scene_optix.cc:240-277 + device_programs.cu:13-47) of n rays
  * against the scene: global triangle id (-1 on miss), t, barycentrics u, v

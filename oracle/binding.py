@@ -29,7 +29,8 @@ class OracleScene(C.Structure):
 
 class OracleCamera(C.Structure):
     _fields_ = [("look_from", C.c_float * 3), ("look_at", C.c_float * 3), ("up", C.c_float * 3),
-                ("vfov", C.c_float), ("height", C.c_int), ("width", C.c_int)]
+                ("vfov", C.c_float), ("height", C.c_int), ("width", C.c_int), ("kind", C.c_int),
+                ("view_width", C.c_float), ("d_params", C.c_float * 10)]
 
 
 class OracleConfig(C.Structure):
@@ -96,12 +97,21 @@ class Scene:
         self.s = s
 
 
-def camera(params):
+def camera(params, d_params=None):
+    """params = look_from(3), look_at(3), up(3), vfov | view_width, height, width[, kind];
+    d_params: tangent direction of the first 10 (renderD JVP w.r.t. the camera)."""
     c = OracleCamera()
     for i in range(3):
         c.look_from[i], c.look_at[i], c.up[i] = params[i], params[3 + i], params[6 + i]
-    c.vfov = params[9]
+    c.kind = int(params[12]) if len(params) > 12 else 0
+    if c.kind == 1:
+        c.view_width = params[9]
+    else:
+        c.vfov = params[9]
     c.height, c.width = int(params[10]), int(params[11])
+    if d_params is not None:
+        for i in range(10):
+            c.d_params[i] = float(d_params[i])
     return c
 
 
@@ -159,14 +169,36 @@ def camera_matrices(params):
     return outs, pos
 
 
-def camera_rays(params, spp: int):
-    cam = camera(params)
+def camera_rays(params, spp: int, d_params=None):
+    """Camera rays of every lane; with d_params also the origins' tangents."""
+    cam = camera(params, d_params)
     N = spp * int(params[10]) * int(params[11])
     o = np.zeros((N, 3), np.float32)
     d = np.zeros((N, 3), np.float32)
     pix = np.zeros(N, np.int32)
-    load().oracle_camera_rays(C.byref(cam), C.c_uint32(spp), _p(o), _p(d), _p(pix))
-    return o, d, pix
+    do = np.zeros((N, 3), np.float32) if d_params is not None else None
+    load().oracle_camera_rays(C.byref(cam), C.c_uint32(spp), _p(o), _p(d), _p(pix), _p(do))
+    return (o, d, pix) if d_params is None else (o, d, pix, do)
+
+
+def point_rays(params, pts, d_params, dpts):
+    """Rays through world points (origin, direction) and the origins' tangents."""
+    cam = camera(params, d_params)
+    pts = np.ascontiguousarray(pts, np.float32)
+    dpts = np.ascontiguousarray(dpts, np.float32)
+    o = np.zeros_like(pts)
+    d = np.zeros_like(pts)
+    do = np.zeros_like(pts)
+    load().oracle_point_rays(C.byref(cam), _p(pts), _p(dpts), pts.shape[0], _p(o), _p(d), _p(do))
+    return o, d, do
+
+
+def camera_tangents(params, d_params):
+    """Tangents of camera2world, world2camera, raster2camera along d_params."""
+    cam = camera(params, d_params)
+    outs = [np.zeros((4, 4), np.float32) for _ in range(3)]
+    load().oracle_camera_tangents(C.byref(cam), *[_p(o) for o in outs])
+    return outs
 
 
 def project_points(params, pts):
@@ -222,8 +254,10 @@ def lbvh(tri9: np.ndarray):
     return morton, order, child, parent, box
 
 
-def render(scene: Scene, cfg, cam_params, mode: int = 0, boundary: bool = True, lane_L: bool = False):
-    """mode 0: renderC image.  mode 1: (image, d_interior, d_primary, d_direct) JVP images."""
+def render(scene: Scene, cfg, cam_params, mode: int = 0, boundary: bool = True, lane_L: bool = False,
+           d_camera=None):
+    """mode 0: renderC image.  mode 1: (image, d_interior, d_primary, d_direct) JVP images
+    along the scene's d_* tangents and the camera tangent d_camera (10 floats)."""
     H, W = int(cam_params[10]), int(cam_params[11])
     img = np.zeros((H, W, 3), np.float32)
     d_int = np.zeros_like(img) if mode == 1 else None
@@ -231,7 +265,7 @@ def render(scene: Scene, cfg, cam_params, mode: int = 0, boundary: bool = True, 
     d_dir = np.zeros_like(img) if (mode == 1 and boundary) else None
     N = int(cfg[1]) * H * W
     lL = np.zeros((N, 3), np.float32) if lane_L else None
-    c, k = config(cfg), camera(cam_params)
+    c, k = config(cfg), camera(cam_params, d_camera)
     load().oracle_render(C.byref(scene.s), C.byref(c), C.byref(k), mode, _p(img), _p(d_int), _p(d_pri), _p(d_dir),
                          _p(lL))
     if mode == 0:

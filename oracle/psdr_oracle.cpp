@@ -42,6 +42,9 @@ struct oracle_camera {
   float look_from[3], look_at[3], up[3];
   float vfov;
   int height, width;
+  int kind;          // 0 perspective, 1 orthographic
+  float view_width;  // orthographic
+  float d_params[10];  // tangent direction of look_from, look_at, up, vfov | view_width
 };
 
 struct oracle_config {
@@ -596,8 +599,8 @@ struct BoundaryOut {
 // samples), so the boundary terms can be compared with finite differences.
 bool g_fd_measure = false;
 
-// eval_primary_boundary (src/path_integrator.cc:202-251). The camera is not a
-// differentiated input here, so the camera-origin term is zero.
+// eval_primary_boundary (src/path_integrator.cc:202-251), with the
+// camera-origin term (o - detach o) . (n (x) [1, 1, -1]) of :240-244.
 template <class F>
 V3<F> eval_primary(const Scene<F>& sc, const Scene<float>& scv, const Camera& cam, Pcg32& rng, float s1, uint32_t nL,
                    uint32_t D, bool ad, BoundaryOut* bo) {
@@ -610,8 +613,9 @@ V3<F> eval_primary(const Scene<F>& sc, const Scene<float>& scv, const Camera& ca
   const F pdf_ep = F(pdf_e) / E.len;
   const V3<float> epv = detach(ep);
   bool valid = cam.ray_with_point(epv, &bo->px, &bo->py);
-  const V3<float> o = cam.pos;
-  const V3<float> d = normalize(epv - o);
+  V3<float> o, d;
+  V3<Dual> o_ad;
+  cam.point_ray(epv, V3<float>{tan_(ep.x), tan_(ep.y), tan_(ep.z)}, &o, &d, &o_ad);
   const V3<float> n = cross(d, normalize(detach(E.p1) - detach(E.p0)));
   const V3<float> da = d - n * REAL_EPS;
   const V3<float> db = d + n * REAL_EPS;
@@ -624,7 +628,8 @@ V3<F> eval_primary(const Scene<F>& sc, const Scene<float>& scv, const Camera& ca
   const V3<float> dL = La - Lb;
   V3<F> value;
   if (ad) {
-    const F xn = dot(ep - lift<F>(epv), lift<F>(n));
+    const V3<float> nm{n.x, n.y, -n.z};
+    const F xn = dot(ep - lift<F>(epv), lift<F>(n)) + dot(origin_as<F>(o, o_ad) - lift<F>(o), lift<F>(nm));
     value = (lift<F>(dL) * xn) / F(detach(pdf_ep));
     if (g_fd_measure && valid)
       value = value * F(static_cast<float>(cam.raster_jacobian(epv, d) / (static_cast<double>(cam.H) * cam.W)));
@@ -664,8 +669,10 @@ V3<F> eval_direct(const Scene<F>& sc, const Camera& cam, float sx, float sy, flo
   const V3<float> xd = detach(id0.p);
   const bool vp = valid ? cam.ray_with_point(xd, &px, &py) : false;
   valid = valid && vp;
-  const V3<float> cd = normalize(xd - cam.pos);
-  const Its<F> i2 = sc.hit(lift<F>(cam.pos), lift<F>(cd), valid);
+  V3<float> co, cd;
+  V3<Dual> co_ad;
+  cam.point_ray(xd, V3<float>{0.f, 0.f, 0.f}, &co, &cd, &co_ad);
+  const Its<F> i2 = sc.hit(lift<F>(co), lift<F>(cd), valid);
   valid = valid && i2.valid && !i2.is_light && squared_norm(detach(i2.p) - xd) < REAL_EPS;
   bo->px = px;
   bo->py = py;
@@ -761,9 +768,10 @@ void render_interior(const Scene<F>& sc, const Camera& cam, const oracle_config&
       const float u0 = rng.next_float();
       const float u1 = rng.next_float();
       V3<float> o, d;
+      V3<Dual> o_ad;
       int x, y;
-      cam.ray(static_cast<uint32_t>(lane), spp, u0, u1, &o, &d, &x, &y);
-      const Its<F> its = sc.hit(lift<F>(o), lift<F>(d), true);
+      cam.ray(static_cast<uint32_t>(lane), spp, u0, u1, &o, &d, &x, &y, &o_ad);
+      const Its<F> its = sc.hit(origin_as<F>(o, o_ad), lift<F>(d), true);
       V3<F> L = trace2light<F>(sc, rng, its, its.valid, cfg.mis_light_samples, cfg.mis_bsdf_samples,
                                cfg.max_bounce, cfg.enable_light_visable != 0);
       L = L * its.jac;
@@ -883,6 +891,8 @@ void render_direct(const Scene<F>& sc, const Camera& cam, const oracle_config& c
 Camera make_camera(const oracle_camera* c) {
   Camera cam;
   cam.build(c->look_from, c->look_at, c->up, c->vfov, c->height, c->width);
+  if (c->kind == 1) cam.build_ortho(c->view_width);
+  cam.build_tangent(c->look_from, c->look_at, c->up, c->kind == 1 ? c->view_width : c->vfov, c->d_params);
   return cam;
 }
 
@@ -949,22 +959,51 @@ void oracle_camera_matrices(const oracle_camera* c, float* c2w, float* w2c, floa
   pos[2] = cam.pos.z;
 }
 
-// Camera rays of every lane of Sampler(spp*H*W): the first two draws of each lane.
-void oracle_camera_rays(const oracle_camera* c, uint32_t spp, float* o, float* d, int* pix) {
+// Camera rays of every lane of Sampler(spp*H*W): the first two draws of each
+// lane; do (optional): the origins' tangents along c->d_params.
+void oracle_camera_rays(const oracle_camera* c, uint32_t spp, float* o, float* d, int* pix, float* dorig) {
   const Camera cam = make_camera(c);
   const int64_t N = static_cast<int64_t>(spp) * cam.H * cam.W;
   for (int64_t lane = 0; lane < N; ++lane) {
     Pcg32 rng = lane_rng(static_cast<uint32_t>(lane));
     const float u0 = rng.next_float(), u1 = rng.next_float();
     V3<float> oo, dd;
+    V3<Dual> oad;
     int x, y;
-    cam.ray(static_cast<uint32_t>(lane), spp, u0, u1, &oo, &dd, &x, &y);
+    cam.ray(static_cast<uint32_t>(lane), spp, u0, u1, &oo, &dd, &x, &y, &oad);
     for (int a = 0; a < 3; ++a) {
       o[3 * lane + a] = oo[a];
       d[3 * lane + a] = dd[a];
+      if (dorig) dorig[3 * lane + a] = oad[a].d;
     }
     pix[lane] = y * cam.W + x;
   }
+}
+
+// Rays of gen_camera_ray_with_point for n points and the tangents of their
+// origins along c->d_params and the points' tangents dpts.
+void oracle_point_rays(const oracle_camera* c, const float* pts, const float* dpts, int n, float* o, float* d,
+                       float* dorig) {
+  const Camera cam = make_camera(c);
+  for (int i = 0; i < n; ++i) {
+    V3<float> oo, dd;
+    V3<Dual> oad;
+    cam.point_ray(V3<float>{pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]},
+                  V3<float>{dpts[3 * i], dpts[3 * i + 1], dpts[3 * i + 2]}, &oo, &dd, &oad);
+    for (int a = 0; a < 3; ++a) {
+      o[3 * i + a] = oo[a];
+      d[3 * i + a] = dd[a];
+      dorig[3 * i + a] = oad[a].d;
+    }
+  }
+}
+
+// Tangents of camera2world, world2camera and raster2camera along c->d_params.
+void oracle_camera_tangents(const oracle_camera* c, float* dC, float* dW, float* dR) {
+  const Camera cam = make_camera(c);
+  std::memcpy(dC, cam.dC.m, 64);
+  std::memcpy(dW, cam.dW.m, 64);
+  std::memcpy(dR, cam.dR.m, 64);
 }
 
 void oracle_project_points(const oracle_camera* c, const float* pts, int n, int* pix_xy, int* valid) {

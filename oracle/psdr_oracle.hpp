@@ -157,6 +157,10 @@ template <class F> V3<F> normalize(const V3<F>& a) {
 }
 template <class F> V3<float> detach(const V3<F>& a) { return {detach(a.x), detach(a.y), detach(a.z)}; }
 template <class F> V3<F> lift(const V3<float>& a) { return {F(a.x), F(a.y), F(a.z)}; }
+// a ray origin on scalar F: the float value, or the Dual carrying the camera tangent
+template <class F> V3<F> origin_as(const V3<float>& o, const V3<Dual>& o_ad);
+template <> inline V3<float> origin_as<float>(const V3<float>& o, const V3<Dual>&) { return o; }
+template <> inline V3<Dual> origin_as<Dual>(const V3<float>&, const V3<Dual>& o_ad) { return o_ad; }
 
 struct V2 {
   float x = 0.f, y = 0.f;
@@ -377,22 +381,53 @@ inline V3<float> xform_dir(const Mat4& M, const V3<float>& d) {
           (M.m[2][0] * d.x + M.m[2][1] * d.y) + M.m[2][2] * d.z};
 }
 
+// camera2world (look_at, src/camera.h:33-53: columns right, up', dir, origin)
+// and, for the orthographic camera, camera2raster = screen2raster * ortho_mat
+// (src/camera.h:19-31, 302-312), on any scalar (Dual: along a parameter tangent).
+template <class F>
+void camera_frame(const V3<F>& o, const V3<F>& t, const V3<F>& u, F C[4][4]) {
+  const V3<F> dir = normalize(o - t);
+  const V3<F> right = cross(normalize(u), dir);
+  const V3<F> nup = cross(dir, right);
+  const V3<F> cols[4] = {right, nup, dir, o};
+  for (int r = 0; r < 4; ++r)
+    for (int c = 0; c < 4; ++c) C[r][c] = (r < 3) ? cols[c][r] : F(c == 3 ? 1.f : 0.f);
+}
+template <class F>
+void ortho_raster(F view_width, int H, int W, F M[4][4]) {
+  const F hw = view_width / F(2.0f);
+  const F hh = (F(static_cast<float>(H)) * hw) / F(static_cast<float>(W));
+  const F l = F(0.f) - hw, r = hw, b = F(0.f) - hh, t = hh;
+  const float n = -0.01f, f = -1000.f;
+  // scale * translate, then S(W/2, -H/2, 1) T(1, -1, 0)
+  const F sx = F(2.0f) / (r - l), sy = F(2.0f) / (t - b);
+  const float sz = 2.0f / (n - f), tz = -(n + f) / 2.0f;
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) M[i][j] = F(0.f);
+  M[0][0] = F(0.5f * static_cast<float>(W)) * sx;
+  M[0][3] = F(0.5f * static_cast<float>(W));
+  M[1][1] = F(-0.5f * static_cast<float>(H)) * sy;
+  M[1][3] = F(0.5f * static_cast<float>(H));
+  M[2][2] = F(sz);
+  M[2][3] = F(sz * tz);
+  M[3][3] = F(1.f);
+}
+
 struct Camera {
   int H = 0, W = 0;
+  bool ortho = false;
   Mat4 c2w, w2c, c2r, r2c;
   V3<float> pos;
+  V3<float> look;  // transform_dir(camera2world, (0, 0, -1))
+  // tangents of camera2world, world2camera and raster2camera along the
+  // parameter direction of oracle_camera::d_params (d(A^-1) = -A^-1 dA A^-1)
+  Mat4 dC{}, dW{}, dR{};
 
   void build(const float from[3], const float at[3], const float up[3], float vfov, int h, int w) {
     H = h;
     W = w;
-    // look_at (src/camera.h:33-53): columns = right, up', dir, origin
     V3<float> o{from[0], from[1], from[2]}, t{at[0], at[1], at[2]}, u{up[0], up[1], up[2]};
-    V3<float> dir = normalize(o - t);
-    V3<float> right = cross(normalize(u), dir);
-    V3<float> nup = cross(dir, right);
-    float cols[4][3] = {{right.x, right.y, right.z}, {nup.x, nup.y, nup.z}, {dir.x, dir.y, dir.z}, {o.x, o.y, o.z}};
-    for (int r = 0; r < 4; ++r)
-      for (int c = 0; c < 4; ++c) c2w.m[r][c] = (r < 3) ? cols[c][r] : (c == 3 ? 1.f : 0.f);
+    camera_frame<float>(o, t, u, c2w.m);
     w2c = mat_inverse(c2w);
     // perspective (src/camera.h:7-17), near=-0.01, far=-1000
     const float n = -0.01f, f = -1000.f;
@@ -422,18 +457,92 @@ struct Camera {
     c2r = mat_mul(s2r, P);
     r2c = mat_inverse(c2r);
     pos = xform_pos(c2w, V3<float>{0.f, 0.f, 0.f});
+    look = xform_dir(c2w, V3<float>{0.f, 0.f, -1.f});
   }
-  // gen_camera_ray_ (src/camera.h:235-258)
-  void ray(uint32_t lane, uint32_t spp, float u0, float u1, V3<float>* o, V3<float>* d, int* px, int* py) const {
+  // OrthographicCamera (src/camera.h:279-312): the projection of ortho_mat
+  void build_ortho(float view_width) {
+    ortho = true;
+    ortho_raster<float>(view_width, H, W, c2r.m);
+    r2c = mat_inverse(c2r);
+  }
+  // parameter tangent (look_from, look_at, up, vfov | view_width)
+  void build_tangent(const float from[3], const float at[3], const float up[3], float extent, const float dp[10]) {
+    V3<Dual> o, t, u;
+    for (int i = 0; i < 3; ++i) {
+      o[i] = Dual(from[i], dp[i]);
+      t[i] = Dual(at[i], dp[3 + i]);
+      u[i] = Dual(up[i], dp[6 + i]);
+    }
+    Dual Cd[4][4];
+    camera_frame<Dual>(o, t, u, Cd);
+    for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 4; ++j) dC.m[i][j] = Cd[i][j].d;
+    dW = neg_sandwich(w2c, dC);
+    if (ortho) {
+      Dual Md[4][4];
+      ortho_raster<Dual>(Dual(extent, dp[9]), H, W, Md);
+      Mat4 dM;
+      for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) dM.m[i][j] = Md[i][j].d;
+      dR = neg_sandwich(r2c, dM);
+    }
+  }
+  static Mat4 neg_sandwich(const Mat4& A, const Mat4& dM) {
+    Mat4 r = mat_mul(mat_mul(A, dM), A);
+    for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 4; ++j) r.m[i][j] = -r.m[i][j];
+    return r;
+  }
+  // origin o = camera2world (x, y, 0, 1) with its tangent: dC (x, y, 0, 1) + C (dx, dy, 0, 0)
+  V3<Dual> origin_with_tangent(const V3<float>& o, float x, float y, float dx, float dy) const {
+    V3<Dual> r;
+    for (int i = 0; i < 3; ++i) {
+      const float t = ((dC.m[i][0] * x + dC.m[i][1] * y) + dC.m[i][3]) + (c2w.m[i][0] * dx + c2w.m[i][1] * dy);
+      r[i] = Dual(o[i], t);
+    }
+    return r;
+  }
+  // gen_camera_ray_ (src/camera.h:235-258 perspective, 314-335 orthographic)
+  void ray(uint32_t lane, uint32_t spp, float u0, float u1, V3<float>* o, V3<float>* d, int* px, int* py,
+           V3<Dual>* o_ad = nullptr) const {
     const uint32_t pix = spp > 1 ? lane / spp : lane;
     const int x = static_cast<int>(pix % static_cast<uint32_t>(W));
     const int y = static_cast<int>(pix / static_cast<uint32_t>(W));
-    V3<float> p{u0 + static_cast<float>(x), u1 + static_cast<float>(y), 1.0f};
-    p = xform_pos(r2c, p);
-    *d = normalize(xform_dir(c2w, p));
-    *o = pos;
+    const V3<float> f{u0 + static_cast<float>(x), u1 + static_cast<float>(y), 1.0f};
+    const V3<float> p = xform_pos(r2c, f);
+    if (ortho) {
+      *o = xform_pos(c2w, V3<float>{p.x, p.y, 0.f});
+      *d = look;
+      if (o_ad) {
+        const float dx = ((dR.m[0][0] * f.x + dR.m[0][1] * f.y) + dR.m[0][2]) + dR.m[0][3];
+        const float dy = ((dR.m[1][0] * f.x + dR.m[1][1] * f.y) + dR.m[1][2]) + dR.m[1][3];
+        *o_ad = origin_with_tangent(*o, p.x, p.y, dx, dy);
+      }
+    } else {
+      *d = normalize(xform_dir(c2w, p));
+      *o = pos;
+      if (o_ad) *o_ad = origin_with_tangent(pos, 0.f, 0.f, 0.f, 0.f);
+    }
     *px = x;
     *py = y;
+  }
+  // ray of gen_camera_ray_with_point_ (src/camera.h:266-275, 351-356) with
+  // the origin's tangent given the point's tangent dpt
+  void point_ray(const V3<float>& pt, const V3<float>& dpt, V3<float>* o, V3<float>* d, V3<Dual>* o_ad) const {
+    if (!ortho) {
+      *o = pos;
+      *d = normalize(pt - pos);
+      *o_ad = origin_with_tangent(pos, 0.f, 0.f, 0.f, 0.f);
+      return;
+    }
+    const V3<float> pc = xform_pos(w2c, pt);
+    *o = xform_pos(c2w, V3<float>{pc.x, pc.y, 0.f});
+    *d = look;
+    float dq[2];
+    for (int r = 0; r < 2; ++r)
+      dq[r] = (((dW.m[r][0] * pt.x + dW.m[r][1] * pt.y) + dW.m[r][2] * pt.z) + dW.m[r][3]) +
+              ((w2c.m[r][0] * dpt.x + w2c.m[r][1] * dpt.y) + w2c.m[r][2] * dpt.z);
+    *o_ad = origin_with_tangent(*o, pc.x, pc.y, dq[0], dq[1]);
   }
   // Raster area per unit world area of a small patch at x perpendicular to the
   // unit direction n (finite differences of the projection in double).  Not part

@@ -65,10 +65,20 @@ def cam_params(res, look_from=(0.0, 1.0, 5.0), vfov=math.pi / 4):
     return list(look_from) + [0.0, 0.0, 0.0, 0.0, 1.0, 0.0, vfov, float(res), float(res)]
 
 
-def camera(device, res, look_from=(0.0, 1.0, 5.0), vfov=math.pi / 4):
-    return dr.PerspectiveCamera(torch.tensor(look_from, device=device), torch.tensor([0.0, 0.0, 0.0], device=device),
-                                torch.tensor([0.0, 1.0, 0.0], device=device), torch.tensor([vfov], device=device),
-                                res, res)
+def camera(device, res, look_from=(0.0, 1.0, 5.0), vfov=math.pi / 4, requires_grad=False):
+    t = lambda v: torch.tensor(v, device=device, requires_grad=requires_grad)
+    return dr.PerspectiveCamera(t(list(look_from)), t([0.0, 0.0, 0.0]), t([0.0, 1.0, 0.0]), t([vfov]), res, res)
+
+
+# orthographic camera of the same view: a 4.5-wide window looking at the origin
+def ortho_params(res, look_from=(0.0, 1.0, 5.0), view_width=4.5):
+    return list(look_from) + [0.0, 0.0, 0.0, 0.0, 1.0, 0.0, view_width, float(res), float(res), 1.0]
+
+
+def ortho_camera(device, res, look_from=(0.0, 1.0, 5.0), view_width=4.5, requires_grad=False):
+    t = lambda v: torch.tensor(v, device=device, requires_grad=requires_grad)
+    return dr.OrthographicCamera(t(list(look_from)), t([0.0, 0.0, 0.0]), t([0.0, 1.0, 0.0]), t([view_width]), res,
+                                 res)
 
 
 def cpu_flat(**kw):

@@ -37,10 +37,10 @@ def test_pcg_lane_draws(gpu_device, oracle):
     np.testing.assert_array_equal(out.cpu().numpy().view(np.uint32), want)
 
 
-@pytest.mark.parametrize("res,spp", [(32, 1), (64, 4)])
-def test_camera_rays(gpu_device, oracle, res, spp):
+@pytest.mark.parametrize("res,spp,kind", [(32, 1, "persp"), (64, 4, "persp"), (32, 2, "ortho")])
+def test_camera_rays(gpu_device, oracle, res, spp, kind):
     lib = N.load()
-    cp = H.cam_params(res)
+    cp = H.cam_params(res) if kind == "persp" else H.ortho_params(res)
     N_ = spp * res * res
     o = torch.empty((N_, 3), device=gpu_device)
     d = torch.empty((N_, 3), device=gpu_device)
@@ -116,6 +116,11 @@ def test_closest_hit_matches_brute_force(gpu_device, oracle, level):
     d = tgt - o
     d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
     o[:1000] = [0, 1, 5]  # camera-like origins
+    # axis-parallel rays: one or two direction components exactly +-0
+    k = np.arange(1000, 31000)
+    d[k, k % 3] = np.where(k % 2 == 0, 0.0, -0.0)
+    d[k[::3], (k[::3] + 1) % 3] = 0.0
+    d[k] /= np.linalg.norm(d[k], axis=1, keepdims=True)
     ro, rd = torch.tensor(o, device=gpu_device), torch.tensor(d, device=gpu_device)
     tri = torch.empty(n, dtype=torch.int32, device=gpu_device)
     t, u, v = (torch.empty(n, device=gpu_device) for _ in range(3))
@@ -156,13 +161,16 @@ CASES = [
     dict(name="bsdf-normal-map", kw=dict(level=2, bsdf_rough=0.5, tex_res=8, normal_res=8), res=24,
          cfg=dict(spp_interior=2, max_bounce=3)),
     dict(name="npass3", kw=dict(level=1), res=16, cfg=dict(spp_interior=3, max_bounce=2, n_pass=3)),
+    dict(name="ortho-rough", kw=dict(level=2, bsdf_rough=0.5), res=24, cfg=dict(spp_interior=2, max_bounce=3),
+         ortho=True),
 ]
 
 
 @pytest.mark.parametrize("case", CASES, ids=[c["name"] for c in CASES])
 def test_forward_radiance_matches_oracle(gpu_device, oracle, case):
     meshes, lights, _ = H.objects(gpu_device, **case["kw"])
-    scene = dr.Scene([H.camera(gpu_device, case["res"])], meshes, lights)
+    cam = (H.ortho_camera if case.get("ortho") else H.camera)(gpu_device, case["res"])
+    scene = dr.Scene([cam], meshes, lights)
     integ = dr.PathIntegrator(**case["cfg"])
     img = integ.renderC(scene)[0]
     ns, flat = _native(scene)

@@ -19,9 +19,10 @@ from tests import helpers as H
 pytestmark = pytest.mark.gpu
 
 
-def _setup(dev, kw, res, cfg):
+def _setup(dev, kw, res, cfg, ortho=False):
     meshes, lights, leaves = H.objects(dev, requires_grad=True, **kw)
-    scene = dr.Scene([H.camera(dev, res)], meshes, lights)
+    cam = (H.ortho_camera if ortho else H.camera)(dev, res)
+    scene = dr.Scene([cam], meshes, lights)
     integ = dr.PathIntegrator(**cfg)
     return scene, integ
 
@@ -33,12 +34,13 @@ def _gpu_vjp(scene, integ, g, terms):
     gv = torch.zeros_like(flat.vertices)
     gt = torch.zeros_like(flat.texels)
     ge = torch.zeros((max(flat.light_emit.shape[0], 1), 3), device=gv.device)
+    gc = torch.zeros(N.PSDR_CAMERA_GRAD, device=gv.device)
     c, k = ops.make_config(integ.cfg), ops.make_camera(cam)
     N.check(ns.lib.psdr_render_backward(ns.handle, ctypes.byref(c), ctypes.byref(k), g.data_ptr(), gv.data_ptr(),
-                                        gt.data_ptr(), ge.data_ptr(), terms, N.current_stream()))
+                                        gt.data_ptr(), ge.data_ptr(), gc.data_ptr(), terms, N.current_stream()))
     torch.cuda.synchronize()
     return gv.cpu().numpy().astype(np.float64), gt.cpu().numpy().astype(np.float64), \
-        ge.cpu().numpy()[: flat.light_emit.shape[0]].astype(np.float64)
+        ge.cpu().numpy()[: flat.light_emit.shape[0]].astype(np.float64), gc.cpu().numpy().astype(np.float64)
 
 
 def _check(oracle, scene, integ, grads, g, term, block, seed, tol):
@@ -47,12 +49,13 @@ def _check(oracle, scene, integ, grads, g, term, block, seed, tol):
     wv = rng.normal(size=tuple(flat.vertices.shape)).astype(np.float32) if block in ("v", "all") else None
     wt = rng.normal(size=(flat.texels.numel(),)).astype(np.float32) if block in ("t", "all") else None
     we = rng.normal(size=tuple(flat.light_emit.shape)).astype(np.float32) if block in ("e", "all") else None
+    wc = rng.normal(size=N.PSDR_CAMERA_GRAD).astype(np.float32) if block in ("c", "all") else None
     osc = oracle.Scene(flat, d_vertices=wv, d_texels=wt, d_emit=we)
     img, d_int, d_pri, d_dir = oracle.render(osc, integ.cfg, scene.cameras[0].params(), mode=1,
-                                             boundary=(term != N.PSDR_TERM_INTERIOR))
+                                             boundary=(term != N.PSDR_TERM_INTERIOR), d_camera=wc)
     d = {N.PSDR_TERM_INTERIOR: d_int, N.PSDR_TERM_PRIMARY: d_pri, N.PSDR_TERM_DIRECT: d_dir}[term]
     lhs = float((g.cpu().numpy().astype(np.float64) * d.astype(np.float64)).sum())
-    gv, gt, ge = grads
+    gv, gt, ge, gc = grads
     rhs = 0.0
     if wv is not None:
         rhs += float((gv * wv).sum())
@@ -60,6 +63,8 @@ def _check(oracle, scene, integ, grads, g, term, block, seed, tol):
         rhs += float((gt * wt).sum())
     if we is not None:
         rhs += float((ge * we).sum())
+    if wc is not None:
+        rhs += float((gc * wc).sum())
     scale = max(abs(lhs), abs(rhs), 1e-12)
     print(f"term {term} block {block}: JVP {lhs:.6e}  VJP {rhs:.6e}  rel {abs(lhs - rhs) / scale:.2e}")
     assert abs(lhs - rhs) <= tol * scale, (lhs, rhs)
@@ -100,6 +105,36 @@ def test_boundary_vjp_matches_oracle_jvp(gpu_device, oracle, term):
     _check(oracle, scene, integ, grads, g, term, "v", 200, 2e-3)
 
 
+# Camera parameters reach the image through the camera vertex's wi (the
+# DiffuseBsdf's roughness term) and the primary boundary's origin term; an
+# orthographic origin also moves with the edge point (vertex block).
+CAMERA_CASES = [
+    dict(name="interior-persp", term=N.PSDR_TERM_INTERIOR, ortho=False),
+    dict(name="interior-ortho", term=N.PSDR_TERM_INTERIOR, ortho=True),
+    dict(name="primary-persp", term=N.PSDR_TERM_PRIMARY, ortho=False),
+    dict(name="primary-ortho", term=N.PSDR_TERM_PRIMARY, ortho=True),
+    dict(name="direct-ortho", term=N.PSDR_TERM_DIRECT, ortho=True),
+]
+
+
+@pytest.mark.parametrize("case", CAMERA_CASES, ids=[c["name"] for c in CAMERA_CASES])
+def test_camera_vjp_matches_oracle_jvp(gpu_device, oracle, case):
+    term = case["term"]
+    scene, integ = _setup(gpu_device, dict(level=2, bsdf_rough=0.5), 24,
+                          dict(spp_interior=2, max_bounce=2, spp_primary_edge=2, spp_secondary_edge=2),
+                          ortho=case["ortho"])
+    g = torch.randn((24, 24, 3), generator=torch.Generator().manual_seed(13)).to(gpu_device)
+    grads = _gpu_vjp(scene, integ, g, term)
+    assert np.isfinite(grads[3]).all()
+    if term != N.PSDR_TERM_DIRECT:
+        assert np.abs(grads[3]).sum() > 0
+        _check(oracle, scene, integ, grads, g, term, "c", 300, 2e-3)
+    else:
+        assert np.abs(grads[3]).sum() == 0  # the shadow term does not see the camera
+    if term != N.PSDR_TERM_INTERIOR:
+        _check(oracle, scene, integ, grads, g, term, "v", 301, 2e-3)
+
+
 def test_renderD_autograd_end_to_end(gpu_device, oracle):
     meshes, lights, leaves = H.objects(gpu_device, level=2, tex_res=8, requires_grad=True)
     scene = dr.Scene([H.camera(gpu_device, 24)], meshes, lights)
@@ -127,7 +162,7 @@ def test_all_terms_equal_sum_of_terms(gpu_device):
     parts = [_gpu_vjp(scene, integ, g, t) for t in (N.PSDR_TERM_INTERIOR, N.PSDR_TERM_PRIMARY, N.PSDR_TERM_DIRECT)]
     for rep in range(2):
         tot = _gpu_vjp(scene, integ, g, N.PSDR_TERM_ALL)
-        for b in range(3):
+        for b in range(4):
             ref = sum(p[b] for p in parts)
             scale = max(np.abs(ref).max(), 1e-12)
             assert np.abs(tot[b] - ref).max() <= 1e-4 * scale, (rep, b, np.abs(tot[b] - ref).max(), scale)
@@ -174,3 +209,54 @@ def test_scene_update_refreshes_gradient_masks(gpu_device):
     assert leaves["color"].grad.abs().sum() > 0
     assert leaves["sphere_v"].grad.abs().sum() > 0
     assert leaves["emit"].grad.abs().sum() > 0
+
+
+def test_multi_camera_renderD_matches_single_camera_sum(gpu_device):
+    """renderD over several cameras (more than the scene keeps forward states
+    for) gives each camera's image and the sum of the single-camera gradients,
+    camera parameters included; a perspective camera's look_at, up and vfov get
+    no gradient (its origin is its position)."""
+    meshes, lights, leaves = H.objects(gpu_device, level=2, bsdf_rough=0.5, tex_res=8, requires_grad=True)
+    views = [(0.0, 1.0, 5.0), (1.5, 1.2, 4.5), (-2.0, 0.8, 4.6), (0.3, 2.5, 4.4), (0.0, 0.6, 5.2)]
+    cams = [H.camera(gpu_device, 16, look_from=v, requires_grad=True) for v in views[:3]] + \
+        [H.ortho_camera(gpu_device, 16, look_from=v, requires_grad=True) for v in views[3:]]
+    integ = dr.PathIntegrator(spp_interior=2, max_bounce=2, spp_primary_edge=2, spp_secondary_edge=2)
+    targets = [torch.rand((16, 16, 3), generator=torch.Generator().manual_seed(20 + i)).to(gpu_device)
+               for i in range(len(cams))]
+    cam_leaves = [(c.look_from, c.look_at, c.up, c._extent) for c in cams]
+
+    def grads():
+        out = {k: v.grad.detach().cpu().numpy().copy() for k, v in leaves.items()}
+        for i, cl in enumerate(cam_leaves):
+            out[f"cam{i}"] = np.concatenate([t.grad.detach().cpu().numpy().reshape(-1) for t in cl])
+        return out
+
+    def clear():
+        for v in list(leaves.values()) + [t for cl in cam_leaves for t in cl]:
+            v.grad = None
+
+    clear()
+    imgs = integ.renderD(dr.Scene(cams, meshes, lights))
+    sum(((im - t) ** 2).sum() for im, t in zip(imgs, targets)).backward()
+    torch.cuda.synchronize()
+    together = grads()
+    ref = None
+    for i, cam in enumerate(cams):
+        clear()
+        for cl in cam_leaves:
+            for t in cl:
+                t.grad = torch.zeros_like(t)
+        img = integ.renderD(dr.Scene([cam], meshes, lights))[0]
+        np.testing.assert_array_equal(img.detach().cpu().numpy(), imgs[i].detach().cpu().numpy())
+        ((img - targets[i]) ** 2).sum().backward()
+        torch.cuda.synchronize()
+        g = grads()
+        ref = g if ref is None else {k: ref[k] + g[k] for k in ref}
+    for k in ref:
+        scale = max(np.abs(ref[k]).max(), 1e-12)
+        assert np.abs(together[k] - ref[k]).max() <= 1e-4 * scale, (k, np.abs(together[k] - ref[k]).max(), scale)
+    for i in range(3):
+        assert np.abs(together[f"cam{i}"][:3]).sum() > 0
+        np.testing.assert_array_equal(together[f"cam{i}"][3:], 0.0)
+    for i in range(3, 5):
+        assert np.isfinite(together[f"cam{i}"]).all() and np.abs(together[f"cam{i}"]).sum() > 0

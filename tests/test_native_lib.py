@@ -30,10 +30,10 @@ def test_abi_version_and_error_string():
 
 
 def test_struct_sizes_match_header():
-    # psdr_object: 14 int32; psdr_config: 14 uint32; psdr_camera: 10 floats + 2 int32
+    # psdr_object: 14 int32; psdr_config: 14 uint32; psdr_camera: 11 floats + 3 int32
     assert ctypes.sizeof(N.PsdrObject) == 56
     assert ctypes.sizeof(N.PsdrConfig) == 56
-    assert ctypes.sizeof(N.PsdrCamera) == 48
+    assert ctypes.sizeof(N.PsdrCamera) == 56
 
 
 def test_argument_errors_without_gpu():

@@ -11,6 +11,11 @@ from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "libpsdr_hip.so"
 
+PSDR_OK = 0
+PSDR_ERR_ARG = -1
+PSDR_ERR_HIP = -2
+PSDR_ERR_STATE = -3
+PSDR_ERR_NOMEM = -4
 PSDR_MAT_DIFFUSE = 0
 PSDR_MAT_DIFFUSE_BSDF = 1
 PSDR_MAT_LIGHT = 2
@@ -25,8 +30,11 @@ PSDR_TERM_INTERIOR = 1
 PSDR_TERM_PRIMARY = 2
 PSDR_TERM_DIRECT = 4
 PSDR_TERM_ALL = 7
+PSDR_CAMERA_PERSPECTIVE = 0
+PSDR_CAMERA_ORTHOGRAPHIC = 1
+PSDR_CAMERA_GRAD = 10
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 
 class PsdrObject(C.Structure):
@@ -91,6 +99,8 @@ class PsdrCamera(C.Structure):
         ("vfov", C.c_float),
         ("height", C.c_int32),
         ("width", C.c_int32),
+        ("kind", C.c_int32),
+        ("view_width", C.c_float),
     ]
 
 
@@ -104,14 +114,15 @@ _SIGS = {
     "psdr_scene_destroy": (C.c_int, [_VP]),
     "psdr_scene_info": (C.c_int, [_VP, C.POINTER(_I32), C.POINTER(_I32), C.POINTER(_I32)]),
     "psdr_render_forward": (C.c_int, [_VP, C.POINTER(PsdrConfig), C.POINTER(PsdrCamera), _VP, _I32, _VP]),
-    "psdr_render_backward": (C.c_int, [_VP, C.POINTER(PsdrConfig), C.POINTER(PsdrCamera), _VP, _VP, _VP, _VP, _I32,
-                                       _VP]),
+    "psdr_render_backward": (C.c_int, [_VP, C.POINTER(PsdrConfig), C.POINTER(PsdrCamera), _VP, _VP, _VP, _VP, _VP,
+                                       _I32, _VP]),
     "psdr_render_forward_plain": (C.c_int, [_VP, C.POINTER(PsdrConfig), C.POINTER(PsdrCamera), _VP, _I32, _VP]),
     "psdr_render_backward_plain": (C.c_int, [_VP, C.POINTER(PsdrConfig), C.POINTER(PsdrCamera), _VP, _VP, _VP, _VP,
-                                             _I32, _VP]),
+                                             _VP, _I32, _VP]),
     "psdr_pcg_lane_draws": (C.c_int, [C.c_uint32, _I32, _I32, _VP, _VP]),
     "psdr_camera_rays": (C.c_int, [C.POINTER(PsdrCamera), C.c_uint32, _VP, _VP, _VP, _VP]),
     "psdr_camera_matrices": (C.c_int, [C.POINTER(PsdrCamera), _VP, _VP, _VP, _VP, _VP]),
+    "psdr_camera_param_vjp": (C.c_int, [C.POINTER(PsdrCamera), _VP, _VP]),
     "psdr_scene_trace": (C.c_int, [_VP, _VP, _VP, _I32, _VP, _VP, _VP, _VP, _VP]),
     "psdr_scene_derived": (C.c_int, [_VP, _VP, _VP, _VP, _VP]),
     "psdr_scene_samplers": (C.c_int, [_VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP]),

@@ -4,19 +4,17 @@ Same class names, constructor arguments, shapes, dtypes, defaults and
 assertions as the reference wrapper; `PathIntegrator.renderC` / `renderD`
 return a list of (H, W, 3) fp32 images, one per camera (row 0 = top), and
 `renderD`'s images are differentiable w.r.t. mesh / light vertices, albedo,
-roughness and normal-map texels and light emission through torch autograd.
+roughness and normal-map texels, light emission and camera parameters through
+torch autograd.
 
 Differences, each deliberate:
   * 2-D (H, W) textures are accepted as 1-channel maps (the reference's `==`
     typo at src/python/torch_psdr.py:233 rejects them);
-  * OrthographicCamera, EnvLight and SH9Light raise NotImplementedError;
-  * camera parameters are constants: they get no gradient (a warning is given
-    when one requires grad);
+  * EnvLight and SH9Light raise NotImplementedError;
   * the stray `from turtle import forward` (src/python/torch_psdr.py:2) is gone.
 """
 from __future__ import annotations
 
-import warnings
 from typing import List, Union
 
 import torch
@@ -46,37 +44,61 @@ class VarContex:
 
 
 class BaseCamera(VarContex):
-    """Base camera class (src/python/torch_psdr.py:106-110)."""
+    """Base camera class (src/python/torch_psdr.py:106-110).
+
+    The parameters (look_from, look_at, up and vfov or view_width) may require
+    grad: their gradient flows, as in the reference, only through ray origins
+    (the camera vertex's direction to the viewer and the primary boundary's
+    origin term)."""
+
+    kind = 0
+
+    def _init(self, look_from: Tensor, look_at: Tensor, up: Tensor, extent: Tensor, height: int, width: int):
+        super().__init__()
+        assert look_from.size() == (3,)
+        assert look_at.size() == (3,)
+        assert up.size() == (3,)
+        assert extent.size() == (1,)
+        assert height > 0 and width > 0
+        for t in (look_from, look_at, up, extent):
+            self.append_variable(t)
+        self.look_from, self.look_at, self.up = look_from, look_at, up
+        self.height = int(height)
+        self.width = int(width)
+        self._extent = extent
+
+    def param_tensor(self) -> Tensor:
+        """The PSDR_CAMERA_GRAD parameters as one (differentiable) fp32 vector."""
+        parts = [self.look_from.reshape(3), self.look_at.reshape(3), self.up.reshape(3), self._extent.reshape(1)]
+        dev = next((t.device for t in parts if t.device.type != "cpu"), parts[0].device)
+        return torch.cat([t.to(device=dev, dtype=torch.float32) for t in parts])
+
+    def params(self) -> List[float]:
+        vals = self.param_tensor().detach().cpu().tolist()
+        return vals + [float(self.height), float(self.width), float(self.kind)]
 
 
 class PerspectiveCamera(BaseCamera):
     """PerspectiveCamera(look_from (3,), look_at (3,), up (3,), vfov (1,) [rad], height, width)
     (src/python/torch_psdr.py:113-158; matrices src/camera.h:7-53, near -0.01, far -1000)."""
 
-    def __init__(self, look_from: Tensor, look_at: Tensor, up: Tensor, vfov: Tensor, height: int, width: int):
-        super().__init__()
-        assert look_from.size() == (3,)
-        assert look_at.size() == (3,)
-        assert up.size() == (3,)
-        assert vfov.size() == (1,)
-        assert height > 0 and width > 0
-        for t in (look_from, look_at, up, vfov):
-            self.append_variable(t)
-        if self.torch_vars:
-            warnings.warn("torch_psdr_amd: camera parameters are treated as constants (no camera gradients)")
-        self.look_from, self.look_at, self.up, self.vfov = look_from, look_at, up, vfov
-        self.height = int(height)
-        self.width = int(width)
+    kind = 0
 
-    def params(self) -> List[float]:
-        vals = torch.cat([self.look_from.detach().reshape(3), self.look_at.detach().reshape(3),
-                          self.up.detach().reshape(3), self.vfov.detach().reshape(1)]).float().cpu().tolist()
-        return vals + [float(self.height), float(self.width)]
+    def __init__(self, look_from: Tensor, look_at: Tensor, up: Tensor, vfov: Tensor, height: int, width: int):
+        self._init(look_from, look_at, up, vfov, height, width)
+        self.vfov = vfov
 
 
 class OrthographicCamera(BaseCamera):
+    """OrthographicCamera(look_from (3,), look_at (3,), up (3,), view_width (1,), height, width)
+    (src/python/torch_psdr.py:161-206; projection src/camera.h:279-362): parallel
+    rays along the view direction from a view_width-wide window on the camera plane."""
+
+    kind = 1
+
     def __init__(self, look_from: Tensor, look_at: Tensor, up: Tensor, view_width: Tensor, height: int, width: int):
-        raise NotImplementedError("OrthographicCamera is not implemented by torch_psdr_amd")
+        self._init(look_from, look_at, up, view_width, height, width)
+        self.view_width = view_width
 
 
 class Texture(VarContex):
@@ -290,5 +312,6 @@ class PathIntegrator:
 
     def renderD(self, scene: Scene) -> List[Tensor]:
         f = scene.flat
-        return [torch.ops.psdr.render(scene.token, cam.params(), self.cfg, f.vertices, f.texels, f.light_emit, True)
+        return [torch.ops.psdr.render(scene.token, cam.params(), self.cfg, f.vertices, f.texels, f.light_emit,
+                                      cam.param_tensor(), True)
                 for cam in scene.cameras]

@@ -20,6 +20,7 @@
 
 #include <cstring>
 
+#include "cam_grad.h"
 #include "grad_cache.h"
 #include "render.h"
 #include "shading.h"
@@ -57,6 +58,7 @@ struct AdjParams {
   float* grad_v;
   float* grad_t;
   float* grad_e;
+  float* cam_acc;  // camera matrix adjoints (CAM_ACC floats), or null
   int hot_n;  // 3*n_lights + 4*n_objects if it fits in HOT_MAX, else 0
 };
 
@@ -455,7 +457,9 @@ __device__ DepthOut depth_step(const AdjParams& P, float* hot, const Its& its, R
 template <int DMAX>
 __global__ __launch_bounds__(ADJ_THREADS) void k_adjoint(AdjParams P) {
   __shared__ float hot[HOT_MAX];
+  __shared__ float cacc[CAM_ACC];
   for (int i = threadIdx.x; i < P.hot_n; i += blockDim.x) hot[i] = 0.f;
+  cam_acc_init(cacc);
   adj_cache().init();
   __syncthreads();
   const SceneView& sv = P.sv;
@@ -474,7 +478,18 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adjoint(AdjParams P) {
     int t0;
     float u0, v0;
     draw_cached(P, 0, lane, &t0, &u0, &v0);
-    const Its its = reconstruct(sv, t0, u0, v0, P.cam.pos, true);
+    // camera ray origin: the camera position, or (orthographic) the film
+    // point's spot on the camera plane, from the lane's first two draws
+    f3 O0 = P.cam.pos, film = splat(0.f);
+    if (P.cam.ortho) {
+      Rng r0 = lane_rng(static_cast<uint32_t>(lane));
+      const float a = r0.next(), b = r0.next();
+      int pixel;
+      film = film_point(P.cam, static_cast<uint32_t>(lane), P.spp, a, b, &pixel);
+      const f3 fp = xform_pos(P.cam.r2c, film);
+      O0 = xform_pos(P.cam.c2w, f3{fp.x, fp.y, 0.0f});
+    }
+    const Its its = reconstruct(sv, t0, u0, v0, O0, true);
     if (!its.valid) continue;
     // per-depth state recorded by the forward render: PCG state and throughput
     // at each vertex, and the vertices themselves (the extension hits)
@@ -504,8 +519,8 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adjoint(AdjParams P) {
     VertexAdj pend = zero_adj();
     f3 pendO = splat(0.f);
     for (int d = n - 1; d >= 0; --d) {
-      f3 O = P.cam.pos;
-      if (d > 0) O = reconstruct(sv, vt[d - 1], vu[d - 1], vv[d - 1], P.cam.pos, true).p;
+      f3 O = O0;
+      if (d > 0) O = reconstruct(sv, vt[d - 1], vu[d - 1], vv[d - 1], O0, true).p;
       const Its vx = reconstruct(sv, vt[d], vu[d], vv[d], O, true);
       Rng r{rs[d], rng.inc};
       const DepthOut o = depth_step(P, hot, vx, r, d, lane, true, g * Ts[d], A * Ts[d]);
@@ -527,9 +542,13 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adjoint(AdjParams P) {
     // camera vertex: its own adjoints plus render_interior's L * J0 (J0 = A / detach(A))
     pend.area = pend.area + dot(g, L) / its.area;
     tri_vjp(P, vt[0], vu[0], vv[0], flip_of(sv, its), pend);
+    // the camera ray origin, through the camera vertex's wi
+    if (P.cam_acc && (pendO.x != 0.f || pendO.y != 0.f || pendO.z != 0.f))
+      (void)cam_origin_vjp(P.cam, pendO, film, P.cam.r2c, CAM_R, cacc);
   }
   __syncthreads();
   adj_cache().flush(P.grad_v, P.grad_t);
+  if (P.cam_acc) cam_acc_flush(cacc, P.cam_acc);
   for (int i = threadIdx.x; i < P.hot_n; i += blockDim.x) {
     const float v = hot[i];
     if (v == 0.f) continue;
@@ -543,6 +562,11 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adjoint(AdjParams P) {
       else atomicAdd(P.grad_t + ob.rough_off, v);
     }
   }
+}
+
+// one thread: camera matrix adjoints -> parameter gradients
+__global__ void k_camera_grad(psdr_camera cam, const float* __restrict__ acc, float* __restrict__ grad) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) camera_param_vjp(cam, acc, grad);
 }
 
 inline int blocks(int64_t n, int t) { return static_cast<int>((n + t - 1) / t); }
@@ -575,7 +599,7 @@ static int aux_fork(psdr_scene* s, hipStream_t st) {
 }
 
 int boundary_prepare_async(psdr_scene* s, const psdr_config& cfg, const psdr_camera& cam) {
-  s->bprep = false;
+  s->cur->bprep = false;
   if (s->n_edges == 0 || s->plain || !s->aux.fork) return PSDR_OK;
   for (int i = 0; i < 2; ++i) PSDR_HIP_CHECK(hipStreamWaitEvent(s->aux.st[i], s->aux.fork, 0));
   int rc = PSDR_OK;
@@ -583,34 +607,56 @@ int boundary_prepare_async(psdr_scene* s, const psdr_config& cfg, const psdr_cam
   if (cfg.spp_primary_edge > 0) rc = boundary_primary_prepare(s, cfg, cam, s->aux.st[0]);
   if (!rc && cfg.spp_secondary_edge > 0) rc = boundary_direct_prepare(s, cfg, cam, s->aux.st[1]);
   if (rc) return rc;
-  s->bprep = true;
-  s->bprep_cfg = cfg;
-  s->bprep_cam = cam;
+  s->cur->bprep = true;
+  return PSDR_OK;
+}
+
+static int camera_grad_begin(psdr_scene* s, float* grad_cam, hipStream_t st, float** acc) {
+  *acc = nullptr;
+  if (!grad_cam) return PSDR_OK;
+  if (s->cam_acc.ensure(4ull * CAM_ACC)) return PSDR_ERR_NOMEM;
+  *acc = s->cam_acc.as<float>();
+  return fill_f32(*acc, CAM_ACC, 0.f, st);
+}
+
+static int camera_grad_end(const psdr_camera& camd, const float* acc, float* grad_cam, hipStream_t st) {
+  if (!grad_cam) return PSDR_OK;
+  hipLaunchKernelGGL(k_camera_grad, dim3(1), dim3(1), 0, st, camd, acc, grad_cam);
+  PSDR_HIP_CHECK(hipGetLastError());
   return PSDR_OK;
 }
 
 int render_backward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& camd, const float* dL_dI,
-                    float* grad_v, float* grad_t, float* grad_e, int terms, hipStream_t st) {
+                    float* grad_v, float* grad_t, float* grad_e, float* grad_cam, int terms, hipStream_t st) {
   const bool ti = (terms & PSDR_TERM_INTERIOR) != 0;
   const bool tp = (terms & PSDR_TERM_PRIMARY) != 0 && s->n_edges > 0 && cfg.spp_primary_edge > 0;
   const bool td = (terms & PSDR_TERM_DIRECT) != 0 && s->n_edges > 0 && cfg.spp_secondary_edge > 0;
   if (s->stats.ensure(4ull * (3 * static_cast<size_t>(s->n_vertices) + s->n_texels + 3 * s->n_lights + 64)))
     return PSDR_ERR_NOMEM;
   float* gv = grad_v ? grad_v : s->stats.as<float>();
+  RenderSlot* slot = s->find_slot(cfg, camd);
+  if (!slot) {
+    set_error(PSDR_ERR_STATE, "render_backward: no matching forward render with keep=1");
+    return PSDR_ERR_STATE;
+  }
+  s->cur = slot;
+  float* acc;
+  int rc0 = camera_grad_begin(s, grad_cam, st, &acc);
+  if (rc0) return rc0;
   if (s->plain) {
-    int rc = ti ? interior_backward(s, cfg, camd, dL_dI, grad_v, grad_t, grad_e, st) : PSDR_OK;
+    int rc = ti ? interior_backward(s, cfg, camd, dL_dI, grad_v, grad_t, grad_e, acc, st) : PSDR_OK;
     if (!rc && tp) rc = boundary_primary_prepare(s, cfg, camd, st);
-    if (!rc && tp) rc = boundary_primary_finish(s, cfg, camd, dL_dI, gv, st);
+    if (!rc && tp) rc = boundary_primary_finish(s, cfg, camd, dL_dI, gv, acc, st);
     if (!rc && td) rc = boundary_direct_prepare(s, cfg, camd, st);
     if (!rc && td) rc = boundary_direct_finish(s, cfg, camd, dL_dI, gv, st);
+    if (!rc) rc = camera_grad_end(camd, acc, grad_cam, st);
     return rc;
   }
-  const bool prepared = s->bprep && std::memcmp(&s->bprep_cfg, &cfg, sizeof(cfg)) == 0 &&
-                        std::memcmp(&s->bprep_cam, &camd, sizeof(camd)) == 0;
+  const bool prepared = slot->bprep;
   int rc = (tp || td) ? aux_fork(s, st) : PSDR_OK;
   if (rc) return rc;
   // launch order by host enqueue cost: the single interior launch first
-  int rci = ti ? interior_backward(s, cfg, camd, dL_dI, grad_v, grad_t, grad_e, st) : PSDR_OK;
+  int rci = ti ? interior_backward(s, cfg, camd, dL_dI, grad_v, grad_t, grad_e, acc, st) : PSDR_OK;
   int rcd = PSDR_OK, rcp = PSDR_OK;
   if (td) {
     if (!prepared) rcd = boundary_direct_prepare(s, cfg, camd, s->aux.st[1]);
@@ -618,18 +664,19 @@ int render_backward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& ca
   }
   if (tp) {
     if (!prepared) rcp = boundary_primary_prepare(s, cfg, camd, s->aux.st[0]);
-    if (!rcp) rcp = boundary_primary_finish(s, cfg, camd, dL_dI, gv, s->aux.st[0]);
+    if (!rcp) rcp = boundary_primary_finish(s, cfg, camd, dL_dI, gv, acc, s->aux.st[0]);
   }
   // join even after an error so no work is left running unordered
   const int rcj = (tp || td) ? aux_join(s, st) : PSDR_OK;
   if (rci) return rci;
   if (rcp) return rcp;
   if (rcd) return rcd;
-  return rcj;
+  if (rcj) return rcj;
+  return camera_grad_end(camd, acc, grad_cam, st);
 }
 
 int interior_backward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& camd, const float* dL_dI,
-                      float* grad_v, float* grad_t, float* grad_e, hipStream_t st) {
+                      float* grad_v, float* grad_t, float* grad_e, float* cam_acc, hipStream_t st) {
   AdjParams P;
   P.sv = s->view();
   P.cam = make_camera(camd);
@@ -638,14 +685,15 @@ int interior_backward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& 
   P.nB = cfg.mis_bsdf_samples;
   P.elv = cfg.enable_light_visable ? 1 : 0;
   P.spp = cfg.spp_interior;
-  P.Kmax = s->last_K;
-  P.N = static_cast<int64_t>(s->last_N);
-  const int64_t slots = 1 + static_cast<int64_t>(s->last_D) * s->last_K;
-  P.ctri = s->hit_cache.as<int>();
+  const RenderSlot& R = *s->cur;
+  P.Kmax = R.K;
+  P.N = static_cast<int64_t>(R.N);
+  const int64_t slots = 1 + static_cast<int64_t>(R.D) * R.K;
+  P.ctri = R.hit_cache.as<int>();
   P.cu = reinterpret_cast<const float*>(P.ctri + slots * P.N);
   P.cv = P.cu + slots * P.N;
-  const int64_t Dr = std::max(s->last_D, 1);
-  P.crs = s->replay.as<uint64_t>();
+  const int64_t Dr = std::max(R.D, 1);
+  P.crs = R.replay.as<uint64_t>();
   P.cT = reinterpret_cast<const float*>(P.crs + Dr * P.N);
   P.cL = P.cT + 3 * Dr * P.N;
   P.cinfo = reinterpret_cast<const int*>(P.cL + 3 * P.N);
@@ -657,6 +705,7 @@ int interior_backward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& 
   P.grad_v = grad_v ? grad_v : sink;
   P.grad_t = grad_t ? grad_t : sink;
   P.grad_e = grad_e ? grad_e : sink;
+  P.cam_acc = cam_acc;
   const int hot = 3 * s->n_lights + 4 * s->n_objects;
   P.hot_n = hot <= HOT_MAX ? hot : 0;
   const int grid = std::max(1, std::min(blocks(P.N, ADJ_THREADS), 256 * 16));

@@ -25,6 +25,7 @@
 
 #include <algorithm>
 
+#include "cam_grad.h"
 #include "grad_cache.h"
 #include "render.h"
 #include "shading.h"
@@ -137,8 +138,8 @@ __global__ void k_prim_setup(SceneView sv, CameraK cam, int n, uint64_t* __restr
   const EdgeRec E = load_edge(sv, e);
   const f3 ep = E.p0 + u * (E.p1 - E.p0);
   int px, py;
-  const bool valid = camera_project(cam, ep, &px, &py);
-  const f3 d = normalize(ep - cam.pos);
+  f3 o, d;
+  const bool valid = camera_point_ray(cam, ep, &px, &py, &o, &d);
   const f3 nn = cross(d, normalize(E.p1 - E.p0));
   if (keys) {
     keys[i] = valid ? (spread_bits16(static_cast<uint32_t>(px)) | (spread_bits16(static_cast<uint32_t>(py)) << 1))
@@ -169,14 +170,16 @@ __global__ void k_prim_rays(SceneView sv, CameraK cam, int n, const uint32_t* __
   // the same expressions as k_prim_setup, so the rays are bit-identical
   const EdgeRec E = load_edge(sv, rec.edge[i]);
   const f3 ep = E.p0 + rec.t[i] * (E.p1 - E.p0);
-  const f3 d = normalize(ep - cam.pos);
+  int px, py;
+  f3 o, d;
+  (void)camera_point_ray(cam, ep, &px, &py, &o, &d);
   const f3 nn = f3{rec.n[i], rec.n[rec.cap + i], rec.n[2 * rec.cap + i]};
   f3 da = d - nn * REAL_EPS, db = d + nn * REAL_EPS;
   if (rec.pix[i] < 0) da = db = splat(__builtin_nanf(""));
-  rays.o[3 * p] = cam.pos.x; rays.o[3 * p + 1] = cam.pos.y; rays.o[3 * p + 2] = cam.pos.z;
+  rays.o[3 * p] = o.x; rays.o[3 * p + 1] = o.y; rays.o[3 * p + 2] = o.z;
   rays.d[3 * p] = da.x; rays.d[3 * p + 1] = da.y; rays.d[3 * p + 2] = da.z;
   const int j = n + p;
-  rays.o[3 * j] = cam.pos.x; rays.o[3 * j + 1] = cam.pos.y; rays.o[3 * j + 2] = cam.pos.z;
+  rays.o[3 * j] = o.x; rays.o[3 * j + 1] = o.y; rays.o[3 * j + 2] = o.z;
   rays.d[3 * j] = db.x; rays.d[3 * j + 1] = db.y; rays.d[3 * j + 2] = db.z;
 }
 
@@ -188,8 +191,9 @@ __global__ void k_prim_check(SceneView sv, CameraK cam, int n, QueueView q0, Pri
   if (rec.pix[i] < 0) return;
   const EdgeRec E = load_edge(sv, rec.edge[i]);
   const f3 ep = E.p0 + rec.t[i] * (E.p1 - E.p0);
-  const Its a = reconstruct(sv, rays.tri[p], rays.u[p], rays.v[p], cam.pos, true);
-  const Its b = reconstruct(sv, rays.tri[n + p], rays.u[n + p], rays.v[n + p], cam.pos, true);
+  const f3 o = f3{rays.o[3 * p], rays.o[3 * p + 1], rays.o[3 * p + 2]};
+  const Its a = reconstruct(sv, rays.tri[p], rays.u[p], rays.v[p], o, true);
+  const Its b = reconstruct(sv, rays.tri[n + p], rays.u[n + p], rays.v[n + p], o, true);
   const bool ok = a.valid && b.valid && (norm(a.p - ep) < 10.0f * REAL_EPS || norm(b.p - ep) < 10.0f * REAL_EPS);
   if (!ok) {
     rays.tri[p] = -1;
@@ -206,9 +210,10 @@ __global__ void k_prim_requeue(int n, QueueView q0, const uint64_t* __restrict__
 }
 
 template <bool PRE>
-__device__ __forceinline__ void prim_finish_one(const SceneView& sv, int i, PrimRecs& rec, const float* LA,
-                                                const float* LB, const uint64_t* rng_b, uint64_t* state, float* mass,
-                                                const float* dLdI, uint32_t spp, float* grad_v, EdgeCache* gc) {
+__device__ __forceinline__ void prim_finish_one(const SceneView& sv, const CameraK& cam, int i, PrimRecs& rec,
+                                                const float* LA, const float* LB, const uint64_t* rng_b,
+                                                uint64_t* state, float* mass, const float* dLdI, uint32_t spp,
+                                                float* grad_v, EdgeCache* gc, float* cacc) {
   if (PRE) state[i] = rng_b[i];
   const int pix = rec.pix[i];
   if (pix < 0) return;
@@ -230,7 +235,15 @@ __device__ __forceinline__ void prim_finish_one(const SceneView& sv, int i, Prim
   if (gx == 0.f) return;
   const f3 nn = f3{rec.n[i], rec.n[rec.cap + i], rec.n[2 * rec.cap + i]};
   const EdgeRec E = load_edge(sv, rec.edge[i]);
-  scatter_edge(sv, grad_v, E, rec.t[i], nn * gx, gc);
+  f3 gep = nn * gx;
+  // origin term (o - detach o) . (n (x) [1, 1, -1]); an orthographic origin is
+  // the edge point's spot on the camera plane, so it also moves with the edge
+  if (cacc || cam.ortho) {
+    const f3 go = f3{nn.x, nn.y, -nn.z} * gx;
+    const f3 ep = E.p0 + rec.t[i] * (E.p1 - E.p0);
+    gep = gep + cam_origin_vjp(cam, go, ep, cam.w2c, CAM_W, cacc);
+  }
+  scatter_edge(sv, grad_v, E, rec.t[i], gep, gc);
 }
 
 // Position p handles lane order[p] (the processing order of the pass), so a
@@ -241,20 +254,24 @@ __global__ __launch_bounds__(BT) void k_prim_finish(SceneView sv, CameraK cam, i
                                                     const float* __restrict__ LB, const uint64_t* __restrict__ rng_b,
                                                     uint64_t* __restrict__ state, float* __restrict__ mass,
                                                     const float* __restrict__ dLdI, uint32_t spp,
-                                                    float* __restrict__ grad_v) {
+                                                    float* __restrict__ grad_v, float* __restrict__ cam_acc) {
   __shared__ int gk[PRE ? 1 : EDGE_SLOTS];
   __shared__ float gvals[PRE ? 1 : 3 * EDGE_SLOTS];
+  __shared__ float cacc[PRE ? 1 : CAM_ACC];
   EdgeCache gc{gk, gvals};
   if (!PRE) {
     gc.init();
+    cam_acc_init(cacc);
     __syncthreads();
   }
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p < n)
-    prim_finish_one<PRE>(sv, static_cast<int>(order[p]), rec, LA, LB, rng_b, state, mass, dLdI, spp, grad_v, &gc);
+    prim_finish_one<PRE>(sv, cam, static_cast<int>(order[p]), rec, LA, LB, rng_b, state, mass, dLdI, spp, grad_v,
+                         &gc, cam_acc ? cacc : nullptr);
   if (!PRE) {
     __syncthreads();
     gc.flush(grad_v, grad_v);
+    if (cam_acc) cam_acc_flush(cacc, cam_acc);
   }
 }
 
@@ -307,12 +324,12 @@ __device__ DirectOut eval_direct(const SceneView& sv, const CameraK& cam, float 
   valid = d0.valid && !d0.is_light;
   if (!valid) return r;
   int px, py;
-  if (!camera_project(cam, d0.p, &px, &py)) return r;
-  const f3 cd = normalize(d0.p - cam.pos);
-  const float tc = norm(d0.p - cam.pos);
-  h = closest_hit<BSTACK>(sv.node, sv.leaf, sv.n_tris, cam.pos, cd, stk, spill, spill_stride, vis_tmax(tc),
+  f3 co, cd;
+  if (!camera_point_ray(cam, d0.p, &px, &py, &co, &cd)) return r;
+  const float tc = norm(d0.p - co);
+  h = closest_hit<BSTACK>(sv.node, sv.leaf, sv.n_tris, co, cd, stk, spill, spill_stride, vis_tmax(tc),
                           vis_tocc(tc));
-  const Its i2 = reconstruct(sv, h.tri, h.u, h.v, cam.pos, true);
+  const Its i2 = reconstruct(sv, h.tri, h.u, h.v, co, true);
   if (!(i2.valid && !i2.is_light && sqnorm(i2.p - d0.p) < REAL_EPS)) return r;
   const float pdf_xw = ((s0.d2 * pdf_ep) * pdf_light) / dot(s0.n, dir);
   // boundary_term
@@ -491,6 +508,17 @@ __global__ void k_seed_states(uint64_t* st, int n) {
   if (i < n) st[i] = lane_rng(static_cast<uint32_t>(i)).state;
 }
 
+}  // namespace
+
+int fill_f32(float* p, int n, float v, hipStream_t st) {
+  if (n <= 0) return PSDR_OK;
+  hipLaunchKernelGGL(k_fill_f32, dim3((n + BT - 1) / BT), dim3(BT), 0, st, p, n, v);
+  PSDR_HIP_CHECK(hipGetLastError());
+  return PSDR_OK;
+}
+
+namespace {
+
 int g_cus_b = 0;
 int cus() {
   if (!g_cus_b) {
@@ -509,38 +537,39 @@ int cus() {
 // round over n = cells, which also accumulates the cell masses).
 static int primary_pass(psdr_scene* s, const psdr_config& cfg, const CameraK& cam, bool pre, int n, int res,
                         const float* cpmf, const float* ccmf, float* mass, hipStream_t st) {
+  RenderSlot& slot = *s->cur;
   const SceneView sv = s->view();
   const uint32_t nL = cfg.mis_light_samples;
   const int Kmax = static_cast<int>(nL + 1 + 1);
   const int64_t ray_cap = static_cast<int64_t>(n) * Kmax;
-  if (s->b_rec.ensure(PRIM_BYTES_PER * n) || s->b_rays.ensure(RAY_BYTES_PER * 2 * n) || s->b_hits.ensure(12ull * 2 * n) ||
-      s->b_q0.ensure(QUEUE_BYTES_PER * n) || s->b_LA.ensure(12ull * n) || s->b_LB.ensure(12ull * n) ||
-      s->b_rngA.ensure(8ull * n) || s->b_rngB.ensure(8ull * n) || s->p_qa.ensure(QUEUE_BYTES_PER * n) ||
-      s->p_qb.ensure(QUEUE_BYTES_PER * n) || s->p_rays.ensure(RAY_BYTES_PER * ray_cap) ||
-      s->p_hits.ensure(12ull * ray_cap) || s->p_rays2.ensure(RAY_BYTES_PER * ray_cap) ||
-      s->p_hits2.ensure(12ull * ray_cap) || s->b_counters.ensure(4 * 256))
+  if (slot.b_rec.ensure(PRIM_BYTES_PER * n) || slot.b_rays.ensure(RAY_BYTES_PER * 2 * n) || slot.b_hits.ensure(12ull * 2 * n) ||
+      slot.b_q0.ensure(QUEUE_BYTES_PER * n) || slot.b_LA.ensure(12ull * n) || slot.b_LB.ensure(12ull * n) ||
+      slot.b_rngA.ensure(8ull * n) || slot.b_rngB.ensure(8ull * n) || slot.p_qa.ensure(QUEUE_BYTES_PER * n) ||
+      slot.p_qb.ensure(QUEUE_BYTES_PER * n) || slot.p_rays.ensure(RAY_BYTES_PER * ray_cap) ||
+      slot.p_hits.ensure(12ull * ray_cap) || slot.p_rays2.ensure(RAY_BYTES_PER * ray_cap) ||
+      slot.p_hits2.ensure(12ull * ray_cap) || slot.b_counters.ensure(4 * 256))
     return PSDR_ERR_NOMEM;
-  PrimRecs rec = carve_prim(s->b_rec, n);
-  RaysView r0 = carve_rays(s->b_rays, s->b_hits, 2ll * n);
-  const QueueView q0 = carve_queue(s->b_q0, n);
-  const QueueView qs[2] = {carve_queue(s->p_qa, n), carve_queue(s->p_qb, n)};
-  const RaysView rs[2] = {carve_rays(s->p_rays, s->p_hits, ray_cap), carve_rays(s->p_rays2, s->p_hits2, ray_cap)};
-  int* cnt = s->b_counters.as<int>();
+  PrimRecs rec = carve_prim(slot.b_rec, n);
+  RaysView r0 = carve_rays(slot.b_rays, slot.b_hits, 2ll * n);
+  const QueueView q0 = carve_queue(slot.b_q0, n);
+  const QueueView qs[2] = {carve_queue(slot.p_qa, n), carve_queue(slot.p_qb, n)};
+  const RaysView rs[2] = {carve_rays(slot.p_rays, slot.p_hits, ray_cap), carve_rays(slot.p_rays2, slot.p_hits2, ray_cap)};
+  int* cnt = slot.b_counters.as<int>();
   int rc = zero_ints(cnt, 256, st);
   if (rc) return rc;
-  uint64_t* state = s->b_state.as<uint64_t>();
+  uint64_t* state = slot.b_state.as<uint64_t>();
   // the lane's PCG state after the setup draw; pass B's output overwrites it later
-  uint64_t* rng0 = s->b_rngB.as<uint64_t>();
+  uint64_t* rng0 = slot.b_rngB.as<uint64_t>();
   // render pass: process lanes in pixel-Morton order (the preprocess lanes are
   // hypercube cells, already in edge order)
   const uint32_t* order = nullptr;
   uint32_t *k0 = nullptr, *v0 = nullptr;
   const bool sorted = !pre && !s->plain;
   if (sorted) {
-    if (s->b_keys.ensure(8ull * n) || s->b_vals.ensure(8ull * n) || s->b_hist.ensure(4 * radix_hist_words(n)))
+    if (slot.b_keys.ensure(8ull * n) || slot.b_vals.ensure(8ull * n) || slot.b_hist.ensure(4 * radix_hist_words(n)))
       return PSDR_ERR_NOMEM;
-    k0 = s->b_keys.as<uint32_t>();
-    v0 = s->b_vals.as<uint32_t>();
+    k0 = slot.b_keys.as<uint32_t>();
+    v0 = slot.b_vals.as<uint32_t>();
   }
   if (pre)
     hipLaunchKernelGGL(k_prim_setup<true>, dim3(blocks(n, BT)), dim3(BT), 0, st, sv, cam, n, state, res, cpmf, ccmf,
@@ -552,15 +581,15 @@ static int primary_pass(psdr_scene* s, const psdr_config& cfg, const CameraK& ca
   if (sorted) {
     // keys are 2 x 16-bit Morton codes; a 2 x 11-bit code needs 3 byte passes
     const int passes = (cam.W <= 2048 && cam.H <= 2048) ? 3 : 4;
-    rc = radix_sort_pairs(k0, v0, k0 + n, v0 + n, s->b_hist.as<uint32_t>(), n, passes, st);
+    rc = radix_sort_pairs(k0, v0, k0 + n, v0 + n, slot.b_hist.as<uint32_t>(), n, passes, st);
     if (rc) return rc;
     order = (passes & 1) ? v0 + n : v0;
   }
   hipLaunchKernelGGL(k_prim_rays, dim3(blocks(n, BT)), dim3(BT), 0, st, sv, cam, n, order, rec, rng0, r0, q0, cnt);
   PSDR_HIP_CHECK(hipGetLastError());
   const int tgrid = trace_grid(2ll * n);
-  if (s->p_spill.ensure(std::max(s->p_spill.bytes, spill_bytes(tgrid)))) return PSDR_ERR_NOMEM;
-  rc = trace_rays(sv, r0.o, r0.d, 2 * n, nullptr, 1, 0, nullptr, 0, r0.tri, nullptr, r0.u, r0.v, s->p_spill.as<int>(),
+  if (slot.p_spill.ensure(std::max(slot.p_spill.bytes, spill_bytes(tgrid)))) return PSDR_ERR_NOMEM;
+  rc = trace_rays(sv, r0.o, r0.d, 2 * n, nullptr, 1, 0, nullptr, 0, r0.tri, nullptr, r0.u, r0.v, slot.p_spill.as<int>(),
                   tgrid, st);
   if (rc) return rc;
   if (s->plain) PSDR_HIP_CHECK(hipStreamSynchronize(st));
@@ -574,24 +603,24 @@ static int primary_pass(psdr_scene* s, const psdr_config& cfg, const CameraK& ca
   wa.ad_jac = 0;
   wa.keep = 0;
   wa.cache = CacheView{nullptr, nullptr, nullptr, n};
-  wa.lane_L = s->b_LA.as<float>();
-  wa.lane_rng_out = s->b_rngA.as<uint64_t>();
+  wa.lane_L = slot.b_LA.as<float>();
+  wa.lane_rng_out = slot.b_rngA.as<uint64_t>();
   wa.counters = cnt;
   wa.plain = s->plain ? 1 : 0;
-  wa.spill = &s->p_spill;
+  wa.spill = &slot.p_spill;
   rc = run_depths(s, wa, q0, r0, qs, rs, n, st);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_prim_requeue, dim3(blocks(n, BT)), dim3(BT), 0, st, n, q0, s->b_rngA.as<uint64_t>(),
+  hipLaunchKernelGGL(k_prim_requeue, dim3(blocks(n, BT)), dim3(BT), 0, st, n, q0, slot.b_rngA.as<uint64_t>(),
                      cnt + 128);
-  wa.lane_L = s->b_LB.as<float>();
-  wa.lane_rng_out = s->b_rngB.as<uint64_t>();
+  wa.lane_L = slot.b_LB.as<float>();
+  wa.lane_rng_out = slot.b_rngB.as<uint64_t>();
   wa.counters = cnt + 128;
   rc = run_depths(s, wa, q0, offset_rays(r0, n), qs, rs, n, st);
   if (rc) return rc;
   if (pre) {
     hipLaunchKernelGGL(k_prim_finish<true>, dim3(blocks(n, BT)), dim3(BT), 0, st, sv, cam, n, q0.lane, rec,
-                       s->b_LA.as<float>(), s->b_LB.as<float>(), s->b_rngB.as<uint64_t>(), state, mass, nullptr,
-                       cfg.spp_primary_edge, nullptr);
+                       slot.b_LA.as<float>(), slot.b_LB.as<float>(), slot.b_rngB.as<uint64_t>(), state, mass, nullptr,
+                       cfg.spp_primary_edge, nullptr, nullptr);
     PSDR_HIP_CHECK(hipGetLastError());
   }
   return PSDR_OK;
@@ -608,43 +637,46 @@ static int edge_lanes(uint32_t spp, const CameraK& cam, const char* what) {
 }
 
 int boundary_primary_prepare(psdr_scene* s, const psdr_config& cfg, const psdr_camera& camd, hipStream_t st) {
+  RenderSlot& slot = *s->cur;
   if (s->n_edges == 0) return PSDR_OK;
   const CameraK cam = make_camera(camd);
   const int R = static_cast<int>(cfg.primary_edge_hypercube_resolution);
   const int n = edge_lanes(cfg.spp_primary_edge, cam, "primary boundary");
   if (n < 0) return PSDR_ERR_ARG;
-  if (s->b_state.ensure(8ull * std::max(R, 1)) || s->hc_mass.ensure(4ull * R) || s->hc_pmf.ensure(4ull * R) ||
-      s->hc_cmf.ensure(4ull * R) || s->b_tmp.ensure(4ull * (2 * (R / 256 + 2) + 16)))
+  if (slot.b_state.ensure(8ull * std::max(R, 1)) || slot.hc_mass.ensure(4ull * R) || slot.hc_pmf.ensure(4ull * R) ||
+      slot.hc_cmf.ensure(4ull * R) || slot.b_tmp.ensure(4ull * (2 * (R / 256 + 2) + 16)))
     return PSDR_ERR_NOMEM;
-  hipLaunchKernelGGL(k_seed_states, dim3(blocks(R, BT)), dim3(BT), 0, st, s->b_state.as<uint64_t>(), R);
-  hipLaunchKernelGGL(k_fill_f32, dim3(blocks(R, BT)), dim3(BT), 0, st, s->hc_mass.as<float>(), R, 0.f);
+  hipLaunchKernelGGL(k_seed_states, dim3(blocks(R, BT)), dim3(BT), 0, st, slot.b_state.as<uint64_t>(), R);
+  hipLaunchKernelGGL(k_fill_f32, dim3(blocks(R, BT)), dim3(BT), 0, st, slot.hc_mass.as<float>(), R, 0.f);
   for (uint32_t r = 0; r < cfg.primary_edge_preprocess_rounds; ++r) {
-    const int rc = primary_pass(s, cfg, cam, true, R, R, nullptr, nullptr, s->hc_mass.as<float>(), st);
+    const int rc = primary_pass(s, cfg, cam, true, R, R, nullptr, nullptr, slot.hc_mass.as<float>(), st);
     if (rc) return rc;
   }
-  int rc = sampler_tables(s->hc_mass.as<float>(), R, s->hc_pmf.as<float>(), s->hc_cmf.as<float>(), nullptr,
-                          s->b_tmp.as<float>(), st);
+  int rc = sampler_tables(slot.hc_mass.as<float>(), R, slot.hc_pmf.as<float>(), slot.hc_cmf.as<float>(), nullptr,
+                          slot.b_tmp.as<float>(), st);
   if (rc) return rc;
-  return primary_pass(s, cfg, cam, false, n, R, s->hc_pmf.as<float>(), s->hc_cmf.as<float>(), nullptr, st);
+  return primary_pass(s, cfg, cam, false, n, R, slot.hc_pmf.as<float>(), slot.hc_cmf.as<float>(), nullptr, st);
 }
 
 int boundary_primary_finish(psdr_scene* s, const psdr_config& cfg, const psdr_camera& camd, const float* dL_dI,
-                            float* grad_v, hipStream_t st) {
+                            float* grad_v, float* cam_acc, hipStream_t st) {
+  RenderSlot& slot = *s->cur;
   if (s->n_edges == 0) return PSDR_OK;
   const CameraK cam = make_camera(camd);
   const int n = edge_lanes(cfg.spp_primary_edge, cam, "primary boundary");
   if (n < 0) return PSDR_ERR_ARG;
   if (n == 0) return PSDR_OK;
-  const PrimRecs rec = carve_prim(s->b_rec, n);
-  const QueueView q0 = carve_queue(s->b_q0, n);
+  const PrimRecs rec = carve_prim(slot.b_rec, n);
+  const QueueView q0 = carve_queue(slot.b_q0, n);
   hipLaunchKernelGGL(k_prim_finish<false>, dim3(blocks(n, BT)), dim3(BT), 0, st, s->view(), cam, n, q0.lane, rec,
-                     s->b_LA.as<float>(), s->b_LB.as<float>(), s->b_rngB.as<uint64_t>(), nullptr, nullptr, dL_dI,
-                     cfg.spp_primary_edge, grad_v);
+                     slot.b_LA.as<float>(), slot.b_LB.as<float>(), slot.b_rngB.as<uint64_t>(), nullptr, nullptr, dL_dI,
+                     cfg.spp_primary_edge, grad_v, cam_acc);
   PSDR_HIP_CHECK(hipGetLastError());
   return PSDR_OK;
 }
 
 int boundary_direct_prepare(psdr_scene* s, const psdr_config& cfg, const psdr_camera& camd, hipStream_t st) {
+  RenderSlot& slot = *s->cur;
   if (s->n_edges == 0) return PSDR_OK;
   const CameraK cam = make_camera(camd);
   const SceneView sv = s->view();
@@ -660,53 +692,54 @@ int boundary_direct_prepare(psdr_scene* s, const psdr_config& cfg, const psdr_ca
   if (n < 0) return PSDR_ERR_ARG;
   const int cells = static_cast<int>(cells64);
   const int grid = std::max(1, std::min(blocks(std::max(cells, n), BT), cus() * 8));
-  if (s->d_state.ensure(8ull * cells) || s->d_mass.ensure(4ull * cells) || s->d_pmf.ensure(4ull * cells) ||
-      s->d_cmf.ensure(4ull * cells) || s->d_tmp.ensure(4ull * (2 * (cells / 256 + 2) + 16)) ||
-      s->d_spill.ensure(static_cast<size_t>(grid) * BT * (STACK_TOTAL - BSTACK) * 4) ||
-      s->d_keys.ensure(8ull * n) || s->d_vals.ensure(8ull * n) || s->d_hist.ensure(4 * radix_hist_words(n)) ||
-      s->d_rec.ensure(48ull * std::max(n, 1)))
+  if (slot.d_state.ensure(8ull * cells) || slot.d_mass.ensure(4ull * cells) || slot.d_pmf.ensure(4ull * cells) ||
+      slot.d_cmf.ensure(4ull * cells) || slot.d_tmp.ensure(4ull * (2 * (cells / 256 + 2) + 16)) ||
+      slot.d_spill.ensure(static_cast<size_t>(grid) * BT * (STACK_TOTAL - BSTACK) * 4) ||
+      slot.d_keys.ensure(8ull * n) || slot.d_vals.ensure(8ull * n) || slot.d_hist.ensure(4 * radix_hist_words(n)) ||
+      slot.d_rec.ensure(48ull * std::max(n, 1)))
     return PSDR_ERR_NOMEM;
-  uint64_t* state = s->d_state.as<uint64_t>();
+  uint64_t* state = slot.d_state.as<uint64_t>();
   hipLaunchKernelGGL(k_seed_states, dim3(blocks(cells, BT)), dim3(BT), 0, st, state, cells);
-  hipLaunchKernelGGL(k_fill_f32, dim3(blocks(cells, BT)), dim3(BT), 0, st, s->d_mass.as<float>(), cells, 0.f);
+  hipLaunchKernelGGL(k_fill_f32, dim3(blocks(cells, BT)), dim3(BT), 0, st, slot.d_mass.as<float>(), cells, 0.f);
   for (uint32_t r = 0; r < cfg.secondary_edge_preprocess_rounds; ++r) {
     hipLaunchKernelGGL(k_direct<true>, dim3(grid), dim3(BT), 0, st, sv, cam, cells, state, r0, r1, r2, nullptr, nullptr,
-                       s->d_mass.as<float>(), 0u, nullptr, s->d_spill.as<int>(), nullptr);
+                       slot.d_mass.as<float>(), 0u, nullptr, slot.d_spill.as<int>(), nullptr);
     PSDR_HIP_CHECK(hipGetLastError());
   }
-  int rc = sampler_tables(s->d_mass.as<float>(), cells, s->d_pmf.as<float>(), s->d_cmf.as<float>(), nullptr,
-                          s->d_tmp.as<float>(), st);
+  int rc = sampler_tables(slot.d_mass.as<float>(), cells, slot.d_pmf.as<float>(), slot.d_cmf.as<float>(), nullptr,
+                          slot.d_tmp.as<float>(), st);
   if (rc) return rc;
   if (n == 0) return PSDR_OK;
   // process the lanes in cell order (edge position major): neighbouring lanes
   // of a wave then trace from neighbouring edge points
   const uint32_t* order = nullptr;
   if (!s->plain) {
-    uint32_t* k0 = s->d_keys.as<uint32_t>();
-    uint32_t* v0 = s->d_vals.as<uint32_t>();
-    hipLaunchKernelGGL(k_direct_keys, dim3(blocks(n, BT)), dim3(BT), 0, st, n, cells, s->d_pmf.as<float>(),
-                       s->d_cmf.as<float>(), k0, v0);
+    uint32_t* k0 = slot.d_keys.as<uint32_t>();
+    uint32_t* v0 = slot.d_vals.as<uint32_t>();
+    hipLaunchKernelGGL(k_direct_keys, dim3(blocks(n, BT)), dim3(BT), 0, st, n, cells, slot.d_pmf.as<float>(),
+                       slot.d_cmf.as<float>(), k0, v0);
     PSDR_HIP_CHECK(hipGetLastError());
     const int passes = cells <= (1 << 24) ? 3 : 4;
-    rc = radix_sort_pairs(k0, v0, k0 + n, v0 + n, s->d_hist.as<uint32_t>(), n, passes, st);
+    rc = radix_sort_pairs(k0, v0, k0 + n, v0 + n, slot.d_hist.as<uint32_t>(), n, passes, st);
     if (rc) return rc;
     order = (passes & 1) ? v0 + n : v0;
   }
   hipLaunchKernelGGL(k_direct<false>, dim3(grid), dim3(BT), 0, st, sv, cam, n, state, r0, r1, r2,
-                     s->d_pmf.as<float>(), s->d_cmf.as<float>(), nullptr, cfg.spp_secondary_edge,
-                     s->d_rec.as<float>(), s->d_spill.as<int>(), order);
+                     slot.d_pmf.as<float>(), slot.d_cmf.as<float>(), nullptr, cfg.spp_secondary_edge,
+                     slot.d_rec.as<float>(), slot.d_spill.as<int>(), order);
   PSDR_HIP_CHECK(hipGetLastError());
   return PSDR_OK;
 }
 
 int boundary_direct_finish(psdr_scene* s, const psdr_config& cfg, const psdr_camera& camd, const float* dL_dI,
                            float* grad_v, hipStream_t st) {
+  RenderSlot& slot = *s->cur;
   if (s->n_edges == 0) return PSDR_OK;
   const CameraK cam = make_camera(camd);
   const int n = edge_lanes(cfg.spp_secondary_edge, cam, "direct boundary");
   if (n < 0) return PSDR_ERR_ARG;
   if (n == 0) return PSDR_OK;
-  hipLaunchKernelGGL(k_direct_scatter, dim3(blocks(n, BT)), dim3(BT), 0, st, s->view(), n, s->d_rec.as<float>(), dL_dI,
+  hipLaunchKernelGGL(k_direct_scatter, dim3(blocks(n, BT)), dim3(BT), 0, st, s->view(), n, slot.d_rec.as<float>(), dL_dI,
                      grad_v);
   PSDR_HIP_CHECK(hipGetLastError());
   return PSDR_OK;

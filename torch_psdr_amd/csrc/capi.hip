@@ -9,6 +9,7 @@
 #include <string>
 #include <vector>
 
+#include "cam_grad.h"
 #include "render.h"
 #include "scene.h"
 
@@ -28,7 +29,7 @@ int DevBuf::ensure(size_t n) {
   if (hipMalloc(&p, n) != hipSuccess) {
     p = nullptr;
     bytes = 0;
-    hipGetLastError();
+    (void)hipGetLastError();
     set_error(PSDR_ERR_NOMEM, "hipMalloc of " + std::to_string(n) + " bytes failed");
     return PSDR_ERR_NOMEM;
   }
@@ -36,17 +37,17 @@ int DevBuf::ensure(size_t n) {
   return PSDR_OK;
 }
 void DevBuf::release() {
-  if (p) hipFree(p);
+  if (p) (void)hipFree(p);
   p = nullptr;
   bytes = 0;
 }
 
 AuxStreams::~AuxStreams() {
   for (int i = 0; i < 2; ++i) {
-    if (st[i]) hipStreamDestroy(st[i]);
-    if (join[i]) hipEventDestroy(join[i]);
+    if (st[i]) (void)hipStreamDestroy(st[i]);
+    if (join[i]) (void)hipEventDestroy(join[i]);
   }
-  if (fork) hipEventDestroy(fork);
+  if (fork) (void)hipEventDestroy(fork);
 }
 
 int AuxStreams::ensure() {
@@ -127,17 +128,38 @@ CameraK make_camera(const psdr_camera& c) {
     for (int col = 0; col < 4; ++col) k.c2w.m[r][col] = r < 3 ? get(cols[col], r) : (col == 3 ? 1.f : 0.f);
   k.w2c = mat_inverse(k.c2w);
   const float n = -0.01f, f = -1000.f;
-  const float inv_nf = 1.0f / (n - f);
-  float sh, ch;
-  psdr_sincos(c.vfov * 0.5f, &sh, &ch);
-  const float cot = ch / sh;
   Mat4 P;
   std::memset(&P, 0, sizeof(P));
-  P.m[0][0] = -cot;
-  P.m[1][1] = -cot;
-  P.m[2][2] = (n + f) * inv_nf;
-  P.m[2][3] = ((-2.f * f) * n) * inv_nf;
-  P.m[3][2] = 1.f;
+  k.ortho = c.kind == PSDR_CAMERA_ORTHOGRAPHIC ? 1 : 0;
+  if (k.ortho) {
+    // ortho_mat (src/camera.h:302-312) = scale * translate of orthographic()
+    // (src/camera.h:19-31): half extents view_width / 2 and H * (view_width / 2) / W
+    const float hw = c.view_width / 2.0f;
+    const float hh = (static_cast<float>(c.height) * hw) / static_cast<float>(c.width);
+    const float l = -hw, r = hw, b = -hh, t = hh;
+    Mat4 Sc, Tr;
+    std::memset(&Sc, 0, sizeof(Sc));
+    std::memset(&Tr, 0, sizeof(Tr));
+    for (int i = 0; i < 4; ++i) Tr.m[i][i] = 1.f;
+    Tr.m[0][3] = -(l + r) / 2.0f;
+    Tr.m[1][3] = -(b + t) / 2.0f;
+    Tr.m[2][3] = -(n + f) / 2.0f;
+    Sc.m[0][0] = 2.0f / (r - l);
+    Sc.m[1][1] = 2.0f / (t - b);
+    Sc.m[2][2] = 2.0f / (n - f);
+    Sc.m[3][3] = 1.f;
+    P = mat_mul(Sc, Tr);
+  } else {
+    const float inv_nf = 1.0f / (n - f);
+    float sh, ch;
+    psdr_sincos(c.vfov * 0.5f, &sh, &ch);
+    const float cot = ch / sh;
+    P.m[0][0] = -cot;
+    P.m[1][1] = -cot;
+    P.m[2][2] = (n + f) * inv_nf;
+    P.m[2][3] = ((-2.f * f) * n) * inv_nf;
+    P.m[3][2] = 1.f;
+  }
   Mat4 S, T;
   std::memset(&S, 0, sizeof(S));
   std::memset(&T, 0, sizeof(T));
@@ -151,6 +173,7 @@ CameraK make_camera(const psdr_camera& c) {
   k.c2r = mat_mul(mat_mul(S, T), P);
   k.r2c = mat_inverse(k.c2r);
   k.pos = xform_pos(k.c2w, f3{0.f, 0.f, 0.f});
+  k.look = xform_dir(k.c2w, f3{0.f, 0.f, -1.f});
   return k;
 }
 
@@ -253,6 +276,36 @@ static int check_desc(const psdr_scene_desc* d) {
 }  // namespace psdr
 
 using namespace psdr;
+
+RenderSlot* psdr_scene::find_slot(const psdr_config& cfg, const psdr_camera& cam) {
+  for (RenderSlot& r : slots)
+    if (r.have_cache && std::memcmp(&r.cfg, &cfg, sizeof(cfg)) == 0 && std::memcmp(&r.cam, &cam, sizeof(cam)) == 0) {
+      r.used = ++clock;
+      return &r;
+    }
+  return nullptr;
+}
+
+RenderSlot* psdr_scene::claim_slot(const psdr_config& cfg, const psdr_camera& cam) {
+  RenderSlot* best = nullptr;
+  for (RenderSlot& r : slots) {
+    if (r.have_cache && std::memcmp(&r.cfg, &cfg, sizeof(cfg)) == 0 && std::memcmp(&r.cam, &cam, sizeof(cam)) == 0) {
+      best = &r;
+      break;
+    }
+    if (!best || (!r.have_cache && best->have_cache) || (r.have_cache == best->have_cache && r.used < best->used))
+      best = &r;
+  }
+  best->used = ++clock;
+  return best;
+}
+
+void psdr_scene::invalidate_slots() {
+  for (RenderSlot& r : slots) {
+    r.have_cache = false;
+    r.bprep = false;
+  }
+}
 
 SceneView psdr_scene::view() const {
   SceneView v;
@@ -435,7 +488,11 @@ static bool cfg_ok(const psdr_config* c) {
          c->secondary_edge_hypercube_resolution[0] > 0 && c->secondary_edge_hypercube_resolution[1] > 0 &&
          c->secondary_edge_hypercube_resolution[2] > 0;
 }
-static bool cam_ok(const psdr_camera* c) { return c && c->height > 0 && c->width > 0; }
+static bool cam_ok(const psdr_camera* c) {
+  if (!c || c->height <= 0 || c->width <= 0) return false;
+  if (c->kind == PSDR_CAMERA_PERSPECTIVE) return true;
+  return c->kind == PSDR_CAMERA_ORTHOGRAPHIC && c->view_width > 0.f;
+}
 
 extern "C" {
 
@@ -471,8 +528,7 @@ int psdr_scene_update(psdr_scene* s, const psdr_scene_desc* desc, void* stream) 
   // a boundary prepass still running on the auxiliary streams reads the scene
   PSDR_TRY(aux_join(s, as_stream(stream)));
   PSDR_TRY(scene_set_buffers(s, desc, as_stream(stream)));
-  s->have_cache = false;
-  s->bprep = false;
+  s->invalidate_slots();
   return scene_derive(s, as_stream(stream));
 }
 
@@ -525,13 +581,13 @@ int psdr_render_forward(psdr_scene* s, const psdr_config* cfg, const psdr_camera
 }
 
 int psdr_render_backward(psdr_scene* s, const psdr_config* cfg, const psdr_camera* cam, const float* dL_dI,
-                         float* grad_vertices, float* grad_texels, float* grad_emit, int32_t terms, void* stream) {
+                         float* grad_vertices, float* grad_texels, float* grad_emit, float* grad_camera,
+                         int32_t terms, void* stream) {
   if (!s || !dL_dI || !cfg_ok(cfg) || !cam_ok(cam)) {
     set_error(PSDR_ERR_ARG, "render_backward: bad argument");
     return PSDR_ERR_ARG;
   }
-  if (!s->have_cache || std::memcmp(&s->last_cfg, cfg, sizeof(psdr_config)) != 0 ||
-      std::memcmp(&s->last_cam, cam, sizeof(psdr_camera)) != 0) {
+  if (!s->find_slot(*cfg, *cam)) {
     set_error(PSDR_ERR_STATE, "render_backward: no matching forward render with keep=1");
     return PSDR_ERR_STATE;
   }
@@ -539,7 +595,8 @@ int psdr_render_backward(psdr_scene* s, const psdr_config* cfg, const psdr_camer
     set_error(PSDR_ERR_ARG, "render_backward: bad terms mask");
     return PSDR_ERR_ARG;
   }
-  return render_backward(s, *cfg, *cam, dL_dI, grad_vertices, grad_texels, grad_emit, terms, as_stream(stream));
+  return render_backward(s, *cfg, *cam, dL_dI, grad_vertices, grad_texels, grad_emit, grad_camera, terms,
+                         as_stream(stream));
 }
 
 namespace {
@@ -561,14 +618,15 @@ int psdr_render_forward_plain(psdr_scene* s, const psdr_config* cfg, const psdr_
 }
 
 int psdr_render_backward_plain(psdr_scene* s, const psdr_config* cfg, const psdr_camera* cam, const float* dL_dI,
-                               float* grad_vertices, float* grad_texels, float* grad_emit, int32_t terms,
-                               void* stream) {
+                               float* grad_vertices, float* grad_texels, float* grad_emit, float* grad_camera,
+                               int32_t terms, void* stream) {
   if (!s) {
     set_error(PSDR_ERR_ARG, "render_backward_plain: bad argument");
     return PSDR_ERR_ARG;
   }
   PlainScope scope(s);
-  return psdr_render_backward(s, cfg, cam, dL_dI, grad_vertices, grad_texels, grad_emit, terms, stream);
+  return psdr_render_backward(s, cfg, cam, dL_dI, grad_vertices, grad_texels, grad_emit, grad_camera, terms,
+                              stream);
 }
 
 int psdr_pcg_lane_draws(uint32_t lane0, int32_t n_lanes, int32_t n_draws, uint32_t* out, void* stream) {
@@ -585,6 +643,15 @@ int psdr_camera_rays(const psdr_camera* cam, uint32_t spp, float* o, float* d, i
     return PSDR_ERR_ARG;
   }
   return camera_rays(make_camera(*cam), spp, o, d, pix, as_stream(stream));
+}
+
+int psdr_camera_param_vjp(const psdr_camera* cam, const float* matrix_adjoints, float* grad) {
+  if (!cam_ok(cam) || !matrix_adjoints || !grad) {
+    set_error(PSDR_ERR_ARG, "camera_param_vjp: bad argument");
+    return PSDR_ERR_ARG;
+  }
+  camera_param_vjp(*cam, matrix_adjoints, grad);
+  return PSDR_OK;
 }
 
 int psdr_camera_matrices(const psdr_camera* cam, float* c2w, float* w2c, float* c2r, float* r2c, float* pos) {

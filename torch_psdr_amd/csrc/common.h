@@ -171,7 +171,9 @@ struct Mat4 {
 struct CameraK {
   Mat4 c2w, w2c, c2r, r2c;
   f3 pos;
+  f3 look;    // transform_dir(camera2world, (0, 0, -1)) (src/camera.h:124)
   int H, W;
+  int ortho;  // OrthographicCamera (src/camera.h:279-362)
 };
 
 PSDR_HD f3 xform_pos(const Mat4& M, f3 p) {
@@ -185,25 +187,49 @@ PSDR_HD f3 xform_dir(const Mat4& M, f3 d) {
             (M.m[1][0] * d.x + M.m[1][1] * d.y) + M.m[1][2] * d.z,
             (M.m[2][0] * d.x + M.m[2][1] * d.y) + M.m[2][2] * d.z};
 }
-// gen_camera_ray_ (src/camera.h:235-258): lane -> pixel = lane / spp.
-PSDR_HD void camera_ray(const CameraK& c, uint32_t lane, uint32_t spp, float u0, float u1, f3* o, f3* d,
-                        int* pixel) {
+// Film point of a lane: lane -> pixel = lane / spp (x fastest) plus the lane's
+// first two draws.
+PSDR_HD f3 film_point(const CameraK& c, uint32_t lane, uint32_t spp, float u0, float u1, int* pixel) {
   const uint32_t pix = spp > 1 ? lane / spp : lane;
   const uint32_t x = pix % static_cast<uint32_t>(c.W);
   const uint32_t y = pix / static_cast<uint32_t>(c.W);
-  f3 p = f3{u0 + static_cast<float>(x), u1 + static_cast<float>(y), 1.0f};
-  p = xform_pos(c.r2c, p);
-  *d = normalize(xform_dir(c.c2w, p));
-  *o = c.pos;
   *pixel = static_cast<int>(pix);
+  return f3{u0 + static_cast<float>(x), u1 + static_cast<float>(y), 1.0f};
 }
-// gen_camera_ray_with_point_ (src/camera.h:260-276): raster pixel + validity.
-PSDR_HD bool camera_project(const CameraK& c, f3 pt, int* px, int* py) {
+// gen_camera_ray_: perspective (src/camera.h:235-258) rays leave the camera
+// position towards the film point; orthographic ones (src/camera.h:314-335)
+// leave the film point's spot on the camera plane along the view direction.
+PSDR_HD void camera_ray(const CameraK& c, uint32_t lane, uint32_t spp, float u0, float u1, f3* o, f3* d,
+                        int* pixel) {
+  const f3 p = xform_pos(c.r2c, film_point(c, lane, spp, u0, u1, pixel));
+  if (c.ortho) {
+    *o = xform_pos(c.c2w, f3{p.x, p.y, 0.0f});
+    *d = c.look;
+  } else {
+    *d = normalize(xform_dir(c.c2w, p));
+    *o = c.pos;
+  }
+}
+// gen_camera_ray_with_point_ (src/camera.h:260-276, 337-361): raster pixel and
+// validity of a world point; with o/d non-null also the camera ray through it.
+PSDR_HD bool camera_point_ray(const CameraK& c, f3 pt, int* px, int* py, f3* o, f3* d) {
   const f3 pc = xform_pos(c.w2c, pt);
   const f3 p = xform_pos(c.c2r, pc);
   *px = static_cast<int>(floorf(p.x));
   *py = static_cast<int>(floorf(p.y));
+  if (o) {
+    if (c.ortho) {
+      *o = xform_pos(c.c2w, f3{pc.x, pc.y, 0.0f});
+      *d = c.look;
+    } else {
+      *o = c.pos;
+      *d = normalize(pt - c.pos);
+    }
+  }
   return pc.z < 0.f && p.x > 0.f && p.x < static_cast<float>(c.W) && p.y > 0.f && p.y < static_cast<float>(c.H);
+}
+PSDR_HD bool camera_project(const CameraK& c, f3 pt, int* px, int* py) {
+  return camera_point_ray(c, pt, px, py, nullptr, nullptr);
 }
 
 // ---------------------------------------------------------------------------

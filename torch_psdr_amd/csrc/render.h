@@ -15,16 +15,19 @@ int lane_draws(uint32_t lane0, int n_lanes, int n_draws, uint32_t* out, hipStrea
 int camera_rays(const CameraK& cam, uint32_t spp, float* o, float* d, int* pix, hipStream_t st);
 int render_forward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& cam, float* out, int keep,
                    hipStream_t st);
+// grad_cam: PSDR_CAMERA_GRAD floats or null; cam_acc: camera matrix adjoints
+// (cam_grad.h) or null
 int render_backward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& cam, const float* dL_dI,
-                    float* grad_v, float* grad_t, float* grad_e, int terms, hipStream_t st);
+                    float* grad_v, float* grad_t, float* grad_e, float* grad_cam, int terms, hipStream_t st);
 int interior_backward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& cam, const float* dL_dI,
-                      float* grad_v, float* grad_t, float* grad_e, hipStream_t st);
+                      float* grad_v, float* grad_t, float* grad_e, float* cam_acc, hipStream_t st);
+int fill_f32(float* p, int n, float v, hipStream_t st);
 // Edge-sampling boundary terms in two parts: prepare (everything that does not
 // depend on dL/dI: hypercube preprocessing, sampler, samples, traversals) and
 // finish (the dL/dI-weighted scatter into vertex gradients).
 int boundary_primary_prepare(psdr_scene* s, const psdr_config& cfg, const psdr_camera& cam, hipStream_t st);
 int boundary_primary_finish(psdr_scene* s, const psdr_config& cfg, const psdr_camera& cam, const float* dL_dI,
-                            float* grad_v, hipStream_t st);
+                            float* grad_v, float* cam_acc, hipStream_t st);
 int boundary_direct_prepare(psdr_scene* s, const psdr_config& cfg, const psdr_camera& cam, hipStream_t st);
 int boundary_direct_finish(psdr_scene* s, const psdr_config& cfg, const psdr_camera& cam, const float* dL_dI,
                            float* grad_v, hipStream_t st);

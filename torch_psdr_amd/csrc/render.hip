@@ -182,6 +182,8 @@ __global__ __launch_bounds__(256) void k_shade(ShadeParams P, QueueView qi, cons
           cache.u[lane] = hu;
           cache.v[lane] = hv;
         }
+        // orthographic rays start on the camera plane, one origin per ray
+        if (P.cam.ortho) O = f3{rin.o[3 * j], rin.o[3 * j + 1], rin.o[3 * j + 2]};
         its = reconstruct(sv, ht, hu, hv, O, true);
         cu = hu;
         cv = hv;
@@ -667,8 +669,15 @@ int render_forward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& cam
   // caller may change afterwards: order it before this render
   int rcj = aux_join(s, st);
   if (rcj) return rcj;
+  // renderD: the forward state goes to this (config, camera)'s slot
+  RenderSlot* slot = nullptr;
+  if (keep) {
+    slot = s->claim_slot(cfg, camd);
+    slot->have_cache = false;
+    slot->bprep = false;
+    s->cur = slot;
+  }
   const bool prep = keep && !s->plain && s->n_edges > 0 && (cfg.spp_primary_edge > 0 || cfg.spp_secondary_edge > 0);
-  s->bprep = false;
   if (prep) {
     // renderD: the boundary terms' dL/dI-independent work goes to the
     // auxiliary streams first (the longest chain, enqueued before this render
@@ -686,13 +695,13 @@ int render_forward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& cam
   if (s->spill.ensure(spill_bytes(tgrid))) return PSDR_ERR_NOMEM;
   const int64_t slots = 1 + static_cast<int64_t>(D) * Kmax;
   CacheView cache{nullptr, nullptr, nullptr, N, nullptr, nullptr, nullptr, nullptr};
-  if (keep) {
-    if (s->hit_cache.ensure(12ull * slots * N) || s->replay.ensure((20ull * std::max(D, 1) + 16ull) * N))
+  if (slot) {
+    if (slot->hit_cache.ensure(12ull * slots * N) || slot->replay.ensure((20ull * std::max(D, 1) + 16ull) * N))
       return PSDR_ERR_NOMEM;
-    cache.tri = s->hit_cache.as<int>();
+    cache.tri = slot->hit_cache.as<int>();
     cache.u = reinterpret_cast<float*>(cache.tri + slots * N);
     cache.v = cache.u + slots * N;
-    cache.rs = s->replay.as<uint64_t>();
+    cache.rs = slot->replay.as<uint64_t>();
     cache.T = reinterpret_cast<float*>(cache.rs + static_cast<int64_t>(std::max(D, 1)) * N);
     cache.L = cache.T + 3ll * std::max(D, 1) * N;
     cache.info = reinterpret_cast<int*>(cache.L + 3ll * N);
@@ -741,9 +750,16 @@ int render_forward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& cam
   s->last_N = N;
   s->last_K = Kmax;
   s->last_D = D;
-  s->have_cache = keep != 0;
   s->last_cfg = cfg;
   s->last_cam = camd;
+  if (slot) {
+    slot->N = N;
+    slot->K = Kmax;
+    slot->D = D;
+    slot->cfg = cfg;
+    slot->cam = camd;
+    slot->have_cache = true;
+  }
   return PSDR_OK;
 }
 

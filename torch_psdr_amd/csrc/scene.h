@@ -106,6 +106,29 @@ struct SceneView {
   int n_edges;
 };
 
+// Forward state of one renderD image kept for its backward pass: the
+// interior's hit records and replay state, and the boundary terms' samples
+// prepared during the forward render (with the scratch of their passes).  A
+// scene keeps RENDER_SLOTS of them, one per distinct (config, camera), so the
+// backward passes of a multi-camera renderD find their forward state.
+struct RenderSlot {
+  DevBuf hit_cache, replay;
+  DevBuf b_rec, b_rays, b_hits, b_q0, b_LA, b_LB, b_rngA, b_rngB, b_state, b_tmp, b_counters, b_keys, b_vals, b_hist;
+  DevBuf hc_mass, hc_pmf, hc_cmf;
+  // primary-boundary wavefront scratch (separate from the interior render's)
+  DevBuf p_qa, p_qb, p_rays, p_hits, p_rays2, p_hits2, p_spill;
+  // direct-boundary scratch (separate from the primary pass's, so both can run at once)
+  DevBuf d_state, d_mass, d_pmf, d_cmf, d_tmp, d_spill, d_keys, d_vals, d_hist, d_rec;
+  bool have_cache = false;  // hit_cache / replay hold the forward render of (cfg, cam)
+  bool bprep = false;       // the boundary prepass for (cfg, cam) has been issued
+  uint64_t N = 0;
+  int K = 0, D = 0;
+  psdr_config cfg{};
+  psdr_camera cam{};
+  uint64_t used = 0;  // last-use stamp (least recently used slot is reused first)
+};
+constexpr int RENDER_SLOTS = 4;
+
 }  // namespace psdr
 
 struct psdr_scene {
@@ -125,36 +148,31 @@ struct psdr_scene {
   // LBVH
   psdr::DevBuf morton, order, child, parent, box, leaf, node, counters, sort_keys, sort_vals, sort_hist, bounds,
       ranges, bvh_boxes;
-  // renderer scratch
-  psdr::DevBuf replay;  // per-lane, per-depth state of the last forward with keep (path replay)
-  psdr::DevBuf lane_L, q_a, q_b, rays, hits, rays2, hits2, spill, hit_cache, counters_r, hc_mass, hc_pmf,
-      hc_cmf, stats;
-  // boundary estimators
-  psdr::DevBuf b_rec, b_rays, b_hits, b_q0, b_LA, b_LB, b_rngA, b_rngB, b_state, b_tmp, b_counters, b_keys, b_vals,
-      b_hist;
-  // primary-boundary wavefront scratch (separate from the interior render's)
-  psdr::DevBuf p_qa, p_qb, p_rays, p_hits, p_rays2, p_hits2, p_spill;
-  // direct-boundary scratch (separate from the primary pass's, so both can run at once)
-  psdr::DevBuf d_state, d_mass, d_pmf, d_cmf, d_tmp, d_spill, d_keys, d_vals, d_hist, d_rec;
+  // renderer scratch (interior forward, main stream)
+  psdr::DevBuf lane_L, q_a, q_b, rays, hits, rays2, hits2, spill, counters_r, stats, cam_acc;
+  // forward state per (config, camera); cur = the slot the current call works on
+  psdr::RenderSlot slots[psdr::RENDER_SLOTS];
+  psdr::RenderSlot* cur = &slots[0];
+  uint64_t clock = 0;
   psdr::AuxStreams aux;
   // Reference launch shape, set for the duration of a *_plain call: every
   // depth's kernels run over all lanes (no compaction), each ray kind is traced
   // by its own launch followed by a host sync, boundary samples are processed in
   // lane order and the gradient terms run one after another.
   bool plain = false;
-  // boundary prepass issued by the last forward render with keep != 0
-  bool bprep = false;
-  psdr_config bprep_cfg{};
-  psdr_camera bprep_cam{};
   int light_tri_total = 0;
-  // last forward (for backward / stats)
+  // last forward render (lane radiance and statistics entry points)
   uint64_t last_N = 0;
   int last_K = 0, last_D = 0;
-  bool have_cache = false;
   psdr_config last_cfg{};
   psdr_camera last_cam{};
 
   psdr::SceneView view() const;
+  // slot holding the forward render of (cfg, cam), or null
+  psdr::RenderSlot* find_slot(const psdr_config& cfg, const psdr_camera& cam);
+  // slot for a new forward render of (cfg, cam): its own, a free one or the least recently used
+  psdr::RenderSlot* claim_slot(const psdr_config& cfg, const psdr_camera& cam);
+  void invalidate_slots();
 };
 
 namespace psdr {

@@ -26,8 +26,22 @@ struct HitRec {
   float t, u, v;
 };
 
-__device__ __forceinline__ float slab_entry(float4 lo_hi_a, float4 lo_hi_b, float4 lo_hi_c, int which, f3 oi,
-                                            f3 inv, float tbest) {
+// Reciprocal ray direction for the slab test.  A component of magnitude below
+// 2^-64 (in particular an exact zero: axis-parallel rays, e.g. an orthographic
+// camera looking along a coordinate plane) is replaced by +-2^-64, so the
+// reciprocal and the products b * inv - o * inv stay finite: with an infinite
+// reciprocal the fused form b * inv + (-o * inv) gives inf - inf = NaN or a
+// wrongly signed infinity and rejects boxes the ray passes through.  The
+// perturbation (below 2^-64 radians) is far inside the padded, widened test.
+PSDR_HD f3 slab_inverse(f3 d) {
+  const float m = 0x1p-64f;
+  const float x = fabsf(d.x) >= m ? d.x : copysignf(m, d.x);
+  const float y = fabsf(d.y) >= m ? d.y : copysignf(m, d.y);
+  const float z = fabsf(d.z) >= m ? d.z : copysignf(m, d.z);
+  return f3{1.0f / x, 1.0f / y, 1.0f / z};
+}
+
+PSDR_HD float slab_entry(float4 lo_hi_a, float4 lo_hi_b, float4 lo_hi_c, int which, f3 oi, f3 inv, float tbest) {
   float lx, ly, lz, hx, hy, hz;
   if (which == 0) {
     lx = lo_hi_a.x; ly = lo_hi_a.y; lz = lo_hi_a.z;
@@ -107,7 +121,7 @@ __device__ __forceinline__ HitRec closest_hit(const float4* __restrict__ node, c
     if (h.tri < 0) h.t = F_INF;
     return h;
   }
-  const f3 inv = f3{1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+  const f3 inv = slab_inverse(d);
   const f3 oi = f3{-(o.x * inv.x), -(o.y * inv.y), -(o.z * inv.z)};
   const int lane = threadIdx.x, nthr = blockDim.x;
   int sp = 0;

@@ -68,7 +68,6 @@ class NativeScene:
         desc = flat.desc(self.keep_alive)
         N.check(self.lib.psdr_scene_create(N.C.byref(desc), N.C.byref(self.handle), N.current_stream()))
         self.flat = flat
-        self.last_forward = None  # (cam, cfg, keep) of the last forward on self.flat
 
     def bind(self, flat: FlatScene) -> None:
         if flat is self.flat:
@@ -78,7 +77,6 @@ class NativeScene:
         N.check(self.lib.psdr_scene_update(self.handle, N.C.byref(desc), N.current_stream()))
         self.keep_alive = keep_alive
         self.flat = flat
-        self.last_forward = None
 
     def info(self):
         nt, ne, dep = N.C.c_int32(), N.C.c_int32(), N.C.c_int32()
@@ -131,12 +129,17 @@ def make_config(cfg: Sequence[int]) -> N.PsdrConfig:
 
 
 def make_camera(cam: Sequence[float]) -> N.PsdrCamera:
+    """cam = look_from(3), look_at(3), up(3), vfov or view_width, height, width[, kind]."""
     c = N.PsdrCamera()
     for i in range(3):
         c.look_from[i] = float(cam[i])
         c.look_at[i] = float(cam[3 + i])
         c.up[i] = float(cam[6 + i])
-    c.vfov = float(cam[9])
+    c.kind = int(cam[12]) if len(cam) > 12 else N.PSDR_CAMERA_PERSPECTIVE
+    if c.kind == N.PSDR_CAMERA_ORTHOGRAPHIC:
+        c.view_width = float(cam[9])
+    else:
+        c.vfov = float(cam[9])
     c.height = int(cam[10])
     c.width = int(cam[11])
     return c
@@ -150,49 +153,58 @@ def forward_native(ns: NativeScene, flat: FlatScene, cam: Sequence[float], cfg: 
     c, k = make_config(cfg), make_camera(cam)
     fn = ns.lib.psdr_render_forward_plain if _plain() else ns.lib.psdr_render_forward
     N.check(fn(ns.handle, N.C.byref(c), N.C.byref(k), out.data_ptr(), int(keep), N.current_stream()))
-    ns.last_forward = (tuple(cam), tuple(cfg), bool(keep))
     return out
 
 
 @torch.library.custom_op("psdr::render", mutates_args=())
 def render(token: int, cam: List[float], cfg: List[int], vertices: torch.Tensor, texels: torch.Tensor,
-           emit: torch.Tensor, keep: bool) -> torch.Tensor:
+           emit: torch.Tensor, camera: torch.Tensor, keep: bool) -> torch.Tensor:
+    """`camera` is the camera's parameter vector (PSDR_CAMERA_GRAD floats, the
+    values of `cam[:10]`): an input only so that autograd routes its gradient."""
     ns, flat = entry(token)
     return forward_native(ns, flat, cam, cfg, keep)
 
 
 @render.register_fake
-def _render_fake(token, cam, cfg, vertices, texels, emit, keep):
+def _render_fake(token, cam, cfg, vertices, texels, emit, camera, keep):
     return vertices.new_empty((int(cam[10]), int(cam[11]), 3))
 
 
 @torch.library.custom_op("psdr::render_backward", mutates_args=())
 def render_backward(token: int, cam: List[float], cfg: List[int], grad: torch.Tensor, vertices: torch.Tensor,
-                    texels: torch.Tensor, emit: torch.Tensor) -> List[torch.Tensor]:
+                    texels: torch.Tensor, emit: torch.Tensor, want_camera: bool) -> List[torch.Tensor]:
     ns, flat = entry(token)
     ns.bind(flat)
-    if ns.last_forward != (tuple(cam), tuple(cfg), True):
-        forward_native(ns, flat, cam, cfg, True)
+    c, k = make_config(cfg), make_camera(cam)
     gv = torch.zeros_like(vertices)
     gt = torch.zeros_like(texels)
     ge = torch.zeros((max(int(emit.shape[0]), 1), 3), dtype=torch.float32, device=emit.device)
-    c, k = make_config(cfg), make_camera(cam)
+    gc = torch.zeros(N.PSDR_CAMERA_GRAD, dtype=torch.float32, device=vertices.device)
     g = grad.contiguous().to(torch.float32)
     fn = ns.lib.psdr_render_backward_plain if _plain() else ns.lib.psdr_render_backward
-    N.check(fn(ns.handle, N.C.byref(c), N.C.byref(k), g.data_ptr(), gv.data_ptr(), gt.data_ptr(), ge.data_ptr(),
-               N.PSDR_TERM_ALL, N.current_stream()))
-    return [gv, gt, ge[: emit.shape[0]].clone()]
+    args = (ns.handle, N.C.byref(c), N.C.byref(k), g.data_ptr(), gv.data_ptr(), gt.data_ptr(), ge.data_ptr(),
+            gc.data_ptr() if want_camera else None, N.PSDR_TERM_ALL, N.current_stream())
+    rc = fn(*args)
+    if rc == N.PSDR_ERR_STATE:
+        # the scene keeps the forward state of its last few (camera, config)
+        # pairs; render this one again when it has been evicted
+        forward_native(ns, flat, cam, cfg, True)
+        rc = fn(*args)
+    N.check(rc)
+    return [gv, gt, ge[: emit.shape[0]].clone(), gc]
 
 
 @render_backward.register_fake
-def _render_backward_fake(token, cam, cfg, grad, vertices, texels, emit):
-    return [torch.empty_like(vertices), torch.empty_like(texels), torch.empty_like(emit)]
+def _render_backward_fake(token, cam, cfg, grad, vertices, texels, emit, want_camera):
+    return [torch.empty_like(vertices), torch.empty_like(texels), torch.empty_like(emit),
+            vertices.new_empty((N.PSDR_CAMERA_GRAD,))]
 
 
 def _setup(ctx, inputs, output):
-    token, cam, cfg, vertices, texels, emit, keep = inputs
+    token, cam, cfg, vertices, texels, emit, camera, keep = inputs
     ctx.token, ctx.cam, ctx.cfg = token, cam, cfg
     ctx.entry = entry(token)  # keeps the scene alive until backward
+    ctx.camera_meta = (camera.device, camera.dtype)
     ctx.save_for_backward(vertices, texels, emit)
 
 
@@ -200,8 +212,10 @@ def _backward(ctx, grad):
     vertices, texels, emit = ctx.saved_tensors
     with _lock:
         _entries.setdefault(ctx.token, ctx.entry)
-    gv, gt, ge = render_backward(ctx.token, ctx.cam, ctx.cfg, grad, vertices, texels, emit)
-    return None, None, None, gv, gt, ge, None
+    want_camera = bool(ctx.needs_input_grad[6])
+    gv, gt, ge, gc = render_backward(ctx.token, ctx.cam, ctx.cfg, grad, vertices, texels, emit, want_camera)
+    gcam = gc.to(device=ctx.camera_meta[0], dtype=ctx.camera_meta[1]) if want_camera else None
+    return None, None, None, gv, gt, ge, gcam, None
 
 
 render.register_autograd(_backward, setup_context=_setup)
