@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define PSDR_ABI_VERSION 3
+#define PSDR_ABI_VERSION 4
 
 /* status codes */
 #define PSDR_OK 0
@@ -34,6 +34,16 @@ extern "C" {
 #define PSDR_MAT_DIFFUSE 0      /* DiffuseMaterial: albedo texture / pi      */
 #define PSDR_MAT_DIFFUSE_BSDF 1 /* DiffuseBsdfMaterial: color, roughness     */
 #define PSDR_MAT_LIGHT 2        /* DiffuseLightMaterial of an AreaLight mesh */
+/* The bounding cube of an environment light (src/emitter.cc:95-130, 218-247;
+ * placed by Scene::Scene, src/scene.cc:31-55): 12 triangles, the LAST object,
+ * at most one.  Its radiance is the cube map at the hit's uv (EnvLight,
+ * albedo_* = the map) or the second-order spherical harmonics of the hit's
+ * direction to the viewer (SH9Light, albedo_offset = 27 coefficient floats, 9
+ * RGB rows in the texel buffer).  It has no edges, its light power is 4*pi and
+ * the traversal tests its triangles for every ray instead of holding them in
+ * the LBVH. */
+#define PSDR_MAT_ENV_MAP 3
+#define PSDR_MAT_ENV_SH9 4
 
 /* One object = one mesh (scene objects are meshes first, then the meshes of area
  * lights, as in Scene::Scene, src/scene.cc:8-20).  Textures live in one packed

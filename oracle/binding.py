@@ -69,7 +69,7 @@ class Scene:
         self.texels = f(flat.texels).astype(np.float32)
         self.emit = f(flat.light_emit).astype(np.float32).reshape(-1)
         objs = flat.objects
-        self.obj_mat = np.array([{0: 0, 1: 1, 2: 2}[o["material"]] for o in objs], dtype=np.int32)
+        self.obj_mat = np.array([{0: 0, 1: 1, 2: 2, 3: 3, 4: 4}[o["material"]] for o in objs], dtype=np.int32)
         self.obj_albedo = np.array([[o["albedo_offset"], o["albedo_h"], o["albedo_w"]] for o in objs],
                                    dtype=np.int32).reshape(-1)
         self.obj_rough = np.array([[o["rough_offset"], o["rough_h"], o["rough_w"]] for o in objs],
@@ -191,6 +191,15 @@ def point_rays(params, pts, d_params, dpts):
     do = np.zeros_like(pts)
     load().oracle_point_rays(C.byref(cam), _p(pts), _p(dpts), pts.shape[0], _p(o), _p(d), _p(do))
     return o, d, do
+
+
+def sh9_eval(coeff, dirs):
+    """SH9Light radiance (n, 3) of unit directions dirs (n, 3) for coeff (9, 3)."""
+    coeff = np.ascontiguousarray(coeff, np.float32).reshape(27)
+    dirs = np.ascontiguousarray(dirs, np.float32)
+    out = np.zeros_like(dirs)
+    load().oracle_sh9_eval(_p(coeff), _p(dirs), dirs.shape[0], _p(out))
+    return out
 
 
 def camera_tangents(params, d_params):

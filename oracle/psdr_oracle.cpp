@@ -222,6 +222,22 @@ struct CpuBvh {
 // ---------------------------------------------------------------------------
 // Intersection record (src/intersection.h, src/scene.cc:106-161)
 // ---------------------------------------------------------------------------
+// SH9Light::eval_impl_ (src/emitter.cc:251-279): sum over k of C_k Y_k(w), the
+// real spherical harmonics up to degree 2 with fp32 constants sqrt(1/(4 pi)),
+// sqrt(3/(4 pi)), sqrt(15/(4 pi)), sqrt(5/(16 pi)), sqrt(15/(16 pi)).
+template <class F> V3<F> sh9_radiance(const F* C, const V3<F>& w) {
+  const float c0 = std::sqrt(REAL_INV_4PI), c1 = std::sqrt(0.75f * REAL_INV_PI);
+  const float c2 = std::sqrt(3.75f * REAL_INV_PI), c3 = std::sqrt(0.3125f * REAL_INV_PI);
+  const float c4 = std::sqrt(0.9375f * REAL_INV_PI);
+  const F xx = w.x * w.x, yy = w.y * w.y, zz = w.z * w.z;
+  const F xy = w.x * w.y, yz = w.y * w.z, zx = w.z * w.x;
+  const F Y[9] = {F(c0),      F(c1) * w.y, F(c1) * w.z, F(c1) * w.x, F(c2) * xy, F(c2) * yz,
+                  F(c3) * ((F(2.0f) * zz - xx) - yy), F(c2) * zx, F(c4) * (xx - yy)};
+  V3<F> r = V3<F>{C[0], C[1], C[2]} * Y[0];
+  for (int k = 1; k < 9; ++k) r = r + V3<F>{C[3 * k], C[3 * k + 1], C[3 * k + 2]} * Y[k];
+  return r;
+}
+
 template <class F> struct Its {
   bool valid = false, is_light = false;
   V3<F> p, n, t, wi;
@@ -241,6 +257,7 @@ template <class F> struct Scene {
   Discrete light_sampler;  // over objects, weight = light power (0 for non-lights)
   Discrete edge_sampler;   // over all edges, weight = length
   float lights_total = 0.f;  // sum of light powers (src/scene.cc:59-60)
+  int n_edge_tris = 0;       // triangles whose edges are sampled (all but an environment cube)
   CpuBvh bvh;
 
   void build(const oracle_scene& s, bool use_tangent) {
@@ -298,12 +315,17 @@ template <class F> struct Scene {
       const V3<float> e = detach(L.emit);
       const float lum = (e.x * 0.2126f + e.y * 0.7152f) + e.z * 0.0722f;
       L.power = (REAL_2PI * lum) * L.total_area;
+      if (ob.mat == MAT_ENV_MAP || ob.mat == MAT_ENV_SH9) L.power = REAL_4PI;  // src/emitter.cc:129, 247
       obj_power[L.obj] = L.power;
     }
     light_sampler.build(obj_power.data(), s.n_objects);
     lights_total = chunked_sum(obj_power.data(), s.n_objects);
-    // edges of every mesh, light meshes included (src/scene.cc:80-101)
-    std::vector<EdgeTopo> topo = edge_topology(s.faces, s.n_tris);
+    // edges of every mesh, light meshes included, not the environment cube
+    // (src/scene.cc:30, 80-101)
+    n_edge_tris = s.n_tris;
+    if (s.n_objects > 0 && (objs.back().mat == MAT_ENV_MAP || objs.back().mat == MAT_ENV_SH9))
+      n_edge_tris = objs.back().tri_begin;
+    std::vector<EdgeTopo> topo = edge_topology(s.faces, n_edge_tris);
     edges.resize(topo.size());
     std::vector<float> lens(topo.size());
     for (size_t e = 0; e < topo.size(); ++e) {
@@ -314,6 +336,18 @@ template <class F> struct Scene {
   }
   static void set(float& x, float v, float) { x = v; }
   static void set(Dual& x, float v, float d) { x = Dual(v, d); }
+
+  // Light::eval of light l at its hit: AreaLight's two-sided constant
+  // (src/emitter.cc:16-23); EnvLight's cube map at the uv (src/emitter.cc:133-140
+  // through DiffuseLightMaterial::emit, src/diffuse_material.h:53-58);
+  // SH9Light's sum over k of C_k Y_k(wi) (src/emitter.cc:251-279).
+  V3<F> radiance(int l, const Its<F>& it) const {
+    const Light<F>& L = lights[l];
+    const Object& ob = objs[L.obj];
+    if (ob.mat == MAT_ENV_MAP) return tex3(ob.albedo, it.uv, texels);
+    if (ob.mat == MAT_ENV_SH9) return sh9_radiance<F>(&texels[ob.albedo.offset], it.wi);
+    return L.emit;
+  }
 
   // Texture lookup (src/texture.h:24-74): nearest texel, rows flipped by the
   // Python wrapper, so image row = H-1-v0.
@@ -467,7 +501,7 @@ V3<F> trace2light(const Scene<F>& sc, Pcg32& rng, const Its<F>& its0, bool valid
   for (uint32_t depth = 0; depth < D; ++depth) {
     {
       const bool hit_light = valid && its.is_light;
-      if (elv && depth == 0 && hit_light) L = L + T * sc.lights[its.light].emit;
+      if (elv && depth == 0 && hit_light) L = L + T * sc.radiance(its.light, its);
       valid = valid && !hit_light;
     }
     for (uint32_t i = 0; i < nL; ++i) {
@@ -485,7 +519,7 @@ V3<F> trace2light(const Scene<F>& sc, Pcg32& rng, const Its<F>& its0, bool valid
       const Its<F> itl = sc.hit(its.p, dir, valid);
       const bool is_hit = itl.valid && itl.is_light && squared_norm(detach(itl.p) - detach(lp)) < REAL_EPS;
       if (!is_hit) continue;
-      const V3<F>& emit = sc.lights[l].emit;
+      const V3<F> emit = sc.radiance(l, itl);
       const V3<F> bsdf = sc.eval(its, dir);
       const F tmp = dot(itl.n, -dir) / itl.d2;
       const F G = dot(its.n, dir) * tmp;
@@ -505,7 +539,7 @@ V3<F> trace2light(const Scene<F>& sc, Pcg32& rng, const Its<F>& its0, bool valid
       const Its<F> itl = sc.hit(its.p, smp.wo, valid);
       const bool is_hit = valid && itl.valid && itl.is_light;
       if (!is_hit) continue;
-      const V3<F>& emit = sc.lights[itl.light].emit;
+      const V3<F> emit = sc.radiance(itl.light, itl);
       const F tmp = dot(itl.n, -smp.wo) / itl.d2;
       const F G = dot(its.n, smp.wo) * tmp;
       const float pdf_bsdf = detach(F(smp.pdf) * tmp);
@@ -679,7 +713,7 @@ V3<F> eval_direct(const Scene<F>& sc, const Camera& cam, float sx, float sy, flo
   if (!valid) return V3<F>(F(0.f));
   const F pdf_xw = ((is0.d2 * pdf_ep) * F(pdf_light)) / dot(is0.n, dir);
   const V3<F> Tb = boundary_term<F>(is0.p, id0.p, ep, is0.n, id0.n, E, ad) / pdf_xw;
-  const V3<float> Ts = detach(sc.lights[is0.light].emit);
+  const V3<float> Ts = detach(sc.radiance(is0.light, is0));
   Its<F> id0c = id0;
   const V3<float> Td = detach(sc.eval(id0c, lift<F>(-cd)));
   V3<F> value = (Tb * lift<F>(Ts)) * lift<F>(Td);
@@ -998,6 +1032,14 @@ void oracle_point_rays(const oracle_camera* c, const float* pts, const float* dp
   }
 }
 
+// SH9 radiance of n unit directions for the 27 coefficients (9 RGB rows).
+void oracle_sh9_eval(const float* coeff, const float* dirs, int n, float* out) {
+  for (int i = 0; i < n; ++i) {
+    const V3<float> r = sh9_radiance<float>(coeff, V3<float>{dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]});
+    for (int a = 0; a < 3; ++a) out[3 * i + a] = r[a];
+  }
+}
+
 // Tangents of camera2world, world2camera and raster2camera along c->d_params.
 void oracle_camera_tangents(const oracle_camera* c, float* dC, float* dW, float* dR) {
   const Camera cam = make_camera(c);
@@ -1030,7 +1072,7 @@ int oracle_prepare(const oracle_scene* s, float* tri_out, int* edge_topo, float*
     }
     tri_out[7 * f + 6] = T.area;
   }
-  std::vector<EdgeTopo> topo = edge_topology(s->faces, s->n_tris);
+  std::vector<EdgeTopo> topo = edge_topology(s->faces, sc.n_edge_tris);
   const int ne = static_cast<int>(topo.size());
   for (int e = 0; e < ne && e < cap_edges; ++e) {
     const EdgeTopo& t = topo[e];

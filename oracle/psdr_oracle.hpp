@@ -92,6 +92,8 @@ constexpr float RAY_TMIN = 1e-3f;
 constexpr float REAL_PI = 3.14159274101257324f;
 constexpr float REAL_2PI = 6.28318548202514648f;
 constexpr float REAL_INV_PI = 0.318309873342514038f;
+constexpr float REAL_4PI = 12.5663709640502930f;     // src/types.h:120
+constexpr float REAL_INV_4PI = 0.0795774683356285f;  // src/types.h:123
 
 // ---------------------------------------------------------------------------
 // sin / cos: Cody-Waite reduction by pi/2 (three-part split) and Taylor
@@ -621,7 +623,9 @@ struct Hit {
 // ---------------------------------------------------------------------------
 // Scene data (templated on the parameter scalar)
 // ---------------------------------------------------------------------------
-enum MatType : int { MAT_DIFFUSE = 0, MAT_DIFFUSE_BSDF = 1, MAT_LIGHT = 2 };
+// MAT_ENV_MAP / MAT_ENV_SH9: an environment light's bounding cube (EnvLight /
+// SH9Light, src/emitter.cc:95-352), the last object.
+enum MatType : int { MAT_DIFFUSE = 0, MAT_DIFFUSE_BSDF = 1, MAT_LIGHT = 2, MAT_ENV_MAP = 3, MAT_ENV_SH9 = 4 };
 
 template <class F> struct Tri {
   V3<F> p0, p1, p2, n, t;

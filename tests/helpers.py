@@ -12,9 +12,11 @@ from torch_psdr_amd.flat import flatten
 
 
 def objects(device, level=1, tex_res=None, tex_seed=1, sphere_vertices=None, light_y=4.0, emit=10.0,
-            bsdf_rough=None, normal_res=None, requires_grad=False):
+            bsdf_rough=None, normal_res=None, requires_grad=False, env=None, env_res=6, area_light=True,
+            ground=True):
     """Sphere (+ optional UV texture, DiffuseBsdf roughness, normal map), ground quad, quad
-    light: (meshes, lights, leaves)."""
+    light, optional environment light (env = "map": a random env_res x 4 env_res/3 cube map,
+    "sh9": random SH coefficients): (meshes, lights, leaves)."""
     Vs, Fs = W.icosphere(level)
     if sphere_vertices is not None:
         Vs = sphere_vertices
@@ -49,16 +51,33 @@ def objects(device, level=1, tex_res=None, tex_seed=1, sphere_vertices=None, lig
             leaves["normal"] = normal
         mat = dr.DiffuseBsdfMaterial(color, rough, normal)
     sphere = dr.Mesh(vs, uv, fs, uvi, mat)
-    Vg, Fg = W.quad(10.0, -1.0)
-    gv = torch.tensor(Vg, device=device, requires_grad=requires_grad)
-    ground_col = torch.tensor([0.5, 0.5, 0.5], device=device, requires_grad=requires_grad)
-    ground = dr.Mesh(gv, None, torch.tensor(Fg, device=device), None, dr.DiffuseMaterial(ground_col))
-    Vl, Fl = W.quad(2.0, light_y, facing_down=True)
-    lv = torch.tensor(Vl, device=device, requires_grad=requires_grad)
-    le = torch.tensor([emit, emit, emit], device=device, requires_grad=requires_grad)
-    light = dr.AreaLight(lv, torch.tensor(Fl, device=device), le)
-    leaves.update(ground_v=gv, ground_color=ground_col, light_v=lv, emit=le)
-    return [sphere, ground], [light], leaves
+    meshes, lights = [sphere], []
+    if ground:
+        Vg, Fg = W.quad(10.0, -1.0)
+        gv = torch.tensor(Vg, device=device, requires_grad=requires_grad)
+        ground_col = torch.tensor([0.5, 0.5, 0.5], device=device, requires_grad=requires_grad)
+        meshes.append(dr.Mesh(gv, None, torch.tensor(Fg, device=device), None, dr.DiffuseMaterial(ground_col)))
+        leaves.update(ground_v=gv, ground_color=ground_col)
+    if area_light:
+        Vl, Fl = W.quad(2.0, light_y, facing_down=True)
+        lv = torch.tensor(Vl, device=device, requires_grad=requires_grad)
+        le = torch.tensor([emit, emit, emit], device=device, requires_grad=requires_grad)
+        lights.append(dr.AreaLight(lv, torch.tensor(Fl, device=device), le))
+        leaves.update(light_v=lv, emit=le)
+    if env is not None:
+        ge = torch.Generator().manual_seed(tex_seed + 200)
+        if env == "map":
+            cube = (0.5 + 1.5 * torch.rand((env_res, 4 * env_res // 3 + 1, 3), generator=ge)).to(device)
+            cube.requires_grad_(requires_grad)
+            leaves["env_map"] = cube
+            lights.append(dr.EnvLight(dr.Texture(cube)))
+        else:
+            coeff = torch.rand((9, 3), generator=ge) * 0.6 - 0.2
+            coeff[0] = 2.0 + torch.rand(3, generator=ge)
+            coeff = coeff.to(device).requires_grad_(requires_grad)
+            leaves["sh9"] = coeff
+            lights.append(dr.SH9Light(coeff))
+    return meshes, lights, leaves
 
 
 def cam_params(res, look_from=(0.0, 1.0, 5.0), vfov=math.pi / 4):
@@ -83,7 +102,10 @@ def ortho_camera(device, res, look_from=(0.0, 1.0, 5.0), view_width=4.5, require
 
 def cpu_flat(**kw):
     meshes, lights, leaves = objects(torch.device("cpu"), **kw)
-    return flatten(meshes, lights, torch.device("cpu")), leaves
+    area = [l for l in lights if isinstance(l, dr.AreaLight)]
+    env = [l for l in lights if not isinstance(l, dr.AreaLight)]
+    cams = [camera(torch.device("cpu"), 8)]
+    return flatten(meshes, area, torch.device("cpu"), env=env[-1] if env else None, cameras=cams), leaves
 
 
 def tri9(flat) -> np.ndarray:

@@ -163,6 +163,11 @@ CASES = [
     dict(name="npass3", kw=dict(level=1), res=16, cfg=dict(spp_interior=3, max_bounce=2, n_pass=3)),
     dict(name="ortho-rough", kw=dict(level=2, bsdf_rough=0.5), res=24, cfg=dict(spp_interior=2, max_bounce=3),
          ortho=True),
+    dict(name="env-map", kw=dict(level=2, env="map"), res=24, cfg=dict(spp_interior=2, max_bounce=3)),
+    dict(name="env-sh9-visible", kw=dict(level=2, env="sh9", bsdf_rough=0.5), res=24,
+         cfg=dict(spp_interior=2, max_bounce=3, enable_light_visable=True)),
+    dict(name="env-sh9-only", kw=dict(level=2, env="sh9", area_light=False), res=24,
+         cfg=dict(spp_interior=2, max_bounce=2)),
 ]
 
 

@@ -81,6 +81,9 @@ INTERIOR_CASES = [
     dict(name="elv", kw=dict(level=1), res=16, cfg=dict(spp_interior=2, max_bounce=2, enable_light_visable=True)),
     dict(name="normal-map", kw=dict(level=2, bsdf_rough=0.5, tex_res=8, normal_res=8), res=16,
          cfg=dict(spp_interior=2, max_bounce=2)),
+    dict(name="env-map", kw=dict(level=2, env="map"), res=16, cfg=dict(spp_interior=2, max_bounce=2)),
+    dict(name="env-sh9", kw=dict(level=2, env="sh9", bsdf_rough=0.5), res=16,
+         cfg=dict(spp_interior=2, max_bounce=2, enable_light_visable=True)),
 ]
 
 
@@ -94,10 +97,12 @@ def test_interior_vjp_matches_oracle_jvp(gpu_device, oracle, case):
         _check(oracle, scene, integ, grads, g, N.PSDR_TERM_INTERIOR, block, 100 + i, 2e-3)
 
 
-@pytest.mark.parametrize("term", [N.PSDR_TERM_PRIMARY, N.PSDR_TERM_DIRECT], ids=["primary", "direct"])
-def test_boundary_vjp_matches_oracle_jvp(gpu_device, oracle, term):
-    scene, integ = _setup(gpu_device, dict(level=2), 32, dict(spp_interior=1, max_bounce=2, spp_primary_edge=2,
-                                                              spp_secondary_edge=2))
+@pytest.mark.parametrize("term,env", [(N.PSDR_TERM_PRIMARY, None), (N.PSDR_TERM_DIRECT, None),
+                                      (N.PSDR_TERM_PRIMARY, "sh9"), (N.PSDR_TERM_DIRECT, "map")],
+                         ids=["primary", "direct", "primary-env-sh9", "direct-env-map"])
+def test_boundary_vjp_matches_oracle_jvp(gpu_device, oracle, term, env):
+    scene, integ = _setup(gpu_device, dict(level=2, env=env), 32,
+                          dict(spp_interior=1, max_bounce=2, spp_primary_edge=2, spp_secondary_edge=2))
     g = torch.randn((32, 32, 3), generator=torch.Generator().manual_seed(11)).to(gpu_device)
     grads = _gpu_vjp(scene, integ, g, term)
     assert np.isfinite(grads[0]).all()
@@ -110,6 +115,7 @@ def test_boundary_vjp_matches_oracle_jvp(gpu_device, oracle, term):
 # orthographic origin also moves with the edge point (vertex block).
 CAMERA_CASES = [
     dict(name="interior-persp", term=N.PSDR_TERM_INTERIOR, ortho=False),
+    dict(name="interior-env-sh9-visible", term=N.PSDR_TERM_INTERIOR, ortho=False, env="sh9"),
     dict(name="interior-ortho", term=N.PSDR_TERM_INTERIOR, ortho=True),
     dict(name="primary-persp", term=N.PSDR_TERM_PRIMARY, ortho=False),
     dict(name="primary-ortho", term=N.PSDR_TERM_PRIMARY, ortho=True),
@@ -120,8 +126,10 @@ CAMERA_CASES = [
 @pytest.mark.parametrize("case", CAMERA_CASES, ids=[c["name"] for c in CAMERA_CASES])
 def test_camera_vjp_matches_oracle_jvp(gpu_device, oracle, case):
     term = case["term"]
-    scene, integ = _setup(gpu_device, dict(level=2, bsdf_rough=0.5), 24,
-                          dict(spp_interior=2, max_bounce=2, spp_primary_edge=2, spp_secondary_edge=2),
+    env = case.get("env")
+    scene, integ = _setup(gpu_device, dict(level=2, bsdf_rough=0.5, env=env), 24,
+                          dict(spp_interior=2, max_bounce=2, spp_primary_edge=2, spp_secondary_edge=2,
+                               enable_light_visable=env is not None),
                           ortho=case["ortho"])
     g = torch.randn((24, 24, 3), generator=torch.Generator().manual_seed(13)).to(gpu_device)
     grads = _gpu_vjp(scene, integ, g, term)

@@ -19,6 +19,8 @@ PSDR_ERR_NOMEM = -4
 PSDR_MAT_DIFFUSE = 0
 PSDR_MAT_DIFFUSE_BSDF = 1
 PSDR_MAT_LIGHT = 2
+PSDR_MAT_ENV_MAP = 3
+PSDR_MAT_ENV_SH9 = 4
 
 PSDR_GRAD_VERTICES = 1
 PSDR_GRAD_ALBEDO = 2
@@ -34,7 +36,7 @@ PSDR_CAMERA_PERSPECTIVE = 0
 PSDR_CAMERA_ORTHOGRAPHIC = 1
 PSDR_CAMERA_GRAD = 10
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 
 class PsdrObject(C.Structure):

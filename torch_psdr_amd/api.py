@@ -10,11 +10,11 @@ torch autograd.
 Differences, each deliberate:
   * 2-D (H, W) textures are accepted as 1-channel maps (the reference's `==`
     typo at src/python/torch_psdr.py:233 rejects them);
-  * EnvLight and SH9Light raise NotImplementedError;
   * the stray `from turtle import forward` (src/python/torch_psdr.py:2) is gone.
 """
 from __future__ import annotations
 
+import warnings
 from typing import List, Union
 
 import torch
@@ -234,13 +234,27 @@ class AreaLight(Light):
 
 
 class EnvLight(Light):
+    """Cube-map environment light (src/python/torch_psdr.py:417-430; src/emitter.cc:95-213): the
+    radiance of a bounding cube around the cameras and meshes, looked up in `cube_map` (a UV3f
+    texture laid out as the cube's unfolded faces) at the hit's uv."""
+
     def __init__(self, cube_map: Texture):
-        raise NotImplementedError("EnvLight is not implemented by torch_psdr_amd")
+        super().__init__()
+        assert cube_map.tex_type == "UV3f"
+        self.cube_map = cube_map
+        self.merge_variables(*cube_map.get_variables())
 
 
 class SH9Light(Light):
+    """Second-order spherical-harmonics environment light (src/python/torch_psdr.py:433-447;
+    src/emitter.cc:215-352): radiance sum_k C_k Y_k(w) of the direction w from the bounding cube
+    back to the shaded point, with the (9, 3) RGB coefficients C."""
+
     def __init__(self, sh9_coeff: Tensor):
-        raise NotImplementedError("SH9Light is not implemented by torch_psdr_amd")
+        super().__init__()
+        assert sh9_coeff.size() == (9, 3)
+        self.sh9_coeff = sh9_coeff
+        self.append_variable(sh9_coeff)
 
 
 class Scene(VarContex):
@@ -258,18 +272,24 @@ class Scene(VarContex):
             self.merge_variables(*m.get_variables())
         for l in lights:
             self.merge_variables(*l.get_variables())
-        for l in lights:
-            if not isinstance(l, AreaLight):
-                raise NotImplementedError("only AreaLight is implemented by torch_psdr_amd")
-        if not meshes and not lights:
-            raise ValueError("Scene needs at least one mesh or light")
-        device = (meshes[0].vertices if meshes else lights[0].vertices).device
+        area = [l for l in lights if isinstance(l, AreaLight)]
+        env = [l for l in lights if isinstance(l, (EnvLight, SH9Light))]
+        if len(area) + len(env) != len(lights):
+            raise TypeError("lights must be AreaLight, EnvLight or SH9Light")
+        if len(env) > 1:
+            # the reference reports this and keeps the last one (src/scene.cc:22-27)
+            warnings.warn("only support no more than one env_light; using the last one")
+        if not meshes and not area:
+            raise ValueError("Scene needs at least one mesh or area light")
+        if env and not cameras:
+            raise ValueError("an environment light needs a camera (its cube is placed around the cameras)")
+        device = (meshes[0].vertices if meshes else area[0].vertices).device
         if device.type != "cuda":
             raise RuntimeError("torch_psdr_amd renders on the GPU: scene tensors must be on a HIP device")
         self.cameras = cameras
         self.meshes = meshes
         self.lights = lights
-        self.flat = flatten(meshes, lights, device)
+        self.flat = flatten(meshes, area, device, env=env[-1] if env else None, cameras=cameras)
         self.token = ops.register(self.flat)
 
     def __del__(self):

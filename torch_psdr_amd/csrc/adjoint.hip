@@ -238,17 +238,72 @@ __device__ void shading_normal_vjp(const AdjParams& P, const Its& it, f3 gN, Ver
     adj_cache().add(P.grad_t, 1, o, f3{dot(gM, it.t), dot(gM, B), dot(gM, it.n)} * 2.0f);
 }
 
-__device__ __forceinline__ void emit_vjp(const AdjParams& P, float* hot, int l, f3 g) {
-  const int obj = P.sv.lights[l].obj;
-  if (!(P.sv.objs[obj].grad_mask & PSDR_GRAD_EMIT)) return;
-  acc_hot(hot, P.hot_n, P.grad_e, 3 * l, 3 * l, g.x);
-  acc_hot(hot, P.hot_n, P.grad_e, 3 * l + 1, 3 * l + 1, g.y);
-  acc_hot(hot, P.hot_n, P.grad_e, 3 * l + 2, 3 * l + 2, g.z);
+// Environment part of radiance_vjp (compiled only into the replay kernel for
+// scenes with an environment light, whose register budget it would otherwise
+// tighten).
+__device__ __forceinline__ f3 env_radiance_vjp(const AdjParams& P, float* hot, int obj, const Its& it, f3 g) {
+  const SceneView& sv = P.sv;
+  const ObjectK& ob = sv.objs[obj];
+  if (ob.mat == PSDR_MAT_ENV_MAP) {
+    if (ob.grad_mask & PSDR_GRAD_ALBEDO) {
+      const int co = tex3_offset(ob.alb_off, ob.alb_h, ob.alb_w, it.uv);
+      if (ob.alb_h == 0) {
+        acc_hot(hot, P.hot_n, P.grad_t, 3 * sv.n_lights + 4 * obj + 0, co + 0, g.x);
+        acc_hot(hot, P.hot_n, P.grad_t, 3 * sv.n_lights + 4 * obj + 1, co + 1, g.y);
+        acc_hot(hot, P.hot_n, P.grad_t, 3 * sv.n_lights + 4 * obj + 2, co + 2, g.z);
+      } else {
+        adj_cache().add(P.grad_t, 1, co, g);
+      }
+    }
+    return splat(0.f);
+  }
+  if (ob.mat == PSDR_MAT_ENV_SH9) {
+    const Sh9Const c = sh9_const();
+    const f3 w = it.wi;
+    const float* C = sv.texels + ob.alb_off;
+    float sk[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) sk[k] = dot(g, f3{C[3 * k], C[3 * k + 1], C[3 * k + 2]});
+    if (ob.grad_mask & PSDR_GRAD_ALBEDO) {
+      const Sh9Basis b = sh9_basis(w);
+      for (int k = 0; k < 9; ++k) {
+        const int o = ob.alb_off + 3 * k;
+        atomicAdd(P.grad_t + o, g.x * b.y[k]);
+        atomicAdd(P.grad_t + o + 1, g.y * b.y[k]);
+        atomicAdd(P.grad_t + o + 2, g.z * b.y[k]);
+      }
+    }
+    // d Y_k / d w of Y = (c0, c1 y, c1 z, c1 x, c2 xy, c2 yz, c3 (2zz - xx - yy), c2 zx, c4 (xx - yy))
+    const float gx = ((sk[3] * c.c1 + sk[4] * c.c2 * w.y) + (sk[7] * c.c2 * w.z - sk[6] * c.c3 * 2.0f * w.x)) +
+                     sk[8] * c.c4 * 2.0f * w.x;
+    const float gy = ((sk[1] * c.c1 + sk[4] * c.c2 * w.x) + (sk[5] * c.c2 * w.z - sk[6] * c.c3 * 2.0f * w.y)) -
+                     sk[8] * c.c4 * 2.0f * w.y;
+    const float gz = (sk[2] * c.c1 + sk[5] * c.c2 * w.y) + (sk[6] * c.c3 * 4.0f * w.z + sk[7] * c.c2 * w.x);
+    return f3{gx, gy, gz};
+  }
+  return splat(0.f);
 }
 
-__device__ __forceinline__ f3 light_emit(const SceneView& sv, int l) {
-  return f3{sv.light_emit[3 * l], sv.light_emit[3 * l + 1], sv.light_emit[3 * l + 2]};
+// VJP of light_radiance(sv, l, it) for the adjoint g into the area light's
+// emission, the cube map's texel or the SH9 coefficients; returns the adjoint
+// of it.wi (nonzero for SH9 only).
+template <bool ENV>
+__device__ __forceinline__ f3 radiance_vjp(const AdjParams& P, float* hot, int l, const Its& it, f3 g) {
+  const SceneView& sv = P.sv;
+  const int obj = sv.lights[l].obj;
+  const ObjectK& ob = sv.objs[obj];
+  if constexpr (ENV) {
+    if (ob.mat == PSDR_MAT_ENV_MAP || ob.mat == PSDR_MAT_ENV_SH9) return env_radiance_vjp(P, hot, obj, it, g);
+  }
+  if (ob.grad_mask & PSDR_GRAD_EMIT) {
+    acc_hot(hot, P.hot_n, P.grad_e, 3 * l, 3 * l, g.x);
+    acc_hot(hot, P.hot_n, P.grad_e, 3 * l + 1, 3 * l + 1, g.y);
+    acc_hot(hot, P.hot_n, P.grad_e, 3 * l + 2, 3 * l + 2, g.z);
+  }
+  return splat(0.f);
 }
+
+__device__ __forceinline__ bool nonzero3(f3 a) { return a.x != 0.f || a.y != 0.f || a.z != 0.f; }
 
 struct DepthOut {
   f3 c;       // sum of weighted NEE + BSDF-MIS values at this depth
@@ -271,7 +326,9 @@ __device__ __forceinline__ void draw_cached(const AdjParams& P, int64_t slot, in
 
 // One depth of trace2light at vertex `its` (state: rng at the depth's first
 // draw).  With grad, adjoints adj_c / adj_t are pushed through this depth.
-__device__ DepthOut depth_step(const AdjParams& P, float* hot, const Its& its, Rng rng, int d, int64_t lane,
+template <bool ENV>
+__device__ __forceinline__ DepthOut depth_step(const AdjParams& P, float* hot, const Its& its, Rng rng, int d,
+                                               int64_t lane,
                                bool grad, f3 adj_c, f3 adj_t) {
   const SceneView& sv = P.sv;
   DepthOut out;
@@ -303,7 +360,7 @@ __device__ DepthOut depth_step(const AdjParams& P, float* hot, const Its& its, R
     draw_cached(P, cslot + k, lane, &ht, &hu, &hv);
     const Its itl = reconstruct(sv, ht, hu, hv, its.p, true);
     if (!(itl.valid && itl.is_light && sqnorm(itl.p - lp.p) < REAL_EPS)) continue;
-    const f3 emit = light_emit(sv, l);
+    const f3 emit = light_radiance(sv, l, itl);
     const f3 bsdf = mat_eval(sv, its, dir);
     const float tmp = dot(itl.n, -dir) / itl.d2;
     const float nd = dot(its.n, dir);
@@ -315,7 +372,7 @@ __device__ DepthOut depth_step(const AdjParams& P, float* hot, const Its& its, R
     out.c = out.c + value * weight;
     if (!grad) continue;
     const f3 av = adj_c * (weight / pdf_light);
-    emit_vjp(P, hot, l, av * bsdf * (G * Jl));
+    const f3 gwl = radiance_vjp<ENV>(P, hot, l, itl, av * bsdf * (G * Jl));
     const f3 gbsdf = av * emit * (G * Jl);
     const float gG = dot(av, emit * bsdf) * Jl;
     const float gJ = dot(av, emit * bsdf) * G;
@@ -333,6 +390,11 @@ __device__ DepthOut depth_step(const AdjParams& P, float* hot, const Its& its, R
     lh.p = gpl;
     out.cur.p = out.cur.p - gpl;
     lh.area = gJ / itl.area;
+    if (nonzero3(gwl)) {  // the light hit's wi = -normalize(itl.p - its.p)
+      const f3 gx = vjp_normalize(itl.p - its.p, -gwl);
+      lh.p = lh.p + gx;
+      out.cur.p = out.cur.p - gx;
+    }
     const f3 gdv = vjp_normalize(dvec, gdir);
     out.cur.p = out.cur.p - gdv;
     VertexAdj ls = zero_adj();
@@ -351,7 +413,7 @@ __device__ DepthOut depth_step(const AdjParams& P, float* hot, const Its& its, R
     const Its itl = reconstruct(sv, ht, hu, hv, its.p, true);
     if (!(itl.valid && itl.is_light)) continue;
     const int l = itl.light;
-    const f3 emit = light_emit(sv, l);
+    const f3 emit = light_radiance(sv, l, itl);
     const float tmp = dot(itl.n, -smp.wo) / itl.d2;
     const float nd = dot(its.n, smp.wo);
     const float G = nd * tmp;
@@ -366,7 +428,7 @@ __device__ DepthOut depth_step(const AdjParams& P, float* hot, const Its& its, R
     out.c = out.c + value * weight;
     if (!grad) continue;
     const f3 av = adj_c * (weight / pdf_bsdf);
-    emit_vjp(P, hot, l, av * smp.bsdf * (G * Jl));
+    const f3 gwl = radiance_vjp<ENV>(P, hot, l, itl, av * smp.bsdf * (G * Jl));
     const f3 gbsdf = av * emit * (G * Jl);
     const float gG = dot(av, emit * smp.bsdf) * Jl;
     const float gJ = dot(av, emit * smp.bsdf) * G;
@@ -384,6 +446,11 @@ __device__ DepthOut depth_step(const AdjParams& P, float* hot, const Its& its, R
     lh.p = gpl;
     out.cur.p = out.cur.p - gpl;
     lh.area = gJ / itl.area;
+    if (nonzero3(gwl)) {  // the light hit's wi = -normalize(itl.p - its.p)
+      const f3 gx = vjp_normalize(itl.p - its.p, -gwl);
+      lh.p = lh.p + gx;
+      out.cur.p = out.cur.p - gx;
+    }
     // wo = normalize(u T + v B + cos N), B = cross(Ng, T), N the shading normal
     const f3 B = cross(its.n, its.t);
     const f3 wu = (smp.u * its.t + smp.v * B) + smp.cos_t * shading_normal(sv, sv.objs[its.obj], its);
@@ -454,7 +521,7 @@ __device__ DepthOut depth_step(const AdjParams& P, float* hot, const Its& its, R
   return out;
 }
 
-template <int DMAX>
+template <int DMAX, bool ENV>
 __global__ __launch_bounds__(ADJ_THREADS) void k_adjoint(AdjParams P) {
   __shared__ float hot[HOT_MAX];
   __shared__ float cacc[CAM_ACC];
@@ -510,8 +577,13 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adjoint(AdjParams P) {
     }
     const f3 L = f3{P.cL[lane], P.cL[P.N + lane], P.cL[2 * P.N + lane]};
     const Its& cur = its;
-    // enable_light_visable: L += T0 * emit at a light camera vertex
-    if (light0) emit_vjp(P, hot, cur.light, g);
+    // enable_light_visable: L += T0 * emit at a light camera vertex; an SH9
+    // light's radiance there depends on the camera origin through wi
+    f3 gO0 = splat(0.f);
+    if (light0) {
+      const f3 gw0 = radiance_vjp<ENV>(P, hot, cur.light, cur, g);
+      if (nonzero3(gw0)) gO0 = -vjp_normalize(cur.p - O0, -gw0);
+    }
     // reverse sweep.  pend = adjoint of vertex d+1 gathered so far (its own
     // depth's terms and, through its wi, vertex d+2's); pendO = adjoint of
     // vertex d's point from vertex d+1's wi.
@@ -523,7 +595,7 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adjoint(AdjParams P) {
       if (d > 0) O = reconstruct(sv, vt[d - 1], vu[d - 1], vv[d - 1], O0, true).p;
       const Its vx = reconstruct(sv, vt[d], vu[d], vv[d], O, true);
       Rng r{rs[d], rng.inc};
-      const DepthOut o = depth_step(P, hot, vx, r, d, lane, true, g * Ts[d], A * Ts[d]);
+      const DepthOut o = depth_step<ENV>(P, hot, vx, r, d, lane, true, g * Ts[d], A * Ts[d]);
       if (o.ext) {
         VertexAdj nx = pend;
         nx.p = nx.p + o.next.p;
@@ -543,8 +615,8 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adjoint(AdjParams P) {
     pend.area = pend.area + dot(g, L) / its.area;
     tri_vjp(P, vt[0], vu[0], vv[0], flip_of(sv, its), pend);
     // the camera ray origin, through the camera vertex's wi
-    if (P.cam_acc && (pendO.x != 0.f || pendO.y != 0.f || pendO.z != 0.f))
-      (void)cam_origin_vjp(P.cam, pendO, film, P.cam.r2c, CAM_R, cacc);
+    pendO = pendO + gO0;
+    if (P.cam_acc && nonzero3(pendO)) (void)cam_origin_vjp(P.cam, pendO, film, P.cam.r2c, CAM_R, cacc);
   }
   __syncthreads();
   adj_cache().flush(P.grad_v, P.grad_t);
@@ -709,9 +781,17 @@ int interior_backward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& 
   const int hot = 3 * s->n_lights + 4 * s->n_objects;
   P.hot_n = hot <= HOT_MAX ? hot : 0;
   const int grid = std::max(1, std::min(blocks(P.N, ADJ_THREADS), 256 * 16));
-  if (P.D <= 4) hipLaunchKernelGGL(k_adjoint<4>, dim3(grid), dim3(ADJ_THREADS), 0, st, P);
-  else if (P.D <= 8) hipLaunchKernelGGL(k_adjoint<8>, dim3(grid), dim3(ADJ_THREADS), 0, st, P);
-  else hipLaunchKernelGGL(k_adjoint<64>, dim3(grid), dim3(ADJ_THREADS), 0, st, P);
+  const bool env = s->n_bvh < s->n_tris;  // an environment light's cube
+  if (P.D <= 4) {
+    if (env) hipLaunchKernelGGL((k_adjoint<4, true>), dim3(grid), dim3(ADJ_THREADS), 0, st, P);
+    else hipLaunchKernelGGL((k_adjoint<4, false>), dim3(grid), dim3(ADJ_THREADS), 0, st, P);
+  } else if (P.D <= 8) {
+    if (env) hipLaunchKernelGGL((k_adjoint<8, true>), dim3(grid), dim3(ADJ_THREADS), 0, st, P);
+    else hipLaunchKernelGGL((k_adjoint<8, false>), dim3(grid), dim3(ADJ_THREADS), 0, st, P);
+  } else {
+    if (env) hipLaunchKernelGGL((k_adjoint<64, true>), dim3(grid), dim3(ADJ_THREADS), 0, st, P);
+    else hipLaunchKernelGGL((k_adjoint<64, false>), dim3(grid), dim3(ADJ_THREADS), 0, st, P);
+  }
   PSDR_HIP_CHECK(hipGetLastError());
   return PSDR_OK;
 }

@@ -314,12 +314,12 @@ __device__ DirectOut eval_direct(const SceneView& sv, const CameraK& cam, float 
   bool valid = (dot(E.nl, dir) * dot(E.nr, dir) < 0.f) && norm(E.nl - E.nr) > REAL_EPS;
   if (!valid) return r;
   const float tl = norm(ep - lp.p);
-  HitRec h = closest_hit<BSTACK>(sv.node, sv.leaf, sv.n_tris, ep, -dir, stk, spill, spill_stride, vis_tmax(tl),
-                                 vis_tocc(tl));
+  HitRec h = closest_hit<BSTACK>(sv.node, sv.leaf, sv.n_bvh, sv.n_tris, ep, -dir, stk, spill, spill_stride,
+                                 vis_tmax(tl), vis_tocc(tl));
   const Its s0 = reconstruct(sv, h.tri, h.u, h.v, ep, true);
   valid = s0.valid && s0.is_light && sqnorm(s0.p - lp.p) < REAL_EPS;
   if (!valid) return r;
-  h = closest_hit<BSTACK>(sv.node, sv.leaf, sv.n_tris, ep, dir, stk, spill, spill_stride);
+  h = closest_hit<BSTACK>(sv.node, sv.leaf, sv.n_bvh, sv.n_tris, ep, dir, stk, spill, spill_stride);
   const Its d0 = reconstruct(sv, h.tri, h.u, h.v, ep, true);
   valid = d0.valid && !d0.is_light;
   if (!valid) return r;
@@ -327,8 +327,8 @@ __device__ DirectOut eval_direct(const SceneView& sv, const CameraK& cam, float 
   f3 co, cd;
   if (!camera_point_ray(cam, d0.p, &px, &py, &co, &cd)) return r;
   const float tc = norm(d0.p - co);
-  h = closest_hit<BSTACK>(sv.node, sv.leaf, sv.n_tris, co, cd, stk, spill, spill_stride, vis_tmax(tc),
-                          vis_tocc(tc));
+  h = closest_hit<BSTACK>(sv.node, sv.leaf, sv.n_bvh, sv.n_tris, co, cd, stk, spill, spill_stride,
+                          vis_tmax(tc), vis_tocc(tc));
   const Its i2 = reconstruct(sv, h.tri, h.u, h.v, co, true);
   if (!(i2.valid && !i2.is_light && sqnorm(i2.p - d0.p) < REAL_EPS)) return r;
   const float pdf_xw = ((s0.d2 * pdf_ep) * pdf_light) / dot(s0.n, dir);
@@ -349,7 +349,7 @@ __device__ DirectOut eval_direct(const SceneView& sv, const CameraK& cam, float 
   const float Jb = ((len_sd * len_sb) * sin_b) / (sin_d * cos_s);
   const float dG = (cos_s * dot(nd, -db)) / (len_sd * len_sd);
   const int ls = s0.light;
-  const f3 Ts = f3{sv.light_emit[3 * ls], sv.light_emit[3 * ls + 1], sv.light_emit[3 * ls + 2]};
+  const f3 Ts = light_radiance(sv, ls, s0);
   const f3 Td = mat_eval(sv, d0, -cd);
   r.pix = py * cam.W + px;
   if (PRE) {

@@ -242,11 +242,18 @@ static int check_desc(const psdr_scene_desc* d) {
       set_error(PSDR_ERR_ARG, "object " + std::to_string(o) + ": triangle range out of bounds");
       return PSDR_ERR_ARG;
     }
-    if (ob.light >= d->n_lights || ob.material < 0 || ob.material > PSDR_MAT_LIGHT) {
+    if (ob.light >= d->n_lights || ob.material < 0 || ob.material > PSDR_MAT_ENV_SH9) {
       set_error(PSDR_ERR_ARG, "object " + std::to_string(o) + ": bad light index or material");
       return PSDR_ERR_ARG;
     }
-    const int64_t alb = ob.albedo_h == 0 ? 3 : 3ll * ob.albedo_h * ob.albedo_w;
+    const bool env = ob.material == PSDR_MAT_ENV_MAP || ob.material == PSDR_MAT_ENV_SH9;
+    if (env && (o != d->n_objects - 1 || ob.light < 0 || ob.tri_count != 12 ||
+                ob.tri_begin + ob.tri_count != d->n_tris)) {
+      set_error(PSDR_ERR_ARG, "object " + std::to_string(o) +
+                                  ": an environment light's cube must be the last object, a light, 12 triangles");
+      return PSDR_ERR_ARG;
+    }
+    const int64_t alb = ob.material == PSDR_MAT_ENV_SH9 ? 27 : (ob.albedo_h == 0 ? 3 : 3ll * ob.albedo_h * ob.albedo_w);
     if (ob.material != PSDR_MAT_LIGHT && (ob.albedo_offset < 0 || ob.albedo_offset + alb > d->n_texels)) {
       set_error(PSDR_ERR_ARG, "object " + std::to_string(o) + ": albedo texture out of bounds");
       return PSDR_ERR_ARG;
@@ -287,16 +294,21 @@ RenderSlot* psdr_scene::find_slot(const psdr_config& cfg, const psdr_camera& cam
 }
 
 RenderSlot* psdr_scene::claim_slot(const psdr_config& cfg, const psdr_camera& cam) {
+  // the slot last used for (cfg, cam), valid or not (its buffers already have
+  // the right sizes: a training loop's scene update must not move renders to a
+  // new slot), else one never used, else the least recently used
   RenderSlot* best = nullptr;
-  for (RenderSlot& r : slots) {
-    if (r.have_cache && std::memcmp(&r.cfg, &cfg, sizeof(cfg)) == 0 && std::memcmp(&r.cam, &cam, sizeof(cam)) == 0) {
+  for (RenderSlot& r : slots)
+    if (r.used && std::memcmp(&r.cfg, &cfg, sizeof(cfg)) == 0 && std::memcmp(&r.cam, &cam, sizeof(cam)) == 0) {
       best = &r;
       break;
     }
-    if (!best || (!r.have_cache && best->have_cache) || (r.have_cache == best->have_cache && r.used < best->used))
-      best = &r;
-  }
+  if (!best)
+    for (RenderSlot& r : slots)
+      if (!best || r.used < best->used) best = &r;
   best->used = ++clock;
+  best->cfg = cfg;
+  best->cam = cam;
   return best;
 }
 
@@ -316,6 +328,7 @@ SceneView psdr_scene::view() const {
   v.leaf = leaf.as<float4>();
   v.node = node.as<float4>();
   v.n_tris = n_tris;
+  v.n_bvh = n_bvh;
   v.objs = objs.as<ObjectK>();
   v.n_objects = n_objects;
   v.lights = lights.as<LightK>();
@@ -420,7 +433,13 @@ static int scene_create_impl(const psdr_scene_desc* d, psdr_scene* s, hipStream_
         set_error(PSDR_ERR_ARG, "face " + std::to_string(f) + " references a vertex out of range");
         return PSDR_ERR_ARG;
       }
-  std::vector<int> topo = edge_topology(hfaces, nt);
+  // an environment light's cube (the last object) has no edges and stays out of the LBVH
+  s->n_bvh = nt;
+  if (d->n_objects > 0) {
+    const psdr_object& last = d->objects[d->n_objects - 1];
+    if (last.material == PSDR_MAT_ENV_MAP || last.material == PSDR_MAT_ENV_SH9) s->n_bvh = last.tri_begin;
+  }
+  std::vector<int> topo = edge_topology(hfaces, s->n_bvh);
   s->n_edges = static_cast<int>(topo.size() / 6);
   // objects / lights
   s->objs_host.resize(d->n_objects);
@@ -548,7 +567,7 @@ int psdr_scene_info(const psdr_scene* s, int32_t* n_tris, int32_t* n_edges, int3
   if (n_edges) *n_edges = s->n_edges;
   if (bvh_depth) {
     // depth = longest parent chain (host walk; diagnostic only)
-    const int n = s->n_tris;
+    const int n = s->n_bvh;
     int depth = 0;
     if (n >= 2) {
       std::vector<int> parent(2 * n - 1);
@@ -743,7 +762,7 @@ int psdr_scene_bvh(const psdr_scene* s, uint32_t* morton, int32_t* order, int32_
     set_error(PSDR_ERR_ARG, "scene is NULL");
     return PSDR_ERR_ARG;
   }
-  const int n = s->n_tris;
+  const int n = s->n_bvh;
   if (n < 2) return PSDR_OK;
   hipStream_t st = as_stream(stream);
   const hipMemcpyKind k = hipMemcpyDeviceToDevice;

@@ -17,6 +17,8 @@ constexpr float REAL_EPS = 1e-5f;                    // src/types.h:114
 constexpr float RAY_TMIN = 1e-3f;                    // src/cuda/device_programs.cu:8
 constexpr float REAL_PI = 3.14159274101257324f;      // src/types.h:118
 constexpr float REAL_2PI = 6.28318548202514648f;     // src/types.h:119
+constexpr float REAL_4PI = 12.5663709640502930f;     // src/types.h:120
+constexpr float REAL_INV_4PI = 0.0795774683356285f;  // src/types.h:123
 constexpr float REAL_INV_PI = 0.318309873342514038f; // src/types.h:121
 constexpr float F_INF = __builtin_huge_valf();
 

@@ -92,7 +92,7 @@ __global__ __launch_bounds__(TRACE_THREADS) void k_trace(SceneView sv, const flo
       tocc = vis_tocc(t);
     }
     const HitRec h =
-        closest_hit<STACK_LDS>(sv.node, sv.leaf, sv.n_tris, o, d, stack, spill + gtid, stride, tmax, tocc);
+        closest_hit<STACK_LDS>(sv.node, sv.leaf, sv.n_bvh, sv.n_tris, o, d, stack, spill + gtid, stride, tmax, tocc);
     otri[i] = h.tri;
     if (ot) ot[i] = h.tri >= 0 ? h.t : -1.f;
     ou[i] = h.u;
@@ -220,7 +220,7 @@ __global__ __launch_bounds__(256) void k_shade(ShadeParams P, QueueView qi, cons
           const Its itl = reconstruct(sv, ht, hu, hv, ip.p, true);
           const bool is_hit = itl.valid && itl.is_light && sqnorm(itl.p - lp.p) < REAL_EPS;
           if (!is_hit) continue;
-          const f3 emit = f3{sv.light_emit[3 * l], sv.light_emit[3 * l + 1], sv.light_emit[3 * l + 2]};
+          const f3 emit = light_radiance(sv, l, itl);
           const f3 bsdf = mat_eval(sv, ip, dir);
           const float tmp = dot(itl.n, -dir) / itl.d2;
           const float G = dot(ip.n, dir) * tmp;
@@ -247,7 +247,7 @@ __global__ __launch_bounds__(256) void k_shade(ShadeParams P, QueueView qi, cons
           const Its itl = reconstruct(sv, ht, hu, hv, ip.p, true);
           if (!(itl.valid && itl.is_light)) continue;
           const int l = itl.light;
-          const f3 emit = f3{sv.light_emit[3 * l], sv.light_emit[3 * l + 1], sv.light_emit[3 * l + 2]};
+          const f3 emit = light_radiance(sv, l, itl);
           const float tmp = dot(itl.n, -smp.wo) / itl.d2;
           const float G = dot(ip.n, smp.wo) * tmp;
           const float pdf_bsdf = smp.pdf * tmp;
@@ -296,7 +296,7 @@ __global__ __launch_bounds__(256) void k_shade(ShadeParams P, QueueView qi, cons
         if (its.is_light) {
           if (P.elv && d == 0) {
             const int l = its.light;
-            L = L + T * f3{sv.light_emit[3 * l], sv.light_emit[3 * l + 1], sv.light_emit[3 * l + 2]};
+            L = L + T * light_radiance(sv, l, its);
             light0 = true;
           }
           alive = false;

@@ -85,6 +85,7 @@ struct SceneView {
   const float4* leaf;
   const float4* node;
   int n_tris;
+  int n_bvh;  // triangles in the LBVH: [0, n_bvh); the rest (an environment cube) are tested by every ray
   const ObjectK* objs;
   int n_objects;
   const LightK* lights;
@@ -133,6 +134,7 @@ constexpr int RENDER_SLOTS = 4;
 
 struct psdr_scene {
   int n_vertices = 0, n_tris = 0, n_objects = 0, n_lights = 0, n_texels = 0, n_edges = 0;
+  int n_bvh = 0;  // triangles with edges and in the LBVH: all but an environment light's cube
   std::vector<psdr_object> objects;
   std::vector<psdr::ObjectK> objs_host;  // source of the last object-table upload
   std::vector<int> light_obj;

@@ -247,7 +247,8 @@ __global__ void k_light_areas(const float4* __restrict__ geom, const LightK* __r
   }
 }
 
-// AreaLight power (src/emitter.cc:10-11) and the per-object light weights.
+// AreaLight power (src/emitter.cc:10-11), the environment lights' fixed 4*pi
+// (src/emitter.cc:129, 247) and the per-object light weights.
 __global__ void k_light_power(const ObjectK* __restrict__ objs, int n_objects, const float* __restrict__ emit,
                               const float* __restrict__ light_area, float* __restrict__ light_power,
                               float* __restrict__ obj_w) {
@@ -256,8 +257,12 @@ __global__ void k_light_power(const ObjectK* __restrict__ objs, int n_objects, c
   const int l = objs[o].light;
   float w = 0.f;
   if (l >= 0) {
-    const float lum = (emit[3 * l] * 0.2126f + emit[3 * l + 1] * 0.7152f) + emit[3 * l + 2] * 0.0722f;
-    w = (REAL_2PI * lum) * light_area[l];
+    if (objs[o].mat == PSDR_MAT_ENV_MAP || objs[o].mat == PSDR_MAT_ENV_SH9) {
+      w = REAL_4PI;
+    } else {
+      const float lum = (emit[3 * l] * 0.2126f + emit[3 * l + 1] * 0.7152f) + emit[3 * l + 2] * 0.0722f;
+      w = (REAL_2PI * lum) * light_area[l];
+    }
     light_power[l] = w;
   }
   obj_w[o] = w;
@@ -480,7 +485,8 @@ __global__ void k_karras(const uint32_t* __restrict__ ms, int n, int* __restrict
 }
 
 // Leaf records in sorted order, the sorted order itself and each leaf's padded box.
-__global__ void k_leaf(const float4* __restrict__ geom, const uint32_t* __restrict__ sorted, int n,
+// Triangle ids are first + f for records geom[f].
+__global__ void k_leaf(const float4* __restrict__ geom, const uint32_t* __restrict__ sorted, int n, int first,
                        float4* __restrict__ leaf, uint32_t* __restrict__ order, float* __restrict__ lbox) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
@@ -497,7 +503,7 @@ __global__ void k_leaf(const float4* __restrict__ geom, const uint32_t* __restri
   const float4 a = geom[5 * f], b = geom[5 * f + 1], c = geom[5 * f + 2];
   const f3 p0 = xyz(a), p1 = xyz(b), p2 = xyz(c);
   const f3 e1 = p1 - p0, e2 = p2 - p0;
-  leaf[3 * k] = make_float4(p0.x, p0.y, p0.z, __int_as_float(f));
+  leaf[3 * k] = make_float4(p0.x, p0.y, p0.z, __int_as_float(first + f));
   leaf[3 * k + 1] = make_float4(e1.x, e1.y, e1.z, 0.f);
   leaf[3 * k + 2] = make_float4(e2.x, e2.y, e2.z, 0.f);
 }
@@ -751,10 +757,16 @@ int radix_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, uin
 }
 
 int build_lbvh(psdr_scene* s, hipStream_t st) {
-  const int n = s->n_tris;
+  const int n = s->n_bvh;
+  if (s->n_tris > n) {
+    // leaves tested by every ray (an environment cube), after the tree's own
+    hipLaunchKernelGGL(k_leaf, dim3(1), dim3(64), 0, st, s->tri_geom.as<float4>() + 5ll * n, nullptr, s->n_tris - n,
+                       n, s->leaf.as<float4>() + 3ll * n, nullptr, nullptr);
+    PSDR_HIP_CHECK(hipGetLastError());
+  }
   if (n < 2) {
     if (n == 1) {
-      hipLaunchKernelGGL(k_leaf, dim3(1), dim3(64), 0, st, s->tri_geom.as<float4>(), nullptr, 1,
+      hipLaunchKernelGGL(k_leaf, dim3(1), dim3(64), 0, st, s->tri_geom.as<float4>(), nullptr, 1, 0,
                          s->leaf.as<float4>(), nullptr, nullptr);
       PSDR_HIP_CHECK(hipGetLastError());
     }
@@ -780,7 +792,7 @@ int build_lbvh(psdr_scene* s, hipStream_t st) {
   int2* range = s->ranges.as<int2>();
   hipLaunchKernelGGL(k_karras, dim3(blocks(n - 1, 256)), dim3(256), 0, st, k0, n, s->child.as<int>(),
                      s->parent.as<int>(), range);
-  hipLaunchKernelGGL(k_leaf, dim3(blocks(n, 256)), dim3(256), 0, st, s->tri_geom.as<float4>(), v0, n,
+  hipLaunchKernelGGL(k_leaf, dim3(blocks(n, 256)), dim3(256), 0, st, s->tri_geom.as<float4>(), v0, n, 0,
                      s->leaf.as<float4>(), s->order.as<uint32_t>(), lbox);
   const int chunks = (n + 63) / 64;
   hipLaunchKernelGGL(k_chunk_boxes, dim3(blocks(64ll * chunks, 256)), dim3(256), 0, st, lbox, n, cbox);

@@ -72,6 +72,55 @@ __device__ __forceinline__ int tex1_offset(int off, int h, int w, f2 uv) {
 }
 __device__ __forceinline__ f3 tex3(const float* tex, int o) { return f3{tex[o], tex[o + 1], tex[o + 2]}; }
 
+// Second-order real spherical harmonics Y_0..Y_8 of a unit direction
+// (SH9Light::eval_impl_, src/emitter.cc:251-268): the constants are the fp32
+// square roots of 1/(4 pi), 3/(4 pi), 15/(4 pi), 5/(16 pi), 15/(16 pi).
+struct Sh9Const {
+  float c0, c1, c2, c3, c4;
+};
+__device__ __forceinline__ Sh9Const sh9_const() {
+  return Sh9Const{sqrtf(REAL_INV_4PI), sqrtf(0.75f * REAL_INV_PI), sqrtf(3.75f * REAL_INV_PI),
+                  sqrtf(0.3125f * REAL_INV_PI), sqrtf(0.9375f * REAL_INV_PI)};
+}
+struct Sh9Basis {
+  float y[9];
+};
+__device__ __forceinline__ Sh9Basis sh9_basis(f3 w) {
+  const Sh9Const c = sh9_const();
+  const float xx = w.x * w.x, yy = w.y * w.y, zz = w.z * w.z;
+  const float xy = w.x * w.y, yz = w.y * w.z, zx = w.z * w.x;
+  Sh9Basis b;
+  b.y[0] = c.c0;
+  b.y[1] = c.c1 * w.y;
+  b.y[2] = c.c1 * w.z;
+  b.y[3] = c.c1 * w.x;
+  b.y[4] = c.c2 * xy;
+  b.y[5] = c.c2 * yz;
+  b.y[6] = c.c3 * ((2.0f * zz - xx) - yy);
+  b.y[7] = c.c2 * zx;
+  b.y[8] = c.c4 * (xx - yy);
+  return b;
+}
+
+// Radiance of light l at its hit `it` (Light::eval): an AreaLight's constant
+// emission, two-sided (src/emitter.cc:16-23); an EnvLight's cube map at the
+// hit's uv (DiffuseLightMaterial::emit, src/diffuse_material.h:53-58,
+// src/emitter.cc:133-140); an SH9Light's sum_k C_k Y_k(wi) over the 9 RGB
+// coefficient rows (src/emitter.cc:269-279), summed in k order.
+__device__ __forceinline__ f3 light_radiance(const SceneView& sv, int l, const Its& it) {
+  const ObjectK& ob = sv.objs[sv.lights[l].obj];
+  if (ob.mat == PSDR_MAT_ENV_MAP) return tex3(sv.texels, tex3_offset(ob.alb_off, ob.alb_h, ob.alb_w, it.uv));
+  if (ob.mat == PSDR_MAT_ENV_SH9) {
+    const Sh9Basis b = sh9_basis(it.wi);
+    const float* c = sv.texels + ob.alb_off;
+    f3 r = f3{c[0], c[1], c[2]} * b.y[0];
+#pragma unroll
+    for (int k = 1; k < 9; ++k) r = r + f3{c[3 * k], c[3 * k + 1], c[3 * k + 2]} * b.y[k];
+    return r;
+  }
+  return f3{sv.light_emit[3 * l], sv.light_emit[3 * l + 1], sv.light_emit[3 * l + 2]};
+}
+
 // DiffuseBsdf shading normal (src/diffuse_bsdf.cc:13-19, 61-68): the
 // geometric normal, or with a normal map normalize(TBN (2 n - 1)) where TBN has
 // the columns T, B = Ng x T, Ng (the matrix-vector product summed in that order).

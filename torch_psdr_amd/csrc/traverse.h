@@ -104,20 +104,30 @@ __device__ __forceinline__ void leaf_test(const float4* __restrict__ leaf, int k
 
 // stack_lds: this block's LDS stack base ([STACK_LDS][blockDim]); spill: this
 // thread's global overflow area with stride `spill_stride` between entries.
-template <int STACK_LDS>
 // tmax / tocc: for a visibility query (is the closest hit the target point,
 // within the reference's squared-distance tolerance REAL_EPS?) hits beyond
 // tmax cannot matter, and a hit before tocc already decides "not visible", so
 // traversal stops there and returns that hit instead of the closest one.
 // Defaults (inf, -inf) give the plain closest hit.
+// Leaves [n_tris, n_all) (an environment light's bounding cube, whose
+// triangles span the whole scene and would only widen every node box) are not
+// in the tree: every ray tests them after the traversal.
+__device__ __forceinline__ void extra_leaves(const float4* __restrict__ leaf, int n_tris, int n_all, f3 o, f3 d,
+                                             float tocc, HitRec& h) {
+  if (h.t < tocc && h.tri >= 0) return;
+  for (int k = n_tris; k < n_all; ++k) leaf_test(leaf, k, o, d, h);
+}
+
+template <int STACK_LDS>
 __device__ __forceinline__ HitRec closest_hit(const float4* __restrict__ node, const float4* __restrict__ leaf,
-                                              int n_tris, f3 o, f3 d, int* stack_lds, int* spill,
+                                              int n_tris, int n_all, f3 o, f3 d, int* stack_lds, int* spill,
                                               int spill_stride, float tmax = F_INF, float tocc = -F_INF) {
   HitRec h{-1, F_INF, -1.f, -1.f};
-  if (!(d.x == d.x) || n_tris <= 0) return h;
+  if (!(d.x == d.x) || n_all <= 0) return h;
   h.t = tmax;
-  if (n_tris == 1) {
-    leaf_test(leaf, 0, o, d, h);
+  if (n_tris <= 1) {
+    if (n_tris == 1) leaf_test(leaf, 0, o, d, h);
+    extra_leaves(leaf, n_tris, n_all, o, d, tocc, h);
     if (h.tri < 0) h.t = F_INF;
     return h;
   }
@@ -163,6 +173,7 @@ __device__ __forceinline__ HitRec closest_hit(const float4* __restrict__ node, c
       ni = (sp < STACK_LDS) ? stack_lds[sp * nthr + lane] : spill[(sp - STACK_LDS) * spill_stride];
     }
   }
+  extra_leaves(leaf, n_tris, n_all, o, d, tocc, h);
   if (h.tri < 0) h.t = F_INF;
   return h;
 }

@@ -18,6 +18,46 @@ from . import _native as N
 # default corner uvs when a mesh has none (src/mesh.cc:31-36)
 _DEFAULT_UV = (0.0, 0.0, 0.0, 1.0, 1.0, 0.0)
 
+# The environment lights' bounding cube (EnvLight::set_aabb / SH9Light::set_aabb,
+# src/emitter.cc:98-130, 218-247): corner k has x from bit 2, y from bit 1, z
+# from bit 0 of k (0 = min, 1 = max); the faces and the cube map's uv layout
+# (a 4 x 3 grid of unit squares, u step 1/4, v step 1/3) are the reference's,
+# since the cube map texture is addressed through them.
+_CUBE_FACES = ((0, 1, 2), (3, 2, 1), (4, 5, 6), (7, 6, 5), (0, 4, 1), (5, 1, 4),
+               (2, 6, 3), (7, 3, 6), (4, 0, 6), (2, 6, 0), (5, 1, 7), (3, 7, 1))
+_CUBE_UV_IDX = ((2, 3, 7), (8, 7, 3), (5, 4, 10), (9, 10, 4), (0, 1, 3), (4, 3, 1),
+                (12, 13, 8), (9, 8, 13), (5, 6, 10), (11, 10, 6), (4, 3, 9), (8, 9, 3))
+
+
+def _cube_uvs():
+    u, v = 0.25, 1.0 / 3.0
+    f32 = lambda x: float(torch.tensor(x, dtype=torch.float32))
+    u, v = f32(u), f32(v)
+    us = (u, 2 * u, 0, u, 2 * u, 3 * u, 4 * u, 0, u, 2 * u, 3 * u, 4 * u, u, 2 * u)
+    vs = (0, 0, v, v, v, v, v, 2 * v, 2 * v, 2 * v, 2 * v, 2 * v, 1, 1)
+    return [(f32(a), f32(b)) for a, b in zip(us, vs)]
+
+
+def _env_cube_vertices(cameras, meshes, device) -> torch.Tensor:
+    """Scene::Scene's cube (src/scene.cc:31-50): the box of the camera positions
+    and the meshes' vertex boxes, grown to its centre +- 1000 x its largest
+    extent; all fp32 elementwise ops, detached."""
+    with torch.no_grad():
+        pos = [c.look_from.detach().to(device=device, dtype=torch.float32).reshape(3) for c in cameras]
+        pmin, pmax = pos[0], pos[0]
+        for p in pos[1:]:
+            pmin, pmax = torch.minimum(pmin, p), torch.maximum(pmax, p)
+        for m in meshes:
+            v = m.vertices.detach().to(torch.float32)
+            pmin = torch.minimum(pmin, v.min(0).values)
+            pmax = torch.maximum(pmax, v.max(0).values)
+        d = torch.tensor(1000.0, dtype=torch.float32, device=device) * (pmax - pmin).max()
+        c = (pmin + pmax) * 0.5
+        lo, hi = c - d, c + d
+        corners = [torch.stack([(hi if (k >> 2) & 1 else lo)[0], (hi if (k >> 1) & 1 else lo)[1],
+                                (hi if k & 1 else lo)[2]]) for k in range(8)]
+        return torch.stack(corners)
+
 
 @dataclass
 class FlatScene:
@@ -73,8 +113,9 @@ def _tex_entry(tex, pieces: list, offset: int):
     return offset, tex.height, tex.width, offset + flat.numel()
 
 
-def flatten(meshes, lights, device) -> FlatScene:
-    """meshes: list of api.Mesh; lights: list of api.AreaLight."""
+def flatten(meshes, lights, device, env=None, cameras=()) -> FlatScene:
+    """meshes: list of api.Mesh; lights: list of api.AreaLight; env: an
+    api.EnvLight / api.SH9Light (its bounding cube becomes the last object)."""
     objs = [(m, None) for m in meshes] + [(l.mesh_proxy(), l) for l in lights]
     verts, faces, uvs, tobj = [], [], [], []
     objects, light_obj = [], []
@@ -135,6 +176,36 @@ def flatten(meshes, lights, device) -> FlatScene:
                     None if m.uvs is None else (m.uvs.data_ptr(), m.uvs._version)))
         voff += nv
         toff += nf
+    if env is not None:
+        oi = len(objs)
+        verts.append(_env_cube_vertices(cameras, meshes, device))
+        faces.append(torch.tensor(_CUBE_FACES, dtype=torch.int32, device=device) + voff)
+        rec = dict(light=len(light_obj), rough_offset=0, rough_h=0, rough_w=0, tri_begin=toff, tri_count=12,
+                   grad_mask=0, normal_offset=-1, normal_h=0, normal_w=0)
+        if hasattr(env, "cube_map"):
+            uvt = torch.tensor(_cube_uvs(), dtype=torch.float32, device=device)
+            uvs.append(uvt[torch.tensor(_CUBE_UV_IDX, device=device).long()].reshape(12, 6))
+            a_off, a_h, a_w, tex_off = _tex_entry(env.cube_map, tex_pieces, tex_off)
+            rec.update(material=N.PSDR_MAT_ENV_MAP, albedo_offset=a_off, albedo_h=a_h, albedo_w=a_w)
+            if env.cube_map.data.requires_grad:
+                rec["grad_mask"] |= N.PSDR_GRAD_ALBEDO
+            key.append(("env_map",))
+        else:
+            uvs.append(torch.tensor(_DEFAULT_UV, dtype=torch.float32, device=device).expand(12, 6))
+            coeff = env.sh9_coeff.to(device=device, dtype=torch.float32).reshape(27)
+            tex_pieces.append(coeff)
+            rec.update(material=N.PSDR_MAT_ENV_SH9, albedo_offset=tex_off, albedo_h=0, albedo_w=0)
+            tex_off += 27
+            if env.sh9_coeff.requires_grad:
+                rec["grad_mask"] |= N.PSDR_GRAD_ALBEDO
+            key.append(("env_sh9",))
+        tobj.append(torch.full((12,), oi, dtype=torch.int32, device=device))
+        light_obj.append(oi)
+        emits.append(torch.zeros(3, dtype=torch.float32, device=device))
+        objects.append(rec)
+        vranges.append((voff, voff + 8))
+        voff += 8
+        toff += 12
     texels = torch.cat(tex_pieces) if tex_pieces else torch.zeros(1, dtype=torch.float32, device=device)
     emit = torch.stack(emits) if emits else torch.zeros((0, 3), dtype=torch.float32, device=device)
     return FlatScene(
