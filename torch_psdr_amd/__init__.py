@@ -7,7 +7,8 @@
 from .api import (AreaLight, BaseCamera, DiffuseBsdfMaterial, DiffuseMaterial, EnvLight, Light, Material, Mesh,
                   OrthographicCamera, PathIntegrator, PerspectiveCamera, PrincipledMaterial, Scene, SH9Light,
                   Texture, VarContex)
+from .io import load_obj, read_mtl, read_obj
 
 __all__ = ["AreaLight", "BaseCamera", "DiffuseBsdfMaterial", "DiffuseMaterial", "EnvLight", "Light", "Material",
            "Mesh", "OrthographicCamera", "PathIntegrator", "PerspectiveCamera", "PrincipledMaterial", "Scene",
-           "SH9Light", "Texture", "VarContex"]
+           "SH9Light", "Texture", "VarContex", "load_obj", "read_mtl", "read_obj"]
