@@ -193,6 +193,15 @@ def point_rays(params, pts, d_params, dpts):
     return o, d, do
 
 
+def render_lanes(scene: Scene, cfg, cam_params, lanes):
+    """Interior radiance (n, 3) of the listed lanes of a renderC."""
+    lanes = np.ascontiguousarray(lanes, np.int64)
+    out = np.zeros((lanes.size, 3), np.float32)
+    c, k = config(cfg), camera(cam_params)
+    load().oracle_render_lanes(C.byref(scene.s), C.byref(c), C.byref(k), _p(lanes), C.c_int64(lanes.size), _p(out))
+    return out
+
+
 def sh9_eval(coeff, dirs):
     """SH9Light radiance (n, 3) of unit directions dirs (n, 3) for coeff (9, 3)."""
     coeff = np.ascontiguousarray(coeff, np.float32).reshape(27)

@@ -787,6 +787,26 @@ void film_add(Film<F>& film, const std::vector<V3<F>>& value, const std::vector<
   }
 }
 
+// One lane of render_interior (src/path_integrator.cc:359-368): camera ray
+// from the lane's first two draws, hit, trace2light, times the jacobian.
+template <class F>
+V3<F> lane_radiance(const Scene<F>& sc, const Camera& cam, const oracle_config& cfg, int64_t lane, int* pix,
+                    bool* valid) {
+  Pcg32 rng = lane_rng(static_cast<uint32_t>(lane));
+  const float u0 = rng.next_float();
+  const float u1 = rng.next_float();
+  V3<float> o, d;
+  V3<Dual> o_ad;
+  int x, y;
+  cam.ray(static_cast<uint32_t>(lane), cfg.spp_interior, u0, u1, &o, &d, &x, &y, &o_ad);
+  const Its<F> its = sc.hit(origin_as<F>(o, o_ad), lift<F>(d), true);
+  V3<F> L = trace2light<F>(sc, rng, its, its.valid, cfg.mis_light_samples, cfg.mis_bsdf_samples, cfg.max_bounce,
+                           cfg.enable_light_visable != 0);
+  *pix = y * cam.W + x;
+  *valid = its.valid;
+  return L * its.jac;
+}
+
 // render_interior (src/path_integrator.cc:355-372)
 template <class F>
 void render_interior(const Scene<F>& sc, const Camera& cam, const oracle_config& cfg, Film<F>& film,
@@ -798,25 +818,17 @@ void render_interior(const Scene<F>& sc, const Camera& cam, const oracle_config&
   std::vector<char> ok(N);
   parallel_for(N, [&](int64_t b, int64_t e) {
     for (int64_t lane = b; lane < e; ++lane) {
-      Pcg32 rng = lane_rng(static_cast<uint32_t>(lane));
-      const float u0 = rng.next_float();
-      const float u1 = rng.next_float();
-      V3<float> o, d;
-      V3<Dual> o_ad;
-      int x, y;
-      cam.ray(static_cast<uint32_t>(lane), spp, u0, u1, &o, &d, &x, &y, &o_ad);
-      const Its<F> its = sc.hit(origin_as<F>(o, o_ad), lift<F>(d), true);
-      V3<F> L = trace2light<F>(sc, rng, its, its.valid, cfg.mis_light_samples, cfg.mis_bsdf_samples,
-                               cfg.max_bounce, cfg.enable_light_visable != 0);
-      L = L * its.jac;
+      int p;
+      bool valid;
+      const V3<F> L = lane_radiance<F>(sc, cam, cfg, lane, &p, &valid);
       vals[lane] = L;
-      pix[lane] = y * cam.W + x;
-      ok[lane] = its.valid;
+      pix[lane] = p;
+      ok[lane] = valid;
       if (lane_L) {
         const V3<float> v = detach(L);
-        lane_L[3 * lane] = its.valid ? v.x : 0.f;
-        lane_L[3 * lane + 1] = its.valid ? v.y : 0.f;
-        lane_L[3 * lane + 2] = its.valid ? v.z : 0.f;
+        lane_L[3 * lane] = valid ? v.x : 0.f;
+        lane_L[3 * lane + 1] = valid ? v.y : 0.f;
+        lane_L[3 * lane + 2] = valid ? v.z : 0.f;
       }
     }
   });
@@ -1030,6 +1042,24 @@ void oracle_point_rays(const oracle_camera* c, const float* pts, const float* dp
       dorig[3 * i + a] = oad[a].d;
     }
   }
+}
+
+// Interior radiance of the n lanes listed in `lanes` (0 where the camera ray
+// misses), for parity checks of renders too large for a full oracle image.
+int oracle_render_lanes(const oracle_scene* s, const oracle_config* cfg, const oracle_camera* c,
+                        const int64_t* lanes, int64_t n, float* out) {
+  const Camera cam = make_camera(c);
+  Scene<float> scv;
+  scv.build(*s, false);
+  parallel_for(n, [&](int64_t b, int64_t e) {
+    for (int64_t i = b; i < e; ++i) {
+      int p;
+      bool valid;
+      const V3<float> L = lane_radiance<float>(scv, cam, *cfg, lanes[i], &p, &valid);
+      for (int a = 0; a < 3; ++a) out[3 * i + a] = valid ? L[a] : 0.f;
+    }
+  });
+  return 0;
 }
 
 // SH9 radiance of n unit directions for the 27 coefficients (9 RGB rows).

@@ -8,6 +8,10 @@
   C2  icosphere level 5 (20480 tris), vertices radially perturbed by N(0, 0.02)
       (seed 3), 512x512 texture, ground + light as C0; 512x512, 16 spp, depth 3,
       spp_primary_edge = spp_secondary_edge = 4 (1M boundary samples)
+  C4  1024 x 1024 height-grid terrain (2 * 1023^2 tris), heights from 6-octave
+      fBm value noise (amplitude 1, seed 5), 1024^2 texture (seed 1), two quad
+      area lights; 1024x1024, 64 spp, depth 4; gradients w.r.t. the heights and
+      texels
 Geometry is generated with numpy (float64, then rounded to fp32 once).
 """
 from __future__ import annotations
@@ -159,3 +163,71 @@ def c2_parts(dr, device, res=512, spp=16, spp_edge=4, level=5, tex_res=512):
     return dict(V0=torch.tensor(V0, device=device), Vp=torch.tensor(Vp, device=device),
                 tex_opt=texture(1, tex_res, device), tex_target=texture(2, tex_res, device), scene=scene,
                 integrator=integ, res=res, spp=spp, spp_edge=spp_edge, n_tris=int(F.shape[0]) + 4)
+
+
+def fbm_value_noise(n: int, octaves: int = 6, amplitude: float = 1.0, seed: int = 5) -> np.ndarray:
+    """(n, n) fBm of value noise: octave k samples a (2^(k+2) + 1)^2 lattice of
+    uniform [-1, 1] values (seed) with smoothstep-weighted bilinear interpolation,
+    weight amplitude / 2^k; fp32."""
+    rng = np.random.default_rng(seed)
+    u = np.linspace(0.0, 1.0, n)
+    out = np.zeros((n, n))
+    for k in range(octaves):
+        cells = 2 ** (k + 2)
+        lat = rng.uniform(-1.0, 1.0, (cells + 1, cells + 1))
+        x = u * cells
+        i = np.minimum(x.astype(np.int64), cells - 1)
+        f = x - i
+        w = f * f * (3.0 - 2.0 * f)
+        a = lat[i][:, i] * (1 - w)[None, :] + lat[i][:, i + 1] * w[None, :]
+        b = lat[i + 1][:, i] * (1 - w)[None, :] + lat[i + 1][:, i + 1] * w[None, :]
+        out += (amplitude / 2 ** k) * (a * (1 - w)[:, None] + b * w[:, None])
+    return out.astype(np.float32)
+
+
+def terrain_grid(n: int, size: float = 8.0):
+    """Base vertices (x, 0, z) of an n x n grid over [-size/2, size/2]^2, faces
+    (2 per cell) and per-vertex uvs."""
+    t = np.linspace(-size / 2, size / 2, n)
+    X, Z = np.meshgrid(t, t, indexing="xy")
+    V = np.stack([X.ravel(), np.zeros(n * n), Z.ravel()], 1).astype(np.float32)
+    idx = np.arange(n * n, dtype=np.int64).reshape(n, n)
+    a, b, c, d = idx[:-1, :-1].ravel(), idx[:-1, 1:].ravel(), idx[1:, :-1].ravel(), idx[1:, 1:].ravel()
+    F = np.concatenate([np.stack([a, c, b], 1), np.stack([b, c, d], 1)]).astype(np.int32)
+    g = np.linspace(0.0, 1.0, n)
+    G0, G1 = np.meshgrid(g, g, indexing="xy")
+    uv = np.stack([G0.ravel(), G1.ravel()], 1).astype(np.float32)
+    return V, F, uv
+
+
+def c4_parts(dr, device, n=1024, res=1024, spp=64, depth=4, tex_res=1024, spp_edge=1):
+    """C4: fBm terrain of n x n heights with a texture, lit by two quad lights;
+    `scene(heights, tex)` builds the Scene with vertices (x, height, z)."""
+    V, F, uv = terrain_grid(n)
+    heights = torch.tensor(fbm_value_noise(n), device=device).reshape(-1)
+    base = torch.tensor(V, device=device)
+    faces = torch.tensor(F, device=device)
+    uvs = torch.tensor(uv, device=device)
+    lights = []
+    for x, e in ((-2.0, 12.0), (2.5, 8.0)):
+        Vl, Fl = quad(1.5, 4.0, facing_down=True)
+        Vl = Vl + np.float32([x, 0.0, 0.0])
+        lights.append((torch.tensor(Vl, device=device), torch.tensor(Fl, device=device),
+                       torch.tensor([e, e, e], device=device)))
+    cam = dr.PerspectiveCamera(look_from=torch.tensor([0.0, 6.0, 7.0], device=device),
+                               look_at=torch.tensor([0.0, 0.0, 0.0], device=device),
+                               up=torch.tensor([0.0, 1.0, 0.0], device=device),
+                               vfov=torch.tensor([math.pi / 4.0], device=device), height=res, width=res)
+    integ = dr.PathIntegrator(spp_interior=spp, max_bounce=depth, spp_primary_edge=spp_edge,
+                              spp_secondary_edge=spp_edge)
+
+    def objects(h, tex):
+        verts = torch.stack([base[:, 0], h, base[:, 2]], 1)
+        mesh = dr.Mesh(verts, uvs, faces, faces, dr.DiffuseMaterial(tex))
+        return [cam], [mesh], [dr.AreaLight(v, f, e) for v, f, e in lights]
+
+    def scene(h, tex):
+        return dr.Scene(*objects(h, tex))
+
+    return dict(heights=heights, tex=texture(1, tex_res, device), scene=scene, objects=objects, integrator=integ,
+                res=res, spp=spp, n_tris=int(F.shape[0]) + 4)
