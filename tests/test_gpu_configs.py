@@ -21,8 +21,8 @@ pytestmark = pytest.mark.gpu
 
 
 def test_c1_texture_step_matches_oracle_jvp(gpu_device, oracle):
-    scene, integ, (_, tex) = W.build_scene(dr, gpu_device, level=3, res=256, tex=W.texture(1, 256, gpu_device),
-                                           spp=8, max_bounce=2)
+    tex = W.texture(1, 256, gpu_device).requires_grad_(True)
+    scene, integ, _ = W.build_scene(dr, gpu_device, level=3, res=256, tex=tex, spp=8, max_bounce=2)
     ns, flat = ops.entry(scene.token)
     cam = scene.cameras[0].params()
     g = torch.randn((256, 256, 3), generator=torch.Generator().manual_seed(21)).to(gpu_device)
@@ -34,7 +34,11 @@ def test_c1_texture_step_matches_oracle_jvp(gpu_device, oracle):
     N.check(ns.lib.psdr_render_backward(ns.handle, ctypes.byref(c), ctypes.byref(k), g.data_ptr(), gv.data_ptr(),
                                         gt.data_ptr(), ge.data_ptr(), None, N.PSDR_TERM_ALL, N.current_stream()))
     torch.cuda.synchronize()
-    w = np.random.default_rng(22).normal(size=flat.texels.numel()).astype(np.float32)
+    # tangent on the sphere's texture only (the ground's constant albedo takes no gradient)
+    ob = flat.objects[0]
+    w = np.zeros(flat.texels.numel(), np.float32)
+    n_tex = 3 * ob["albedo_h"] * ob["albedo_w"]
+    w[ob["albedo_offset"]:ob["albedo_offset"] + n_tex] = np.random.default_rng(22).normal(size=n_tex)
     _, d_int, d_pri, d_dir = oracle.render(oracle.Scene(flat, d_texels=w), integ.cfg, cam, mode=1)
     lhs = float((g.cpu().numpy().astype(np.float64) * (d_int + d_pri + d_dir)).sum())
     rhs = float((gt.cpu().numpy().astype(np.float64) * w).sum())
