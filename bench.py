@@ -10,6 +10,12 @@ second texture.  A step = Scene rebuild (triangle records, edges, samplers,
 LBVH) + renderD forward + loss + backward (interior path replay, primary and
 direct boundary terms) w.r.t. vertices and texels; synthetic data.
 
+The step runs eagerly (warm-up, the timed steps with the traversal kernel's
+launches timed for the roofline) and is then captured into a HIP graph
+(torch_psdr_amd.graph.CapturedStep) whose replays are timed the same way; by
+default the replay time is the reported step time, the eager one is given
+beside it.
+
 Prints one JSON line.  value = Mrays/s of reference-equivalent rays:
 R*N (N = spp*H*W, R = 1 + D*(L+B+1) = 10) + 20 rays per primary-boundary sample
 + 3 per direct-boundary sample, divided by the step time.
@@ -41,6 +47,9 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=3,
                     help="steps of the reference-shaped baseline path to time (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU-baseline sample time")
+    ap.add_argument("--graph", choices=("value", "report", "off"), default="value",
+                    help="the step captured into a HIP graph and replayed: reported as the value (the eager "
+                         "step's time beside it), reported beside the eager value, or not run")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device-to-device copy (scales rocprof FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -79,7 +88,9 @@ def main():
         img = integ.renderD(scene)[0]
         loss = ((img - target) ** 2).mean()
         loss.backward()
-        return loss
+        # the caller keeps only the value: a live autograd graph would tie the
+        # parameters' gradient accumulators to this step's stream
+        return loss.detach()
 
     for _ in range(args.warmup):
         step()
@@ -109,6 +120,12 @@ def main():
     mrays = rays / (ms * 1e-3) / 1e6
 
     roof = trace_roofline(tr_ms.value, tr_launch.value, tr_bytes.value, default_workload=(res, spp) == (512, 16))
+    graph = None
+    if args.graph != "off":
+        graph = graph_step_timing(torch, step, args.steps, rays)
+        graph["eager_ms_per_step"] = round(ms, 3)
+        if args.graph == "value":
+            ms, mrays = graph["ms_per_step"], graph["value"]
     plain = None
     if args.plain_steps > 0:
         plain = plain_baseline(torch, ops, step, verts, tex, args.plain_steps, rays)
@@ -138,8 +155,33 @@ def main():
                           "algorithmic_bytes": step_bytes},
         "cpu_baseline": cpu,
         "plain_gpu_path": plain,
+        "graph_step": graph,
     }
     print(json.dumps(out))
+
+
+def graph_step_timing(torch, step, n_steps, rays):
+    """The same step (Scene rebuild with its LBVH, renderD, loss, backward)
+    captured once into a HIP graph and replayed n_steps times."""
+    from torch_psdr_amd.graph import CapturedStep
+    print("graph step: warm-up and capture", file=sys.stderr, flush=True)
+    cs = CapturedStep(step, warmup=1)
+    print("graph step: first replay", file=sys.stderr, flush=True)
+    cs.replay()
+    torch.cuda.synchronize()
+    print("graph step: timed replays", file=sys.stderr, flush=True)
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    wall0 = time.perf_counter()
+    t0.record()
+    for _ in range(n_steps):
+        cs.replay()
+    t1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    ms = max(t0.elapsed_time(t1) / n_steps, wall * 1e3 / n_steps)
+    return {"ms_per_step": round(ms, 3), "value": round(rays / (ms * 1e-3) / 1e6, 3), "unit": "Mrays/s",
+            "steps": n_steps}
 
 
 def plain_baseline(torch, ops, step, verts, tex, n_steps, rays):

@@ -11,6 +11,22 @@
  * unless stated.  Every call returns 0 on success or a negative code; the text
  * of the last error on the calling thread is returned by psdr_last_error().
  * No C++ exception crosses this boundary.
+ *
+ * Graph capture (no reference counterpart; the reference synchronises after
+ * every ray cast, src/scene_optix.cc:275): psdr_scene_update,
+ * psdr_render_forward and psdr_render_backward may be recorded into a HIP graph
+ * by stream capture of `stream` (hipStreamBeginCapture, torch.cuda.graph) once
+ * the same calls, with the same sizes, have run eagerly on the scene: they then
+ * allocate nothing and never synchronise the host, and the LBVH rebuild of an
+ * update is part of the graph.  The boundary terms of a renderD run on two
+ * auxiliary streams forked from and joined back into the captured stream by the
+ * forward / backward pair, so a capture must hold both halves.  A call that
+ * would have to grow a scene buffer during capture fails with
+ * PSDR_ERR_CAPTURE; once any capture has happened in the process, a buffer that
+ * grows keeps its old allocation until the process ends (a graph may point at
+ * it).  Replays and eager calls share the scene's buffers: they must not run
+ * concurrently on one scene.  The *_plain entry points synchronise the host and
+ * fail with PSDR_ERR_CAPTURE under capture.
  */
 #ifndef PSDR_HIP_H
 #define PSDR_HIP_H
@@ -21,7 +37,7 @@
 extern "C" {
 #endif
 
-#define PSDR_ABI_VERSION 4
+#define PSDR_ABI_VERSION 5
 
 /* status codes */
 #define PSDR_OK 0
@@ -29,6 +45,7 @@ extern "C" {
 #define PSDR_ERR_HIP -2      /* a HIP runtime call failed */
 #define PSDR_ERR_STATE -3    /* call out of order (e.g. backward without forward) */
 #define PSDR_ERR_NOMEM -4    /* device allocation failed */
+#define PSDR_ERR_CAPTURE -5  /* not possible under stream capture (see above) */
 
 /* Material kinds (src/material.h:24-58). */
 #define PSDR_MAT_DIFFUSE 0      /* DiffuseMaterial: albedo texture / pi      */
@@ -147,6 +164,12 @@ int psdr_scene_create(const psdr_scene_desc* desc, psdr_scene** out, void* strea
 int psdr_scene_update(psdr_scene* scene, const psdr_scene_desc* desc, void* stream);
 
 int psdr_scene_destroy(psdr_scene* scene);
+
+/* Orders the scene's outstanding work on its auxiliary streams (the boundary
+ * prepass of a renderD forward whose backward has not run) before later work
+ * on `stream`.  No reference counterpart: the reference runs everything on one
+ * stream.  A graph replay that uses the scene calls it first. */
+int psdr_scene_join(psdr_scene* scene, void* stream);
 
 /* Counts: triangles, edges, LBVH depth (root = 0). */
 int psdr_scene_info(const psdr_scene* scene, int32_t* n_tris, int32_t* n_edges, int32_t* bvh_depth);

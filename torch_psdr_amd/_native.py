@@ -16,6 +16,7 @@ PSDR_ERR_ARG = -1
 PSDR_ERR_HIP = -2
 PSDR_ERR_STATE = -3
 PSDR_ERR_NOMEM = -4
+PSDR_ERR_CAPTURE = -5
 PSDR_MAT_DIFFUSE = 0
 PSDR_MAT_DIFFUSE_BSDF = 1
 PSDR_MAT_LIGHT = 2
@@ -36,7 +37,7 @@ PSDR_CAMERA_PERSPECTIVE = 0
 PSDR_CAMERA_ORTHOGRAPHIC = 1
 PSDR_CAMERA_GRAD = 10
 
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 
 class PsdrObject(C.Structure):
@@ -114,6 +115,7 @@ _SIGS = {
     "psdr_scene_create": (C.c_int, [C.POINTER(PsdrSceneDesc), C.POINTER(_VP), _VP]),
     "psdr_scene_update": (C.c_int, [_VP, C.POINTER(PsdrSceneDesc), _VP]),
     "psdr_scene_destroy": (C.c_int, [_VP]),
+    "psdr_scene_join": (C.c_int, [_VP, _VP]),
     "psdr_scene_info": (C.c_int, [_VP, C.POINTER(_I32), C.POINTER(_I32), C.POINTER(_I32)]),
     "psdr_render_forward": (C.c_int, [_VP, C.POINTER(PsdrConfig), C.POINTER(PsdrCamera), _VP, _I32, _VP]),
     "psdr_render_backward": (C.c_int, [_VP, C.POINTER(PsdrConfig), C.POINTER(PsdrCamera), _VP, _VP, _VP, _VP, _VP,

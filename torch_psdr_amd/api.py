@@ -43,6 +43,10 @@ class VarContex:
                 self.torch_vars.append(t)
 
 
+def _capturing() -> bool:
+    return torch.cuda.is_available() and torch.cuda.is_current_stream_capturing()
+
+
 class BaseCamera(VarContex):
     """Base camera class (src/python/torch_psdr.py:106-110).
 
@@ -74,7 +78,21 @@ class BaseCamera(VarContex):
         return torch.cat([t.to(device=dev, dtype=torch.float32) for t in parts])
 
     def params(self) -> List[float]:
-        vals = self.param_tensor().detach().cpu().tolist()
+        """Host values of the parameters (+ height, width, kind).  Read from the
+        tensors at every eager call; under HIP graph capture, where a copy from
+        device tensors is not possible, the values of the last eager read are
+        used, provided the tensors have not changed since."""
+        parts = (self.look_from, self.look_at, self.up, self._extent)
+        key = tuple((t.data_ptr(), t._version) for t in parts)
+        if _capturing() and any(t.is_cuda for t in parts):
+            if getattr(self, "_host_key", None) != key:
+                raise RuntimeError("camera parameters changed (or were never read) since the last eager render: a "
+                                   "captured step renders with camera values read before capture, so render once "
+                                   "eagerly first, and keep camera optimisation in eager steps")
+            vals = self._host_vals
+        else:
+            vals = self.param_tensor().detach().cpu().tolist()
+            self._host_vals, self._host_key = vals, key
         return vals + [float(self.height), float(self.width), float(self.kind)]
 
 

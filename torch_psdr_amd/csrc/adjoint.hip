@@ -653,20 +653,37 @@ inline int blocks(int64_t n, int t) { return static_cast<int>((n + t - 1) / t); 
 // the forward render (boundary_prepare_async) and overlap it; the backward then
 // only forms the dL/dI-weighted scatters.
 
+// Joins open auxiliary-stream work into `st`.  Under stream capture only work
+// issued in the same capture can be joined (a captured stream may not wait on an
+// event recorded outside it); work left open by an eager renderD whose backward
+// never ran must be finished before a capture begins.
 int aux_join(psdr_scene* s, hipStream_t st) {
-  if (!s->aux.fork) return PSDR_OK;
+  if (!s->aux.fork || !s->aux_open) return PSDR_OK;
+  if (capture_active() && !s->aux_captured) {
+    set_error(PSDR_ERR_CAPTURE, "boundary work of an eager renderD without backward is still open: run its "
+                                "backward before capturing");
+    return PSDR_ERR_CAPTURE;
+  }
   for (int i = 0; i < 2; ++i) {
     PSDR_HIP_CHECK(hipEventRecord(s->aux.join[i], s->aux.st[i]));
     PSDR_HIP_CHECK(hipStreamWaitEvent(st, s->aux.join[i], 0));
   }
+  s->aux_open = false;
   return PSDR_OK;
 }
 
 static int aux_fork(psdr_scene* s, hipStream_t st) {
   int rc = s->aux.ensure();
   if (rc) return rc;
+  if (s->aux_open && capture_active() != s->aux_captured) {
+    // open work from outside this capture (or a capture left unjoined)
+    rc = aux_join(s, st);
+    if (rc) return rc;
+  }
   PSDR_HIP_CHECK(hipEventRecord(s->aux.fork, st));
   for (int i = 0; i < 2; ++i) PSDR_HIP_CHECK(hipStreamWaitEvent(s->aux.st[i], s->aux.fork, 0));
+  s->aux_open = true;
+  s->aux_captured = capture_active();
   return PSDR_OK;
 }
 
@@ -678,6 +695,8 @@ int boundary_prepare_async(psdr_scene* s, const psdr_config& cfg, const psdr_cam
   // the primary chain is the longest: enqueue it first
   if (cfg.spp_primary_edge > 0) rc = boundary_primary_prepare(s, cfg, cam, s->aux.st[0]);
   if (!rc && cfg.spp_secondary_edge > 0) rc = boundary_direct_prepare(s, cfg, cam, s->aux.st[1]);
+  s->aux_open = true;
+  s->aux_captured = capture_active();
   if (rc) return rc;
   s->cur->bprep = true;
   return PSDR_OK;
@@ -686,7 +705,7 @@ int boundary_prepare_async(psdr_scene* s, const psdr_config& cfg, const psdr_cam
 static int camera_grad_begin(psdr_scene* s, float* grad_cam, hipStream_t st, float** acc) {
   *acc = nullptr;
   if (!grad_cam) return PSDR_OK;
-  if (s->cam_acc.ensure(4ull * CAM_ACC)) return PSDR_ERR_NOMEM;
+  if (s->cam_acc.ensure(4ull * CAM_ACC)) return grow_failed();
   *acc = s->cam_acc.as<float>();
   return fill_f32(*acc, CAM_ACC, 0.f, st);
 }
@@ -704,7 +723,7 @@ int render_backward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& ca
   const bool tp = (terms & PSDR_TERM_PRIMARY) != 0 && s->n_edges > 0 && cfg.spp_primary_edge > 0;
   const bool td = (terms & PSDR_TERM_DIRECT) != 0 && s->n_edges > 0 && cfg.spp_secondary_edge > 0;
   if (s->stats.ensure(4ull * (3 * static_cast<size_t>(s->n_vertices) + s->n_texels + 3 * s->n_lights + 64)))
-    return PSDR_ERR_NOMEM;
+    return grow_failed();
   float* gv = grad_v ? grad_v : s->stats.as<float>();
   RenderSlot* slot = s->find_slot(cfg, camd);
   if (!slot) {
@@ -738,8 +757,9 @@ int render_backward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& ca
     if (!prepared) rcp = boundary_primary_prepare(s, cfg, camd, s->aux.st[0]);
     if (!rcp) rcp = boundary_primary_finish(s, cfg, camd, dL_dI, gv, acc, s->aux.st[0]);
   }
-  // join even after an error so no work is left running unordered
-  const int rcj = (tp || td) ? aux_join(s, st) : PSDR_OK;
+  // join even after an error so no work is left running unordered (also a
+  // prepass of this forward whose terms this call does not form)
+  const int rcj = aux_join(s, st);
   if (rci) return rci;
   if (rcp) return rcp;
   if (rcd) return rcd;
@@ -772,7 +792,7 @@ int interior_backward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& 
   P.dLdI = dL_dI;
   // scratch for absent gradient buffers
   if (s->stats.ensure(4ull * (3 * static_cast<size_t>(s->n_vertices) + s->n_texels + 3 * s->n_lights + 64)))
-    return PSDR_ERR_NOMEM;
+    return grow_failed();
   float* sink = s->stats.as<float>();
   P.grad_v = grad_v ? grad_v : sink;
   P.grad_t = grad_t ? grad_t : sink;

@@ -548,7 +548,7 @@ static int primary_pass(psdr_scene* s, const psdr_config& cfg, const CameraK& ca
       slot.p_qb.ensure(QUEUE_BYTES_PER * n) || slot.p_rays.ensure(RAY_BYTES_PER * ray_cap) ||
       slot.p_hits.ensure(12ull * ray_cap) || slot.p_rays2.ensure(RAY_BYTES_PER * ray_cap) ||
       slot.p_hits2.ensure(12ull * ray_cap) || slot.b_counters.ensure(4 * 256))
-    return PSDR_ERR_NOMEM;
+    return grow_failed();
   PrimRecs rec = carve_prim(slot.b_rec, n);
   RaysView r0 = carve_rays(slot.b_rays, slot.b_hits, 2ll * n);
   const QueueView q0 = carve_queue(slot.b_q0, n);
@@ -567,7 +567,7 @@ static int primary_pass(psdr_scene* s, const psdr_config& cfg, const CameraK& ca
   const bool sorted = !pre && !s->plain;
   if (sorted) {
     if (slot.b_keys.ensure(8ull * n) || slot.b_vals.ensure(8ull * n) || slot.b_hist.ensure(4 * radix_hist_words(n)))
-      return PSDR_ERR_NOMEM;
+      return grow_failed();
     k0 = slot.b_keys.as<uint32_t>();
     v0 = slot.b_vals.as<uint32_t>();
   }
@@ -588,7 +588,7 @@ static int primary_pass(psdr_scene* s, const psdr_config& cfg, const CameraK& ca
   hipLaunchKernelGGL(k_prim_rays, dim3(blocks(n, BT)), dim3(BT), 0, st, sv, cam, n, order, rec, rng0, r0, q0, cnt);
   PSDR_HIP_CHECK(hipGetLastError());
   const int tgrid = trace_grid(2ll * n);
-  if (slot.p_spill.ensure(std::max(slot.p_spill.bytes, spill_bytes(tgrid)))) return PSDR_ERR_NOMEM;
+  if (slot.p_spill.ensure(std::max(slot.p_spill.bytes, spill_bytes(tgrid)))) return grow_failed();
   rc = trace_rays(sv, r0.o, r0.d, 2 * n, nullptr, 1, 0, nullptr, 0, r0.tri, nullptr, r0.u, r0.v, slot.p_spill.as<int>(),
                   tgrid, st);
   if (rc) return rc;
@@ -645,7 +645,7 @@ int boundary_primary_prepare(psdr_scene* s, const psdr_config& cfg, const psdr_c
   if (n < 0) return PSDR_ERR_ARG;
   if (slot.b_state.ensure(8ull * std::max(R, 1)) || slot.hc_mass.ensure(4ull * R) || slot.hc_pmf.ensure(4ull * R) ||
       slot.hc_cmf.ensure(4ull * R) || slot.b_tmp.ensure(4ull * (2 * (R / 256 + 2) + 16)))
-    return PSDR_ERR_NOMEM;
+    return grow_failed();
   hipLaunchKernelGGL(k_seed_states, dim3(blocks(R, BT)), dim3(BT), 0, st, slot.b_state.as<uint64_t>(), R);
   hipLaunchKernelGGL(k_fill_f32, dim3(blocks(R, BT)), dim3(BT), 0, st, slot.hc_mass.as<float>(), R, 0.f);
   for (uint32_t r = 0; r < cfg.primary_edge_preprocess_rounds; ++r) {
@@ -697,7 +697,7 @@ int boundary_direct_prepare(psdr_scene* s, const psdr_config& cfg, const psdr_ca
       slot.d_spill.ensure(static_cast<size_t>(grid) * BT * (STACK_TOTAL - BSTACK) * 4) ||
       slot.d_keys.ensure(8ull * n) || slot.d_vals.ensure(8ull * n) || slot.d_hist.ensure(4 * radix_hist_words(n)) ||
       slot.d_rec.ensure(48ull * std::max(n, 1)))
-    return PSDR_ERR_NOMEM;
+    return grow_failed();
   uint64_t* state = slot.d_state.as<uint64_t>();
   hipLaunchKernelGGL(k_seed_states, dim3(blocks(cells, BT)), dim3(BT), 0, st, state, cells);
   hipLaunchKernelGGL(k_fill_f32, dim3(blocks(cells, BT)), dim3(BT), 0, st, slot.d_mass.as<float>(), cells, 0.f);

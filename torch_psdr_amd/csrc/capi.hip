@@ -4,7 +4,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -22,8 +24,40 @@ void set_error(int code, const std::string& msg) {
   g_last_error = msg;
 }
 
+namespace {
+thread_local bool t_capture = false;
+std::atomic<bool> g_graph_seen{false};
+std::mutex g_retired_mu;
+std::vector<void*> g_retired;  // allocations a captured graph may still use
+}  // namespace
+
+bool capture_active() { return t_capture; }
+
+CaptureScope::CaptureScope(hipStream_t st) : prev(t_capture) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(st, &cs) != hipSuccess) {
+    (void)hipGetLastError();
+    cs = hipStreamCaptureStatusNone;
+  }
+  t_capture = cs == hipStreamCaptureStatusActive;
+  if (t_capture) g_graph_seen = true;
+}
+CaptureScope::~CaptureScope() { t_capture = prev; }
+
 int DevBuf::ensure(size_t n) {
   if (n <= bytes && p) return PSDR_OK;
+  if (t_capture) {
+    set_error(PSDR_ERR_CAPTURE, "a scene buffer would grow to " + std::to_string(n) +
+                                    " bytes during graph capture: run the captured calls once eagerly, with the "
+                                    "same sizes, before capturing them");
+    return PSDR_ERR_CAPTURE;
+  }
+  if (p && g_graph_seen) {
+    std::lock_guard<std::mutex> lk(g_retired_mu);
+    g_retired.push_back(p);
+    p = nullptr;
+    bytes = 0;
+  }
   release();
   if (n == 0) n = 16;
   if (hipMalloc(&p, n) != hipSuccess) {
@@ -52,6 +86,11 @@ AuxStreams::~AuxStreams() {
 
 int AuxStreams::ensure() {
   if (fork) return PSDR_OK;
+  if (t_capture) {
+    set_error(PSDR_ERR_CAPTURE, "auxiliary streams are created on first use: run the captured calls once eagerly "
+                                "before capturing them");
+    return PSDR_ERR_CAPTURE;
+  }
   for (int i = 0; i < 2; ++i) {
     PSDR_HIP_CHECK(hipStreamCreateWithFlags(&st[i], hipStreamNonBlocking));
     PSDR_HIP_CHECK(hipEventCreateWithFlags(&join[i], hipEventDisableTiming));
@@ -544,6 +583,7 @@ int psdr_scene_update(psdr_scene* s, const psdr_scene_desc* desc, void* stream) 
     return PSDR_ERR_ARG;
   }
   PSDR_TRY(check_desc(desc));
+  CaptureScope cap(as_stream(stream));
   // a boundary prepass still running on the auxiliary streams reads the scene
   PSDR_TRY(aux_join(s, as_stream(stream)));
   PSDR_TRY(scene_set_buffers(s, desc, as_stream(stream)));
@@ -556,6 +596,15 @@ int psdr_scene_destroy(psdr_scene* s) {
   (void)hipDeviceSynchronize();
   delete s;  // buffers, streams and events are released by their owners
   return PSDR_OK;
+}
+
+int psdr_scene_join(psdr_scene* s, void* stream) {
+  if (!s) {
+    set_error(PSDR_ERR_ARG, "scene is NULL");
+    return PSDR_ERR_ARG;
+  }
+  CaptureScope cap(as_stream(stream));
+  return aux_join(s, as_stream(stream));
 }
 
 int psdr_scene_info(const psdr_scene* s, int32_t* n_tris, int32_t* n_edges, int32_t* bvh_depth) {
@@ -596,6 +645,7 @@ int psdr_render_forward(psdr_scene* s, const psdr_config* cfg, const psdr_camera
     set_error(PSDR_ERR_ARG, "render_forward: bad argument");
     return PSDR_ERR_ARG;
   }
+  CaptureScope cap(as_stream(stream));
   return render_forward(s, *cfg, *cam, out_rgb, keep, as_stream(stream));
 }
 
@@ -614,6 +664,7 @@ int psdr_render_backward(psdr_scene* s, const psdr_config* cfg, const psdr_camer
     set_error(PSDR_ERR_ARG, "render_backward: bad terms mask");
     return PSDR_ERR_ARG;
   }
+  CaptureScope cap(as_stream(stream));
   return render_backward(s, *cfg, *cam, dL_dI, grad_vertices, grad_texels, grad_emit, grad_camera, terms,
                          as_stream(stream));
 }
@@ -624,6 +675,14 @@ struct PlainScope {  // sets psdr_scene::plain for one call
   explicit PlainScope(psdr_scene* sc) : s(sc) { s->plain = true; }
   ~PlainScope() { s->plain = false; }
 };
+// the reference launch shape synchronises the host after every trace
+int refuse_capture(void* stream, const char* what) {
+  CaptureScope cap(as_stream(stream));
+  if (!capture_active()) return PSDR_OK;
+  set_error(PSDR_ERR_CAPTURE, std::string(what) + ": the plain launch shape synchronises the host and cannot be "
+                                                  "captured into a graph");
+  return PSDR_ERR_CAPTURE;
+}
 }  // namespace
 
 int psdr_render_forward_plain(psdr_scene* s, const psdr_config* cfg, const psdr_camera* cam, float* out_rgb,
@@ -632,6 +691,7 @@ int psdr_render_forward_plain(psdr_scene* s, const psdr_config* cfg, const psdr_
     set_error(PSDR_ERR_ARG, "render_forward_plain: bad argument");
     return PSDR_ERR_ARG;
   }
+  PSDR_TRY(refuse_capture(stream, "render_forward_plain"));
   PlainScope scope(s);
   return psdr_render_forward(s, cfg, cam, out_rgb, keep, stream);
 }
@@ -643,6 +703,7 @@ int psdr_render_backward_plain(psdr_scene* s, const psdr_config* cfg, const psdr
     set_error(PSDR_ERR_ARG, "render_backward_plain: bad argument");
     return PSDR_ERR_ARG;
   }
+  PSDR_TRY(refuse_capture(stream, "render_backward_plain"));
   PlainScope scope(s);
   return psdr_render_backward(s, cfg, cam, dL_dI, grad_vertices, grad_texels, grad_emit, grad_camera, terms,
                               stream);

@@ -603,7 +603,7 @@ int run_depths(psdr_scene* s, const WaveArgs& a, const QueueView& q0, const Rays
   const int64_t ray_cap = static_cast<int64_t>(n) * Kmax;
   const int tgrid = trace_grid(ray_cap);
   DevBuf& spill = a.spill ? *a.spill : s->spill;
-  if (spill.ensure(std::max(spill.bytes, spill_bytes(tgrid)))) return PSDR_ERR_NOMEM;
+  if (spill.ensure(std::max(spill.bytes, spill_bytes(tgrid)))) return grow_failed();
   const int sgrid = std::max(1, std::min(blocks(n, 256), num_cus() * 8));
   QueueView qin = q0;
   RaysView rin = r0;
@@ -682,7 +682,8 @@ int render_forward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& cam
     // renderD: the boundary terms' dL/dI-independent work goes to the
     // auxiliary streams first (the longest chain, enqueued before this render
     // so it starts at once) and overlaps this render and the caller's loss
-    if (s->aux.ensure()) return PSDR_ERR_HIP;
+    const int rca = s->aux.ensure();
+    if (rca) return rca;
     PSDR_HIP_CHECK(hipEventRecord(s->aux.fork, st));
     const int rcp = boundary_prepare_async(s, cfg, camd);
     if (rcp) return rcp;
@@ -690,14 +691,14 @@ int render_forward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& cam
   // buffers
   if (s->lane_L.ensure(12ull * N) || s->q_a.ensure(QUEUE_BYTES_PER * N) || s->q_b.ensure(QUEUE_BYTES_PER * N) ||
       s->rays.ensure(RAY_BYTES_PER * ray_cap) || s->hits.ensure(12ull * ray_cap) || s->counters_r.ensure(4 * COUNTERS))
-    return PSDR_ERR_NOMEM;
+    return grow_failed();
   const int tgrid = trace_grid(ray_cap);
-  if (s->spill.ensure(spill_bytes(tgrid))) return PSDR_ERR_NOMEM;
+  if (s->spill.ensure(spill_bytes(tgrid))) return grow_failed();
   const int64_t slots = 1 + static_cast<int64_t>(D) * Kmax;
   CacheView cache{nullptr, nullptr, nullptr, N, nullptr, nullptr, nullptr, nullptr};
   if (slot) {
     if (slot->hit_cache.ensure(12ull * slots * N) || slot->replay.ensure((20ull * std::max(D, 1) + 16ull) * N))
-      return PSDR_ERR_NOMEM;
+      return grow_failed();
     cache.tri = slot->hit_cache.as<int>();
     cache.u = reinterpret_cast<float*>(cache.tri + slots * N);
     cache.v = cache.u + slots * N;
@@ -723,7 +724,7 @@ int render_forward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& cam
                       rays_a.v, s->spill.as<int>(), tgrid, st);
   if (rc) return rc;
   if (s->plain) PSDR_HIP_CHECK(hipStreamSynchronize(st));
-  if (s->rays2.ensure(RAY_BYTES_PER * ray_cap) || s->hits2.ensure(12ull * ray_cap)) return PSDR_ERR_NOMEM;
+  if (s->rays2.ensure(RAY_BYTES_PER * ray_cap) || s->hits2.ensure(12ull * ray_cap)) return grow_failed();
   RaysView rays_b = carve_rays(s->rays2, s->hits2, ray_cap);
   WaveArgs wa;
   wa.cam = cam;

@@ -34,6 +34,24 @@ void set_error(int code, const std::string& msg);
     }                                                                                     \
   } while (0)
 
+// Stream capture (HIP graphs).  An entry point working on a stream that is
+// being captured sets capture_active() for the duration of the call; buffers
+// must not grow then (ensure() fails with PSDR_ERR_CAPTURE), and once any
+// capture has happened in the process a buffer that grows retires its old
+// allocation (kept until the process ends) instead of freeing it, since a graph
+// may still point at it.
+bool capture_active();
+struct CaptureScope {
+  bool prev;
+  explicit CaptureScope(hipStream_t st);
+  ~CaptureScope();
+  CaptureScope(const CaptureScope&) = delete;
+  CaptureScope& operator=(const CaptureScope&) = delete;
+};
+
+// status of a failed DevBuf::ensure (its message is already set)
+inline int grow_failed() { return capture_active() ? PSDR_ERR_CAPTURE : PSDR_ERR_NOMEM; }
+
 // Grow-only device buffer.
 struct DevBuf {
   void* p = nullptr;
@@ -157,6 +175,9 @@ struct psdr_scene {
   psdr::RenderSlot* cur = &slots[0];
   uint64_t clock = 0;
   psdr::AuxStreams aux;
+  // work issued on the auxiliary streams and not yet joined into a caller's
+  // stream; aux_captured: issued while the caller's stream was being captured
+  bool aux_open = false, aux_captured = false;
   // Reference launch shape, set for the duration of a *_plain call: every
   // depth's kernels run over all lanes (no compaction), each ray kind is traced
   // by its own launch followed by a host sync, boundary samples are processed in

@@ -29,6 +29,33 @@ _CUBE_UV_IDX = ((2, 3, 7), (8, 7, 3), (5, 4, 10), (9, 10, 4), (0, 1, 3), (4, 3, 
                 (12, 13, 8), (9, 8, 13), (5, 6, 10), (11, 10, 6), (4, 3, 9), (8, 9, 3))
 
 
+_CONST: dict = {}
+
+
+def _const(device: torch.device, name: str, make) -> torch.Tensor:
+    """A constant tensor on `device`, built on the host once and cached: a step
+    that rebuilds its Scene then issues no host-to-device copy (such a copy
+    synchronises the host, and cannot be recorded into a HIP graph)."""
+    dev = torch.device(device)
+    if dev.type == "cuda" and dev.index is None:
+        dev = torch.device("cuda", torch.cuda.current_device())
+    key = (dev.type, dev.index, name)
+    t = _CONST.get(key)
+    if t is None:
+        t = make().to(dev)
+        _CONST[key] = t
+    return t
+
+
+def _default_uv(device) -> torch.Tensor:
+    return _const(device, "default_uv", lambda: torch.tensor(_DEFAULT_UV, dtype=torch.float32))
+
+
+def _dev_scalars(values, device) -> torch.Tensor:
+    """A small fp32 vector on `device` made by fill kernels (no host copy)."""
+    return torch.stack([torch.full((), float(x), dtype=torch.float32, device=device) for x in values])
+
+
 def _cube_uvs():
     u, v = 0.25, 1.0 / 3.0
     f32 = lambda x: float(torch.tensor(x, dtype=torch.float32))
@@ -43,15 +70,16 @@ def _env_cube_vertices(cameras, meshes, device) -> torch.Tensor:
     and the meshes' vertex boxes, grown to its centre +- 1000 x its largest
     extent; all fp32 elementwise ops, detached."""
     with torch.no_grad():
-        pos = [c.look_from.detach().to(device=device, dtype=torch.float32).reshape(3) for c in cameras]
-        pmin, pmax = pos[0], pos[0]
-        for p in pos[1:]:
-            pmin, pmax = torch.minimum(pmin, p), torch.maximum(pmax, p)
+        # camera positions from their host copies (BaseCamera.params)
+        pos = [c.params()[:3] for c in cameras]
+        lo3 = [min(p[a] for p in pos) for a in range(3)]
+        hi3 = [max(p[a] for p in pos) for a in range(3)]
+        pmin, pmax = _dev_scalars(lo3, device), _dev_scalars(hi3, device)
         for m in meshes:
             v = m.vertices.detach().to(torch.float32)
             pmin = torch.minimum(pmin, v.min(0).values)
             pmax = torch.maximum(pmax, v.max(0).values)
-        d = torch.tensor(1000.0, dtype=torch.float32, device=device) * (pmax - pmin).max()
+        d = (pmax - pmin).max() * 1000.0
         c = (pmin + pmax) * 0.5
         lo, hi = c - d, c + d
         corners = [torch.stack([(hi if (k >> 2) & 1 else lo)[0], (hi if (k >> 1) & 1 else lo)[1],
@@ -138,7 +166,7 @@ def flatten(meshes, lights, device, env=None, cameras=()) -> FlatScene:
             corner = m.uvs.to(torch.float32)[m.uv_indices.long()]  # (F, 3, 2)
             uvs.append(corner.reshape(nf, 6))
         else:
-            uvs.append(torch.tensor(_DEFAULT_UV, dtype=torch.float32, device=device).expand(nf, 6))
+            uvs.append(_default_uv(device).expand(nf, 6))
         tobj.append(torch.full((nf,), oi, dtype=torch.int32, device=device))
         rec = dict(material=N.PSDR_MAT_DIFFUSE, light=-1, albedo_offset=0, albedo_h=0, albedo_w=0,
                    rough_offset=0, rough_h=0, rough_w=0, tri_begin=toff, tri_count=nf, grad_mask=0,
@@ -179,19 +207,19 @@ def flatten(meshes, lights, device, env=None, cameras=()) -> FlatScene:
     if env is not None:
         oi = len(objs)
         verts.append(_env_cube_vertices(cameras, meshes, device))
-        faces.append(torch.tensor(_CUBE_FACES, dtype=torch.int32, device=device) + voff)
+        faces.append(_const(device, "cube_faces", lambda: torch.tensor(_CUBE_FACES, dtype=torch.int32)) + voff)
         rec = dict(light=len(light_obj), rough_offset=0, rough_h=0, rough_w=0, tri_begin=toff, tri_count=12,
                    grad_mask=0, normal_offset=-1, normal_h=0, normal_w=0)
         if hasattr(env, "cube_map"):
-            uvt = torch.tensor(_cube_uvs(), dtype=torch.float32, device=device)
-            uvs.append(uvt[torch.tensor(_CUBE_UV_IDX, device=device).long()].reshape(12, 6))
+            uvs.append(_const(device, "cube_uv", lambda: torch.tensor(_cube_uvs(), dtype=torch.float32)[
+                torch.tensor(_CUBE_UV_IDX).long()].reshape(12, 6)))
             a_off, a_h, a_w, tex_off = _tex_entry(env.cube_map, tex_pieces, tex_off)
             rec.update(material=N.PSDR_MAT_ENV_MAP, albedo_offset=a_off, albedo_h=a_h, albedo_w=a_w)
             if env.cube_map.data.requires_grad:
                 rec["grad_mask"] |= N.PSDR_GRAD_ALBEDO
             key.append(("env_map",))
         else:
-            uvs.append(torch.tensor(_DEFAULT_UV, dtype=torch.float32, device=device).expand(12, 6))
+            uvs.append(_default_uv(device).expand(12, 6))
             coeff = env.sh9_coeff.to(device=device, dtype=torch.float32).reshape(27)
             tex_pieces.append(coeff)
             rec.update(material=N.PSDR_MAT_ENV_SH9, albedo_offset=tex_off, albedo_h=0, albedo_w=0)

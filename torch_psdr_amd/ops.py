@@ -15,6 +15,7 @@ from __future__ import annotations
 
 import itertools
 import threading
+import weakref
 from typing import Dict, List, Sequence, Tuple
 
 import torch
@@ -58,6 +59,21 @@ def _plain() -> bool:
     return _shape[0] == "plain"
 
 
+# Native scenes used while a HIP graph is being captured (torch_psdr_amd.graph):
+# one set per open capture.
+_capture_sets: List[set] = []
+
+
+def _capturing() -> bool:
+    return torch.cuda.is_available() and torch.cuda.is_current_stream_capturing()
+
+
+def _note_capture(ns: "NativeScene") -> None:
+    if _capture_sets and _capturing():
+        with _lock:
+            _capture_sets[-1].add(ns)
+
+
 class NativeScene:
     """Owns a psdr_scene handle bound to one FlatScene at a time."""
 
@@ -65,18 +81,39 @@ class NativeScene:
         self.lib = N.load()
         self.keep_alive: list = []
         self.handle = N.C.c_void_p()
+        self._bound = None
         desc = flat.desc(self.keep_alive)
         N.check(self.lib.psdr_scene_create(N.C.byref(desc), N.C.byref(self.handle), N.current_stream()))
-        self.flat = flat
+        self._hold(flat)
+
+    def _hold(self, flat: FlatScene) -> None:
+        # The native scene reads the bound buffers until the next bind: keep
+        # their storage alive, detached (an autograd graph kept alive here would
+        # tie later backward passes to the stream of the step that built it).
+        self.buffers = tuple(t.detach() for t in (flat.vertices, flat.texels, flat.light_emit))
+        self._bound = weakref.ref(flat)
+
+    def bound_to(self, flat: FlatScene) -> bool:
+        return self._bound is not None and self._bound() is flat
+
+    def stale(self) -> None:
+        """Derive the scene again at its next bind (its derived data or forward
+        states were rewritten behind the host's back, e.g. by a graph replay)."""
+        self._bound = None
 
     def bind(self, flat: FlatScene) -> None:
-        if flat is self.flat:
+        _note_capture(self)
+        if self.bound_to(flat):
             return
         keep_alive: list = []
         desc = flat.desc(keep_alive)
         N.check(self.lib.psdr_scene_update(self.handle, N.C.byref(desc), N.current_stream()))
         self.keep_alive = keep_alive
-        self.flat = flat
+        self._hold(flat)
+
+    def join(self) -> None:
+        """Order the scene's open auxiliary-stream work before later work on the current stream."""
+        N.check(self.lib.psdr_scene_join(self.handle, N.current_stream()))
 
     def info(self):
         nt, ne, dep = N.C.c_int32(), N.C.c_int32(), N.C.c_int32()
@@ -96,6 +133,9 @@ def register(flat: FlatScene) -> int:
     with _lock:
         ns = _by_topo.get(flat.topo_key)
     if ns is None:
+        if _capturing():
+            raise RuntimeError("a scene topology seen for the first time cannot be set up under HIP graph capture: "
+                               "run the step eagerly before capturing it")
         ns = NativeScene(flat)
         with _lock:
             if len(_by_topo) >= _MAX_CACHED:
@@ -210,10 +250,18 @@ def _setup(ctx, inputs, output):
 
 def _backward(ctx, grad):
     vertices, texels, emit = ctx.saved_tensors
+    # the Scene may be gone by now: its entry is visible to the op for this call only
     with _lock:
-        _entries.setdefault(ctx.token, ctx.entry)
+        added = ctx.token not in _entries
+        if added:
+            _entries[ctx.token] = ctx.entry
     want_camera = bool(ctx.needs_input_grad[6])
-    gv, gt, ge, gc = render_backward(ctx.token, ctx.cam, ctx.cfg, grad, vertices, texels, emit, want_camera)
+    try:
+        gv, gt, ge, gc = render_backward(ctx.token, ctx.cam, ctx.cfg, grad, vertices, texels, emit, want_camera)
+    finally:
+        if added:
+            with _lock:
+                _entries.pop(ctx.token, None)
     gcam = gc.to(device=ctx.camera_meta[0], dtype=ctx.camera_meta[1]) if want_camera else None
     return None, None, None, gv, gt, ge, gcam, None
 
