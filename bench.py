@@ -50,6 +50,9 @@ def main():
     ap.add_argument("--graph", choices=("value", "report", "off"), default="value",
                     help="the step captured into a HIP graph and replayed: reported as the value (the eager "
                          "step's time beside it), reported beside the eager value, or not run")
+    ap.add_argument("--workload", choices=("c2", "c3"), default="c2",
+                    help="c2: the 512^2 / 16 spp vertex + texture step (default); c3: 64 scenes of uneven size, "
+                         "rendered and differentiated in turn on shared buffers")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device-to-device copy (scales rocprof FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -74,6 +77,9 @@ def main():
     from torch_psdr_amd import workloads as W
 
     dev = torch.device("cuda:0")
+    if args.workload == "c3":
+        bench_c3(args, torch, dr, W, dev)
+        return
     parts = W.c2_parts(dr, dev, res=args.res, spp=args.spp)
     integ = parts["integrator"]
     with torch.no_grad():
@@ -158,6 +164,52 @@ def main():
         "graph_step": graph,
     }
     print(json.dumps(out))
+
+
+def bench_c3(args, torch, dr, W, dev):
+    """64 scenes of uneven size (geodesic spheres of 980-184,320 triangles,
+    128^2-1024^2, 4-64 spp, depth 3, one edge sample per pixel for each boundary
+    term): one step renders and differentiates every scene in turn (vertex +
+    texel gradients), all scenes sharing one set of per-render buffers."""
+    t_setup = time.perf_counter()
+    parts = W.c3_parts(dr, dev)
+    print(f"c3: {len(parts)} scenes set up in {time.perf_counter() - t_setup:.1f} s", file=sys.stderr, flush=True)
+
+    def step():
+        for b in parts:
+            b.vertices.grad = None
+            b.tex.grad = None
+            img = b.integrator.renderD(b.scene(b.vertices, b.tex))[0]
+            ((img - b.target) ** 2).mean().backward()
+
+    D, L, B = 3, 1, 1
+    R = 1 + D * (L + B + 1)
+    lanes = sum(b.res * b.res * b.spp for b in parts)
+    edge = sum(b.res * b.res for b in parts)
+    rays = R * lanes + 23 * edge
+    steps = max(1, min(args.steps, 3))
+    for _ in range(max(1, min(args.warmup, 1))):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    ms_eager = (time.perf_counter() - t0) * 1e3 / steps
+    print(f"c3: eager {ms_eager:.1f} ms per step", file=sys.stderr, flush=True)
+    graph = graph_step_timing(torch, step, steps, rays)
+    graph["eager_ms_per_step"] = round(ms_eager, 3)
+    ms = graph["ms_per_step"] if args.graph == "value" else ms_eager
+    print(json.dumps({
+        "metric": "fwd+bwd Mrays/s, 64 scenes of uneven size in turn on shared buffers",
+        "value": round(rays / (ms * 1e-3) / 1e6, 3), "unit": "Mrays/s", "n_gpus": 1, "steps": steps,
+        "warmup": 1, "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "64 geodesic spheres (seed 4), 980-184320 tris, 128^2-1024^2, 4/16/64 spp, depth 3, "
+                               "edge spp 1, vertex + texel gradients",
+                   "scenes": len(parts), "lanes": lanes, "boundary_samples_per_term": edge, "rays_per_step": rays,
+                   "tris": [b.n_tris for b in parts], "res": [b.res for b in parts], "spp": [b.spp for b in parts]},
+        "graph_step": graph}))
 
 
 def graph_step_timing(torch, step, n_steps, rays):

@@ -171,6 +171,17 @@ int psdr_scene_destroy(psdr_scene* scene);
  * stream.  A graph replay that uses the scene calls it first. */
 int psdr_scene_join(psdr_scene* scene, void* stream);
 
+/* Makes `scene` use the per-render buffers (wavefront queues, rays, hits,
+ * forward states, boundary scratch) and auxiliary streams of `with`; its own
+ * are released (synchronising the device when no other scene shared them).
+ * Scenes sharing them render one after another in stream order, as one
+ * scene's calls do, and the buffers grow to the largest render: a batch of
+ * scenes of uneven sizes rendered in turn holds one set instead of one per
+ * scene.  Forward states stay per scene, four per workspace (least recently
+ * used evicted; a backward whose state was evicted returns PSDR_ERR_STATE).
+ * No reference counterpart: the reference renders one scene per call. */
+int psdr_scene_share_workspace(psdr_scene* scene, psdr_scene* with, void* stream);
+
 /* Counts: triangles, edges, LBVH depth (root = 0). */
 int psdr_scene_info(const psdr_scene* scene, int32_t* n_tris, int32_t* n_edges, int32_t* bvh_depth);
 

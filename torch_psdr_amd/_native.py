@@ -116,6 +116,7 @@ _SIGS = {
     "psdr_scene_update": (C.c_int, [_VP, C.POINTER(PsdrSceneDesc), _VP]),
     "psdr_scene_destroy": (C.c_int, [_VP]),
     "psdr_scene_join": (C.c_int, [_VP, _VP]),
+    "psdr_scene_share_workspace": (C.c_int, [_VP, _VP, _VP]),
     "psdr_scene_info": (C.c_int, [_VP, C.POINTER(_I32), C.POINTER(_I32), C.POINTER(_I32)]),
     "psdr_render_forward": (C.c_int, [_VP, C.POINTER(PsdrConfig), C.POINTER(PsdrCamera), _VP, _I32, _VP]),
     "psdr_render_backward": (C.c_int, [_VP, C.POINTER(PsdrConfig), C.POINTER(PsdrCamera), _VP, _VP, _VP, _VP, _VP,

@@ -482,7 +482,7 @@ __device__ __forceinline__ DepthOut depth_step(const AdjParams& P, float* hot, c
         const float pdf = smp.pdf * tmp;
         const float Jn = itn.area / itn.area;
         out.t = ((smp.bsdf * Jn) * G) / pdf;
-        if (grad) {
+        if (grad && nonzero3(adj_t)) {
           const f3 at = adj_t / pdf;
           const f3 gbsdf = at * (Jn * G);
           const float gJ = dot(at, smp.bsdf) * G;
@@ -595,7 +595,13 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adjoint(AdjParams P) {
       if (d > 0) O = reconstruct(sv, vt[d - 1], vu[d - 1], vv[d - 1], O0, true).p;
       const Its vx = reconstruct(sv, vt[d], vu[d], vv[d], O, true);
       Rng r{rs[d], rng.inc};
-      const DepthOut o = depth_step<ENV>(P, hot, vx, r, d, lane, true, g * Ts[d], A * Ts[d]);
+      // A == 0: nothing after this depth reached L.  The throughput of an
+      // extension whose sample has pdf 0 (cos = sqrt(u) = 0, or a grazing next
+      // hit) is 0 / 0; the forward multiplies it only into terms it never
+      // adds (else L is not finite and the lane is dropped), so its adjoint is
+      // 0, not 0 * NaN.
+      const bool tail = nonzero3(A);
+      const DepthOut o = depth_step<ENV>(P, hot, vx, r, d, lane, true, g * Ts[d], tail ? A * Ts[d] : splat(0.f));
       if (o.ext) {
         VertexAdj nx = pend;
         nx.p = nx.p + o.next.p;
@@ -609,7 +615,7 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adjoint(AdjParams P) {
       pend = o.cur;
       pend.p = pend.p + pendO;
       pendO = o.gO;
-      A = g * o.c + A * o.t;
+      A = tail ? g * o.c + A * o.t : g * o.c;
     }
     // camera vertex: its own adjoints plus render_interior's L * J0 (J0 = A / detach(A))
     pend.area = pend.area + dot(g, L) / its.area;
@@ -658,55 +664,55 @@ inline int blocks(int64_t n, int t) { return static_cast<int>((n + t - 1) / t); 
 // event recorded outside it); work left open by an eager renderD whose backward
 // never ran must be finished before a capture begins.
 int aux_join(psdr_scene* s, hipStream_t st) {
-  if (!s->aux.fork || !s->aux_open) return PSDR_OK;
-  if (capture_active() && !s->aux_captured) {
+  if (!s->ws->aux.fork || !s->ws->aux_open) return PSDR_OK;
+  if (capture_active() && !s->ws->aux_captured) {
     set_error(PSDR_ERR_CAPTURE, "boundary work of an eager renderD without backward is still open: run its "
                                 "backward before capturing");
     return PSDR_ERR_CAPTURE;
   }
   for (int i = 0; i < 2; ++i) {
-    PSDR_HIP_CHECK(hipEventRecord(s->aux.join[i], s->aux.st[i]));
-    PSDR_HIP_CHECK(hipStreamWaitEvent(st, s->aux.join[i], 0));
+    PSDR_HIP_CHECK(hipEventRecord(s->ws->aux.join[i], s->ws->aux.st[i]));
+    PSDR_HIP_CHECK(hipStreamWaitEvent(st, s->ws->aux.join[i], 0));
   }
-  s->aux_open = false;
+  s->ws->aux_open = false;
   return PSDR_OK;
 }
 
 static int aux_fork(psdr_scene* s, hipStream_t st) {
-  int rc = s->aux.ensure();
+  int rc = s->ws->aux.ensure();
   if (rc) return rc;
-  if (s->aux_open && capture_active() != s->aux_captured) {
+  if (s->ws->aux_open && capture_active() != s->ws->aux_captured) {
     // open work from outside this capture (or a capture left unjoined)
     rc = aux_join(s, st);
     if (rc) return rc;
   }
-  PSDR_HIP_CHECK(hipEventRecord(s->aux.fork, st));
-  for (int i = 0; i < 2; ++i) PSDR_HIP_CHECK(hipStreamWaitEvent(s->aux.st[i], s->aux.fork, 0));
-  s->aux_open = true;
-  s->aux_captured = capture_active();
+  PSDR_HIP_CHECK(hipEventRecord(s->ws->aux.fork, st));
+  for (int i = 0; i < 2; ++i) PSDR_HIP_CHECK(hipStreamWaitEvent(s->ws->aux.st[i], s->ws->aux.fork, 0));
+  s->ws->aux_open = true;
+  s->ws->aux_captured = capture_active();
   return PSDR_OK;
 }
 
 int boundary_prepare_async(psdr_scene* s, const psdr_config& cfg, const psdr_camera& cam) {
-  s->cur->bprep = false;
-  if (s->n_edges == 0 || s->plain || !s->aux.fork) return PSDR_OK;
-  for (int i = 0; i < 2; ++i) PSDR_HIP_CHECK(hipStreamWaitEvent(s->aux.st[i], s->aux.fork, 0));
+  s->ws->cur->bprep = false;
+  if (s->n_edges == 0 || s->plain || !s->ws->aux.fork) return PSDR_OK;
+  for (int i = 0; i < 2; ++i) PSDR_HIP_CHECK(hipStreamWaitEvent(s->ws->aux.st[i], s->ws->aux.fork, 0));
   int rc = PSDR_OK;
   // the primary chain is the longest: enqueue it first
-  if (cfg.spp_primary_edge > 0) rc = boundary_primary_prepare(s, cfg, cam, s->aux.st[0]);
-  if (!rc && cfg.spp_secondary_edge > 0) rc = boundary_direct_prepare(s, cfg, cam, s->aux.st[1]);
-  s->aux_open = true;
-  s->aux_captured = capture_active();
+  if (cfg.spp_primary_edge > 0) rc = boundary_primary_prepare(s, cfg, cam, s->ws->aux.st[0]);
+  if (!rc && cfg.spp_secondary_edge > 0) rc = boundary_direct_prepare(s, cfg, cam, s->ws->aux.st[1]);
+  s->ws->aux_open = true;
+  s->ws->aux_captured = capture_active();
   if (rc) return rc;
-  s->cur->bprep = true;
+  s->ws->cur->bprep = true;
   return PSDR_OK;
 }
 
 static int camera_grad_begin(psdr_scene* s, float* grad_cam, hipStream_t st, float** acc) {
   *acc = nullptr;
   if (!grad_cam) return PSDR_OK;
-  if (s->cam_acc.ensure(4ull * CAM_ACC)) return grow_failed();
-  *acc = s->cam_acc.as<float>();
+  if (s->ws->cam_acc.ensure(4ull * CAM_ACC)) return grow_failed();
+  *acc = s->ws->cam_acc.as<float>();
   return fill_f32(*acc, CAM_ACC, 0.f, st);
 }
 
@@ -722,15 +728,15 @@ int render_backward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& ca
   const bool ti = (terms & PSDR_TERM_INTERIOR) != 0;
   const bool tp = (terms & PSDR_TERM_PRIMARY) != 0 && s->n_edges > 0 && cfg.spp_primary_edge > 0;
   const bool td = (terms & PSDR_TERM_DIRECT) != 0 && s->n_edges > 0 && cfg.spp_secondary_edge > 0;
-  if (s->stats.ensure(4ull * (3 * static_cast<size_t>(s->n_vertices) + s->n_texels + 3 * s->n_lights + 64)))
+  if (s->ws->stats.ensure(4ull * (3 * static_cast<size_t>(s->n_vertices) + s->n_texels + 3 * s->n_lights + 64)))
     return grow_failed();
-  float* gv = grad_v ? grad_v : s->stats.as<float>();
-  RenderSlot* slot = s->find_slot(cfg, camd);
+  float* gv = grad_v ? grad_v : s->ws->stats.as<float>();
+  RenderSlot* slot = s->ws->find_slot(s, cfg, camd);
   if (!slot) {
     set_error(PSDR_ERR_STATE, "render_backward: no matching forward render with keep=1");
     return PSDR_ERR_STATE;
   }
-  s->cur = slot;
+  s->ws->cur = slot;
   float* acc;
   int rc0 = camera_grad_begin(s, grad_cam, st, &acc);
   if (rc0) return rc0;
@@ -750,12 +756,12 @@ int render_backward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& ca
   int rci = ti ? interior_backward(s, cfg, camd, dL_dI, grad_v, grad_t, grad_e, acc, st) : PSDR_OK;
   int rcd = PSDR_OK, rcp = PSDR_OK;
   if (td) {
-    if (!prepared) rcd = boundary_direct_prepare(s, cfg, camd, s->aux.st[1]);
-    if (!rcd) rcd = boundary_direct_finish(s, cfg, camd, dL_dI, gv, s->aux.st[1]);
+    if (!prepared) rcd = boundary_direct_prepare(s, cfg, camd, s->ws->aux.st[1]);
+    if (!rcd) rcd = boundary_direct_finish(s, cfg, camd, dL_dI, gv, s->ws->aux.st[1]);
   }
   if (tp) {
-    if (!prepared) rcp = boundary_primary_prepare(s, cfg, camd, s->aux.st[0]);
-    if (!rcp) rcp = boundary_primary_finish(s, cfg, camd, dL_dI, gv, acc, s->aux.st[0]);
+    if (!prepared) rcp = boundary_primary_prepare(s, cfg, camd, s->ws->aux.st[0]);
+    if (!rcp) rcp = boundary_primary_finish(s, cfg, camd, dL_dI, gv, acc, s->ws->aux.st[0]);
   }
   // join even after an error so no work is left running unordered (also a
   // prepass of this forward whose terms this call does not form)
@@ -777,7 +783,7 @@ int interior_backward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& 
   P.nB = cfg.mis_bsdf_samples;
   P.elv = cfg.enable_light_visable ? 1 : 0;
   P.spp = cfg.spp_interior;
-  const RenderSlot& R = *s->cur;
+  const RenderSlot& R = *s->ws->cur;
   P.Kmax = R.K;
   P.N = static_cast<int64_t>(R.N);
   const int64_t slots = 1 + static_cast<int64_t>(R.D) * R.K;
@@ -791,9 +797,9 @@ int interior_backward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& 
   P.cinfo = reinterpret_cast<const int*>(P.cL + 3 * P.N);
   P.dLdI = dL_dI;
   // scratch for absent gradient buffers
-  if (s->stats.ensure(4ull * (3 * static_cast<size_t>(s->n_vertices) + s->n_texels + 3 * s->n_lights + 64)))
+  if (s->ws->stats.ensure(4ull * (3 * static_cast<size_t>(s->n_vertices) + s->n_texels + 3 * s->n_lights + 64)))
     return grow_failed();
-  float* sink = s->stats.as<float>();
+  float* sink = s->ws->stats.as<float>();
   P.grad_v = grad_v ? grad_v : sink;
   P.grad_t = grad_t ? grad_t : sink;
   P.grad_e = grad_e ? grad_e : sink;

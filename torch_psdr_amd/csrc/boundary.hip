@@ -537,7 +537,7 @@ int cus() {
 // round over n = cells, which also accumulates the cell masses).
 static int primary_pass(psdr_scene* s, const psdr_config& cfg, const CameraK& cam, bool pre, int n, int res,
                         const float* cpmf, const float* ccmf, float* mass, hipStream_t st) {
-  RenderSlot& slot = *s->cur;
+  RenderSlot& slot = *s->ws->cur;
   const SceneView sv = s->view();
   const uint32_t nL = cfg.mis_light_samples;
   const int Kmax = static_cast<int>(nL + 1 + 1);
@@ -637,7 +637,7 @@ static int edge_lanes(uint32_t spp, const CameraK& cam, const char* what) {
 }
 
 int boundary_primary_prepare(psdr_scene* s, const psdr_config& cfg, const psdr_camera& camd, hipStream_t st) {
-  RenderSlot& slot = *s->cur;
+  RenderSlot& slot = *s->ws->cur;
   if (s->n_edges == 0) return PSDR_OK;
   const CameraK cam = make_camera(camd);
   const int R = static_cast<int>(cfg.primary_edge_hypercube_resolution);
@@ -660,7 +660,7 @@ int boundary_primary_prepare(psdr_scene* s, const psdr_config& cfg, const psdr_c
 
 int boundary_primary_finish(psdr_scene* s, const psdr_config& cfg, const psdr_camera& camd, const float* dL_dI,
                             float* grad_v, float* cam_acc, hipStream_t st) {
-  RenderSlot& slot = *s->cur;
+  RenderSlot& slot = *s->ws->cur;
   if (s->n_edges == 0) return PSDR_OK;
   const CameraK cam = make_camera(camd);
   const int n = edge_lanes(cfg.spp_primary_edge, cam, "primary boundary");
@@ -676,7 +676,7 @@ int boundary_primary_finish(psdr_scene* s, const psdr_config& cfg, const psdr_ca
 }
 
 int boundary_direct_prepare(psdr_scene* s, const psdr_config& cfg, const psdr_camera& camd, hipStream_t st) {
-  RenderSlot& slot = *s->cur;
+  RenderSlot& slot = *s->ws->cur;
   if (s->n_edges == 0) return PSDR_OK;
   const CameraK cam = make_camera(camd);
   const SceneView sv = s->view();
@@ -733,7 +733,7 @@ int boundary_direct_prepare(psdr_scene* s, const psdr_config& cfg, const psdr_ca
 
 int boundary_direct_finish(psdr_scene* s, const psdr_config& cfg, const psdr_camera& camd, const float* dL_dI,
                            float* grad_v, hipStream_t st) {
-  RenderSlot& slot = *s->cur;
+  RenderSlot& slot = *s->ws->cur;
   if (s->n_edges == 0) return PSDR_OK;
   const CameraK cam = make_camera(camd);
   const int n = edge_lanes(cfg.spp_secondary_edge, cam, "direct boundary");

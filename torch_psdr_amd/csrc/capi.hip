@@ -323,22 +323,28 @@ static int check_desc(const psdr_scene_desc* d) {
 
 using namespace psdr;
 
-RenderSlot* psdr_scene::find_slot(const psdr_config& cfg, const psdr_camera& cam) {
+static bool same_render(const RenderSlot& r, const psdr_scene* owner, const psdr_config& cfg,
+                        const psdr_camera& cam) {
+  return r.owner == owner && std::memcmp(&r.cfg, &cfg, sizeof(cfg)) == 0 &&
+         std::memcmp(&r.cam, &cam, sizeof(cam)) == 0;
+}
+
+RenderSlot* Workspace::find_slot(const psdr_scene* owner, const psdr_config& cfg, const psdr_camera& cam) {
   for (RenderSlot& r : slots)
-    if (r.have_cache && std::memcmp(&r.cfg, &cfg, sizeof(cfg)) == 0 && std::memcmp(&r.cam, &cam, sizeof(cam)) == 0) {
+    if (r.have_cache && same_render(r, owner, cfg, cam)) {
       r.used = ++clock;
       return &r;
     }
   return nullptr;
 }
 
-RenderSlot* psdr_scene::claim_slot(const psdr_config& cfg, const psdr_camera& cam) {
-  // the slot last used for (cfg, cam), valid or not (its buffers already have
+RenderSlot* Workspace::claim_slot(const psdr_scene* owner, const psdr_config& cfg, const psdr_camera& cam) {
+  // the slot last used for this render, valid or not (its buffers already have
   // the right sizes: a training loop's scene update must not move renders to a
   // new slot), else one never used, else the least recently used
   RenderSlot* best = nullptr;
   for (RenderSlot& r : slots)
-    if (r.used && std::memcmp(&r.cfg, &cfg, sizeof(cfg)) == 0 && std::memcmp(&r.cam, &cam, sizeof(cam)) == 0) {
+    if (r.used && same_render(r, owner, cfg, cam)) {
       best = &r;
       break;
     }
@@ -348,14 +354,29 @@ RenderSlot* psdr_scene::claim_slot(const psdr_config& cfg, const psdr_camera& ca
   best->used = ++clock;
   best->cfg = cfg;
   best->cam = cam;
+  best->owner = owner;
   return best;
 }
 
-void psdr_scene::invalidate_slots() {
-  for (RenderSlot& r : slots) {
-    r.have_cache = false;
-    r.bprep = false;
+void Workspace::invalidate_slots(const psdr_scene* owner) {
+  for (RenderSlot& r : slots)
+    if (r.owner == owner) {
+      r.have_cache = false;
+      r.bprep = false;
+    }
+}
+
+psdr_scene::~psdr_scene() {
+  if (!ws) return;
+  if (--ws->refs == 0) {
+    delete ws;
+    return;
   }
+  // the shared workspace forgets this scene's renders
+  ws->invalidate_slots(this);
+  for (RenderSlot& r : ws->slots)
+    if (r.owner == this) r.owner = nullptr;
+  if (ws->last_owner == this) ws->last_owner = nullptr;
 }
 
 SceneView psdr_scene::view() const {
@@ -368,7 +389,7 @@ SceneView psdr_scene::view() const {
   v.node = node.as<float4>();
   v.n_tris = n_tris;
   v.n_bvh = n_bvh;
-  v.objs = objs.as<ObjectK>();
+  v.objs = objs_dev;
   v.n_objects = n_objects;
   v.lights = lights.as<LightK>();
   v.n_lights = n_lights;
@@ -404,6 +425,27 @@ static int upload(DevBuf& b, const void* src, size_t bytes, hipStream_t st) {
   return PSDR_OK;
 }
 
+// Points the scene at the device copy of objs_host, uploading it once per
+// distinct content.
+static int bind_object_table(psdr_scene* s, hipStream_t st) {
+  const size_t bytes = sizeof(ObjectK) * s->objs_host.size();
+  for (const auto& t : s->obj_tables)
+    if (t->host.size() == s->objs_host.size() && std::memcmp(t->host.data(), s->objs_host.data(), bytes) == 0) {
+      s->objs_dev = t->dev.as<ObjectK>();
+      return PSDR_OK;
+    }
+  std::unique_ptr<psdr_scene::ObjTable> t(new (std::nothrow) psdr_scene::ObjTable());
+  if (!t) {
+    set_error(PSDR_ERR_NOMEM, "host allocation failed");
+    return PSDR_ERR_NOMEM;
+  }
+  t->host = s->objs_host;
+  PSDR_TRY(upload(t->dev, t->host.data(), bytes, st));
+  s->objs_dev = t->dev.as<ObjectK>();
+  s->obj_tables.push_back(std::move(t));
+  return PSDR_OK;
+}
+
 static ObjectK object_k(const psdr_object& b) {
   return ObjectK{b.material,  b.light,   b.albedo_offset, b.albedo_h,  b.albedo_w,
                  b.rough_offset, b.rough_h, b.rough_w,     b.tri_begin, b.tri_count,
@@ -435,7 +477,7 @@ static int scene_set_buffers(psdr_scene* s, const psdr_scene_desc* d, hipStream_
     s->objects.assign(d->objects, d->objects + d->n_objects);
     s->objs_host.resize(s->objects.size());
     for (size_t o = 0; o < s->objects.size(); ++o) s->objs_host[o] = object_k(s->objects[o]);
-    PSDR_TRY(upload(s->objs, s->objs_host.data(), sizeof(ObjectK) * s->objs_host.size(), st));
+    PSDR_TRY(bind_object_table(s, st));
   }
   s->vertices = d->vertices;
   s->texels = d->texels;
@@ -501,7 +543,7 @@ static int scene_create_impl(const psdr_scene_desc* d, psdr_scene* s, hipStream_
   const size_t tmp_floats = 2 * (wmax / 256 + 2 + static_cast<size_t>(nl) + 2) + 16;
   PSDR_TRY(s->scan_tmp.ensure(4 * (hdr.size() + wmax + tmp_floats)));
   PSDR_HIP_CHECK(hipMemcpyAsync(s->scan_tmp.p, hdr.data(), 4 * hdr.size(), hipMemcpyHostToDevice, st));
-  PSDR_TRY(upload(s->objs, s->objs_host.data(), sizeof(ObjectK) * s->objs_host.size(), st));
+  PSDR_TRY(bind_object_table(s, st));
   PSDR_TRY(upload(s->lights, lk.data(), sizeof(LightK) * lk.size(), st));
   PSDR_TRY(upload(s->edge_topo, topo.data(), 4 * topo.size(), st));
   // derived buffers
@@ -564,6 +606,13 @@ int psdr_scene_create(const psdr_scene_desc* desc, psdr_scene** out, void* strea
   }
   *out = nullptr;
   psdr_scene* s = new (std::nothrow) psdr_scene();
+  if (s) {
+    s->ws = new (std::nothrow) Workspace();
+    if (!s->ws) {
+      delete s;
+      s = nullptr;
+    }
+  }
   if (!s) {
     set_error(PSDR_ERR_NOMEM, "host allocation failed");
     return PSDR_ERR_NOMEM;
@@ -587,7 +636,7 @@ int psdr_scene_update(psdr_scene* s, const psdr_scene_desc* desc, void* stream) 
   // a boundary prepass still running on the auxiliary streams reads the scene
   PSDR_TRY(aux_join(s, as_stream(stream)));
   PSDR_TRY(scene_set_buffers(s, desc, as_stream(stream)));
-  s->invalidate_slots();
+  s->ws->invalidate_slots(s);
   return scene_derive(s, as_stream(stream));
 }
 
@@ -605,6 +654,37 @@ int psdr_scene_join(psdr_scene* s, void* stream) {
   }
   CaptureScope cap(as_stream(stream));
   return aux_join(s, as_stream(stream));
+}
+
+int psdr_scene_share_workspace(psdr_scene* s, psdr_scene* with, void* stream) {
+  if (!s || !with) {
+    set_error(PSDR_ERR_ARG, "scene is NULL");
+    return PSDR_ERR_ARG;
+  }
+  if (s->ws == with->ws) return PSDR_OK;
+  const hipStream_t st = as_stream(stream);
+  CaptureScope cap(st);
+  if (capture_active()) {
+    set_error(PSDR_ERR_CAPTURE, "share_workspace: buffers cannot change hands under graph capture");
+    return PSDR_ERR_CAPTURE;
+  }
+  PSDR_TRY(aux_join(s, st));
+  PSDR_TRY(aux_join(with, st));
+  Workspace* old = s->ws;
+  if (old->refs == 1) {
+    // work queued on any stream may still use its buffers
+    PSDR_HIP_CHECK(hipDeviceSynchronize());
+    delete old;
+  } else {
+    old->invalidate_slots(s);
+    for (RenderSlot& r : old->slots)
+      if (r.owner == s) r.owner = nullptr;
+    if (old->last_owner == s) old->last_owner = nullptr;
+    --old->refs;
+  }
+  s->ws = with->ws;
+  ++s->ws->refs;
+  return PSDR_OK;
 }
 
 int psdr_scene_info(const psdr_scene* s, int32_t* n_tris, int32_t* n_edges, int32_t* bvh_depth) {
@@ -656,7 +736,7 @@ int psdr_render_backward(psdr_scene* s, const psdr_config* cfg, const psdr_camer
     set_error(PSDR_ERR_ARG, "render_backward: bad argument");
     return PSDR_ERR_ARG;
   }
-  if (!s->find_slot(*cfg, *cam)) {
+  if (!s->ws->find_slot(s, *cfg, *cam)) {
     set_error(PSDR_ERR_STATE, "render_backward: no matching forward render with keep=1");
     return PSDR_ERR_STATE;
   }
@@ -760,8 +840,8 @@ int psdr_scene_trace(psdr_scene* s, const float* ro, const float* rd, int32_t n,
   }
   if (n == 0) return PSDR_OK;
   const int grid = trace_grid(n);
-  PSDR_TRY(s->spill.ensure(std::max(s->spill.bytes, spill_bytes(grid))));
-  return trace_rays(s->view(), ro, rd, n, nullptr, 1, 0, nullptr, 0, tri, t, u, v, s->spill.as<int>(), grid,
+  PSDR_TRY(s->ws->spill.ensure(std::max(s->ws->spill.bytes, spill_bytes(grid))));
+  return trace_rays(s->view(), ro, rd, n, nullptr, 1, 0, nullptr, 0, tri, t, u, v, s->ws->spill.as<int>(), grid,
                     as_stream(stream));
 }
 
@@ -836,28 +916,30 @@ int psdr_scene_bvh(const psdr_scene* s, uint32_t* morton, int32_t* order, int32_
 }
 
 int psdr_last_lane_radiance(const psdr_scene* s, float* out, void* stream) {
-  if (!s || !out || s->last_N == 0) {
+  if (!s || !out || s->ws->last_N == 0 || s->ws->last_owner != s) {
     set_error(PSDR_ERR_STATE, "last_lane_radiance: no forward render");
     return PSDR_ERR_STATE;
   }
-  PSDR_HIP_CHECK(hipMemcpyAsync(out, s->lane_L.p, 12ull * s->last_N, hipMemcpyDeviceToDevice, as_stream(stream)));
+  PSDR_HIP_CHECK(hipMemcpyAsync(out, s->ws->lane_L.p, 12ull * s->ws->last_N, hipMemcpyDeviceToDevice,
+                                as_stream(stream)));
   return PSDR_OK;
 }
 
 int psdr_last_render_stats(const psdr_scene* s, int64_t* counts, int32_t n_counts, int64_t* rays_traced) {
-  if (!s || s->last_N == 0) {
+  if (!s || s->ws->last_N == 0 || s->ws->last_owner != s) {
     set_error(PSDR_ERR_STATE, "last_render_stats: no forward render");
     return PSDR_ERR_STATE;
   }
   std::vector<int> c(128, 0);
-  PSDR_HIP_CHECK(hipMemcpy(c.data(), s->counters_r.p, 4 * 128, hipMemcpyDeviceToHost));
+  PSDR_HIP_CHECK(hipMemcpy(c.data(), s->ws->counters_r.p, 4 * 128, hipMemcpyDeviceToHost));
   for (int i = 0; i < n_counts && i < 64; ++i) counts[i] = c[i];
   if (rays_traced) {
     // camera rays + per depth d < D: alive paths x (light + BSDF samples, + 1
     // extension ray except at the last depth)
-    const int64_t nLB = static_cast<int64_t>(s->last_cfg.mis_light_samples) + s->last_cfg.mis_bsdf_samples;
+    const Workspace& w = *s->ws;
+    const int64_t nLB = static_cast<int64_t>(w.last_cfg.mis_light_samples) + w.last_cfg.mis_bsdf_samples;
     int64_t r = c[0];
-    for (int d = 0; d < s->last_D; ++d) r += static_cast<int64_t>(c[d + 1]) * (nLB + (d < s->last_D - 1 ? 1 : 0));
+    for (int d = 0; d < w.last_D; ++d) r += static_cast<int64_t>(c[d + 1]) * (nLB + (d < w.last_D - 1 ? 1 : 0));
     *rays_traced = r;
   }
   return PSDR_OK;

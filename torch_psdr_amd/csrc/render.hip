@@ -602,7 +602,7 @@ int run_depths(psdr_scene* s, const WaveArgs& a, const QueueView& q0, const Rays
   const int Kmax = static_cast<int>(a.nL + a.nB + 1);
   const int64_t ray_cap = static_cast<int64_t>(n) * Kmax;
   const int tgrid = trace_grid(ray_cap);
-  DevBuf& spill = a.spill ? *a.spill : s->spill;
+  DevBuf& spill = a.spill ? *a.spill : s->ws->spill;
   if (spill.ensure(std::max(spill.bytes, spill_bytes(tgrid)))) return grow_failed();
   const int sgrid = std::max(1, std::min(blocks(n, 256), num_cus() * 8));
   QueueView qin = q0;
@@ -672,28 +672,28 @@ int render_forward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& cam
   // renderD: the forward state goes to this (config, camera)'s slot
   RenderSlot* slot = nullptr;
   if (keep) {
-    slot = s->claim_slot(cfg, camd);
+    slot = s->ws->claim_slot(s, cfg, camd);
     slot->have_cache = false;
     slot->bprep = false;
-    s->cur = slot;
+    s->ws->cur = slot;
   }
   const bool prep = keep && !s->plain && s->n_edges > 0 && (cfg.spp_primary_edge > 0 || cfg.spp_secondary_edge > 0);
   if (prep) {
     // renderD: the boundary terms' dL/dI-independent work goes to the
     // auxiliary streams first (the longest chain, enqueued before this render
     // so it starts at once) and overlaps this render and the caller's loss
-    const int rca = s->aux.ensure();
+    const int rca = s->ws->aux.ensure();
     if (rca) return rca;
-    PSDR_HIP_CHECK(hipEventRecord(s->aux.fork, st));
+    PSDR_HIP_CHECK(hipEventRecord(s->ws->aux.fork, st));
     const int rcp = boundary_prepare_async(s, cfg, camd);
     if (rcp) return rcp;
   }
   // buffers
-  if (s->lane_L.ensure(12ull * N) || s->q_a.ensure(QUEUE_BYTES_PER * N) || s->q_b.ensure(QUEUE_BYTES_PER * N) ||
-      s->rays.ensure(RAY_BYTES_PER * ray_cap) || s->hits.ensure(12ull * ray_cap) || s->counters_r.ensure(4 * COUNTERS))
+  if (s->ws->lane_L.ensure(12ull * N) || s->ws->q_a.ensure(QUEUE_BYTES_PER * N) || s->ws->q_b.ensure(QUEUE_BYTES_PER * N) ||
+      s->ws->rays.ensure(RAY_BYTES_PER * ray_cap) || s->ws->hits.ensure(12ull * ray_cap) || s->ws->counters_r.ensure(4 * COUNTERS))
     return grow_failed();
   const int tgrid = trace_grid(ray_cap);
-  if (s->spill.ensure(spill_bytes(tgrid))) return grow_failed();
+  if (s->ws->spill.ensure(spill_bytes(tgrid))) return grow_failed();
   const int64_t slots = 1 + static_cast<int64_t>(D) * Kmax;
   CacheView cache{nullptr, nullptr, nullptr, N, nullptr, nullptr, nullptr, nullptr};
   if (slot) {
@@ -707,25 +707,25 @@ int render_forward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& cam
     cache.L = cache.T + 3ll * std::max(D, 1) * N;
     cache.info = reinterpret_cast<int*>(cache.L + 3ll * N);
   }
-  QueueView qa = carve_queue(s->q_a, N), qb = carve_queue(s->q_b, N);
-  RaysView rays_a = carve_rays(s->rays, s->hits, ray_cap);
+  QueueView qa = carve_queue(s->ws->q_a, N), qb = carve_queue(s->ws->q_b, N);
+  RaysView rays_a = carve_rays(s->ws->rays, s->ws->hits, ray_cap);
   // second ray set: the shade kernel reads the hits of the previous depth while
   // writing the next depth's rays; rays and hits are double-buffered by halves.
-  int* counters = s->counters_r.as<int>();
+  int* counters = s->ws->counters_r.as<int>();
   hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(COUNTERS), 0, st, counters, COUNTERS);
   const SceneView sv = s->view();
-  float* lane_L = s->lane_L.as<float>();
+  float* lane_L = s->ws->lane_L.as<float>();
 
   // stage 0: camera rays + primary hits
   hipLaunchKernelGGL(k_cam, dim3(blocks(N, 256)), dim3(256), 0, st, cam, cfg.spp_interior, N, qa, rays_a,
                      counters + 0);
   PSDR_HIP_CHECK(hipGetLastError());
   int rc = trace_rays(sv, rays_a.o, rays_a.d, N, counters + 0, 1, N, nullptr, 0, rays_a.tri, nullptr, rays_a.u,
-                      rays_a.v, s->spill.as<int>(), tgrid, st);
+                      rays_a.v, s->ws->spill.as<int>(), tgrid, st);
   if (rc) return rc;
   if (s->plain) PSDR_HIP_CHECK(hipStreamSynchronize(st));
-  if (s->rays2.ensure(RAY_BYTES_PER * ray_cap) || s->hits2.ensure(12ull * ray_cap)) return grow_failed();
-  RaysView rays_b = carve_rays(s->rays2, s->hits2, ray_cap);
+  if (s->ws->rays2.ensure(RAY_BYTES_PER * ray_cap) || s->ws->hits2.ensure(12ull * ray_cap)) return grow_failed();
+  RaysView rays_b = carve_rays(s->ws->rays2, s->ws->hits2, ray_cap);
   WaveArgs wa;
   wa.cam = cam;
   wa.D = D;
@@ -748,11 +748,12 @@ int render_forward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& cam
                                                                         : 1;
   hipLaunchKernelGGL(k_film, dim3(blocks(P_, ppb)), dim3(256), 0, st, lane_L, P_, cfg.spp_interior, cfg.n_pass, out);
   PSDR_HIP_CHECK(hipGetLastError());
-  s->last_N = N;
-  s->last_K = Kmax;
-  s->last_D = D;
-  s->last_cfg = cfg;
-  s->last_cam = camd;
+  s->ws->last_owner = s;
+  s->ws->last_N = N;
+  s->ws->last_K = Kmax;
+  s->ws->last_D = D;
+  s->ws->last_cfg = cfg;
+  s->ws->last_cam = camd;
   if (slot) {
     slot->N = N;
     slot->K = Kmax;

@@ -11,6 +11,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -144,9 +145,41 @@ struct RenderSlot {
   int K = 0, D = 0;
   psdr_config cfg{};
   psdr_camera cam{};
+  const psdr_scene* owner = nullptr;  // the scene whose render this slot holds
   uint64_t used = 0;  // last-use stamp (least recently used slot is reused first)
 };
 constexpr int RENDER_SLOTS = 4;
+
+// Per-render device buffers, forward states and auxiliary streams.  Every scene
+// starts with its own; scenes may share one (psdr_scene_share_workspace), so
+// that many scenes rendered one after another hold one set of buffers, sized
+// for the largest render.  Calls on scenes sharing a workspace are ordered on
+// the caller's stream as calls on one scene are.
+struct Workspace {
+  DevBuf lane_L, q_a, q_b, rays, hits, rays2, hits2, spill, counters_r, stats, cam_acc;
+  // forward state per (scene, config, camera); cur = the slot the current call works on
+  RenderSlot slots[RENDER_SLOTS];
+  RenderSlot* cur = &slots[0];
+  uint64_t clock = 0;
+  AuxStreams aux;
+  // work issued on the auxiliary streams and not yet joined into a caller's
+  // stream; aux_captured: issued while the caller's stream was being captured
+  bool aux_open = false, aux_captured = false;
+  // last forward render (lane radiance and statistics entry points)
+  const psdr_scene* last_owner = nullptr;
+  uint64_t last_N = 0;
+  int last_K = 0, last_D = 0;
+  psdr_config last_cfg{};
+  psdr_camera last_cam{};
+  int refs = 1;  // scenes using this workspace
+
+  // slot holding the forward render of (owner, cfg, cam), or null
+  RenderSlot* find_slot(const psdr_scene* owner, const psdr_config& cfg, const psdr_camera& cam);
+  // slot for a new forward render: its own, a free one or the least recently used
+  RenderSlot* claim_slot(const psdr_scene* owner, const psdr_config& cfg, const psdr_camera& cam);
+  // drops the forward states of one scene
+  void invalidate_slots(const psdr_scene* owner);
+};
 
 }  // namespace psdr
 
@@ -154,48 +187,43 @@ struct psdr_scene {
   int n_vertices = 0, n_tris = 0, n_objects = 0, n_lights = 0, n_texels = 0, n_edges = 0;
   int n_bvh = 0;  // triangles with edges and in the LBVH: all but an environment light's cube
   std::vector<psdr_object> objects;
-  std::vector<psdr::ObjectK> objs_host;  // source of the last object-table upload
+  std::vector<psdr::ObjectK> objs_host;  // the bound object table
   std::vector<int> light_obj;
   // caller-owned device buffers of the last create/update
   const float* vertices = nullptr;
   const float* texels = nullptr;
   const float* light_emit = nullptr;
   // topology copies
-  psdr::DevBuf faces, tri_uv, tri_obj, objs, lights, edge_topo;
+  psdr::DevBuf faces, tri_uv, tri_obj, lights, edge_topo;
+  // Device object tables, one per distinct content and never rewritten: a graph
+  // captured with one table keeps reading it while eager calls bind another
+  // (e.g. the same objects with other gradient masks).
+  struct ObjTable {
+    std::vector<psdr::ObjectK> host;
+    psdr::DevBuf dev;
+  };
+  std::vector<std::unique_ptr<ObjTable>> obj_tables;
+  const psdr::ObjectK* objs_dev = nullptr;
   // derived
   psdr::DevBuf tri_geom, edge_geom, edge_pmf, edge_cmf, light_tri_pmf, light_tri_cmf, light_power, light_area,
       lights_total, obj_pmf, obj_cmf, scan_tmp;
   // LBVH
   psdr::DevBuf morton, order, child, parent, box, leaf, node, counters, sort_keys, sort_vals, sort_hist, bounds,
       ranges, bvh_boxes;
-  // renderer scratch (interior forward, main stream)
-  psdr::DevBuf lane_L, q_a, q_b, rays, hits, rays2, hits2, spill, counters_r, stats, cam_acc;
-  // forward state per (config, camera); cur = the slot the current call works on
-  psdr::RenderSlot slots[psdr::RENDER_SLOTS];
-  psdr::RenderSlot* cur = &slots[0];
-  uint64_t clock = 0;
-  psdr::AuxStreams aux;
-  // work issued on the auxiliary streams and not yet joined into a caller's
-  // stream; aux_captured: issued while the caller's stream was being captured
-  bool aux_open = false, aux_captured = false;
+  // per-render buffers, forward states, auxiliary streams (possibly shared)
+  psdr::Workspace* ws = nullptr;
   // Reference launch shape, set for the duration of a *_plain call: every
   // depth's kernels run over all lanes (no compaction), each ray kind is traced
   // by its own launch followed by a host sync, boundary samples are processed in
   // lane order and the gradient terms run one after another.
   bool plain = false;
   int light_tri_total = 0;
-  // last forward render (lane radiance and statistics entry points)
-  uint64_t last_N = 0;
-  int last_K = 0, last_D = 0;
-  psdr_config last_cfg{};
-  psdr_camera last_cam{};
 
+  psdr_scene() = default;
+  psdr_scene(const psdr_scene&) = delete;
+  psdr_scene& operator=(const psdr_scene&) = delete;
+  ~psdr_scene();
   psdr::SceneView view() const;
-  // slot holding the forward render of (cfg, cam), or null
-  psdr::RenderSlot* find_slot(const psdr_config& cfg, const psdr_camera& cam);
-  // slot for a new forward render of (cfg, cam): its own, a free one or the least recently used
-  psdr::RenderSlot* claim_slot(const psdr_config& cfg, const psdr_camera& cam);
-  void invalidate_slots();
 };
 
 namespace psdr {

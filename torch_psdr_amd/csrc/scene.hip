@@ -654,7 +654,7 @@ int scene_derive(psdr_scene* s, hipStream_t st) {
   }
   // light power + light pick over objects
   if (nobj > 0) {
-    hipLaunchKernelGGL(k_light_power, dim3(blocks(nobj, 64)), dim3(64), 0, st, s->objs.as<ObjectK>(), nobj,
+    hipLaunchKernelGGL(k_light_power, dim3(blocks(nobj, 64)), dim3(64), 0, st, s->objs_dev, nobj,
                        s->light_emit, s->light_area.as<float>(), s->light_power.as<float>(), wbuf);
     PSDR_HIP_CHECK(hipGetLastError());
     int* cnt = segi + 2 * nl;  // holds nobj (uploaded at creation)

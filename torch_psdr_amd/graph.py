@@ -16,6 +16,10 @@ captured step is the eager step's kernels, with the eager step's results.
 Rules a captured step keeps (each is checked where it can be):
 - fn() must run eagerly once at the same sizes first (the warm-up does); a
   native buffer that would grow under capture is an error;
+- fn() builds its Scene itself, as the reference's optimisation loops do (the
+  Scene copies the parameters at construction); a Scene built before the
+  capture is derived again before every replay, and refused when it carries an
+  autograd graph;
 - no autograd graph of an earlier eager step may be alive at capture (keep
   only detached values of eager losses): its gradient accumulators would tie
   the captured backward to the eager step's stream;
@@ -55,7 +59,7 @@ class CapturedStep:
         torch.cuda.current_stream().wait_stream(side)
         torch.cuda.synchronize()
         self.graph = torch.cuda.CUDAGraph()
-        used: set = set()
+        used: dict = {}
         with ops._lock:
             ops._capture_sets.append(used)
         try:
@@ -64,8 +68,11 @@ class CapturedStep:
         finally:
             with ops._lock:
                 ops._capture_sets.remove(used)
-        # the native scenes the graph points into stay alive with it
+        # the native scenes the graph points into stay alive with it; scenes
+        # the graph renders with a binding made before the capture are derived
+        # from that binding again before each replay
         self.scenes = list(used)
+        self.prebind = [(ns, rec["flat"]) for ns, rec in used.items() if "flat" in rec]
         self._stale()
 
     def _stale(self) -> None:
@@ -78,6 +85,8 @@ class CapturedStep:
         # eager boundary work still open on a scene's auxiliary streams comes first
         for ns in self.scenes:
             ns.join()
+        for ns, flat in self.prebind:
+            ns.bind(flat)
         self.graph.replay()
         self._stale()
         return self.out

@@ -27,7 +27,7 @@ _lock = threading.Lock()
 _ids = itertools.count(1)
 _entries: Dict[int, Tuple["NativeScene", FlatScene]] = {}
 _by_topo: Dict[tuple, "NativeScene"] = {}
-_MAX_CACHED = 8
+_MAX_CACHED = 128
 # "fused" (default) or "plain": the reference's launch shape, used as the
 # measured baseline (psdr_render_forward_plain / psdr_render_backward_plain).
 # Process-wide, because autograd runs the backward on its own thread.
@@ -60,18 +60,20 @@ def _plain() -> bool:
 
 
 # Native scenes used while a HIP graph is being captured (torch_psdr_amd.graph):
-# one set per open capture.
-_capture_sets: List[set] = []
+# one dict per open capture, NativeScene -> {"flat": the binding made before
+# the capture, when the capture first used the scene without binding it anew}.
+_capture_sets: List[dict] = []
 
 
 def _capturing() -> bool:
     return torch.cuda.is_available() and torch.cuda.is_current_stream_capturing()
 
 
-def _note_capture(ns: "NativeScene") -> None:
-    if _capture_sets and _capturing():
-        with _lock:
-            _capture_sets[-1].add(ns)
+def _capture_record(ns: "NativeScene"):
+    if not (_capture_sets and _capturing()):
+        return None
+    with _lock:
+        return _capture_sets[-1].setdefault(ns, {})
 
 
 class NativeScene:
@@ -102,9 +104,18 @@ class NativeScene:
         self._bound = None
 
     def bind(self, flat: FlatScene) -> None:
-        _note_capture(self)
+        rec = _capture_record(self)
         if self.bound_to(flat):
+            if rec is not None and not rec:
+                # a graph rendering a scene bound before the capture: every
+                # replay derives that binding again first (CapturedStep)
+                if any(t.grad_fn is not None for t in (flat.vertices, flat.texels, flat.light_emit)):
+                    raise RuntimeError("a captured step must build its Scene itself: this Scene was built before "
+                                       "the capture, with an autograd graph the captured backward cannot join")
+                rec["flat"] = flat
             return
+        if rec is not None:
+            rec.setdefault("updated", True)
         keep_alive: list = []
         desc = flat.desc(keep_alive)
         N.check(self.lib.psdr_scene_update(self.handle, N.C.byref(desc), N.current_stream()))
@@ -147,6 +158,19 @@ def register(flat: FlatScene) -> int:
     with _lock:
         _entries[tok] = (ns, flat)
     return tok
+
+
+def share_workspace(scenes) -> None:
+    """Make the native handles of `scenes` (api.Scene objects) share one set of
+    per-render buffers (psdr_scene_share_workspace): scenes rendered one after
+    another then hold one set sized for the largest render, not one each."""
+    handles = []
+    for sc in scenes:
+        ns, _ = entry(sc.token)
+        if all(ns is not h for h in handles):
+            handles.append(ns)
+    for ns in handles[1:]:
+        N.check(ns.lib.psdr_scene_share_workspace(ns.handle, handles[0].handle, N.current_stream()))
 
 
 def unregister(tok: int) -> None:

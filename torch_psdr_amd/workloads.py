@@ -8,6 +8,12 @@
   C2  icosphere level 5 (20480 tris), vertices radially perturbed by N(0, 0.02)
       (seed 3), 512x512 texture, ground + light as C0; 512x512, 16 spp, depth 3,
       spp_primary_edge = spp_secondary_edge = 4 (1M boundary samples)
+  C3  64 scenes, each a geodesic sphere of N = 20 k^2 triangles with log10 N
+      uniform on [3, 5.3] (seed 4), randomly scaled in [0.5, 2] and rotated,
+      vertices radially perturbed by N(0, 0.02), a 256x256 texture, and the C0
+      ground + light; resolution uniform in {128, 256, 512, 1024}^2, spp in
+      {4, 16, 64}, depth 3; target = the unperturbed sphere with a second
+      texture; all scenes share one set of per-render buffers
   C4  1024 x 1024 height-grid terrain (2 * 1023^2 tris), heights from 6-octave
       fBm value noise (amplitude 1, seed 5), 1024^2 texture (seed 1), two quad
       area lights; 1024x1024, 64 spp, depth 4; gradients w.r.t. the heights and
@@ -51,6 +57,62 @@ def icosphere(level: int, radius: float = 1.0):
         faces = nf
     V = (np.stack(verts) * radius).astype(np.float32)
     return V, np.array(faces, dtype=np.int32)
+
+
+def _icosahedron():
+    t = (1.0 + math.sqrt(5.0)) / 2.0
+    V = np.array([(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+                  (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)], dtype=np.float64)
+    F = np.array([(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+                  (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5),
+                  (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)], dtype=np.int64)
+    return V / np.linalg.norm(V, axis=1, keepdims=True), F
+
+
+def geodesic_sphere(k: int, radius: float = 1.0):
+    """Class-I geodesic sphere of frequency k: every icosahedron face split into
+    k^2 triangles (20 k^2 in all, wound as the face), grid points projected to
+    the sphere.  A grid point shared by faces (on an icosahedron edge or vertex)
+    is one vertex.  (V, 3) float32, (F, 3) int32."""
+    base, faces = _icosahedron()
+    I, J = np.meshgrid(np.arange(k + 1), np.arange(k + 1), indexing="ij")
+    m = I + J <= k
+    i, j = I[m], J[m]
+    npts = i.size
+    local = np.full((k + 2, k + 2), -1, dtype=np.int64)
+    local[i, j] = np.arange(npts)
+    # a grid point's key: its nonzero (corner vertex, weight) pairs, sorted
+    big = 12 * (k + 1)
+    w = np.stack([k - i - j, i, j], 1)                       # (npts, 3)
+    keys = []
+    pos = []
+    for a, b, c in faces:
+        ids = np.array([a, b, c])
+        code = np.where(w > 0, ids[None, :] * (k + 1) + w, big)
+        keys.append(np.sort(code, axis=1))
+        pos.append((base[a][None, :] * w[:, :1] + base[b][None, :] * w[:, 1:2] + base[c][None, :] * w[:, 2:]) / k)
+    keys = np.concatenate(keys)
+    pos = np.concatenate(pos)
+    _, first, inv = np.unique(keys, axis=0, return_index=True, return_inverse=True)
+    inv = inv.reshape(-1)
+    V = pos[first]
+    V = V / np.linalg.norm(V, axis=1, keepdims=True) * radius
+    ii, jj = I[I + J < k], J[I + J < k]
+    up = np.stack([local[ii, jj], local[ii + 1, jj], local[ii, jj + 1]], 1)
+    di, dj = I[I + J < k - 1], J[I + J < k - 1]
+    down = np.stack([local[di + 1, dj], local[di + 1, dj + 1], local[di, dj + 1]], 1)
+    tri = np.concatenate([up, down])
+    F = np.concatenate([inv[tri + f * npts] for f in range(faces.shape[0])])
+    return V.astype(np.float32), F.astype(np.int32)
+
+
+def random_rotation(rng: np.random.Generator) -> np.ndarray:
+    """A uniformly random 3x3 rotation (QR of a Gaussian matrix, signs fixed)."""
+    q, r = np.linalg.qr(rng.standard_normal((3, 3)))
+    q = q * np.sign(np.diag(r))[None, :]
+    if np.linalg.det(q) < 0:
+        q[:, 0] = -q[:, 0]
+    return q
 
 
 def sphere_uv(V: np.ndarray) -> np.ndarray:
@@ -231,3 +293,81 @@ def c4_parts(dr, device, n=1024, res=1024, spp=64, depth=4, tex_res=1024, spp_ed
 
     return dict(heights=heights, tex=texture(1, tex_res, device), scene=scene, objects=objects, integrator=integ,
                 res=res, spp=spp, n_tris=int(F.shape[0]) + 4)
+
+
+@dataclass
+class BatchScene:
+    """One scene of the batched configuration: parameters (requires_grad), the
+    target image, the integrator, and `scene(vertices, tex)` building its Scene."""
+    vertices: torch.Tensor
+    tex: torch.Tensor
+    target: Optional[torch.Tensor]
+    integrator: object
+    scene: object
+    res: int
+    spp: int
+    n_tris: int
+
+
+def c3_specs(n_scenes=64, seed=4, res_choices=(128, 256, 512, 1024), spp_choices=(4, 16, 64),
+             log_tris=(3.0, 5.3)):
+    """Per-scene draws of C3 (one generator, seed 4, in scene order): geodesic
+    frequency k (20 k^2 ~ 10^U(log_tris)), scale, rotation, resolution, spp and
+    the generator state for the radial perturbation."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for _ in range(n_scenes):
+        n_target = 10.0 ** rng.uniform(*log_tris)
+        k = max(2, int(round(math.sqrt(n_target / 20.0))))
+        scale = float(rng.uniform(0.5, 2.0))
+        Q = random_rotation(rng)
+        res = int(rng.choice(res_choices))
+        spp = int(rng.choice(spp_choices))
+        specs.append(dict(k=k, n_tris=20 * k * k, scale=scale, rotation=Q, res=res, spp=spp,
+                          noise_seed=int(rng.integers(0, 2 ** 31))))
+    return specs
+
+
+def c3_parts(dr, device, n_scenes=64, seed=4, res_choices=(128, 256, 512, 1024), spp_choices=(4, 16, 64),
+             log_tris=(3.0, 5.3), tex_res=256, depth=3, spp_edge=1, sigma=0.02, targets=True, share=True):
+    """C3: `n_scenes` scenes of uneven size (see the module docstring), their
+    native handles sharing one set of per-render buffers."""
+    from . import ops
+    specs = c3_specs(n_scenes, seed, res_choices, spp_choices, log_tris)
+    Vg, Fg = quad(10.0, -1.0)
+    Vl, Fl = quad(2.0, 4.0, facing_down=True)
+    ground_v, ground_f = torch.tensor(Vg, device=device), torch.tensor(Fg, device=device)
+    ground_c = torch.tensor([0.5] * 3, device=device)
+    light_v, light_f = torch.tensor(Vl, device=device), torch.tensor(Fl, device=device)
+    emit = torch.tensor([10.0] * 3, device=device)
+    out = []
+    for si, sp in enumerate(specs):
+        V, F = geodesic_sphere(sp["k"])
+        res, spp = sp["res"], sp["spp"]
+        r = 1.0 + sigma * np.random.default_rng(sp["noise_seed"]).standard_normal(V.shape[0])
+        V0 = (V.astype(np.float64) @ sp["rotation"].T) * sp["scale"]
+        uv = torch.tensor(sphere_uv(V), device=device)   # the texture moves with the sphere
+        faces = torch.tensor(F, device=device)
+        cam = camera(dr, device, res, res)
+        integ = dr.PathIntegrator(spp_interior=spp, max_bounce=depth, spp_primary_edge=spp_edge,
+                                  spp_secondary_edge=spp_edge)
+
+        def scene(vertices, tex, uv=uv, faces=faces, cam=cam):
+            sphere = dr.Mesh(vertices, uv, faces, faces, dr.DiffuseMaterial(tex))
+            ground = dr.Mesh(ground_v, None, ground_f, None, dr.DiffuseMaterial(ground_c))
+            light = dr.AreaLight(light_v, light_f, emit)
+            return dr.Scene([cam], [sphere, ground], [light])
+
+        v_opt = torch.tensor((V0 * r[:, None]).astype(np.float32), device=device)
+        tex_opt = texture(1000 + 2 * si, tex_res, device)
+        target = None
+        if targets:
+            with torch.no_grad():
+                target = integ.renderC(scene(torch.tensor(V0.astype(np.float32), device=device),
+                                             texture(1001 + 2 * si, tex_res, device)))[0]
+        out.append(BatchScene(v_opt.requires_grad_(True), tex_opt.requires_grad_(True), target, integ, scene,
+                              res, spp, int(F.shape[0]) + 4))
+    # one set of per-render buffers for all scenes, sized by the largest render
+    if share:
+        ops.share_workspace([b.scene(b.vertices, b.tex) for b in out])
+    return out
