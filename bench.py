@@ -14,7 +14,9 @@ The step runs eagerly (warm-up, the timed steps with the traversal kernel's
 launches timed for the roofline) and is then captured into a HIP graph
 (torch_psdr_amd.graph.CapturedStep) whose replays are timed the same way; by
 default the replay time is the reported step time, the eager one is given
-beside it.
+beside it.  Before the timed region the autotuner (torch_psdr_amd.tune) picks
+the launch shapes by timing this step (reported under "autotune"; --no-autotune
+keeps the defaults).
 
 Prints one JSON line.  value = Mrays/s of reference-equivalent rays:
 R*N (N = spp*H*W, R = 1 + D*(L+B+1) = 10) + 20 rays per primary-boundary sample
@@ -44,6 +46,8 @@ def main():
     ap.add_argument("--res", type=int, default=512)
     ap.add_argument("--spp", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="time the default launch shapes instead of the autotuner's choice")
     ap.add_argument("--plain-steps", type=int, default=3,
                     help="steps of the reference-shaped baseline path to time (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU-baseline sample time")
@@ -101,6 +105,17 @@ def main():
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
+    tuned = None
+    if not args.no_autotune:
+        # launch shapes (traversal grid and LDS stack depth, shading / adjoint /
+        # boundary grids) chosen by timing this step, before the timed region
+        from torch_psdr_amd import tune
+        t_tune = time.perf_counter()
+        res_t = tune.autotune(step, repeats=3, rounds=2)
+        tuned = {"tuning": res_t["tuning"], "default": res_t["start"]["tuning"],
+                 "default_ms": round(res_t["start"]["ms"], 3), "tuned_ms": round(res_t["ms"], 3),
+                 "trials": len(res_t["trials"]), "seconds": round(time.perf_counter() - t_tune, 1)}
+        print(f"autotune: {tuned}", file=sys.stderr, flush=True)
     # traversal launches are bracketed by HIP events on the library's stream
     N.check(N.load().psdr_trace_timing(1, None, None, None))
     t0 = torch.cuda.Event(enable_timing=True)
@@ -162,6 +177,7 @@ def main():
         "cpu_baseline": cpu,
         "plain_gpu_path": plain,
         "graph_step": graph,
+        "autotune": tuned,
     }
     print(json.dumps(out))
 

@@ -293,6 +293,26 @@ int psdr_last_render_stats(const psdr_scene* scene, int64_t* counts, int32_t n_c
  * with t, hit written per ray).  Process-wide; not for concurrent callers. */
 int psdr_trace_timing(int32_t enable, double* total_ms, int64_t* launches, int64_t* bytes);
 
+/* Launch shapes of the hot kernels, chosen at run time by the autotuner
+ * (torch_psdr_amd/tune.py) or by hand.  No counterpart in the reference, whose
+ * launch shapes belong to OptiX and Enoki (src/scene_optix.cc:267,
+ * src/path_integrator.cc:80-186).  Process-wide: a change applies to launches
+ * issued afterwards (a captured graph keeps the shapes it was captured with)
+ * and must not race with renders on other threads.  Results do not depend on
+ * the shapes: images are bit-identical, gradients differ only by the order of
+ * their float atomics. */
+typedef struct psdr_tuning {
+  int32_t trace_blocks_per_cu;   /* closest-hit kernel grid, blocks of 256 per CU, 1..64 (default 8) */
+  int32_t trace_stack_lds;       /* traversal stack entries per lane kept in LDS: 8, 16 or 32 (default 16) */
+  int32_t shade_blocks_per_cu;   /* wavefront shading kernel grid, 1..64 (default 8) */
+  int32_t adjoint_blocks_per_cu; /* path-replay adjoint kernel grid, 1..64 (default 16) */
+  int32_t direct_blocks_per_cu;  /* direct-boundary kernel grid, 1..64 (default 8) */
+} psdr_tuning;
+
+int psdr_get_tuning(psdr_tuning* out);
+/* PSDR_ERR_ARG (nothing changed) when a field is outside its range. */
+int psdr_set_tuning(const psdr_tuning* in);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,6 +1,8 @@
 """The C-ABI library loads and exports every entry point include/psdr_hip.h declares (no GPU calls)."""
 import ctypes
 import re
+
+import pytest
 from pathlib import Path
 
 import torch_psdr_amd._native as N
@@ -43,3 +45,30 @@ def test_argument_errors_without_gpu():
     assert b"NULL" in lib.psdr_last_error()
     cam = N.PsdrCamera()
     assert lib.psdr_camera_matrices(ctypes.byref(cam), None, None, None, None, None) == -1  # height 0
+
+
+def test_tuning_roundtrip_and_validation(tmp_path):
+    from torch_psdr_amd import tune
+    assert ctypes.sizeof(N.PsdrTuning) == 20
+    prev = tune.get_tuning()
+    assert prev == tune.DEFAULTS
+    try:
+        tune.set_tuning(trace_blocks_per_cu=4, trace_stack_lds=32)
+        assert tune.get_tuning() == dict(tune.DEFAULTS, trace_blocks_per_cu=4, trace_stack_lds=32)
+        for bad in ({"trace_stack_lds": 12}, {"shade_blocks_per_cu": 0}, {"adjoint_blocks_per_cu": 65}):
+            with pytest.raises(RuntimeError):
+                tune.set_tuning(**bad)
+            # a rejected call changes nothing
+            assert tune.get_tuning() == dict(tune.DEFAULTS, trace_blocks_per_cu=4, trace_stack_lds=32)
+        with pytest.raises(KeyError):
+            tune.set_tuning(stack=8)
+        path = tmp_path / "tuning.json"
+        tune.save("dev|w1", dict(tune.DEFAULTS, direct_blocks_per_cu=2), path)
+        tune.save("dev|w2", dict(tune.DEFAULTS, shade_blocks_per_cu=16), path)
+        assert tune.load("missing", path) is None
+        got = tune.load("dev|w1", path)
+        assert got == dict(tune.DEFAULTS, direct_blocks_per_cu=2) == tune.get_tuning()
+        assert tune.load("dev|w2", path, apply=False)["shade_blocks_per_cu"] == 16
+        assert tune.get_tuning()["shade_blocks_per_cu"] == tune.DEFAULTS["shade_blocks_per_cu"]
+    finally:
+        tune.set_tuning(prev)

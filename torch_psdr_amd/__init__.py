@@ -8,7 +8,8 @@ from .api import (AreaLight, BaseCamera, DiffuseBsdfMaterial, DiffuseMaterial, E
                   OrthographicCamera, PathIntegrator, PerspectiveCamera, PrincipledMaterial, Scene, SH9Light,
                   Texture, VarContex)
 from .io import load_obj, read_mtl, read_obj
+from .tune import autotune, get_tuning, set_tuning
 
 __all__ = ["AreaLight", "BaseCamera", "DiffuseBsdfMaterial", "DiffuseMaterial", "EnvLight", "Light", "Material",
            "Mesh", "OrthographicCamera", "PathIntegrator", "PerspectiveCamera", "PrincipledMaterial", "Scene",
-           "SH9Light", "Texture", "VarContex", "load_obj", "read_mtl", "read_obj"]
+           "SH9Light", "Texture", "VarContex", "autotune", "get_tuning", "load_obj", "read_mtl", "read_obj", "set_tuning"]

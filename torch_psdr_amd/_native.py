@@ -107,6 +107,17 @@ class PsdrCamera(C.Structure):
     ]
 
 
+class PsdrTuning(C.Structure):
+    """Launch shapes (include/psdr_hip.h psdr_tuning)."""
+    _fields_ = [
+        ("trace_blocks_per_cu", C.c_int32),
+        ("trace_stack_lds", C.c_int32),
+        ("shade_blocks_per_cu", C.c_int32),
+        ("adjoint_blocks_per_cu", C.c_int32),
+        ("direct_blocks_per_cu", C.c_int32),
+    ]
+
+
 _VP = C.c_void_p
 _I32 = C.c_int32
 _SIGS = {
@@ -135,6 +146,8 @@ _SIGS = {
     "psdr_last_lane_radiance": (C.c_int, [_VP, _VP, _VP]),
     "psdr_last_render_stats": (C.c_int, [_VP, C.POINTER(C.c_int64), _I32, C.POINTER(C.c_int64)]),
     "psdr_trace_timing": (C.c_int, [_I32, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "psdr_get_tuning": (C.c_int, [C.POINTER(PsdrTuning)]),
+    "psdr_set_tuning": (C.c_int, [C.POINTER(PsdrTuning)]),
 }
 
 EXPORTED = tuple(_SIGS)

@@ -806,7 +806,7 @@ int interior_backward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& 
   P.cam_acc = cam_acc;
   const int hot = 3 * s->n_lights + 4 * s->n_objects;
   P.hot_n = hot <= HOT_MAX ? hot : 0;
-  const int grid = std::max(1, std::min(blocks(P.N, ADJ_THREADS), 256 * 16));
+  const int grid = std::max(1, std::min(blocks(P.N, ADJ_THREADS), num_cus() * tuning().adjoint_bpc.load()));
   const bool env = s->n_bvh < s->n_tris;  // an environment light's cube
   if (P.D <= 4) {
     if (env) hipLaunchKernelGGL((k_adjoint<4, true>), dim3(grid), dim3(ADJ_THREADS), 0, st, P);

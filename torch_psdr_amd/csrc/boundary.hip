@@ -691,7 +691,7 @@ int boundary_direct_prepare(psdr_scene* s, const psdr_config& cfg, const psdr_ca
   const int n = edge_lanes(cfg.spp_secondary_edge, cam, "direct boundary");
   if (n < 0) return PSDR_ERR_ARG;
   const int cells = static_cast<int>(cells64);
-  const int grid = std::max(1, std::min(blocks(std::max(cells, n), BT), cus() * 8));
+  const int grid = std::max(1, std::min(blocks(std::max(cells, n), BT), cus() * tuning().direct_bpc.load()));
   if (slot.d_state.ensure(8ull * cells) || slot.d_mass.ensure(4ull * cells) || slot.d_pmf.ensure(4ull * cells) ||
       slot.d_cmf.ensure(4ull * cells) || slot.d_tmp.ensure(4ull * (2 * (cells / 256 + 2) + 16)) ||
       slot.d_spill.ensure(static_cast<size_t>(grid) * BT * (STACK_TOTAL - BSTACK) * 4) ||

@@ -849,6 +849,45 @@ int psdr_trace_timing(int32_t enable, double* total_ms, int64_t* launches, int64
   return trace_timing(enable, total_ms, launches, bytes);
 }
 
+int psdr_get_tuning(psdr_tuning* out) {
+  if (!out) {
+    set_error(PSDR_ERR_ARG, "get_tuning: output is NULL");
+    return PSDR_ERR_ARG;
+  }
+  const Tuning& t = tuning();
+  out->trace_blocks_per_cu = t.trace_bpc.load();
+  out->trace_stack_lds = t.trace_stack.load();
+  out->shade_blocks_per_cu = t.shade_bpc.load();
+  out->adjoint_blocks_per_cu = t.adjoint_bpc.load();
+  out->direct_blocks_per_cu = t.direct_bpc.load();
+  return PSDR_OK;
+}
+
+int psdr_set_tuning(const psdr_tuning* in) {
+  if (!in) {
+    set_error(PSDR_ERR_ARG, "set_tuning: input is NULL");
+    return PSDR_ERR_ARG;
+  }
+  const int bpc[4] = {in->trace_blocks_per_cu, in->shade_blocks_per_cu, in->adjoint_blocks_per_cu,
+                      in->direct_blocks_per_cu};
+  for (int v : bpc)
+    if (v < 1 || v > TUNE_BPC_MAX) {
+      set_error(PSDR_ERR_ARG, "set_tuning: blocks per CU must lie in 1..64");
+      return PSDR_ERR_ARG;
+    }
+  if (!trace_stack_ok(in->trace_stack_lds)) {
+    set_error(PSDR_ERR_ARG, "set_tuning: trace_stack_lds must be 8, 16 or 32");
+    return PSDR_ERR_ARG;
+  }
+  Tuning& t = tuning();
+  t.trace_bpc = in->trace_blocks_per_cu;
+  t.trace_stack = in->trace_stack_lds;
+  t.shade_bpc = in->shade_blocks_per_cu;
+  t.adjoint_bpc = in->adjoint_blocks_per_cu;
+  t.direct_bpc = in->direct_blocks_per_cu;
+  return PSDR_OK;
+}
+
 int psdr_scene_derived(const psdr_scene* s, float* tri7, int32_t* edge_topo, float* edge_geom, void* stream) {
   if (!s) {
     set_error(PSDR_ERR_ARG, "scene is NULL");

@@ -1,9 +1,22 @@
 // Host-side entry points of the renderer kernels (render.hip, adjoint.hip).
 #pragma once
 
+#include <atomic>
+
 #include "scene.h"
 
 namespace psdr {
+// Launch shapes chosen at run time (psdr_set_tuning), process-wide: grids in
+// blocks per CU, and the traversal stack entries kept in LDS per lane.
+struct Tuning {
+  std::atomic<int> trace_bpc{8}, trace_stack{16}, shade_bpc{8}, adjoint_bpc{16}, direct_bpc{8};
+};
+Tuning& tuning();
+constexpr int TUNE_BPC_MAX = 64;
+// LDS stack depths with a k_trace instance; spill areas are sized for the smallest
+constexpr int TRACE_STACK_MIN = 8;
+bool trace_stack_ok(int entries);
+int num_cus();
 int trace_rays(const SceneView& sv, const float* ro, const float* rd, int n, const int* count_dev, int K, int seg,
                const float* tl, int nvis, int* tri, float* t, float* u, float* v, int* spill, int grid,
                hipStream_t st);

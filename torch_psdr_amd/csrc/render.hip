@@ -34,7 +34,6 @@ namespace {
 constexpr int TRACE_THREADS = 256;
 // render counters: [d] paths entering shade(d)
 constexpr int COUNTERS = 128;
-constexpr int STACK_LDS = 16;
 
 __global__ void k_lane_draws(uint32_t lane0, int n_lanes, int n_draws, uint32_t* out) {
   const int l = blockIdx.x * blockDim.x + threadIdx.x;
@@ -58,6 +57,8 @@ __global__ void k_camera_rays(CameraK cam, uint32_t spp, int64_t N, float* o, fl
 
 // Closest hit of rays [0, *count) (or n if count is null); rays SoA-of-xyz
 // AoS (ro[3*i+k]); outputs per ray.
+// STACK_LDS: traversal stack entries per lane kept in LDS (the rest spill)
+template <int STACK_LDS>
 __global__ __launch_bounds__(TRACE_THREADS) void k_trace(SceneView sv, const float* __restrict__ ro,
                                                          const float* __restrict__ rd, int n,
                                                          const int* __restrict__ count, int K, int seg,
@@ -433,6 +434,9 @@ __global__ __launch_bounds__(256) void k_film(const float* __restrict__ lane_L, 
 inline int blocks(int64_t n, int t) { return static_cast<int>((n + t - 1) / t); }
 
 int g_cus = 0;
+
+}  // namespace
+
 int num_cus() {
   if (g_cus == 0) {
     int dev = 0;
@@ -444,7 +448,12 @@ int num_cus() {
   return g_cus;
 }
 
-}  // namespace
+Tuning& tuning() {
+  static Tuning t;
+  return t;
+}
+
+bool trace_stack_ok(int entries) { return entries == 8 || entries == 16 || entries == 32; }
 
 QueueView carve_queue(DevBuf& b, int cap) {
   QueueView q;
@@ -524,8 +533,21 @@ int trace_rays(const SceneView& sv, const float* ro, const float* rd, int n, con
   const bool timed = tt.on && tt.used < TIMER_PAIRS;
   if (tt.on && !timed) ++tt.dropped;
   if (timed) PSDR_HIP_CHECK(hipEventRecord(tt.ev[2 * tt.used], st));
-  hipLaunchKernelGGL(k_trace, dim3(grid), dim3(TRACE_THREADS), 0, st, sv, ro, rd, n, count_dev, K, seg, tl, nvis, tri,
-                     t, u, v, spill, timed ? tt.bytes : nullptr);
+  unsigned long long* tb = timed ? tt.bytes : nullptr;
+  switch (tuning().trace_stack.load()) {
+    case 8:
+      hipLaunchKernelGGL(k_trace<8>, dim3(grid), dim3(TRACE_THREADS), 0, st, sv, ro, rd, n, count_dev, K, seg, tl, nvis,
+                         tri, t, u, v, spill, tb);
+      break;
+    case 32:
+      hipLaunchKernelGGL(k_trace<32>, dim3(grid), dim3(TRACE_THREADS), 0, st, sv, ro, rd, n, count_dev, K, seg, tl,
+                         nvis, tri, t, u, v, spill, tb);
+      break;
+    default:
+      hipLaunchKernelGGL(k_trace<16>, dim3(grid), dim3(TRACE_THREADS), 0, st, sv, ro, rd, n, count_dev, K, seg, tl,
+                         nvis, tri, t, u, v, spill, tb);
+      break;
+  }
   PSDR_HIP_CHECK(hipGetLastError());
   if (timed) {
     PSDR_HIP_CHECK(hipEventRecord(tt.ev[2 * tt.used + 1], st));
@@ -575,11 +597,13 @@ int trace_timing(int enable, double* total_ms, int64_t* launches, int64_t* bytes
 }
 
 int trace_grid(int64_t max_rays) {
-  const int g = std::max(1, std::min(blocks(max_rays, TRACE_THREADS), num_cus() * 8));
+  const int g = std::max(1, std::min(blocks(max_rays, TRACE_THREADS), num_cus() * tuning().trace_bpc.load()));
   return g;
 }
 
-size_t spill_bytes(int grid) { return static_cast<size_t>(grid) * TRACE_THREADS * (STACK_TOTAL - STACK_LDS) * 4; }
+size_t spill_bytes(int grid) {
+  return static_cast<size_t>(grid) * TRACE_THREADS * (STACK_TOTAL - TRACE_STACK_MIN) * 4;
+}
 
 int lane_draws(uint32_t lane0, int n_lanes, int n_draws, uint32_t* out, hipStream_t st) {
   if (n_lanes <= 0) return PSDR_OK;
@@ -604,7 +628,7 @@ int run_depths(psdr_scene* s, const WaveArgs& a, const QueueView& q0, const Rays
   const int tgrid = trace_grid(ray_cap);
   DevBuf& spill = a.spill ? *a.spill : s->ws->spill;
   if (spill.ensure(std::max(spill.bytes, spill_bytes(tgrid)))) return grow_failed();
-  const int sgrid = std::max(1, std::min(blocks(n, 256), num_cus() * 8));
+  const int sgrid = std::max(1, std::min(blocks(n, 256), num_cus() * tuning().shade_bpc.load()));
   QueueView qin = q0;
   RaysView rin = r0;
   for (int d = 0; d <= a.D; ++d) {
