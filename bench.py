@@ -48,6 +48,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="time the default launch shapes instead of the autotuner's choice")
+    ap.add_argument("--tuning", default=None,
+                    help="launch shapes to time instead of tuning, e.g. trace_blocks_per_cu=16,direct_blocks_per_cu=16")
     ap.add_argument("--plain-steps", type=int, default=3,
                     help="steps of the reference-shaped baseline path to time (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU-baseline sample time")
@@ -106,7 +108,11 @@ def main():
         step()
     torch.cuda.synchronize()
     tuned = None
-    if not args.no_autotune:
+    if args.tuning:
+        from torch_psdr_amd import tune
+        tune.set_tuning({k: int(v) for k, v in (kv.split("=") for kv in args.tuning.split(","))})
+        tuned = {"tuning": tune.get_tuning(), "given": True}
+    elif not args.no_autotune:
         # launch shapes (traversal grid and LDS stack depth, shading / adjoint /
         # boundary grids) chosen by timing this step, before the timed region
         from torch_psdr_amd import tune
@@ -279,7 +285,7 @@ def plain_baseline(torch, ops, step, verts, tex, n_steps, rays):
 # workload (the 18 traversal launches of one step; FETCH_SIZE x 2.0 and
 # WRITE_SIZE x 1.0, the factors a 1 GiB device copy reads for those counters);
 # the commands and per-launch table are in profiles/k_trace_gradient_step_512px_16spp.md.
-TRACE_TRAFFIC_PER_LAUNCH = 62857979
+TRACE_TRAFFIC_PER_LAUNCH = 62707567
 
 
 def trace_roofline(total_ms, launches, bytes_, default_workload):
