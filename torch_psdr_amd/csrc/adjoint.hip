@@ -698,9 +698,10 @@ int boundary_prepare_async(psdr_scene* s, const psdr_config& cfg, const psdr_cam
   if (s->n_edges == 0 || s->plain || !s->ws->aux.fork) return PSDR_OK;
   for (int i = 0; i < 2; ++i) PSDR_HIP_CHECK(hipStreamWaitEvent(s->ws->aux.st[i], s->ws->aux.fork, 0));
   int rc = PSDR_OK;
-  // the primary chain is the longest: enqueue it first
-  if (cfg.spp_primary_edge > 0) rc = boundary_primary_prepare(s, cfg, cam, s->ws->aux.st[0]);
-  if (!rc && cfg.spp_secondary_edge > 0) rc = boundary_direct_prepare(s, cfg, cam, s->ws->aux.st[1]);
+  // the direct chain first: a few launches, one of them long, so it is queued
+  // on the GPU at once; the primary chain is a long run of launches after it
+  if (cfg.spp_secondary_edge > 0) rc = boundary_direct_prepare(s, cfg, cam, s->ws->aux.st[1]);
+  if (!rc && cfg.spp_primary_edge > 0) rc = boundary_primary_prepare(s, cfg, cam, s->ws->aux.st[0]);
   s->ws->aux_open = true;
   s->ws->aux_captured = capture_active();
   if (rc) return rc;

@@ -704,13 +704,14 @@ int render_forward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& cam
   const bool prep = keep && !s->plain && s->n_edges > 0 && (cfg.spp_primary_edge > 0 || cfg.spp_secondary_edge > 0);
   if (prep) {
     // renderD: the boundary terms' dL/dI-independent work goes to the
-    // auxiliary streams first (the longest chain, enqueued before this render
-    // so it starts at once) and overlaps this render and the caller's loss
+    // auxiliary streams, forked here so that it depends on nothing of this
+    // render and overlaps it and the caller's loss.  It is enqueued after this
+    // render's own launches (below): kernels start in the order the host
+    // submits them, and the interior chain (render, loss, adjoint) is the
+    // step's longest, so it must not wait behind the boundary chains' launches.
     const int rca = s->ws->aux.ensure();
     if (rca) return rca;
     PSDR_HIP_CHECK(hipEventRecord(s->ws->aux.fork, st));
-    const int rcp = boundary_prepare_async(s, cfg, camd);
-    if (rcp) return rcp;
   }
   // buffers
   if (s->ws->lane_L.ensure(12ull * N) || s->ws->q_a.ensure(QUEUE_BYTES_PER * N) || s->ws->q_b.ensure(QUEUE_BYTES_PER * N) ||
@@ -772,6 +773,10 @@ int render_forward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& cam
                                                                         : 1;
   hipLaunchKernelGGL(k_film, dim3(blocks(P_, ppb)), dim3(256), 0, st, lane_L, P_, cfg.spp_interior, cfg.n_pass, out);
   PSDR_HIP_CHECK(hipGetLastError());
+  if (prep) {
+    const int rcp = boundary_prepare_async(s, cfg, camd);
+    if (rcp) return rcp;
+  }
   s->ws->last_owner = s;
   s->ws->last_N = N;
   s->ws->last_K = Kmax;
