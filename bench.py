@@ -315,7 +315,7 @@ def plain_baseline(torch, ops, step, verts, tex, n_steps, rays):
 # workload (the 18 traversal launches of one step; FETCH_SIZE x 2.0 and
 # WRITE_SIZE x 1.0, the factors a 1 GiB device copy reads for those counters);
 # the commands and per-launch table are in profiles/k_trace_gradient_step_512px_16spp.md.
-TRACE_TRAFFIC_PER_LAUNCH = 62707567
+TRACE_TRAFFIC_PER_LAUNCH = 63579410
 
 
 def trace_roofline(total_ms, launches, bytes_, default_workload):
