@@ -96,10 +96,15 @@ def main():
     verts = parts["Vp"].clone().requires_grad_(True)
     tex = parts["tex_opt"].clone().requires_grad_(True)
 
+    last = {}
+
     def step():
         verts.grad = None
         tex.grad = None
         scene = parts["scene"](verts, tex)
+        # the native handle only: holding the Scene would keep this step's
+        # autograd graph alive into the graph capture below
+        last["ns"] = ops.entry(scene.token)[0]
         img = integ.renderD(scene)[0]
         loss = ((img - target) ** 2).mean()
         loss.backward()
@@ -150,6 +155,14 @@ def main():
     mrays = rays / (ms * 1e-3) / 1e6
 
     roof = trace_roofline(tr_ms.value, tr_launch.value, tr_bytes.value, default_workload=(res, spp) == (512, 16))
+    # rays the interior render actually traced (after compaction; the last depth's
+    # extension is not traced), beside the reference-equivalent R*N
+    ns = last.pop("ns")
+    counts = (ctypes.c_int64 * (D + 2))()
+    traced = ctypes.c_int64()
+    N.check(N.load().psdr_last_render_stats(ns.handle, counts, D + 2, ctypes.byref(traced)))
+    interior_rays = {"reference_equivalent": R * N_int, "traced": int(traced.value),
+                     "paths_per_depth": [int(c) for c in counts]}
     graph = None
     if args.graph != "off":
         graph = graph_step_timing(torch, step, args.steps, rays)
@@ -177,7 +190,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"vertex+texture gradient step, icosphere L5 (20480 tris) + ground + light, "
                                f"{res}x{res}, {spp} spp, depth 3, edge spp {spe}",
-                   "rays_per_step": rays, "loss": float(loss.detach().cpu()),
+                   "rays_per_step": rays, "interior_rays": interior_rays, "loss": float(loss.detach().cpu()),
                    "grad_abs_max": {"vertices": float(verts.grad.abs().max()), "texels": float(tex.grad.abs().max())}},
         "roofline": roof,
         "step_roofline": {"bound": "hbm", "achieved": round(step_bytes / (ms * 1e-3) / 1e9, 2), "peak": HBM_PEAK_GBS,
