@@ -374,18 +374,29 @@ def c3_parts(dr, device, n_scenes=64, seed=4, res_choices=(128, 256, 512, 1024),
                                              texture(1001 + 2 * si, tex_res, device)))[0]
         out.append(BatchScene(v_opt.requires_grad_(True), tex_opt.requires_grad_(True), target, integ, scene,
                               res, spp, int(F.shape[0]) + 4))
-    if small_groups > 0:
-        loads = [0] * small_groups
-        small = [b for b in out if b.res * b.res * b.spp <= small_lanes]
-        for b in sorted(small, key=lambda b: -b.res * b.res * b.spp):
-            b.group = loads.index(min(loads))
-            loads[b.group] += b.res * b.res * b.spp
+    for b, g in zip(out, assign_groups([b.res * b.res * b.spp for b in out], small_groups, small_lanes)):
+        b.group = g
     # one set of per-render buffers per group (all scenes when there are no
     # groups), sized by the group's largest render
     if share:
         for g in sorted({b.group for b in out}):
             ops.share_workspace([b.scene(b.vertices, b.tex) for b in out if b.group == g])
     return out
+
+
+def assign_groups(lanes, n_groups, small_lanes):
+    """Group of each scene given its lane count: -1 for scenes above
+    small_lanes (or all, when n_groups <= 0); the others dealt largest first
+    to the group with the fewest lanes so far (ties: the lowest group)."""
+    groups = [-1] * len(lanes)
+    if n_groups <= 0:
+        return groups
+    loads = [0] * n_groups
+    for i in sorted((i for i, n in enumerate(lanes) if n <= small_lanes), key=lambda i: (-lanes[i], i)):
+        g = loads.index(min(loads))
+        groups[i] = g
+        loads[g] += lanes[i]
+    return groups
 
 
 def c3_streams(parts, device):
