@@ -59,9 +59,6 @@ def main():
     ap.add_argument("--workload", choices=("c2", "c3"), default="c2",
                     help="c2: the 512^2 / 16 spp vertex + texture step (default); c3: 64 scenes of uneven size, "
                          "rendered and differentiated in turn on shared buffers")
-    ap.add_argument("--c3-groups", type=int, default=0,
-                    help="c3: streams that the small scenes (<= 262144 lanes) run on beside the large ones (0: all in "
-                         "turn on one stream; measured no faster: the large scenes keep every CU busy)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device-to-device copy (scales rocprof FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -131,7 +128,7 @@ def main():
                  "trials": len(res_t["trials"]), "seconds": round(time.perf_counter() - t_tune, 1)}
         print(f"autotune: {tuned}", file=sys.stderr, flush=True)
     # traversal launches are bracketed by HIP events on the library's stream
-    N.check(N.load().psdr_trace_timing(1, None, None, None))
+    N.check(N.load().psdr_trace_timing(1, None, None, None, None))
     t0 = torch.cuda.Event(enable_timing=True)
     t1 = torch.cuda.Event(enable_timing=True)
     wall0 = time.perf_counter()
@@ -141,8 +138,9 @@ def main():
     t1.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - wall0
-    tr_ms, tr_launch, tr_bytes = ctypes.c_double(), ctypes.c_int64(), ctypes.c_int64()
-    N.check(N.load().psdr_trace_timing(0, ctypes.byref(tr_ms), ctypes.byref(tr_launch), ctypes.byref(tr_bytes)))
+    tr_ms, tr_launch, tr_bytes, tr_busy = ctypes.c_double(), ctypes.c_int64(), ctypes.c_int64(), ctypes.c_double()
+    N.check(N.load().psdr_trace_timing(0, ctypes.byref(tr_ms), ctypes.byref(tr_launch), ctypes.byref(tr_bytes),
+                                       ctypes.byref(tr_busy)))
     ms = t0.elapsed_time(t1) / args.steps
     ms = max(ms, wall * 1e3 / args.steps)
 
@@ -154,7 +152,9 @@ def main():
     rays = R * N_int + 20 * N_e + 3 * N_e
     mrays = rays / (ms * 1e-3) / 1e6
 
-    roof = trace_roofline(tr_ms.value, tr_launch.value, tr_bytes.value, default_workload=(res, spp) == (512, 16))
+    from torch_psdr_amd import tune
+    roof = trace_roofline(tr_busy.value, tr_ms.value, tr_launch.value, tr_bytes.value,
+                          default_workload=(res, spp) == (512, 16), shape=tune.get_tuning())
     # rays the interior render actually traced (after compaction; the last depth's
     # extension is not traced), beside the reference-equivalent R*N
     ns = last.pop("ns")
@@ -210,11 +210,8 @@ def bench_c3(args, torch, dr, W, dev):
     term): one step renders and differentiates every scene in turn (vertex +
     texel gradients), all scenes sharing one set of per-render buffers."""
     t_setup = time.perf_counter()
-    parts = W.c3_parts(dr, dev, small_groups=args.c3_groups)
-    streams = W.c3_streams(parts, dev)
-    print(f"c3: {len(parts)} scenes set up in {time.perf_counter() - t_setup:.1f} s "
-          f"({sum(b.group >= 0 for b in parts)} small scenes in {len(streams)} stream groups)",
-          file=sys.stderr, flush=True)
+    parts = W.c3_parts(dr, dev)
+    print(f"c3: {len(parts)} scenes set up in {time.perf_counter() - t_setup:.1f} s", file=sys.stderr, flush=True)
 
     def scene_step(b):
         b.vertices.grad = None
@@ -222,11 +219,8 @@ def bench_c3(args, torch, dr, W, dev):
         img = b.integrator.renderD(b.scene(b.vertices, b.tex))[0]
         ((img - b.target) ** 2).mean().backward()
 
-    def step():  # small scenes on their own streams beside the large ones (eager only)
-        W.c3_step(parts, streams, scene_step)
-
-    def step_in_turn():  # every scene in turn on one stream: the form captured into a graph
-        W.c3_step(parts, None, scene_step)
+    def step():  # every scene in turn on one stream
+        W.c3_step(parts, scene_step)
 
     D, L, B = 3, 1, 1
     R = 1 + D * (L + B + 1)
@@ -234,33 +228,22 @@ def bench_c3(args, torch, dr, W, dev):
     edge = sum(b.res * b.res for b in parts)
     rays = R * lanes + 23 * edge
     steps = max(1, min(args.steps, 3))
-
-    def eager_ms(fn, what):
-        for _ in range(max(1, min(args.warmup, 1))):
-            fn()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            fn()
-        torch.cuda.synchronize()
-        ms_ = (time.perf_counter() - t0) * 1e3 / steps
-        print(f"c3: eager {what} {ms_:.1f} ms per step", file=sys.stderr, flush=True)
-        return ms_
-
-    ms_turn = eager_ms(step_in_turn, "in turn")
-    ms_grouped = eager_ms(step, f"small scenes on {len(streams)} streams") if streams else None
+    for _ in range(max(1, min(args.warmup, 1))):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    ms_eager = (time.perf_counter() - t0) * 1e3 / steps
+    print(f"c3: eager {ms_eager:.1f} ms per step", file=sys.stderr, flush=True)
     graph = None
-    if args.graph != "off" and not streams:
-        # captured only with every scene on one shared set of buffers (--c3-groups 0): a capture spanning
-        # the small-scene groups' buffers and auxiliary streams has not been run
-        graph = graph_step_timing(torch, step_in_turn, steps, rays)
-        graph["eager_ms_per_step"] = round(ms_turn, 3)
-    # the value: the fastest of the measured forms of the same step
-    forms = {"eager, scenes in turn": ms_turn}
-    if ms_grouped is not None:
-        forms[f"eager, small scenes on {len(streams)} streams"] = ms_grouped
+    if args.graph != "off":
+        graph = graph_step_timing(torch, step, steps, rays)
+        graph["eager_ms_per_step"] = round(ms_eager, 3)
+    forms = {"eager": ms_eager}
     if graph is not None and args.graph == "value":
-        forms["graph replay, scenes in turn"] = graph["ms_per_step"]
+        forms["graph replay"] = graph["ms_per_step"]
     form, ms = min(forms.items(), key=lambda kv: kv[1])
     print(json.dumps({
         "metric": "fwd+bwd Mrays/s, 64 scenes of uneven size in turn on shared buffers",
@@ -271,7 +254,6 @@ def bench_c3(args, torch, dr, W, dev):
                                "edge spp 1, vertex + texel gradients",
                    "scenes": len(parts), "lanes": lanes, "boundary_samples_per_term": edge, "rays_per_step": rays,
                    "tris": [b.n_tris for b in parts], "res": [b.res for b in parts], "spp": [b.spp for b in parts],
-                   "small_scene_groups": len(streams), "small_scenes": sum(b.group >= 0 for b in parts),
                    "form": form},
         "forms_ms_per_step": {k: round(v, 3) for k, v in forms.items()},
         "graph_step": graph}))
@@ -324,25 +306,40 @@ def plain_baseline(torch, ops, step, verts, tex, n_steps, rays):
             "steps": n_steps, "grad_max_rel_diff_vs_fused": diff}
 
 
-# HBM bytes per k_trace launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this
-# workload (the 18 traversal launches of one step; FETCH_SIZE x 2.0 and
-# WRITE_SIZE x 1.0, the factors a 1 GiB device copy reads for those counters);
-# the commands and per-launch table are in profiles/k_trace_gradient_step_512px_16spp.md.
-TRACE_TRAFFIC_PER_LAUNCH = 63579410
+# HBM bytes per k_trace launch on the default workload from two rocprofv3 --pmc
+# runs (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch, summed over the step's
+# traversal launches), each scaled by what the counter reads for a 1 GiB
+# device-to-device copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0), at the launch shape
+# named here; another shape reports null.  The commands and per-launch table are
+# in profiles/k_trace_gradient_step_512px_16spp.md.
+TRACE_TRAFFIC = {"bytes_per_launch": 63579410,
+                 "shape": {"trace_blocks_per_cu": 32, "trace_stack_lds": 8}}
 
 
-def trace_roofline(total_ms, launches, bytes_, default_workload):
-    """Dominant kernel (closest-hit LBVH traversal): its launches inside the timed
-    region, timed with HIP events on the launch stream.  achieved = algorithmic
-    bytes (24 B ray read + 12 B hit written per ray) / summed launch time."""
-    sec = total_ms * 1e-3
+def trace_roofline(busy_ms, event_ms, launches, bytes_, default_workload, shape):
+    """Dominant kernel (closest-hit LBVH traversal): its launches inside the
+    timed region.  achieved = algorithmic bytes (24 B ray read + 12 B hit
+    written per ray) / summed execution time, each launch's execution time
+    being its latest workgroup end minus its earliest workgroup start on the
+    device clock (what a kernel trace reports as the dispatch's duration).
+    The HIP-event durations of the same launches are given beside it: they
+    also count time a launch waits for CUs held by the boundary terms'
+    streams, and understate the kernel's rate."""
+    sec = busy_ms * 1e-3
     ach = bytes_ / sec / 1e9 if sec > 0 else 0.0
+    ev_sec = event_ms * 1e-3
+    ev_ach = bytes_ / ev_sec / 1e9 if ev_sec > 0 else 0.0
+    same_shape = all(shape.get(k) == v for k, v in TRACE_TRAFFIC["shape"].items())
     return {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(ach / HBM_PEAK_GBS, 5),
-            "traffic": TRACE_TRAFFIC_PER_LAUNCH if default_workload else None,
+            "traffic": TRACE_TRAFFIC["bytes_per_launch"] if (default_workload and same_shape) else None,
+            "traffic_profile_shape": TRACE_TRAFFIC["shape"],
             "kernel": "k_trace (closest-hit LBVH traversal)", "launches": launches,
             "algorithmic_bytes_per_launch": round(bytes_ / max(launches, 1)),
-            "avg_launch_ms": round(total_ms / max(launches, 1), 4),
+            "avg_launch_ms": round(busy_ms / max(launches, 1), 4),
+            "timing": "in-kernel span (earliest workgroup start to latest workgroup end)",
+            "event_timed": {"avg_launch_ms": round(event_ms / max(launches, 1), 4), "achieved": round(ev_ach, 2),
+                            "frac": round(ev_ach / HBM_PEAK_GBS, 5)},
             "grays_per_s": round(bytes_ / 36.0 / sec / 1e9, 3) if sec > 0 else 0.0}
 
 
@@ -358,6 +355,37 @@ def step_algorithmic_bytes(n_int, n_e, spp, D, L, B):
     return int(round((b_fwd + b_bwd) * n_int + 72.0 * 23 * n_e))
 
 
+def host_cpus():
+    """The host's CPUs as this process sees them: logical CPUs (nproc), the
+    CPUs it may run on (affinity), the cgroup CPU quota if one is set, and the
+    model name.  The oracle runs one thread per CPU it may use: the affinity
+    count, or the quota (rounded up) when that is smaller."""
+    import math
+    nproc = os.cpu_count() or 1
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = nproc
+    quota = None
+    try:
+        q, period = Path("/sys/fs/cgroup/cpu.max").read_text().split()[:2]
+        if q != "max":
+            quota = int(q) / int(period)
+    except (OSError, ValueError):
+        pass
+    model = None
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    threads = affinity if quota is None else max(1, min(affinity, math.ceil(quota)))
+    return {"nproc": nproc, "affinity": affinity, "cgroup_cpu_quota": quota, "model": model,
+            "threads_used": threads}
+
+
 def cpu_baseline(parts, spp, target_s):
     """The CPU oracle (a port of the reference algorithm) timed on this host on a
     bounded sample: the same scene at a reduced resolution with the same spp,
@@ -370,7 +398,8 @@ def cpu_baseline(parts, spp, target_s):
         from torch_psdr_amd.flat import flatten
         import torch_psdr_amd as dr
         from torch_psdr_amd import workloads as W
-        cores = min(os.cpu_count() or 1, 16)
+        host = host_cpus()
+        cores = host["threads_used"]
         O.set_threads(cores)
         cpu = torch.device("cpu")
         V0, F = W.icosphere(5)
@@ -405,9 +434,11 @@ def cpu_baseline(parts, spp, target_s):
         n_e = 4 * sample_res * sample_res
         rays = reps * (10 * n_int + 23 * n_e)
         return {"value": round(rays / sec / 1e6, 3), "unit": "Mrays/s", "cores": cores, "kind": "port",
-                "sample": f"oracle renderD forward + tangent (interior, primary and direct boundary) at "
+                "host": host,
+                "sample": f"oracle renderD forward + forward-mode derivative along one direction of every vertex "
+                          f"and texel (interior, primary and direct boundary; the oracle has no reverse mode) at "
                           f"{sample_res}x{sample_res}, {spp} spp, edge spp 4, same scene, {reps} run(s); "
-                          f"{sec:.1f} s"}
+                          f"{sec:.1f} s on {cores} threads"}
     except Exception as e:  # the baseline is reported, never required
         return {"value": None, "unit": "Mrays/s", "cores": None, "kind": "port", "sample": f"failed: {e}"}
 

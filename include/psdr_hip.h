@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define PSDR_ABI_VERSION 5
+#define PSDR_ABI_VERSION 6
 
 /* status codes */
 #define PSDR_OK 0
@@ -286,12 +286,17 @@ int psdr_last_render_stats(const psdr_scene* scene, int64_t* counts, int32_t n_c
 
 /* Live timing of the closest-hit traversal kernel (no reference counterpart;
  * used to measure it inside a timed region).  enable = 1 synchronises the
- * device, clears the totals and starts recording a HIP event pair around every
- * traversal launch on its stream (at most 4096 launches).  enable = 0 stops,
- * synchronises and returns the summed launch durations in milliseconds, the
- * number of launches and their algorithmic bytes (24 B ray read + 12 B, or 16 B
- * with t, hit written per ray).  Process-wide; not for concurrent callers. */
-int psdr_trace_timing(int32_t enable, double* total_ms, int64_t* launches, int64_t* bytes);
+ * device, clears the totals and starts timing every traversal launch (at most
+ * 4096 launches) two ways: a HIP event pair around the launch on its stream,
+ * and, inside the kernel, the earliest workgroup start and latest workgroup end
+ * on the device's constant-rate wall clock.  enable = 0 stops, synchronises and
+ * returns the summed event durations (total_ms: they include time a launch
+ * waits for CUs held by work on other streams), the summed in-kernel spans
+ * (busy_ms: execution only, what a kernel trace reports as the dispatch's
+ * duration), the number of launches and their algorithmic bytes (24 B ray read
+ * + 12 B, or 16 B with t, hit written per ray).  Any output may be NULL.
+ * Process-wide; not for concurrent callers. */
+int psdr_trace_timing(int32_t enable, double* total_ms, int64_t* launches, int64_t* bytes, double* busy_ms);
 
 /* Launch shapes of the hot kernels, chosen at run time by the autotuner
  * (torch_psdr_amd/tune.py) or by hand.  No counterpart in the reference, whose

@@ -134,6 +134,18 @@ def set_fd_measure(on: bool):
     load().oracle_set_fd_measure(int(bool(on)))
 
 
+def diffuse_bsdf(color, sigma, n, v, l) -> np.ndarray:
+    """DiffuseBsdf value (the oracle's restatement of src/diffuse_bsdf.cc:57-89)
+    for arrays of cases: color (k,3), sigma (k,), shading normal, direction to
+    the viewer and to the light (k,3 each) -> (k,3)."""
+    color, n, v, l = (np.asarray(a, np.float32).reshape(-1, 3) for a in (color, n, v, l))
+    sigma = np.asarray(sigma, np.float32).reshape(-1, 1)
+    cases = np.ascontiguousarray(np.concatenate([color, sigma, n, v, l], 1), dtype=np.float32)
+    out = np.zeros((cases.shape[0], 3), np.float32)
+    load().oracle_diffuse_bsdf(_p(cases), cases.shape[0], _p(out))
+    return out
+
+
 def pcg_kat(initstate: int, initseq: int, n: int) -> np.ndarray:
     out = np.zeros(n, dtype=np.uint32)
     load().oracle_pcg_kat(C.c_uint64(initstate), C.c_uint64(initseq), n, _p(out))

@@ -247,6 +247,25 @@ template <class F> struct Its {
   int obj = -1, tri = -1, light = -1;
 };
 
+// DiffuseBsdfMaterial::eval_ (src/diffuse_bsdf.cc:57-89) for shading normal N,
+// direction to the viewer V and to the light L: color (a + b t) with
+// a = 1 / (pi + (3 pi - 4) / 6 sigma), b = sigma a, t = V.L - max(N.L, 0)
+// max(N.V, 0), and t replaced by max(N.L, N.V) + REAL_EPS where it is positive.
+template <class F>
+V3<F> diffuse_bsdf_value(const V3<F>& color, const F& sigma, const V3<F>& N, const V3<F>& Vw, const V3<F>& L) {
+  const F div = F(1.0f) / (F(REAL_PI) + F((3.0f * REAL_PI - 4.0f) / 6.0f) * sigma);
+  const F a = div;
+  const F b = sigma * div;
+  F NdotL = dot(N, L);
+  if (val(NdotL) < 0.f) NdotL = F(0.f);
+  F NdotV = dot(N, Vw);
+  if (val(NdotV) < 0.f) NdotV = F(0.f);
+  const F VdotL = dot(Vw, L);
+  F t = VdotL - NdotL * NdotV;
+  if (val(t) > 0.f) t = (val(NdotL) >= val(NdotV) ? NdotL : NdotV) + F(REAL_EPS);
+  return color * (a + b * t);
+}
+
 template <class F> struct Scene {
   std::vector<V3<F>> V;
   std::vector<Tri<F>> tris;
@@ -418,21 +437,8 @@ template <class F> struct Scene {
   }
   // DiffuseBsdf shading value (src/diffuse_bsdf.cc:42-75)
   V3<F> bsdf_diffuse_bsdf(const Object& ob, const Its<F>& it, const V3<F>& L) const {
-    const V3<F> color = tex3(ob.albedo, it.uv, texels);
-    const F sigma = tex1(ob.rough, it.uv, texels);
-    const F div = F(1.0f) / (F(REAL_PI) + F((3.0f * REAL_PI - 4.0f) / 6.0f) * sigma);
-    const F a = div;
-    const F b = sigma * div;
-    const V3<F> N = shading_normal(ob, it);
-    const V3<F>& Vw = it.wi;
-    F NdotL = dot(N, L);
-    if (val(NdotL) < 0.f) NdotL = F(0.f);
-    F NdotV = dot(N, Vw);
-    if (val(NdotV) < 0.f) NdotV = F(0.f);
-    const F VdotL = dot(Vw, L);
-    F t = VdotL - NdotL * NdotV;
-    if (val(t) > 0.f) t = (val(NdotL) >= val(NdotV) ? NdotL : NdotV) + F(REAL_EPS);
-    return color * (a + b * t);
+    return diffuse_bsdf_value(tex3(ob.albedo, it.uv, texels), tex1(ob.rough, it.uv, texels), shading_normal(ob, it),
+                              it.wi, L);
   }
   // Material::eval (diffuse_material.cc:29-34 / diffuse_bsdf.cc:42-75)
   V3<F> eval(const Its<F>& it, const V3<F>& wo) const {
@@ -1277,6 +1283,18 @@ int oracle_lbvh(const float* tri9, int n, uint32_t* morton, int* order, int* chi
 // split into interior / primary-boundary / direct-boundary parts (any may be
 // NULL to skip that estimator).  lane_L (optional): per-lane interior radiance.
 void oracle_set_fd_measure(int on) { g_fd_measure = on != 0; }
+
+// DiffuseBsdf value for n cases of (color[3], sigma, N[3], V[3], L[3]) -> out[3].
+void oracle_diffuse_bsdf(const float* cases, int n, float* out) {
+  for (int i = 0; i < n; ++i) {
+    const float* c = cases + 13 * i;
+    const V3<float> v = diffuse_bsdf_value(V3<float>(c[0], c[1], c[2]), c[3], V3<float>(c[4], c[5], c[6]),
+                                           V3<float>(c[7], c[8], c[9]), V3<float>(c[10], c[11], c[12]));
+    out[3 * i] = v.x;
+    out[3 * i + 1] = v.y;
+    out[3 * i + 2] = v.z;
+  }
+}
 
 int oracle_render(const oracle_scene* s, const oracle_config* cfg, const oracle_camera* c, int mode, float* img,
                   float* d_int, float* d_pri, float* d_dir, float* lane_L) {

@@ -124,66 +124,6 @@ def test_captured_batched_step_matches_eager():
             parts[rep].integrator.renderC(parts[rep].scene(parts[rep].vertices.detach(), parts[rep].tex.detach()))
 
 
-def _grouped_parts():
-    seq = W.c3_parts(dr, DEV, **SMALL)
-    grp = W.c3_parts(dr, DEV, small_groups=2, small_lanes=2000, **SMALL)
-    assert {b.group for b in grp} == {-1, 0, 1}  # large scenes in turn and two small groups
-    return seq, grp
-
-
-def test_small_scene_groups_on_streams_match_scenes_in_turn():
-    """c3_step: small scenes in groups on their own streams (with their own
-    buffers) give the images of scenes rendered in turn on one stream, bit for
-    bit, and the same gradients up to the order of float atomics."""
-    seq, grp = _grouped_parts()
-    streams = W.c3_streams(grp, DEV)
-    assert len(streams) == 2
-    out = {}
-
-    def scene_step(b):
-        b.vertices.grad = None
-        b.tex.grad = None
-        img = b.integrator.renderD(b.scene(b.vertices, b.tex))[0]
-        ((img - b.target) ** 2).sum().backward()
-        out[id(b)] = img
-
-    want = []
-    W.c3_step(seq, None, scene_step)
-    for b in seq:
-        want.append((out[id(b)].detach().clone(), b.vertices.grad.clone(), b.tex.grad.clone()))
-    for rep in range(2):
-        W.c3_step(grp, streams, scene_step)
-        torch.cuda.synchronize()
-        for i, (b, (wi, wv, wt)) in enumerate(zip(grp, want)):
-            assert torch.equal(out[id(b)], wi), (rep, i)
-            _close(b.vertices.grad, wv, ("vertices, grouped", rep, i))
-            _close(b.tex.grad, wt, ("texels, grouped", rep, i))
-
-def test_grouped_step_refuses_capture(monkeypatch):
-    """The grouped step raises under stream capture (see the skipped test
-    below) before it issues any work; checked without capturing anything."""
-    _, grp = _grouped_parts()
-    streams = W.c3_streams(grp, DEV)
-    calls = []
-    monkeypatch.setattr(torch.cuda, "is_current_stream_capturing", lambda: True)
-    with pytest.raises(RuntimeError, match="cannot be captured"):
-        W.c3_step(grp, streams, calls.append)
-    assert calls == []
-
-
-@pytest.mark.skip(reason="capturing a step whose small scenes run on their own torch streams ended in a "
-                         "segmentation fault inside torch.cuda.CUDAGraph.capture_end (hipStreamEndCapture) on the "
-                         "GPU, cause not found; c3_step refuses capture, the captured form runs every scene in turn")
-def test_small_scene_groups_captured():
-    seq, grp = _grouped_parts()
-    streams = W.c3_streams(grp, DEV)
-
-    def step():
-        W.c3_step(grp, streams, lambda b: b.integrator.renderD(b.scene(b.vertices, b.tex))[0].sum().backward())
-
-    CapturedStep(step, warmup=1).replay()
-
-
 def test_zero_pdf_extension_keeps_gradients_finite(oracle):
     """Scene 1 of the batched configuration (256^2, 16 spp, 64,980-triangle
     sphere of radius ~2) has a lane whose extension sample has pdf 0: its

@@ -1,5 +1,10 @@
 """The benchmark configurations at their real sizes (run with -m gpu).
 
+C2 (the benchmarked step: perturbed level-5 sphere, 512^2 texture, 512^2, 16
+spp, depth 3, 4 edge samples per pixel for each boundary term, i.e. 1 M
+samples over 10,000 and 10,000 x 6 x 6 hypercube cells): renderD's image equals
+renderC's, sampled lanes equal the oracle's, and per gradient term
+VJP(GPU) . w == g . JVP(oracle) with g the L2 loss's dL/dI.
 C1 (texture-gradient step, 256^2, 8 spp, depth 2): VJP(GPU) . w == g . JVP(oracle)
 over all 196,608 texel channels.  C4 (2.09 M-triangle terrain, 1024^2, 64 spp,
 depth 4, two lights): the full renderD step runs within 64 GB of device memory,
@@ -18,6 +23,74 @@ from torch_psdr_amd import ops
 from torch_psdr_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
+
+
+def test_c2_full_scale(gpu_device, oracle):
+    p = W.c2_parts(dr, gpu_device)
+    integ = p["integrator"]
+    assert (p["res"], p["spp"], p["spp_edge"], integ.cfg[5]) == (512, 16, 4, 3)
+    with torch.no_grad():
+        target = integ.renderC(p["scene"](p["V0"], p["tex_target"]))[0]
+    verts = p["Vp"].clone().requires_grad_(True)
+    tex = p["tex_opt"].clone().requires_grad_(True)
+    scene = p["scene"](verts, tex)
+    assert scene.flat.n_tris == 20484
+    ns, flat = ops.entry(scene.token)
+    cam = scene.cameras[0].params()
+    img_c = integ.renderC(scene)[0]
+    img = ops.forward_native(ns, flat, cam, integ.cfg, True)
+    torch.cuda.synchronize()
+    print("C2: forward done", flush=True)
+    assert torch.equal(img, img_c)
+    assert torch.isfinite(img).all() and img.mean() > 0
+    # (i) lanes through the sphere, its shadow on the ground and the background
+    n_lanes = 16 * 512 * 512
+    lanes = torch.empty((n_lanes, 3), device=gpu_device)
+    N.check(ns.lib.psdr_last_lane_radiance(ns.handle, lanes.data_ptr(), N.current_stream()))
+    torch.cuda.synchronize()
+    starts = [(row * 512 + col) * 16 for row, col in ((256, 224), (400, 200), (330, 40), (120, 256))]
+    idx = np.concatenate([np.arange(s, s + 64 * 16) for s in starts])
+    want = oracle.render_lanes(oracle.Scene(flat), integ.cfg, cam, idx).astype(np.float64)
+    got = lanes[torch.tensor(idx, device=gpu_device)].cpu().numpy().astype(np.float64)
+    rel = np.abs(got - want) / np.maximum(np.abs(want), 1e-3)
+    print(f"C2 lanes: bit-exact {(got == want).mean():.4f}, mean rel {rel.mean():.2e}, max rel {rel.max():.2e}",
+          flush=True)
+    assert want.sum() > 0
+    assert rel.mean() <= 1e-4 and rel.max() <= 1e-3
+    # (ii) per term, with g = dL/dI of the L2 loss mean((I - target)^2)
+    g = (2.0 / img.numel()) * (img - target)
+    c, k = ops.make_config(integ.cfg), ops.make_camera(cam)
+    terms = {}
+    for name, term in (("interior", N.PSDR_TERM_INTERIOR), ("primary", N.PSDR_TERM_PRIMARY),
+                       ("direct", N.PSDR_TERM_DIRECT)):
+        gv = torch.zeros_like(flat.vertices)
+        gt = torch.zeros_like(flat.texels)
+        N.check(ns.lib.psdr_render_backward(ns.handle, ctypes.byref(c), ctypes.byref(k), g.data_ptr(),
+                                            gv.data_ptr(), gt.data_ptr(), None, None, term, N.current_stream()))
+        torch.cuda.synchronize()
+        assert torch.isfinite(gv).all() and torch.isfinite(gt).all(), name
+        terms[name] = (gv.cpu().numpy().astype(np.float64), gt.cpu().numpy().astype(np.float64))
+    print("C2: backward terms done", flush=True)
+    # tangent: the sphere's vertices and its texture (the parameters of the step)
+    rng = np.random.default_rng(23)
+    nv = verts.shape[0]
+    wv = np.zeros(tuple(flat.vertices.shape), np.float32)
+    wv[:nv] = rng.normal(size=(nv, 3))
+    ob = flat.objects[0]
+    n_tex = 3 * ob["albedo_h"] * ob["albedo_w"]
+    wt = np.zeros(flat.texels.numel(), np.float32)
+    wt[ob["albedo_offset"]:ob["albedo_offset"] + n_tex] = rng.normal(size=n_tex)
+    _, d_int, d_pri, d_dir = oracle.render(oracle.Scene(flat, d_vertices=wv.reshape(-1), d_texels=wt), integ.cfg,
+                                           cam, mode=1)
+    gg = g.cpu().numpy().astype(np.float64)
+    for name, d in (("interior", d_int), ("primary", d_pri), ("direct", d_dir)):
+        gv, gt = terms[name]
+        lhs = float((gg * d.astype(np.float64)).sum())
+        rhs = float((gv * wv.astype(np.float64)).sum() + (gt * wt.astype(np.float64)).sum())
+        print(f"C2 {name}: JVP {lhs:.6e} VJP {rhs:.6e} rel {abs(lhs - rhs) / max(abs(lhs), abs(rhs)):.2e}",
+              flush=True)
+        assert lhs != 0.0 and np.isfinite(lhs)
+        assert abs(lhs - rhs) <= 2e-3 * max(abs(lhs), abs(rhs)), name
 
 
 def test_c1_texture_step_matches_oracle_jvp(gpu_device, oracle):

@@ -199,3 +199,30 @@ def test_c0_full_resolution(gpu_device, oracle):
     w = oracle.render(oracle.Scene(flat), integ.cfg, scene.cameras[0].params(), mode=0)
     exact = _compare_images(img.cpu().numpy(), w, "C0 256x256 4spp")
     assert exact > 0.999
+
+
+@pytest.mark.parametrize("bad", ["past-the-end", "negative", "other-object"])
+def test_scene_create_rejects_a_bad_tri_obj(gpu_device, bad):
+    """tri_obj entries index the object table in every kernel: an entry outside
+    [0, n_objects), or naming an object whose triangle range does not hold the
+    triangle, is refused at creation with PSDR_ERR_ARG."""
+    from torch_psdr_amd.flat import flatten
+    meshes, lights, _ = H.objects(gpu_device, level=1)
+    flat = flatten(meshes, lights, gpu_device)
+    n_obj = len(meshes) + len(lights)
+    t = flat.tri_obj.clone()
+    t[3] = {"past-the-end": n_obj, "negative": -1, "other-object": 1}[bad]  # triangle 3 is the sphere's
+    flat.tri_obj = t
+    keep = []
+    desc = flat.desc(keep)
+    handle = N.C.c_void_p()
+    lib = N.load()
+    assert lib.psdr_scene_create(N.C.byref(desc), N.C.byref(handle), _stream()) == N.PSDR_ERR_ARG
+    assert b"tri_obj" in lib.psdr_last_error()
+    assert not handle
+    # the untouched table is accepted
+    flat.tri_obj = t.clone().fill_(0)
+    flat.tri_obj[:] = flatten(meshes, lights, gpu_device).tri_obj
+    desc = flat.desc(keep)
+    N.check(lib.psdr_scene_create(N.C.byref(desc), N.C.byref(handle), _stream()))
+    lib.psdr_scene_destroy(handle)

@@ -75,16 +75,3 @@ def test_batched_configuration_draws():
         Q = s["rotation"]
         np.testing.assert_allclose(Q @ Q.T, np.eye(3), atol=1e-12)
         assert abs(np.linalg.det(Q) - 1.0) < 1e-12 and 0.5 <= s["scale"] <= 2.0
-
-
-def test_small_scene_groups_are_balanced():
-    from torch_psdr_amd.workloads import assign_groups
-    lanes = [1 << 20, 4096, 65536, 16384, 65536, 262144, 1 << 22, 1024]
-    g = assign_groups(lanes, 2, 1 << 18)
-    assert [g[i] for i in (0, 6)] == [-1, -1]           # above the limit: in turn
-    assert all(x in (0, 1) for i, x in enumerate(g) if i not in (0, 6))
-    loads = [sum(n for n, x in zip(lanes, g) if x == k) for k in (0, 1)]
-    assert sum(loads) == 262144 + 2 * 65536 + 16384 + 4096 + 1024
-    assert abs(loads[0] - loads[1]) <= 262144             # greedy, largest first
-    assert g[5] == 0 and g[2] == 1 and g[4] == 1          # 262144 -> 0, then both 65536 -> 1
-    assert assign_groups(lanes, 0, 1 << 18) == [-1] * len(lanes)

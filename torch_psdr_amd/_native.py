@@ -37,7 +37,7 @@ PSDR_CAMERA_PERSPECTIVE = 0
 PSDR_CAMERA_ORTHOGRAPHIC = 1
 PSDR_CAMERA_GRAD = 10
 
-ABI_VERSION = 5
+ABI_VERSION = 6
 
 
 class PsdrObject(C.Structure):
@@ -145,7 +145,8 @@ _SIGS = {
     "psdr_scene_bvh": (C.c_int, [_VP, _VP, _VP, _VP, _VP, _VP, _VP]),
     "psdr_last_lane_radiance": (C.c_int, [_VP, _VP, _VP]),
     "psdr_last_render_stats": (C.c_int, [_VP, C.POINTER(C.c_int64), _I32, C.POINTER(C.c_int64)]),
-    "psdr_trace_timing": (C.c_int, [_I32, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "psdr_trace_timing": (C.c_int, [_I32, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                    C.POINTER(C.c_double)]),
     "psdr_get_tuning": (C.c_int, [C.POINTER(PsdrTuning)]),
     "psdr_set_tuning": (C.c_int, [C.POINTER(PsdrTuning)]),
 }

@@ -505,8 +505,11 @@ static int scene_create_impl(const psdr_scene_desc* d, psdr_scene* s, hipStream_
     PSDR_HIP_CHECK(hipMemcpyAsync(s->tri_uv.p, d->tri_uv, 24ull * nt, hipMemcpyDeviceToDevice, st));
     PSDR_HIP_CHECK(hipMemcpyAsync(s->tri_obj.p, d->tri_obj, 4ull * nt, hipMemcpyDeviceToDevice, st));
   }
-  std::vector<int> hfaces(3ull * nt);
-  if (nt) PSDR_HIP_CHECK(hipMemcpyAsync(hfaces.data(), d->faces, 12ull * nt, hipMemcpyDeviceToHost, st));
+  std::vector<int> hfaces(3ull * nt), hobj(nt);
+  if (nt) {
+    PSDR_HIP_CHECK(hipMemcpyAsync(hfaces.data(), d->faces, 12ull * nt, hipMemcpyDeviceToHost, st));
+    PSDR_HIP_CHECK(hipMemcpyAsync(hobj.data(), d->tri_obj, 4ull * nt, hipMemcpyDeviceToHost, st));
+  }
   PSDR_HIP_CHECK(hipStreamSynchronize(st));
   for (int f = 0; f < nt; ++f)
     for (int k = 0; k < 3; ++k)
@@ -514,6 +517,17 @@ static int scene_create_impl(const psdr_scene_desc* d, psdr_scene* s, hipStream_
         set_error(PSDR_ERR_ARG, "face " + std::to_string(f) + " references a vertex out of range");
         return PSDR_ERR_ARG;
       }
+  // kernels index the object table with tri_obj[t] and assume t lies in that
+  // object's triangle range
+  for (int t = 0; t < nt; ++t) {
+    const int o = hobj[t];
+    if (o < 0 || o >= d->n_objects || t < d->objects[o].tri_begin ||
+        t >= d->objects[o].tri_begin + d->objects[o].tri_count) {
+      set_error(PSDR_ERR_ARG, "triangle " + std::to_string(t) + ": tri_obj " + std::to_string(o) +
+                                  " is not an object whose triangle range holds it");
+      return PSDR_ERR_ARG;
+    }
+  }
   // an environment light's cube (the last object) has no edges and stays out of the LBVH
   s->n_bvh = nt;
   if (d->n_objects > 0) {
@@ -592,6 +606,39 @@ static bool cam_ok(const psdr_camera* c) {
   if (!c || c->height <= 0 || c->width <= 0) return false;
   if (c->kind == PSDR_CAMERA_PERSPECTIVE) return true;
   return c->kind == PSDR_CAMERA_ORTHOGRAPHIC && c->view_width > 0.f;
+}
+
+// Size limits of one render.  The wavefront kernels index ray buffers in 32-bit
+// arithmetic as 3 * i with i < lanes * (rays per lane): the interior render has
+// N = spp * H * W lanes with L + B + 1 rays each, the primary boundary
+// max(spp_primary_edge * H * W, hypercube cells) lanes with L + 2 rays each (one
+// light, one BSDF and one extension ray per depth, two camera-side rays), the
+// direct boundary spp_secondary_edge * H * W lanes over at most 2^28 cells.
+// Checked before any buffer is touched, so it needs no device.
+static int check_render_sizes(const psdr_config& c, const psdr_camera& cam) {
+  constexpr int64_t lim = 0x7fffffff;
+  const int64_t px = static_cast<int64_t>(cam.height) * cam.width;
+  const char* why = nullptr;
+  if (c.mis_light_samples > 4096 || c.mis_bsdf_samples > 4096) {
+    why = "mis_light_samples and mis_bsdf_samples must be at most 4096";
+  } else {
+    const int64_t ki = 1 + int64_t{c.mis_light_samples} + c.mis_bsdf_samples;
+    const int64_t kp = 2 + int64_t{c.mis_light_samples};
+    const int64_t ni = int64_t{c.spp_interior} * px, np = int64_t{c.spp_primary_edge} * px;
+    const int64_t nd = int64_t{c.spp_secondary_edge} * px;
+    const int64_t cells = int64_t{c.secondary_edge_hypercube_resolution[0]} *
+                          c.secondary_edge_hypercube_resolution[1] * c.secondary_edge_hypercube_resolution[2];
+    if (ni > lim / 4 || 3 * ki * ni > lim)
+      why = "interior: 3 * spp_interior * H * W * (mis_light_samples + mis_bsdf_samples + 1) exceeds 2^31 - 1";
+    else if (3 * kp * std::max<int64_t>(np, c.primary_edge_hypercube_resolution) > lim)
+      why = "primary boundary: 3 * max(spp_primary_edge * H * W, primary_edge_hypercube_resolution) * "
+            "(mis_light_samples + 2) exceeds 2^31 - 1";
+    else if (nd > lim / 8 || cells > lim / 8)
+      why = "direct boundary: spp_secondary_edge * H * W and the hypercube cells must each be below 2^28";
+  }
+  if (!why) return PSDR_OK;
+  set_error(PSDR_ERR_ARG, std::string("render: sizes out of range: ") + why);
+  return PSDR_ERR_ARG;
 }
 
 extern "C" {
@@ -721,7 +768,12 @@ int psdr_scene_info(const psdr_scene* s, int32_t* n_tris, int32_t* n_edges, int3
 
 int psdr_render_forward(psdr_scene* s, const psdr_config* cfg, const psdr_camera* cam, float* out_rgb, int32_t keep,
                         void* stream) {
-  if (!s || !out_rgb || !cfg_ok(cfg) || !cam_ok(cam)) {
+  if (!cfg_ok(cfg) || !cam_ok(cam)) {
+    set_error(PSDR_ERR_ARG, "render_forward: bad argument");
+    return PSDR_ERR_ARG;
+  }
+  PSDR_TRY(check_render_sizes(*cfg, *cam));
+  if (!s || !out_rgb) {
     set_error(PSDR_ERR_ARG, "render_forward: bad argument");
     return PSDR_ERR_ARG;
   }
@@ -732,7 +784,12 @@ int psdr_render_forward(psdr_scene* s, const psdr_config* cfg, const psdr_camera
 int psdr_render_backward(psdr_scene* s, const psdr_config* cfg, const psdr_camera* cam, const float* dL_dI,
                          float* grad_vertices, float* grad_texels, float* grad_emit, float* grad_camera,
                          int32_t terms, void* stream) {
-  if (!s || !dL_dI || !cfg_ok(cfg) || !cam_ok(cam)) {
+  if (!cfg_ok(cfg) || !cam_ok(cam)) {
+    set_error(PSDR_ERR_ARG, "render_backward: bad argument");
+    return PSDR_ERR_ARG;
+  }
+  PSDR_TRY(check_render_sizes(*cfg, *cam));
+  if (!s || !dL_dI) {
     set_error(PSDR_ERR_ARG, "render_backward: bad argument");
     return PSDR_ERR_ARG;
   }
@@ -845,8 +902,8 @@ int psdr_scene_trace(psdr_scene* s, const float* ro, const float* rd, int32_t n,
                     as_stream(stream));
 }
 
-int psdr_trace_timing(int32_t enable, double* total_ms, int64_t* launches, int64_t* bytes) {
-  return trace_timing(enable, total_ms, launches, bytes);
+int psdr_trace_timing(int32_t enable, double* total_ms, int64_t* launches, int64_t* bytes, double* busy_ms) {
+  return trace_timing(enable, total_ms, launches, bytes, busy_ms);
 }
 
 int psdr_get_tuning(psdr_tuning* out) {

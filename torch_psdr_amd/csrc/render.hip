@@ -66,8 +66,13 @@ __global__ __launch_bounds__(TRACE_THREADS) void k_trace(SceneView sv, const flo
                                                          int* __restrict__ otri,
                                                          float* __restrict__ ot, float* __restrict__ ou,
                                                          float* __restrict__ ov, int* __restrict__ spill,
-                                                         unsigned long long* __restrict__ timed_bytes) {
+                                                         unsigned long long* __restrict__ timed_bytes,
+                                                         unsigned long long* __restrict__ span) {
   __shared__ int stack[STACK_LDS * TRACE_THREADS];
+  // timed launch: span[0] = earliest workgroup start, span[1] = latest
+  // workgroup end on the device's constant-rate clock, so span[1] - span[0] is
+  // the launch's execution time without the time it waited for CUs
+  if (span && threadIdx.x == 0) atomicMin(&span[0], static_cast<unsigned long long>(wall_clock64()));
   // rays are K segments of `per` rays each, segment k starting at k * seg
   // (one segment per ray kind, so a wave traces rays of one kind)
   const int per = count ? *count : n;
@@ -98,6 +103,19 @@ __global__ __launch_bounds__(TRACE_THREADS) void k_trace(SceneView sv, const flo
     if (ot) ot[i] = h.tri >= 0 ? h.t : -1.f;
     ou[i] = h.u;
     ov[i] = h.v;
+  }
+  if (span) {
+    __syncthreads();
+    if (threadIdx.x == 0) atomicMax(&span[1], static_cast<unsigned long long>(wall_clock64()));
+  }
+}
+
+// Span slots of timed launches: (start, end) = (max, 0) before a launch.
+__global__ void k_span_init(unsigned long long* span, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    span[2 * i] = ~0ull;
+    span[2 * i + 1] = 0ull;
   }
 }
 
@@ -517,6 +535,8 @@ struct TraceTimer {
   size_t used = 0;             // pairs recorded since the last start
   int64_t dropped = 0;         // launches beyond the event pool
   unsigned long long* bytes = nullptr;
+  unsigned long long* span = nullptr;  // [pair][start, end] device clock of each launch
+  double clock_khz = 0.0;
 };
 TraceTimer& trace_timer() {
   static TraceTimer t;
@@ -534,18 +554,19 @@ int trace_rays(const SceneView& sv, const float* ro, const float* rd, int n, con
   if (tt.on && !timed) ++tt.dropped;
   if (timed) PSDR_HIP_CHECK(hipEventRecord(tt.ev[2 * tt.used], st));
   unsigned long long* tb = timed ? tt.bytes : nullptr;
+  unsigned long long* sp = timed ? tt.span + 2 * tt.used : nullptr;
   switch (tuning().trace_stack.load()) {
     case 8:
       hipLaunchKernelGGL(k_trace<8>, dim3(grid), dim3(TRACE_THREADS), 0, st, sv, ro, rd, n, count_dev, K, seg, tl, nvis,
-                         tri, t, u, v, spill, tb);
+                         tri, t, u, v, spill, tb, sp);
       break;
     case 32:
       hipLaunchKernelGGL(k_trace<32>, dim3(grid), dim3(TRACE_THREADS), 0, st, sv, ro, rd, n, count_dev, K, seg, tl,
-                         nvis, tri, t, u, v, spill, tb);
+                         nvis, tri, t, u, v, spill, tb, sp);
       break;
     default:
       hipLaunchKernelGGL(k_trace<16>, dim3(grid), dim3(TRACE_THREADS), 0, st, sv, ro, rd, n, count_dev, K, seg, tl,
-                         nvis, tri, t, u, v, spill, tb);
+                         nvis, tri, t, u, v, spill, tb, sp);
       break;
   }
   PSDR_HIP_CHECK(hipGetLastError());
@@ -556,17 +577,30 @@ int trace_rays(const SceneView& sv, const float* ro, const float* rd, int n, con
   return PSDR_OK;
 }
 
-int trace_timing(int enable, double* total_ms, int64_t* launches, int64_t* bytes) {
+int trace_timing(int enable, double* total_ms, int64_t* launches, int64_t* bytes, double* busy_ms) {
   TraceTimer& tt = trace_timer();
   std::lock_guard<std::mutex> lock(tt.mu);
   if (enable) {
     if (tt.ev.empty()) {
+      int dev = 0, khz = 0;
+      PSDR_HIP_CHECK(hipGetDevice(&dev));
+      PSDR_HIP_CHECK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+      if (khz <= 0) {
+        set_error(PSDR_ERR_HIP, "trace_timing: the device reports no wall-clock rate");
+        return PSDR_ERR_HIP;
+      }
+      tt.clock_khz = khz;
       tt.ev.resize(2 * TIMER_PAIRS);
       for (auto& e : tt.ev) PSDR_HIP_CHECK(hipEventCreate(&e));
       PSDR_HIP_CHECK(hipMalloc(&tt.bytes, sizeof(unsigned long long)));
+      PSDR_HIP_CHECK(hipMalloc(&tt.span, 2 * TIMER_PAIRS * sizeof(unsigned long long)));
     }
     PSDR_HIP_CHECK(hipDeviceSynchronize());
     PSDR_HIP_CHECK(hipMemset(tt.bytes, 0, sizeof(unsigned long long)));
+    hipLaunchKernelGGL(k_span_init, dim3(blocks(TIMER_PAIRS, 256)), dim3(256), 0, nullptr, tt.span,
+                       static_cast<int>(TIMER_PAIRS));
+    PSDR_HIP_CHECK(hipGetLastError());
+    PSDR_HIP_CHECK(hipDeviceSynchronize());
     tt.used = 0;
     tt.dropped = 0;
     tt.on = true;
@@ -586,10 +620,17 @@ int trace_timing(int enable, double* total_ms, int64_t* launches, int64_t* bytes
   }
   unsigned long long b = 0;
   PSDR_HIP_CHECK(hipMemcpy(&b, tt.bytes, sizeof(b), hipMemcpyDeviceToHost));
+  std::vector<unsigned long long> span(2 * tt.used);
+  if (tt.used) PSDR_HIP_CHECK(hipMemcpy(span.data(), tt.span, span.size() * sizeof(unsigned long long),
+                                        hipMemcpyDeviceToHost));
+  double ticks = 0.0;
+  for (size_t i = 0; i < tt.used; ++i)
+    if (span[2 * i + 1] >= span[2 * i]) ticks += static_cast<double>(span[2 * i + 1] - span[2 * i]);
   if (tt.dropped) {
     set_error(PSDR_ERR_STATE, "trace_timing: more launches than the event pool holds");
     return PSDR_ERR_STATE;
   }
+  if (busy_ms) *busy_ms = ticks / tt.clock_khz;
   if (total_ms) *total_ms = ms;
   if (launches) *launches = static_cast<int64_t>(tt.used);
   if (bytes) *bytes = static_cast<int64_t>(b);

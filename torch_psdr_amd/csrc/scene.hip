@@ -10,7 +10,9 @@
 //    codes of padded triangle-box centres (bit 30 set for triangles whose largest
 //    padded extent is at most 1/8 of the centre-bounds extent), a stable LSD radix sort (LDS digit
 //    histograms + wave-ballot multisplit ranking), the Karras (2012) hierarchy
-//    over keys (morton << 32 | sorted position), and an atomic bottom-up refit.
+//    over keys (morton << 32 | sorted position), and node boxes computed as the
+//    union of each internal node's sorted leaf range (chunk boxes, then one wave
+//    per node; no inter-workgroup atomics).
 //    The integer output is deterministic and matches the CPU restatement exactly.
 #include <hip/hip_runtime.h>
 

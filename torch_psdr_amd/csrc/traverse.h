@@ -5,9 +5,12 @@
 // two boxes); leaves are single triangles stored in LBVH order as
 // {p0, id}{e1}{e2}.  The traversal stack lives in LDS (STACK_LDS entries per
 // lane, laid out [entry][lane] so a wave's pushes hit distinct banks) and
-// overflows into a per-thread global area.  A Karras tree over 62-bit keys has at
-// most 62 internal levels and the stack holds at most one entry per level, so a
-// total of 64 entries never overflows.
+// overflows into a per-thread global area.  The tree is a Karras hierarchy over
+// 64-bit keys (31-bit Morton code << 32 | sorted position; the top bit is always
+// zero): an internal node's common-prefix length lies in [1, 63] and grows
+// strictly from parent to child, so a root-to-leaf path has at most 63 internal
+// nodes, and the stack, which holds at most one entry per level, never needs
+// more than the 64 entries it has.
 //
 // Result contract: smallest t >= 1e-3 (Moller-Trumbore, fixed op order); an
 // exactly equal t keeps the smaller global triangle id, so the answer does not

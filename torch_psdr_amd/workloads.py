@@ -307,7 +307,6 @@ class BatchScene:
     res: int
     spp: int
     n_tris: int
-    group: int = -1  # -1: rendered in turn on the caller's stream; g >= 0: small-scene group g (c3_step)
 
 
 def c3_specs(n_scenes=64, seed=4, res_choices=(128, 256, 512, 1024), spp_choices=(4, 16, 64),
@@ -330,15 +329,10 @@ def c3_specs(n_scenes=64, seed=4, res_choices=(128, 256, 512, 1024), spp_choices
 
 
 def c3_parts(dr, device, n_scenes=64, seed=4, res_choices=(128, 256, 512, 1024), spp_choices=(4, 16, 64),
-             log_tris=(3.0, 5.3), tex_res=256, depth=3, spp_edge=1, sigma=0.02, targets=True, share=True,
-             small_groups=0, small_lanes=1 << 18):
+             log_tris=(3.0, 5.3), tex_res=256, depth=3, spp_edge=1, sigma=0.02, targets=True, share=True):
     """C3: `n_scenes` scenes of uneven size (see the module docstring), their
-    native handles sharing one set of per-render buffers.
-
-    small_groups > 0: scenes of at most `small_lanes` lanes are dealt into that
-    many groups of about equal lane counts (BatchScene.group), each group with a
-    set of buffers of its own, so that c3_step can render the groups on their
-    own streams beside the large scenes."""
+    native handles sharing one set of per-render buffers (rendered in turn,
+    the buffers grow to the largest render)."""
     from . import ops
     specs = c3_specs(n_scenes, seed, res_choices, spp_choices, log_tris)
     Vg, Fg = quad(10.0, -1.0)
@@ -374,67 +368,13 @@ def c3_parts(dr, device, n_scenes=64, seed=4, res_choices=(128, 256, 512, 1024),
                                              texture(1001 + 2 * si, tex_res, device)))[0]
         out.append(BatchScene(v_opt.requires_grad_(True), tex_opt.requires_grad_(True), target, integ, scene,
                               res, spp, int(F.shape[0]) + 4))
-    for b, g in zip(out, assign_groups([b.res * b.res * b.spp for b in out], small_groups, small_lanes)):
-        b.group = g
-    # one set of per-render buffers per group (all scenes when there are no
-    # groups), sized by the group's largest render
     if share:
-        for g in sorted({b.group for b in out}):
-            ops.share_workspace([b.scene(b.vertices, b.tex) for b in out if b.group == g])
+        ops.share_workspace([b.scene(b.vertices, b.tex) for b in out])
     return out
 
 
-def assign_groups(lanes, n_groups, small_lanes):
-    """Group of each scene given its lane count: -1 for scenes above
-    small_lanes (or all, when n_groups <= 0); the others dealt largest first
-    to the group with the fewest lanes so far (ties: the lowest group)."""
-    groups = [-1] * len(lanes)
-    if n_groups <= 0:
-        return groups
-    loads = [0] * n_groups
-    for i in sorted((i for i, n in enumerate(lanes) if n <= small_lanes), key=lambda i: (-lanes[i], i)):
-        g = loads.index(min(loads))
-        groups[i] = g
-        loads[g] += lanes[i]
-    return groups
-
-
-def c3_streams(parts, device):
-    """One stream per small-scene group of `parts` (default priority)."""
-    n = max((b.group for b in parts), default=-1) + 1
-    return [torch.cuda.Stream(device=device) for _ in range(n)]
-
-
-def c3_step(parts, streams, scene_step):
-    """One step over every scene: scene_step(b) (render, loss, backward) for the
-    scenes outside the groups in turn on the caller's stream and, interleaved
-    with them, each small-scene group in turn on its own stream, so the small
-    scenes' short, latency-bound launches run beside the large scenes' instead
-    of after them.  The group streams fork from and join back into the caller's
-    stream, so the step is ordered like any other work on it.  With no streams
-    every scene runs in turn on the caller's stream (whatever its group), which
-    is also the form to capture into a graph."""
-    if not streams:
-        for b in parts:  # every scene in turn on the caller's stream
-            scene_step(b)
-        return
-    if torch.cuda.is_current_stream_capturing():
-        # capturing the group streams' fork and join has ended in a segmentation
-        # fault inside the HIP runtime's capture end: refuse instead
-        raise RuntimeError("c3_step: a step with small-scene streams cannot be captured into a graph; "
-                           "capture c3_step(parts, None, ...) (every scene in turn) instead")
-    main = torch.cuda.current_stream()
-    for st in streams:
-        st.wait_stream(main)
-    large = [b for b in parts if b.group < 0]
-    groups = [[b for b in parts if b.group == g] for g in range(len(streams))]
-    rounds = max([len(large)] + [len(g) for g in groups])
-    for k in range(rounds):
-        if k < len(large):
-            scene_step(large[k])
-        for g, st in enumerate(streams):
-            if k < len(groups[g]):
-                with torch.cuda.stream(st):
-                    scene_step(groups[g][k])
-    for st in streams:
-        main.wait_stream(st)
+def c3_step(parts, scene_step):
+    """One step over every scene: scene_step(b) (render, loss, backward) for
+    each scene in turn on the caller's stream."""
+    for b in parts:
+        scene_step(b)
