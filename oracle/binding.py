@@ -146,6 +146,13 @@ def diffuse_bsdf(color, sigma, n, v, l) -> np.ndarray:
     return out
 
 
+def set_direct_unit_normal(on: bool):
+    """Direct boundary term with the receiver's shadow-line direction and normal
+    normalised and the shadow point's motion projected on that normal by a dot
+    product (for finite-difference checks; off = the reference's formula)."""
+    load().oracle_set_direct_unit_normal(int(bool(on)))
+
+
 def pcg_kat(initstate: int, initseq: int, n: int) -> np.ndarray:
     out = np.zeros(n, dtype=np.uint32)
     load().oracle_pcg_kat(C.c_uint64(initstate), C.c_uint64(initseq), n, _p(out))

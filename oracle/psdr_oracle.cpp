@@ -597,6 +597,11 @@ template <class F> V3<F> hit_triangle(const V3<float>& o, const V3<F>& d, const 
   return (lift<F>(p0) + u * lift<F>(e1)) + v * lift<F>(e2);
 }
 
+// Test-only switch (see oracle_set_direct_unit_normal): the direct boundary
+// term with the shadow line's direction and normal on the receiver normalised
+// and the shadow point's motion projected on that normal by a dot product.
+bool g_direct_unit = false;
+
 template <class F>
 V3<F> boundary_term(const V3<F>& xs, const V3<F>& xd, const V3<F>& xb, const V3<F>& ns, const V3<F>& nd,
                     const Edge<F>& E, bool ad) {
@@ -606,7 +611,8 @@ V3<F> boundary_term(const V3<F>& xs, const V3<F>& xd, const V3<F>& xb, const V3<
   V3<float> ev = normalize(detach(E.p1) - detach(E.p0));
   if (dot(cross(dir_, ev), detach(E.out)) < 0.f) ev = -ev;
   const V3<float> n_plane = cross(dir_, ev);
-  const V3<float> edge_proj = cross(nd_, n_plane);
+  V3<float> edge_proj = cross(nd_, n_plane);
+  if (g_direct_unit) edge_proj = normalize(edge_proj);
   const float cos_s = std::fabs(dot(ns_, dir_));
   const float cos_b = dot(ev, dir_);
   const float cos_d = dot(edge_proj, dir_);
@@ -621,7 +627,10 @@ V3<F> boundary_term(const V3<F>& xs, const V3<F>& xd, const V3<F>& xb, const V3<
     const V3<float> nep = cross(nd_, edge_proj);
     V3<F> xh = hit_triangle<F>(xs_, dir, xd_, edge_proj, nep);
     xh = xh - lift<F>(detach(xh));
-    fb = xh * lift<F>(nep * dG);
+    if (g_direct_unit)
+      fb = V3<F>(dot(xh, lift<F>(nep)) * F(dG));
+    else
+      fb = xh * lift<F>(nep * dG);
   } else {
     fb = V3<F>(F(dG));
   }
@@ -1283,6 +1292,7 @@ int oracle_lbvh(const float* tri9, int n, uint32_t* morton, int* order, int* chi
 // split into interior / primary-boundary / direct-boundary parts (any may be
 // NULL to skip that estimator).  lane_L (optional): per-lane interior radiance.
 void oracle_set_fd_measure(int on) { g_fd_measure = on != 0; }
+void oracle_set_direct_unit_normal(int on) { g_direct_unit = on != 0; }
 
 // DiffuseBsdf value for n cases of (color[3], sigma, N[3], V[3], L[3]) -> out[3].
 void oracle_diffuse_bsdf(const float* cases, int n, float* out) {
