@@ -254,9 +254,30 @@ int psdr_camera_param_vjp(const psdr_camera* cam, const float* matrix_adjoints, 
 
 /* Closest hit (src/scene_optix.cc:240-277 + device_programs.cu:13-47) of n rays
  * against the scene: global triangle id (-1 on miss), t, barycentrics u, v
- * (weights of p1, p2; -1 on miss).  All device arrays. */
+ * (weights of p1, p2; -1 on miss).  All device arrays; t may be NULL.  Grows
+ * the scene's traversal-stack overflow buffer when the launch needs more. */
 int psdr_scene_trace(psdr_scene* scene, const float* ro, const float* rd, int32_t n, int32_t* tri, float* t,
                      float* u, float* v, void* stream);
+
+/* Largest ray count of psdr_scene_trace and psdr_scene_trace_noalloc: ray
+ * buffers are indexed as 3 * i in 32-bit arithmetic. */
+#define PSDR_NOALLOC_MAX_RAYS 715827882
+
+/* psdr_scene_trace without any allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and results as psdr_scene_trace (bit-identical outputs); the
+ * traversal stacks' overflow area is a fixed 32 MiB array of the library on
+ * each device, used with a fixed launch of 1024 workgroups, instead of a scene
+ * buffer grown on demand.  The first call on a device (n may be 0) looks the
+ * array up, which may load the library's device code and with it the array;
+ * make that call at set-up.  Calls no hipMalloc, hipFree or allocating library
+ * routine.  n above
+ * PSDR_NOALLOC_MAX_RAYS is refused with PSDR_ERR_ARG.  Every call on a device
+ * shares that device's array: calls that overlap on one device (on different
+ * streams or threads) race and must be ordered by the caller. */
+int psdr_scene_trace_noalloc(psdr_scene* scene, const float* ro, const float* rd, int32_t n, int32_t* tri, float* t,
+                             float* u, float* v, void* stream);
 
 /* Derived data for parity tests (device outputs):
  * tri7 [n_tris*7] = normal(3), tangent(3), area;

@@ -36,6 +36,7 @@ PSDR_TERM_ALL = 7
 PSDR_CAMERA_PERSPECTIVE = 0
 PSDR_CAMERA_ORTHOGRAPHIC = 1
 PSDR_CAMERA_GRAD = 10
+PSDR_NOALLOC_MAX_RAYS = 715827882
 
 ABI_VERSION = 6
 
@@ -140,6 +141,7 @@ _SIGS = {
     "psdr_camera_matrices": (C.c_int, [C.POINTER(PsdrCamera), _VP, _VP, _VP, _VP, _VP]),
     "psdr_camera_param_vjp": (C.c_int, [C.POINTER(PsdrCamera), _VP, _VP]),
     "psdr_scene_trace": (C.c_int, [_VP, _VP, _VP, _I32, _VP, _VP, _VP, _VP, _VP]),
+    "psdr_scene_trace_noalloc": (C.c_int, [_VP, _VP, _VP, _I32, _VP, _VP, _VP, _VP, _VP]),
     "psdr_scene_derived": (C.c_int, [_VP, _VP, _VP, _VP, _VP]),
     "psdr_scene_samplers": (C.c_int, [_VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP]),
     "psdr_scene_bvh": (C.c_int, [_VP, _VP, _VP, _VP, _VP, _VP, _VP]),

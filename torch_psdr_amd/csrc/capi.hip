@@ -14,6 +14,7 @@
 #include "cam_grad.h"
 #include "render.h"
 #include "scene.h"
+#include "traverse.h"
 
 namespace psdr {
 
@@ -641,6 +642,48 @@ static int check_render_sizes(const psdr_config& c, const psdr_camera& cam) {
   return PSDR_ERR_ARG;
 }
 
+// Scratch of psdr_scene_trace_noalloc: the overflow area of the traversal
+// stacks (entries 32..63 of each lane's stack; 0..31 live in LDS) of
+// NOALLOC_BLOCKS persistent workgroups, 32 MiB per device, held by every device
+// the library runs on until the process ends.  Nothing in it must start at
+// zero: a lane writes a slot (push) before it reads it (pop) in every
+// traversal.  Host code uses the per-device address from hipGetSymbolAddress,
+// never the array's name (that is a host-side stand-in, not a device address).
+namespace {
+constexpr int NOALLOC_BLOCKS = 1024;
+constexpr int NOALLOC_STACK_LDS = 32;
+constexpr size_t NOALLOC_SCRATCH_INTS =
+    static_cast<size_t>(NOALLOC_BLOCKS) * TRACE_BLOCK_THREADS * (STACK_TOTAL - NOALLOC_STACK_LDS);
+static __device__ __align__(256) int scratch[NOALLOC_SCRATCH_INTS];
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_noalloc_mu;
+int* g_noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// The current device's copy of `scratch`, looked up once per device.
+int noalloc_scratch(int** out) {
+  int dev = -1;
+  PSDR_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) {
+    set_error(PSDR_ERR_ARG, "scene_trace_noalloc: device ordinal beyond the library's per-device table");
+    return PSDR_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (!g_noalloc_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      (void)hipGetLastError();
+      set_error(PSDR_ERR_HIP, std::string("scene_trace_noalloc: hipGetSymbolAddress of the scratch array failed: ") +
+                                  hipGetErrorString(e));
+      return PSDR_ERR_HIP;
+    }
+    g_noalloc_ptr[dev] = static_cast<int*>(p);
+  }
+  *out = g_noalloc_ptr[dev];
+  return PSDR_OK;
+}
+}  // namespace
+
 extern "C" {
 
 const char* psdr_last_error(void) { return g_last_error.c_str(); }
@@ -891,8 +934,8 @@ int psdr_camera_matrices(const psdr_camera* cam, float* c2w, float* w2c, float* 
 
 int psdr_scene_trace(psdr_scene* s, const float* ro, const float* rd, int32_t n, int32_t* tri, float* t, float* u,
                      float* v, void* stream) {
-  if (!s || n < 0 || (n > 0 && (!ro || !rd || !tri || !u || !v))) {
-    set_error(PSDR_ERR_ARG, "scene_trace: bad argument");
+  if (!s || n < 0 || n > PSDR_NOALLOC_MAX_RAYS || (n > 0 && (!ro || !rd || !tri || !u || !v))) {
+    set_error(PSDR_ERR_ARG, "scene_trace: bad argument (n must lie in [0, (2^31 - 1) / 3])");
     return PSDR_ERR_ARG;
   }
   if (n == 0) return PSDR_OK;
@@ -900,6 +943,26 @@ int psdr_scene_trace(psdr_scene* s, const float* ro, const float* rd, int32_t n,
   PSDR_TRY(s->ws->spill.ensure(std::max(s->ws->spill.bytes, spill_bytes(grid))));
   return trace_rays(s->view(), ro, rd, n, nullptr, 1, 0, nullptr, 0, tri, t, u, v, s->ws->spill.as<int>(), grid,
                     as_stream(stream));
+}
+
+int psdr_scene_trace_noalloc(psdr_scene* s, const float* ro, const float* rd, int32_t n, int32_t* tri, float* t,
+                             float* u, float* v, void* stream) {
+  if (!s || n < 0 || (n > 0 && (!ro || !rd || !tri || !u || !v))) {
+    set_error(PSDR_ERR_ARG, "scene_trace_noalloc: bad argument");
+    return PSDR_ERR_ARG;
+  }
+  if (n > PSDR_NOALLOC_MAX_RAYS) {
+    set_error(PSDR_ERR_ARG, "scene_trace_noalloc: n exceeds PSDR_NOALLOC_MAX_RAYS (ray buffers are indexed as 3 * i "
+                            "in 32-bit arithmetic)");
+    return PSDR_ERR_ARG;
+  }
+  // set-up on the first call on this device (also with n == 0): the lookup may
+  // load the library's device code, which places the array
+  int* spill = nullptr;
+  PSDR_TRY(noalloc_scratch(&spill));
+  if (n == 0) return PSDR_OK;
+  return trace_rays(s->view(), ro, rd, n, nullptr, 1, 0, nullptr, 0, tri, t, u, v, spill, NOALLOC_BLOCKS,
+                    as_stream(stream), NOALLOC_STACK_LDS);
 }
 
 int psdr_trace_timing(int32_t enable, double* total_ms, int64_t* launches, int64_t* bytes, double* busy_ms) {

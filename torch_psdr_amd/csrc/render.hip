@@ -31,7 +31,7 @@ namespace psdr {
 
 namespace {
 
-constexpr int TRACE_THREADS = 256;
+constexpr int TRACE_THREADS = TRACE_BLOCK_THREADS;
 // render counters: [d] paths entering shade(d)
 constexpr int COUNTERS = 128;
 
@@ -547,7 +547,7 @@ constexpr size_t TIMER_PAIRS = 4096;
 
 int trace_rays(const SceneView& sv, const float* ro, const float* rd, int n, const int* count_dev, int K, int seg,
                const float* tl, int nvis, int* tri, float* t, float* u, float* v, int* spill, int grid,
-               hipStream_t st) {
+               hipStream_t st, int stack_lds) {
   TraceTimer& tt = trace_timer();
   std::lock_guard<std::mutex> lock(tt.mu);
   const bool timed = tt.on && tt.used < TIMER_PAIRS;
@@ -555,7 +555,7 @@ int trace_rays(const SceneView& sv, const float* ro, const float* rd, int n, con
   if (timed) PSDR_HIP_CHECK(hipEventRecord(tt.ev[2 * tt.used], st));
   unsigned long long* tb = timed ? tt.bytes : nullptr;
   unsigned long long* sp = timed ? tt.span + 2 * tt.used : nullptr;
-  switch (tuning().trace_stack.load()) {
+  switch (stack_lds ? stack_lds : tuning().trace_stack.load()) {
     case 8:
       hipLaunchKernelGGL(k_trace<8>, dim3(grid), dim3(TRACE_THREADS), 0, st, sv, ro, rd, n, count_dev, K, seg, tl, nvis,
                          tri, t, u, v, spill, tb, sp);
