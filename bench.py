@@ -18,6 +18,11 @@ beside it.  Before the timed region the autotuner (torch_psdr_amd.tune) picks
 the launch shapes by timing this step (reported under "autotune"; --no-autotune
 keeps the defaults).
 
+After the timed regions: the same step in the reference's launch shape (the
+plain GPU path), the gradient error of both shapes against directional central
+differences of the loss (grad_error), one step with the traversal's counting
+instance (traversal_counters) and the CPU oracle on a bounded sample.
+
 Prints one JSON line.  value = Mrays/s of reference-equivalent rays:
 R*N (N = spp*H*W, R = 1 + D*(L+B+1) = 10) + 20 rays per primary-boundary sample
 + 3 per direct-boundary sample, divided by the step time.
@@ -59,6 +64,8 @@ def main():
     ap.add_argument("--workload", choices=("c2", "c3"), default="c2",
                     help="c2: the 512^2 / 16 spp vertex + texture step (default); c3: 64 scenes of uneven size, "
                          "rendered and differentiated in turn on shared buffers")
+    ap.add_argument("--no-grad-error", action="store_true",
+                    help="skip the gradient-error leg (directional central differences of the loss)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device-to-device copy (scales rocprof FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -170,9 +177,16 @@ def main():
         if args.graph == "value":
             ms, mrays = graph["ms_per_step"], graph["value"]
     plain = None
+    grads = {}
     if args.plain_steps > 0:
-        plain = plain_baseline(torch, ops, step, verts, tex, args.plain_steps, rays)
+        plain = plain_baseline(torch, ops, step, verts, tex, args.plain_steps, rays, grads)
         plain["speedup"] = round(plain["ms_per_step"] / ms, 2)
+    if "fused" not in grads and not args.no_grad_error:
+        step()
+        torch.cuda.synchronize()
+        grads["fused"] = (verts.grad.clone(), tex.grad.clone())
+    gerr = None if args.no_grad_error else grad_error(torch, parts, integ, target, verts, tex, grads)
+    counters = trace_counter_pass(torch, N, step)
     step_bytes = step_algorithmic_bytes(N_int, N_e, spp, D, L, B)
     cpu = None if args.no_cpu_baseline else cpu_baseline(parts, spp, args.cpu_seconds)
     out = {
@@ -198,6 +212,8 @@ def main():
                           "algorithmic_bytes": step_bytes},
         "cpu_baseline": cpu,
         "plain_gpu_path": plain,
+        "grad_error": gerr,
+        "traversal_counters": counters,
         "graph_step": graph,
         "autotune": tuned,
     }
@@ -283,7 +299,63 @@ def graph_step_timing(torch, step, n_steps, rays):
             "steps": n_steps}
 
 
-def plain_baseline(torch, ops, step, verts, tex, n_steps, rays):
+def trace_counter_pass(torch, N, step):
+    """One more eager step with the traversal's counting instance (outside the
+    timed region): rays traced, BVH nodes fetched and triangles tested per ray,
+    and the closest-hit rays whose hit is a near tie (closest and second-closest
+    triangle within 4 ulp) or grazes an edge (barycentrics within 1e-6), the
+    rays whose identity another implementation's arithmetic could change."""
+    lib = N.load()
+    N.check(lib.psdr_trace_counters(1, None, 0))
+    step()
+    torch.cuda.synchronize()
+    c = (ctypes.c_int64 * N.PSDR_TRACE_COUNTERS)()
+    N.check(lib.psdr_trace_counters(0, c, N.PSDR_TRACE_COUNTERS))
+    rays = max(int(c[0]), 1)
+    return {"rays": int(c[0]), "nodes_per_ray": round(c[1] / rays, 3), "triangles_per_ray": round(c[2] / rays, 3),
+            "exact_tests_per_ray": round(c[3] / rays, 3), "hits": int(c[6]), "closest_hit_rays": int(c[7]),
+            "near_ties": int(c[4]), "grazing": int(c[5])}
+
+
+def grad_error(torch, parts, integ, target, verts, tex, grads, n_dirs=3, h_tex=1e-2, h_vert=1e-4, seed=17):
+    """Gradient error against central differences of the same estimator (same
+    sample streams): along n_dirs random directions w per parameter block, the
+    relative L2 error over the directions of dL.w (renderD's gradient) against
+    (L(p + h w) - L(p - h w)) / 2h with L the step's L2 loss rendered by renderC,
+    in double.  Texel directions are N(0, 1) per texel channel; vertex
+    directions N(0, 1) per coordinate.  The texel part of the estimator is
+    smooth in the texels, so its error is the estimator's own; the vertex part
+    carries the reference's boundary normalisation (samples summed per pixel
+    over the edge spp only, hypercube cell pmf used as a density), which a
+    difference quotient of the image does not, so its error is large by
+    construction and equal for both launch shapes."""
+    g = torch.Generator().manual_seed(seed)
+
+    def loss(v, t):
+        with torch.no_grad():
+            img = integ.renderC(parts["scene"](v, t))[0]
+            return float(((img.double() - target.double()) ** 2).mean())
+
+    out = {}
+    for name, p, h in (("texels", tex, h_tex), ("vertices", verts, h_vert)):
+        dirs = [torch.randn(p.shape, generator=g).to(p.device) for _ in range(n_dirs)]
+        fd = []
+        for w in dirs:
+            plus = loss(verts.detach() + h * w, tex.detach()) if name == "vertices" else loss(verts.detach(), tex.detach() + h * w)
+            minus = loss(verts.detach() - h * w, tex.detach()) if name == "vertices" else loss(verts.detach(), tex.detach() - h * w)
+            fd.append((plus - minus) / (2 * h))
+        fd_n = sum(x * x for x in fd) ** 0.5
+        rec = {"h": h, "directions": n_dirs, "fd": [round(x, 9) for x in fd]}
+        for shape, gg in grads.items():
+            gp = gg[0] if name == "vertices" else gg[1]
+            an = [float((gp.double() * w.double()).sum()) for w in dirs]
+            err = sum((a - b) ** 2 for a, b in zip(an, fd)) ** 0.5 / max(fd_n, 1e-300)
+            rec[shape] = {"rel_l2_error": round(err, 6), "directional": [round(x, 9) for x in an]}
+        out[name] = rec
+    return out
+
+
+def plain_baseline(torch, ops, step, verts, tex, n_steps, rays, grads):
     """The same step in the reference's launch shape (full-width launches without
     compaction, one synchronised trace launch per ray kind, boundary samples in
     lane order, gradient terms one after another).  Both shapes compute the same
@@ -292,10 +364,12 @@ def plain_baseline(torch, ops, step, verts, tex, n_steps, rays):
     step()
     torch.cuda.synchronize()
     g_fused = (verts.grad.clone(), tex.grad.clone())
+    grads["fused"] = g_fused
     with ops.launch_shape("plain"):
         step()  # warm-up, also the comparison step
         torch.cuda.synchronize()
         g_plain = (verts.grad.clone(), tex.grad.clone())
+        grads["plain"] = g_plain
         t0 = time.perf_counter()
         for _ in range(n_steps):
             step()
@@ -307,13 +381,15 @@ def plain_baseline(torch, ops, step, verts, tex, n_steps, rays):
 
 
 # HBM bytes per k_trace launch on the default workload from two rocprofv3 --pmc
-# runs (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch, summed over the step's
+# runs (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch, summed over one step's 18
 # traversal launches), each scaled by what the counter reads for a 1 GiB
-# device-to-device copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0), at the launch shape
-# named here; another shape reports null.  The commands and per-launch table are
-# in profiles/k_trace_gradient_step_512px_16spp.md.
-TRACE_TRAFFIC = {"bytes_per_launch": 63579410,
-                 "shape": {"trace_blocks_per_cu": 32, "trace_stack_lds": 8}}
+# device-to-device copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0), at the launch
+# shapes named here; another shape reports null.  Commands and per-launch table:
+# profiles/k_trace_gradient_step_512px_16spp.md.
+TRACE_TRAFFIC = [
+    {"shape": {"trace_blocks_per_cu": 16, "trace_stack_lds": 16}, "bytes_per_launch": 63177711},
+    {"shape": {"trace_blocks_per_cu": 32, "trace_stack_lds": 8}, "bytes_per_launch": 63579410},
+]
 
 
 def trace_roofline(busy_ms, event_ms, launches, bytes_, default_workload, shape):
@@ -329,11 +405,11 @@ def trace_roofline(busy_ms, event_ms, launches, bytes_, default_workload, shape)
     ach = bytes_ / sec / 1e9 if sec > 0 else 0.0
     ev_sec = event_ms * 1e-3
     ev_ach = bytes_ / ev_sec / 1e9 if ev_sec > 0 else 0.0
-    same_shape = all(shape.get(k) == v for k, v in TRACE_TRAFFIC["shape"].items())
+    measured = [t for t in TRACE_TRAFFIC if all(shape.get(k) == v for k, v in t["shape"].items())]
     return {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(ach / HBM_PEAK_GBS, 5),
-            "traffic": TRACE_TRAFFIC["bytes_per_launch"] if (default_workload and same_shape) else None,
-            "traffic_profile_shape": TRACE_TRAFFIC["shape"],
+            "traffic": measured[0]["bytes_per_launch"] if (default_workload and measured) else None,
+            "traffic_profiled_shapes": [t["shape"] for t in TRACE_TRAFFIC],
             "kernel": "k_trace (closest-hit LBVH traversal)", "launches": launches,
             "algorithmic_bytes_per_launch": round(bytes_ / max(launches, 1)),
             "avg_launch_ms": round(busy_ms / max(launches, 1), 4),

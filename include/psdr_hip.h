@@ -319,6 +319,23 @@ int psdr_last_render_stats(const psdr_scene* scene, int64_t* counts, int32_t n_c
  * Process-wide; not for concurrent callers. */
 int psdr_trace_timing(int32_t enable, double* total_ms, int64_t* launches, int64_t* bytes, double* busy_ms);
 
+/* Traversal counters (no reference counterpart; a profiling mode, also the
+ * count of closest hits whose identity depends on rounding).  enable = 1
+ * synchronises the device, clears the totals and makes every later traversal
+ * launch use a counting instance of the kernel (same results, slower: it
+ * prunes against the second-closest hit); enable = 0 stops, synchronises and
+ * writes up to n_counts totals: [0] rays, [1] BVH nodes fetched, [2] triangles
+ * tested (division-free prefilter), [3] exact ray-triangle tests, [4]
+ * closest-hit rays whose closest and second-closest triangle distances lie
+ * within 4 ulp (a near tie: another triangle test order or OptiX's arithmetic
+ * could pick the other one), [5] closest-hit rays whose hit barycentrics lie
+ * within 1e-6 of an edge (grazing: a neighbour could claim the hit), [6] rays
+ * that hit, [7] closest-hit rays (visibility queries stop at the first
+ * occluder and are not classified).  Process-wide; not for concurrent
+ * callers. */
+#define PSDR_TRACE_COUNTERS 8
+int psdr_trace_counters(int32_t enable, int64_t* counts, int32_t n_counts);
+
 /* Launch shapes of the hot kernels, chosen at run time by the autotuner
  * (torch_psdr_amd/tune.py) or by hand.  No counterpart in the reference, whose
  * launch shapes belong to OptiX and Enoki (src/scene_optix.cc:267,

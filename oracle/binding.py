@@ -278,6 +278,18 @@ def closest_hit(tri9: np.ndarray, ro: np.ndarray, rd: np.ndarray, brute: bool = 
     return tri, t, u, v
 
 
+def hit_classes(tri9: np.ndarray, ro: np.ndarray, rd: np.ndarray) -> np.ndarray:
+    """Per ray: bit 0 = the closest and second-closest triangle distances lie
+    within 4 ulp (near tie), bit 1 = the hit's barycentrics lie within 1e-6 of
+    an edge (grazing)."""
+    tri9 = np.ascontiguousarray(tri9, np.float32).reshape(-1, 9)
+    ro = np.ascontiguousarray(ro, np.float32)
+    rd = np.ascontiguousarray(rd, np.float32)
+    cls = np.zeros(ro.shape[0], np.int32)
+    load().oracle_hit_classes(_p(tri9), tri9.shape[0], _p(ro), _p(rd), ro.shape[0], _p(cls))
+    return cls
+
+
 def lbvh(tri9: np.ndarray):
     tri9 = np.ascontiguousarray(tri9, np.float32).reshape(-1, 9)
     n = tri9.shape[0]

@@ -108,11 +108,19 @@ struct CpuBvh {
     idx.resize(n);
     std::iota(idx.begin(), idx.end(), 0);
     std::vector<float> blo(3 * n), bhi(3 * n), cen(3 * n);
+    // Boxes are padded beyond the triangle by more than the rounding of the
+    // exact test, which can accept a ray passing outside the triangle by a few
+    // ulp of the coordinates involved (rays through vertices): relative to the
+    // box's own coordinates and to the scene's extent.
+    float scale = 0.f;
+    for (int i = 0; i < n; ++i)
+      for (int a = 0; a < 3; ++a)
+        scale = std::max(scale, std::max(std::fabs(P0[i][a]), std::max(std::fabs(P1[i][a]), std::fabs(P2[i][a]))));
     for (int i = 0; i < n; ++i)
       for (int a = 0; a < 3; ++a) {
         float x0 = P0[i][a], x1 = P1[i][a], x2 = P2[i][a];
         float lo = std::min(x0, std::min(x1, x2)), hi = std::max(x0, std::max(x1, x2));
-        float pad = (std::fabs(lo) + std::fabs(hi)) * 1e-6f + 1e-30f;
+        float pad = (std::fabs(lo) + std::fabs(hi)) * 1e-5f + scale * 1e-6f + 1e-30f;
         blo[3 * i + a] = lo - pad;
         bhi[3 * i + a] = hi + pad;
         cen[3 * i + a] = 0.5f * (lo + hi);
@@ -192,6 +200,42 @@ struct CpuBvh {
               h.u = u;
               h.v = v;
             }
+          }
+        }
+      } else {
+        stack[sp++] = nd.left;
+        stack[sp++] = nd.right;
+      }
+    }
+    return h;
+  }
+
+  // Closest hit plus the second-smallest t among the other triangles the ray
+  // hits (boxes pruned against that second distance).
+  Hit closest2(const V3<float>& o, const V3<float>& d, float* t2) const {
+    Hit h;
+    *t2 = std::numeric_limits<float>::infinity();
+    if (n == 0) return h;
+    const V3<float> inv{1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+    int stack[128];
+    int sp = 0;
+    stack[sp++] = 0;
+    while (sp) {
+      const Node& nd = nodes[stack[--sp]];
+      if (!box_hit(nd, o, inv, *t2)) continue;
+      if (nd.left < 0) {
+        for (int k = nd.first; k < nd.first + nd.count; ++k) {
+          const int tri = idx[k];
+          float t, u, v;
+          if (!ray_tri(o, d, p0[tri], e1[tri], e2[tri], &t, &u, &v)) continue;
+          if (t < h.t || (t == h.t && tri < h.tri)) {
+            *t2 = std::min(*t2, h.t);
+            h.t = t;
+            h.tri = tri;
+            h.u = u;
+            h.v = v;
+          } else {
+            *t2 = std::min(*t2, t);
           }
         }
       } else {
@@ -1182,6 +1226,38 @@ void oracle_closest_hit(const float* tri9, int n_tris, const float* ro, const fl
       t[i] = h.tri >= 0 ? h.t : -1.f;
       u[i] = h.u;
       v[i] = h.v;
+    }
+  });
+}
+
+// Per-ray class of the closest hit: bit 0 = near tie (closest and
+// second-closest triangle distances within 4 ulp), bit 1 = grazing (hit
+// barycentrics within 1e-6 of an edge).
+void oracle_hit_classes(const float* tri9, int n_tris, const float* ro, const float* rd, int n_rays, int* cls) {
+  std::vector<V3<float>> P0(n_tris), P1(n_tris), P2(n_tris);
+  for (int f = 0; f < n_tris; ++f) {
+    const float* q = tri9 + 9 * f;
+    P0[f] = {q[0], q[1], q[2]};
+    P1[f] = {q[3], q[4], q[5]};
+    P2[f] = {q[6], q[7], q[8]};
+  }
+  CpuBvh bvh;
+  bvh.build(P0, P1, P2);
+  parallel_for(n_rays, [&](int64_t b, int64_t e) {
+    for (int64_t i = b; i < e; ++i) {
+      const V3<float> o{ro[3 * i], ro[3 * i + 1], ro[3 * i + 2]};
+      const V3<float> d{rd[3 * i], rd[3 * i + 1], rd[3 * i + 2]};
+      float t2;
+      const Hit h = bvh.closest2(o, d, &t2);
+      int c = 0;
+      if (h.tri >= 0) {
+        int32_t a, b2;
+        std::memcpy(&a, &h.t, 4);
+        std::memcpy(&b2, &t2, 4);
+        if (std::isfinite(t2) && b2 - a <= 4) c |= 1;
+        if (std::fmin(std::fmin(h.u, h.v), 1.0f - h.u - h.v) < 1e-6f) c |= 2;
+      }
+      cls[i] = c;
     }
   });
 }

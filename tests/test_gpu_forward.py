@@ -121,20 +121,41 @@ def test_closest_hit_matches_brute_force(gpu_device, oracle, level):
     d[k, k % 3] = np.where(k % 2 == 0, 0.0, -0.0)
     d[k[::3], (k[::3] + 1) % 3] = 0.0
     d[k] /= np.linalg.norm(d[k], axis=1, keepdims=True)
+    # rays aimed at mesh vertices from the camera-like origin: grazing hits
+    # (barycentrics at a corner), where another triangle test could claim the hit
+    Vs = flat.vertices.detach().cpu().numpy()
+    kv = np.arange(31000, 41000)
+    d[kv] = Vs[rng.integers(0, Vs.shape[0] - 8, kv.size)] - o[kv]
+    d[kv] /= np.linalg.norm(d[kv], axis=1, keepdims=True)
     ro, rd = torch.tensor(o, device=gpu_device), torch.tensor(d, device=gpu_device)
     tri = torch.empty(n, dtype=torch.int32, device=gpu_device)
     t, u, v = (torch.empty(n, device=gpu_device) for _ in range(3))
-    N.check(ns.lib.psdr_scene_trace(ns.handle, ro.data_ptr(), rd.data_ptr(), n, tri.data_ptr(), t.data_ptr(),
-                                    u.data_ptr(), v.data_ptr(), _stream()))
+    lib = ns.lib
+    N.check(lib.psdr_trace_counters(1, None, 0))
+    N.check(lib.psdr_scene_trace(ns.handle, ro.data_ptr(), rd.data_ptr(), n, tri.data_ptr(), t.data_ptr(),
+                                 u.data_ptr(), v.data_ptr(), _stream()))
+    cnt = (ctypes.c_int64 * N.PSDR_TRACE_COUNTERS)()
+    N.check(lib.psdr_trace_counters(0, cnt, N.PSDR_TRACE_COUNTERS))
     torch.cuda.synchronize()
     w_tri, w_t, w_u, w_v = oracle.closest_hit(H.tri9(flat), o, d, brute=(level < 5))
+    cls = oracle.hit_classes(H.tri9(flat), o, d)
     g_tri = tri.cpu().numpy()
-    mism = (g_tri != w_tri).sum()
-    print(f"level {level}: {n} rays, hits {(w_tri >= 0).sum()}, id mismatches {mism}")
-    assert mism == 0
-    np.testing.assert_array_equal(t.cpu().numpy(), w_t)
-    np.testing.assert_array_equal(u.cpu().numpy(), w_u)
-    np.testing.assert_array_equal(v.cpu().numpy(), w_v)
+    mism = g_tri != w_tri
+    ties, grazing = int((cls & 1).sum()), int(((cls & 2) > 0).sum())
+    print(f"level {level}: {n} rays, hits {(w_tri >= 0).sum()}, id mismatches {mism.sum()}, near ties {ties}, "
+          f"grazing {grazing}; GPU: nodes/ray {cnt[1] / cnt[0]:.2f}, triangles/ray {cnt[2] / cnt[0]:.2f}, "
+          f"exact tests/ray {cnt[3] / cnt[0]:.2f}")
+    # the counting traversal classifies the same rays as the oracle
+    assert (cnt[0], cnt[7], cnt[6]) == (n, n, int((w_tri >= 0).sum()))
+    assert (cnt[4], cnt[5]) == (ties, grazing)
+    assert grazing >= 1000 and ties >= 500  # vertex-aimed rays that reach their vertex (shared by 5-6 triangles)
+    # an id may differ from the oracle's only on a near tie or a grazing hit
+    # (the contract); t, u, v of every other ray are bit-identical
+    assert (cls[mism] != 0).all()
+    ok = ~mism
+    np.testing.assert_array_equal(t.cpu().numpy()[ok], w_t[ok])
+    np.testing.assert_array_equal(u.cpu().numpy()[ok], w_u[ok])
+    np.testing.assert_array_equal(v.cpu().numpy()[ok], w_v[ok])
 
 
 def _compare_images(g, w, tag):

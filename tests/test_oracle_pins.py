@@ -569,3 +569,35 @@ def test_normal_map_texel_jvp_matches_central_difference(oracle):
     fd = (ip - im) / (2 * h)
     assert np.abs(d_int).max() > 1e-3
     assert np.abs(fd - d_int).max() < 2e-3 * max(1.0, np.abs(fd).max())
+
+
+def test_hit_classes_flag_ties_and_grazing_hits(oracle):
+    """oracle.hit_classes: a triangle duplicated at the same place gives every
+    ray through it an exact tie (bit 0); a ray through a vertex or along the
+    shared edge of two triangles grazes (bit 1); rays through the middle of a
+    quad in front of a distant wall are neither."""
+    quad = np.array([[-1, -1, 0], [1, -1, 0], [1, 1, 0], [-1, 1, 0]], np.float32)
+    wall = quad * 4 + np.array([0, 0, -5], np.float32)
+    tris = np.stack([quad[[0, 1, 2]], quad[[0, 2, 3]], wall[[0, 1, 2]], wall[[0, 2, 3]],
+                     quad[[0, 1, 2]] + np.array([3, 0, 0], np.float32)])  # the last: a second quad, off to the side
+    tris = np.concatenate([tris, tris[4:5]])  # ... duplicated: exact ties
+    tri9 = tris.reshape(-1, 9)
+    rng = np.random.default_rng(3)
+    n = 600
+    o = np.tile(np.array([[0, 0, 5]], np.float32), (n, 1))
+    tgt = np.zeros((n, 3), np.float32)
+    tgt[:200, :2] = rng.uniform(-0.8, 0.8, (200, 2))         # quad interiors (away from the diagonal below)
+    tgt[:200, 1] = np.where(np.abs(tgt[:200, 0] - tgt[:200, 1]) < 0.05, -0.7, tgt[:200, 1])
+    a, b = rng.uniform(-0.8, 0.8, 200), rng.uniform(-0.8, 0.8, 200)
+    lo, hi = np.minimum(a, b), np.maximum(a, b)
+    tgt[200:400, 0], tgt[200:400, 1] = hi + 3, lo - 0.1   # inside the duplicated triangle (x - 3 > y)
+    tgt[400:500, 0] = tgt[400:500, 1] = rng.uniform(-0.9, 0.9, 100)  # along the shared diagonal edge
+    tgt[500:600, :2] = quad[rng.integers(0, 4, 100), :2]             # the quad's corners (vertices)
+    d = tgt - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    cls = oracle.hit_classes(tri9, o, d.astype(np.float32))
+    tri, t, u, v = oracle.closest_hit(tri9, o, d.astype(np.float32), brute=True)
+    assert (tri[:200] < 2).all() and (cls[:200] == 0).all()
+    assert (cls[200:400] & 1).all() and (tri[200:400] == 4).all()   # tie kept by the smaller id
+    assert (cls[400:500] & 2).all()
+    assert (cls[500:600] & 2).all()

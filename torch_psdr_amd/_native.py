@@ -37,6 +37,7 @@ PSDR_CAMERA_PERSPECTIVE = 0
 PSDR_CAMERA_ORTHOGRAPHIC = 1
 PSDR_CAMERA_GRAD = 10
 PSDR_NOALLOC_MAX_RAYS = 715827882
+PSDR_TRACE_COUNTERS = 8
 
 ABI_VERSION = 6
 
@@ -149,6 +150,7 @@ _SIGS = {
     "psdr_last_render_stats": (C.c_int, [_VP, C.POINTER(C.c_int64), _I32, C.POINTER(C.c_int64)]),
     "psdr_trace_timing": (C.c_int, [_I32, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                     C.POINTER(C.c_double)]),
+    "psdr_trace_counters": (C.c_int, [_I32, C.POINTER(C.c_int64), _I32]),
     "psdr_get_tuning": (C.c_int, [C.POINTER(PsdrTuning)]),
     "psdr_set_tuning": (C.c_int, [C.POINTER(PsdrTuning)]),
 }

@@ -969,6 +969,14 @@ int psdr_trace_timing(int32_t enable, double* total_ms, int64_t* launches, int64
   return trace_timing(enable, total_ms, launches, bytes, busy_ms);
 }
 
+int psdr_trace_counters(int32_t enable, int64_t* counts, int32_t n_counts) {
+  if (!enable && (!counts || n_counts < 0)) {
+    set_error(PSDR_ERR_ARG, "trace_counters: counts is NULL");
+    return PSDR_ERR_ARG;
+  }
+  return trace_counters(enable, counts, n_counts);
+}
+
 int psdr_get_tuning(psdr_tuning* out) {
   if (!out) {
     set_error(PSDR_ERR_ARG, "get_tuning: output is NULL");

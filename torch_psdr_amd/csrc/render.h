@@ -27,6 +27,7 @@ int trace_rays(const SceneView& sv, const float* ro, const float* rd, int n, con
 constexpr unsigned long long RAY_BYTES_PER = 28;  // ray buffers: origin, direction, light distance
 int trace_grid(int64_t max_rays);
 int trace_timing(int enable, double* total_ms, int64_t* launches, int64_t* bytes, double* busy_ms);
+int trace_counters(int enable, int64_t* out, int n_out);
 size_t spill_bytes(int grid);
 int lane_draws(uint32_t lane0, int n_lanes, int n_draws, uint32_t* out, hipStream_t st);
 int camera_rays(const CameraK& cam, uint32_t spp, float* o, float* d, int* pix, hipStream_t st);

@@ -58,7 +58,12 @@ __global__ void k_camera_rays(CameraK cam, uint32_t spp, int64_t N, float* o, fl
 // Closest hit of rays [0, *count) (or n if count is null); rays SoA-of-xyz
 // AoS (ro[3*i+k]); outputs per ray.
 // STACK_LDS: traversal stack entries per lane kept in LDS (the rest spill)
-template <int STACK_LDS>
+// STATS (profiling launches only): per-launch counters of rays, nodes fetched,
+// triangles tested (division-free prefilter, then the exact test), closest-hit
+// rays whose hit is a near tie or grazes an edge (traverse.h near_tie /
+// grazing), and hits; the results equal those of the plain instance.
+constexpr int TRACE_COUNTERS = 8;
+template <int STACK_LDS, bool STATS>
 __global__ __launch_bounds__(TRACE_THREADS) void k_trace(SceneView sv, const float* __restrict__ ro,
                                                          const float* __restrict__ rd, int n,
                                                          const int* __restrict__ count, int K, int seg,
@@ -67,8 +72,14 @@ __global__ __launch_bounds__(TRACE_THREADS) void k_trace(SceneView sv, const flo
                                                          float* __restrict__ ot, float* __restrict__ ou,
                                                          float* __restrict__ ov, int* __restrict__ spill,
                                                          unsigned long long* __restrict__ timed_bytes,
-                                                         unsigned long long* __restrict__ span) {
+                                                         unsigned long long* __restrict__ span,
+                                                         unsigned long long* __restrict__ counters) {
   __shared__ int stack[STACK_LDS * TRACE_THREADS];
+  __shared__ unsigned long long cnt_lds[TRACE_COUNTERS];
+  if (STATS) {
+    if (threadIdx.x < TRACE_COUNTERS) cnt_lds[threadIdx.x] = 0;
+    __syncthreads();
+  }
   // timed launch: span[0] = earliest workgroup start, span[1] = latest
   // workgroup end on the device's constant-rate clock, so span[1] - span[0] is
   // the launch's execution time without the time it waited for CUs
@@ -97,8 +108,23 @@ __global__ __launch_bounds__(TRACE_THREADS) void k_trace(SceneView sv, const flo
       tmax = vis_tmax(t);
       tocc = vis_tocc(t);
     }
-    const HitRec h =
-        closest_hit<STACK_LDS>(sv.node, sv.leaf, sv.n_bvh, sv.n_tris, o, d, stack, spill + gtid, stride, tmax, tocc);
+    HitRec h;
+    if (STATS) {
+      RayStats st;
+      h = closest_hit<STACK_LDS>(sv.node, sv.leaf, sv.n_bvh, sv.n_tris, o, d, stack, spill + gtid, stride, tmax, tocc,
+                                 st);
+      const bool closest = k >= nvis;  // visibility queries stop early: no second distance
+      atomicAdd(&cnt_lds[0], 1ull);
+      atomicAdd(&cnt_lds[1], static_cast<unsigned long long>(st.nodes));
+      atomicAdd(&cnt_lds[2], static_cast<unsigned long long>(st.leaves));
+      atomicAdd(&cnt_lds[3], static_cast<unsigned long long>(st.exacts));
+      if (closest && near_tie(h, st.t2)) atomicAdd(&cnt_lds[4], 1ull);
+      if (closest && grazing(h)) atomicAdd(&cnt_lds[5], 1ull);
+      if (h.tri >= 0) atomicAdd(&cnt_lds[6], 1ull);
+      if (closest) atomicAdd(&cnt_lds[7], 1ull);
+    } else {
+      h = closest_hit<STACK_LDS>(sv.node, sv.leaf, sv.n_bvh, sv.n_tris, o, d, stack, spill + gtid, stride, tmax, tocc);
+    }
     otri[i] = h.tri;
     if (ot) ot[i] = h.tri >= 0 ? h.t : -1.f;
     ou[i] = h.u;
@@ -107,6 +133,10 @@ __global__ __launch_bounds__(TRACE_THREADS) void k_trace(SceneView sv, const flo
   if (span) {
     __syncthreads();
     if (threadIdx.x == 0) atomicMax(&span[1], static_cast<unsigned long long>(wall_clock64()));
+  }
+  if (STATS) {
+    __syncthreads();
+    if (threadIdx.x < TRACE_COUNTERS) atomicAdd(&counters[threadIdx.x], cnt_lds[threadIdx.x]);
   }
 }
 
@@ -537,6 +567,8 @@ struct TraceTimer {
   unsigned long long* bytes = nullptr;
   unsigned long long* span = nullptr;  // [pair][start, end] device clock of each launch
   double clock_khz = 0.0;
+  bool count_on = false;                   // launches use the counting instance
+  unsigned long long* counters = nullptr;  // TRACE_COUNTERS totals
 };
 TraceTimer& trace_timer() {
   static TraceTimer t;
@@ -555,20 +587,25 @@ int trace_rays(const SceneView& sv, const float* ro, const float* rd, int n, con
   if (timed) PSDR_HIP_CHECK(hipEventRecord(tt.ev[2 * tt.used], st));
   unsigned long long* tb = timed ? tt.bytes : nullptr;
   unsigned long long* sp = timed ? tt.span + 2 * tt.used : nullptr;
+  unsigned long long* cn = tt.count_on ? tt.counters : nullptr;
+#define PSDR_LAUNCH_TRACE(S, STATS)                                                                                  \
+  hipLaunchKernelGGL((k_trace<S, STATS>), dim3(grid), dim3(TRACE_THREADS), 0, st, sv, ro, rd, n, count_dev, K, seg, tl, \
+                     nvis, tri, t, u, v, spill, tb, sp, cn)
   switch (stack_lds ? stack_lds : tuning().trace_stack.load()) {
     case 8:
-      hipLaunchKernelGGL(k_trace<8>, dim3(grid), dim3(TRACE_THREADS), 0, st, sv, ro, rd, n, count_dev, K, seg, tl, nvis,
-                         tri, t, u, v, spill, tb, sp);
+      if (cn) PSDR_LAUNCH_TRACE(8, true);
+      else PSDR_LAUNCH_TRACE(8, false);
       break;
     case 32:
-      hipLaunchKernelGGL(k_trace<32>, dim3(grid), dim3(TRACE_THREADS), 0, st, sv, ro, rd, n, count_dev, K, seg, tl,
-                         nvis, tri, t, u, v, spill, tb, sp);
+      if (cn) PSDR_LAUNCH_TRACE(32, true);
+      else PSDR_LAUNCH_TRACE(32, false);
       break;
     default:
-      hipLaunchKernelGGL(k_trace<16>, dim3(grid), dim3(TRACE_THREADS), 0, st, sv, ro, rd, n, count_dev, K, seg, tl,
-                         nvis, tri, t, u, v, spill, tb, sp);
+      if (cn) PSDR_LAUNCH_TRACE(16, true);
+      else PSDR_LAUNCH_TRACE(16, false);
       break;
   }
+#undef PSDR_LAUNCH_TRACE
   PSDR_HIP_CHECK(hipGetLastError());
   if (timed) {
     PSDR_HIP_CHECK(hipEventRecord(tt.ev[2 * tt.used + 1], st));
@@ -634,6 +671,29 @@ int trace_timing(int enable, double* total_ms, int64_t* launches, int64_t* bytes
   if (total_ms) *total_ms = ms;
   if (launches) *launches = static_cast<int64_t>(tt.used);
   if (bytes) *bytes = static_cast<int64_t>(b);
+  return PSDR_OK;
+}
+
+int trace_counters(int enable, int64_t* out, int n_out) {
+  TraceTimer& tt = trace_timer();
+  std::lock_guard<std::mutex> lock(tt.mu);
+  if (enable) {
+    if (!tt.counters) PSDR_HIP_CHECK(hipMalloc(&tt.counters, TRACE_COUNTERS * sizeof(unsigned long long)));
+    PSDR_HIP_CHECK(hipDeviceSynchronize());
+    PSDR_HIP_CHECK(hipMemset(tt.counters, 0, TRACE_COUNTERS * sizeof(unsigned long long)));
+    PSDR_HIP_CHECK(hipDeviceSynchronize());
+    tt.count_on = true;
+    return PSDR_OK;
+  }
+  if (!tt.count_on) {
+    set_error(PSDR_ERR_STATE, "trace_counters: counting was not started");
+    return PSDR_ERR_STATE;
+  }
+  tt.count_on = false;
+  PSDR_HIP_CHECK(hipDeviceSynchronize());
+  unsigned long long c[TRACE_COUNTERS];
+  PSDR_HIP_CHECK(hipMemcpy(c, tt.counters, sizeof(c), hipMemcpyDeviceToHost));
+  for (int i = 0; i < n_out && i < TRACE_COUNTERS; ++i) out[i] = static_cast<int64_t>(c[i]);
   return PSDR_OK;
 }
 

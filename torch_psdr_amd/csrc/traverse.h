@@ -89,20 +89,56 @@ PSDR_HD bool tri_maybe_hit(f3 o, f3 d, f3 p0, f3 e1, f3 e2, float tbest) {
   return true;
 }
 
-__device__ __forceinline__ void leaf_test(const float4* __restrict__ leaf, int k, f3 o, f3 d, HitRec& h) {
+// Traversal statistics of a profiling launch.  NoStats compiles to nothing.
+// RayStats counts the nodes fetched and the triangles tested, and keeps the
+// second-smallest t among the triangles the ray hits; traversal then prunes
+// against that second distance instead of the closest one, so no triangle
+// that could tie with the closest hit is skipped.  The closest hit itself is
+// the same either way (equal t keeps the smaller id).
+struct NoStats {
+  static constexpr bool on = false;
+  __device__ __forceinline__ float prune(float t) const { return t; }
+  __device__ __forceinline__ void node() {}
+  __device__ __forceinline__ void leaf() {}
+  __device__ __forceinline__ void exact() {}
+  __device__ __forceinline__ void second(float) {}
+};
+struct RayStats {
+  static constexpr bool on = true;
+  float t2 = F_INF;
+  int nodes = 0, leaves = 0, exacts = 0;
+  __device__ __forceinline__ float prune(float) const { return t2; }
+  __device__ __forceinline__ void node() { ++nodes; }
+  __device__ __forceinline__ void leaf() { ++leaves; }
+  __device__ __forceinline__ void exact() { ++exacts; }
+  __device__ __forceinline__ void second(float t) { t2 = fminf(t2, t); }
+};
+
+template <class S = NoStats>
+__device__ __forceinline__ void leaf_test(const float4* __restrict__ leaf, int k, f3 o, f3 d, HitRec& h,
+                                          S& st) {
   const float4 a = leaf[3 * k], b = leaf[3 * k + 1], c = leaf[3 * k + 2];
   float t, u, v;
   const f3 p0{a.x, a.y, a.z}, e1{b.x, b.y, b.z}, e2{c.x, c.y, c.z};
-  if (!tri_maybe_hit(o, d, p0, e1, e2, h.t)) return;
+  st.leaf();
+  if (!tri_maybe_hit(o, d, p0, e1, e2, st.prune(h.t))) return;
+  st.exact();
   if (ray_tri(o, d, p0, e1, e2, &t, &u, &v)) {
     const int id = __float_as_int(a.w);
     if (t < h.t || (t == h.t && id < h.tri)) {
+      st.second(h.t);
       h.t = t;
       h.tri = id;
       h.u = u;
       h.v = v;
+    } else {
+      st.second(t);
     }
   }
+}
+__device__ __forceinline__ void leaf_test(const float4* __restrict__ leaf, int k, f3 o, f3 d, HitRec& h) {
+  NoStats st;
+  leaf_test(leaf, k, o, d, h, st);
 }
 
 // stack_lds: this block's LDS stack base ([STACK_LDS][blockDim]); spill: this
@@ -115,22 +151,23 @@ __device__ __forceinline__ void leaf_test(const float4* __restrict__ leaf, int k
 // Leaves [n_tris, n_all) (an environment light's bounding cube, whose
 // triangles span the whole scene and would only widen every node box) are not
 // in the tree: every ray tests them after the traversal.
+template <class S = NoStats>
 __device__ __forceinline__ void extra_leaves(const float4* __restrict__ leaf, int n_tris, int n_all, f3 o, f3 d,
-                                             float tocc, HitRec& h) {
+                                             float tocc, HitRec& h, S& st) {
   if (h.t < tocc && h.tri >= 0) return;
-  for (int k = n_tris; k < n_all; ++k) leaf_test(leaf, k, o, d, h);
+  for (int k = n_tris; k < n_all; ++k) leaf_test(leaf, k, o, d, h, st);
 }
 
-template <int STACK_LDS>
+template <int STACK_LDS, class S = NoStats>
 __device__ __forceinline__ HitRec closest_hit(const float4* __restrict__ node, const float4* __restrict__ leaf,
                                               int n_tris, int n_all, f3 o, f3 d, int* stack_lds, int* spill,
-                                              int spill_stride, float tmax = F_INF, float tocc = -F_INF) {
+                                              int spill_stride, float tmax, float tocc, S& st) {
   HitRec h{-1, F_INF, -1.f, -1.f};
   if (!(d.x == d.x) || n_all <= 0) return h;
   h.t = tmax;
   if (n_tris <= 1) {
-    if (n_tris == 1) leaf_test(leaf, 0, o, d, h);
-    extra_leaves(leaf, n_tris, n_all, o, d, tocc, h);
+    if (n_tris == 1) leaf_test(leaf, 0, o, d, h, st);
+    extra_leaves(leaf, n_tris, n_all, o, d, tocc, h, st);
     if (h.tri < 0) h.t = F_INF;
     return h;
   }
@@ -143,15 +180,16 @@ __device__ __forceinline__ HitRec closest_hit(const float4* __restrict__ node, c
   for (;;) {
     if (static_cast<unsigned>(ni) >= static_cast<unsigned>(n_internal)) break;  // malformed tree guard
     const float4 a = node[4 * ni], b = node[4 * ni + 1], c = node[4 * ni + 2], e = node[4 * ni + 3];
+    st.node();
     int cl = __float_as_int(e.x), cr = __float_as_int(e.y);
-    float tl = slab_entry(a, b, c, 0, oi, inv, h.t);
-    float tr = slab_entry(a, b, c, 1, oi, inv, h.t);
+    float tl = slab_entry(a, b, c, 0, oi, inv, st.prune(h.t));
+    float tr = slab_entry(a, b, c, 1, oi, inv, st.prune(h.t));
     if (tl != F_INF && cl < 0) {
-      if ((cl & 0x7fffffff) < n_tris) leaf_test(leaf, cl & 0x7fffffff, o, d, h);
+      if ((cl & 0x7fffffff) < n_tris) leaf_test(leaf, cl & 0x7fffffff, o, d, h, st);
       tl = F_INF;
     }
     if (tr != F_INF && cr < 0) {
-      if ((cr & 0x7fffffff) < n_tris) leaf_test(leaf, cr & 0x7fffffff, o, d, h);
+      if ((cr & 0x7fffffff) < n_tris) leaf_test(leaf, cr & 0x7fffffff, o, d, h, st);
       tr = F_INF;
     }
     if (h.t < tocc && h.tri >= 0) break;  // occluded before the target
@@ -176,9 +214,27 @@ __device__ __forceinline__ HitRec closest_hit(const float4* __restrict__ node, c
       ni = (sp < STACK_LDS) ? stack_lds[sp * nthr + lane] : spill[(sp - STACK_LDS) * spill_stride];
     }
   }
-  extra_leaves(leaf, n_tris, n_all, o, d, tocc, h);
+  extra_leaves(leaf, n_tris, n_all, o, d, tocc, h, st);
   if (h.tri < 0) h.t = F_INF;
   return h;
+}
+
+template <int STACK_LDS>
+__device__ __forceinline__ HitRec closest_hit(const float4* __restrict__ node, const float4* __restrict__ leaf,
+                                              int n_tris, int n_all, f3 o, f3 d, int* stack_lds, int* spill,
+                                              int spill_stride, float tmax = F_INF, float tocc = -F_INF) {
+  NoStats st;
+  return closest_hit<STACK_LDS>(node, leaf, n_tris, n_all, o, d, stack_lds, spill, spill_stride, tmax, tocc, st);
+}
+
+// A closest hit within 4 ulp of the next triangle the ray hits (an order
+// between them that another triangle test could flip), and one whose
+// barycentrics lie within 1e-6 of an edge (a neighbour could claim it).
+__device__ __forceinline__ bool near_tie(const HitRec& h, float t2) {
+  return h.tri >= 0 && t2 < F_INF && __float_as_int(t2) - __float_as_int(h.t) <= 4;
+}
+__device__ __forceinline__ bool grazing(const HitRec& h) {
+  return h.tri >= 0 && fminf(fminf(h.u, h.v), 1.0f - h.u - h.v) < 1e-6f;
 }
 
 // Window of a visibility query toward a target at distance t: a hit at least
