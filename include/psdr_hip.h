@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define PSDR_ABI_VERSION 6
+#define PSDR_ABI_VERSION 7
 
 /* status codes */
 #define PSDR_OK 0
@@ -193,6 +193,19 @@ int psdr_scene_info(const psdr_scene* scene, int32_t* n_tris, int32_t* n_edges, 
 int psdr_render_forward(psdr_scene* scene, const psdr_config* cfg, const psdr_camera* cam, float* out_rgb,
                         int32_t keep, void* stream);
 
+/* The images of several cameras of one scene in one wavefront
+ * (PathIntegrator::render_ renders the cameras of a scene one after another,
+ * src/path_integrator.cc:385-403): every stage is one launch over the lanes of
+ * all cameras, concatenated (camera c's lanes keep their own indices 0 ..
+ * spp*H_c*W_c - 1, so each image equals psdr_render_forward's for that
+ * camera).  out_rgb[c] receives camera c's (H_c, W_c, 3) image.  With keep != 0
+ * each camera's forward state is kept for psdr_render_backward with that
+ * camera.  At most PSDR_MAX_CAMERAS cameras per call (a scene keeps the forward
+ * states of as many (config, camera) pairs). */
+#define PSDR_MAX_CAMERAS 4
+int psdr_render_forward_multi(psdr_scene* scene, const psdr_config* cfg, const psdr_camera* cams, int32_t n_cams,
+                              float* const* out_rgb, int32_t keep, void* stream);
+
 /* Terms of renderD's image derivative (src/path_integrator.cc:381-406). */
 #define PSDR_TERM_INTERIOR 1 /* render_interior (path replay)                  */
 #define PSDR_TERM_PRIMARY 2  /* render_primary_boundary (silhouette edges)     */
@@ -297,7 +310,8 @@ int psdr_scene_samplers(const psdr_scene* scene, float* light_pmf, float* light_
 int psdr_scene_bvh(const psdr_scene* scene, uint32_t* morton, int32_t* order, int32_t* child, int32_t* parent,
                    float* box, void* stream);
 
-/* Per-lane interior radiance of the last forward render (device [N*3]). */
+/* Per-lane interior radiance of the last forward render (device [N*3]; the
+ * lanes of every camera of a psdr_render_forward_multi, concatenated). */
 int psdr_last_lane_radiance(const psdr_scene* scene, float* out, void* stream);
 
 /* Counters of the last forward render: active paths entering each depth
@@ -350,7 +364,11 @@ typedef struct psdr_tuning {
   int32_t shade_blocks_per_cu;   /* wavefront shading kernel grid, 1..64 (default 8) */
   int32_t adjoint_blocks_per_cu; /* path-replay adjoint kernel grid, 1..64 (default 16) */
   int32_t direct_blocks_per_cu;  /* direct-boundary kernel grid, 1..64 (default 8) */
+  int32_t tile_lanes;            /* lanes per wavefront tile of a forward render: 0 = all lanes at once
+                                    (default), else >= PSDR_MIN_TILE_LANES; the queue, ray and hit buffers
+                                    hold one tile, so their memory is O(tile) instead of O(spp*H*W) */
 } psdr_tuning;
+#define PSDR_MIN_TILE_LANES 65536
 
 int psdr_get_tuning(psdr_tuning* out);
 /* PSDR_ERR_ARG (nothing changed) when a field is outside its range. */

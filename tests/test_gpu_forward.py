@@ -247,3 +247,35 @@ def test_scene_create_rejects_a_bad_tri_obj(gpu_device, bad):
     desc = flat.desc(keep)
     N.check(lib.psdr_scene_create(N.C.byref(desc), N.C.byref(handle), _stream()))
     lib.psdr_scene_destroy(handle)
+
+
+
+@pytest.mark.parametrize("tile", [0, 65536])
+def test_cameras_in_one_wavefront_match_single_renders(gpu_device, oracle, tile):
+    """renderC over four cameras of different sizes and kinds renders them in
+    one wavefront (psdr_render_forward_multi); each image equals the camera's
+    own render bit for bit, also with lane tiles that straddle cameras."""
+    from torch_psdr_amd import tune
+    prev = tune.get_tuning()
+    try:
+        tune.set_tuning(tile_lanes=tile)
+        meshes, lights, _ = H.objects(gpu_device, level=3, tex_res=16, bsdf_rough=0.5)
+        cams = [H.camera(gpu_device, 64), H.camera(gpu_device, 96, look_from=(1.5, 1.2, 4.5)),
+                H.ortho_camera(gpu_device, 48, look_from=(0.3, 2.5, 4.4)),
+                H.camera(gpu_device, 80, look_from=(-2.0, 0.8, 4.6))]
+        integ = dr.PathIntegrator(spp_interior=8, max_bounce=3)
+        together = integ.renderC(dr.Scene(cams, meshes, lights))
+        torch.cuda.synchronize()
+        for i, cam in enumerate(cams):
+            alone = integ.renderC(dr.Scene([cam], meshes, lights))[0]
+            torch.cuda.synchronize()
+            assert together[i].shape == alone.shape
+            np.testing.assert_array_equal(together[i].cpu().numpy(), alone.cpu().numpy())
+            assert float(alone.mean()) > 0
+        # one camera against the oracle as well
+        scene = dr.Scene([cams[2]], meshes, lights)
+        ns, flat = _native(scene)
+        ref = oracle.render(oracle.Scene(flat), integ.cfg, cams[2].params(), mode=0)
+        _compare_images(together[2].cpu().numpy(), ref, "ortho camera in a 4-camera wavefront")
+    finally:
+        tune.set_tuning(prev)

@@ -79,3 +79,40 @@ def test_autotune_picks_from_the_space_and_applies_it(gpu_device, restore_tuning
         assert all(shape[k] in vals for k, vals in space.items())
     key = tune.workload_key(dr.Scene([cam], meshes, lights), integ)
     assert "lanes16384" in key and "tris" in key
+
+
+def _tile_step(dev, res=96, spp=16):
+    meshes, lights, leaves = H.objects(dev, level=3, tex_res=16, requires_grad=True)
+    scene = dr.Scene([H.camera(dev, res)], meshes, lights)
+    integ = dr.PathIntegrator(spp_interior=spp, max_bounce=3, spp_primary_edge=1, spp_secondary_edge=1)
+    img = integ.renderD(scene)[0]
+    loss = (img * torch.linspace(0.5, 1.5, img.numel(), device=dev).view_as(img)).sum()
+    loss.backward()
+    torch.cuda.synchronize()
+    import ctypes
+    import torch_psdr_amd._native as N
+    from torch_psdr_amd import ops
+    ns, _ = ops.entry(scene.token)
+    counts = (ctypes.c_int64 * 5)()
+    traced = ctypes.c_int64()
+    N.check(ns.lib.psdr_last_render_stats(ns.handle, counts, 5, ctypes.byref(traced)))
+    return (img.detach().cpu().numpy(), leaves["sphere_v"].grad.cpu().numpy(), leaves["color"].grad.cpu().numpy(),
+            list(counts), traced.value)
+
+
+def test_lane_tiles_do_not_change_results(gpu_device, restore_tuning):
+    """The forward render in lane tiles (147,456 lanes in tiles of 65,536 and
+    100,000: the last tile partial) gives the untiled image bit for bit, the
+    same per-depth path counts and rays, and the same gradients up to the
+    order of their float atomics."""
+    tune.set_tuning(tune.DEFAULTS)
+    img0, gv0, gt0, c0, r0 = _tile_step(gpu_device)
+    assert c0[0] == 96 * 96 * 16 and r0 > c0[0]
+    for tile in (65536, 100000):
+        print("tile", tile, flush=True)
+        tune.set_tuning(tile_lanes=tile)
+        img, gv, gt, c, r = _tile_step(gpu_device)
+        np.testing.assert_array_equal(img, img0)
+        assert (c, r) == (c0, r0)
+        np.testing.assert_allclose(gv, gv0, rtol=1e-4, atol=1e-5 * np.abs(gv0).max())
+        np.testing.assert_allclose(gt, gt0, rtol=1e-4, atol=1e-5 * np.abs(gt0).max())

@@ -106,13 +106,14 @@ def test_render_accepts_the_largest_configured_sizes():
 
 def test_tuning_roundtrip_and_validation(tmp_path):
     from torch_psdr_amd import tune
-    assert ctypes.sizeof(N.PsdrTuning) == 20
+    assert ctypes.sizeof(N.PsdrTuning) == 24
     prev = tune.get_tuning()
     assert prev == tune.DEFAULTS
     try:
         tune.set_tuning(trace_blocks_per_cu=4, trace_stack_lds=32)
         assert tune.get_tuning() == dict(tune.DEFAULTS, trace_blocks_per_cu=4, trace_stack_lds=32)
-        for bad in ({"trace_stack_lds": 12}, {"shade_blocks_per_cu": 0}, {"adjoint_blocks_per_cu": 65}):
+        for bad in ({"trace_stack_lds": 12}, {"shade_blocks_per_cu": 0}, {"adjoint_blocks_per_cu": 65},
+                    {"tile_lanes": 1000}, {"tile_lanes": -1}):
             with pytest.raises(RuntimeError):
                 tune.set_tuning(**bad)
             # a rejected call changes nothing

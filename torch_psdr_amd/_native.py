@@ -38,8 +38,10 @@ PSDR_CAMERA_ORTHOGRAPHIC = 1
 PSDR_CAMERA_GRAD = 10
 PSDR_NOALLOC_MAX_RAYS = 715827882
 PSDR_TRACE_COUNTERS = 8
+PSDR_MAX_CAMERAS = 4
+PSDR_MIN_TILE_LANES = 65536
 
-ABI_VERSION = 6
+ABI_VERSION = 7
 
 
 class PsdrObject(C.Structure):
@@ -117,6 +119,7 @@ class PsdrTuning(C.Structure):
         ("shade_blocks_per_cu", C.c_int32),
         ("adjoint_blocks_per_cu", C.c_int32),
         ("direct_blocks_per_cu", C.c_int32),
+        ("tile_lanes", C.c_int32),
     ]
 
 
@@ -132,6 +135,8 @@ _SIGS = {
     "psdr_scene_share_workspace": (C.c_int, [_VP, _VP, _VP]),
     "psdr_scene_info": (C.c_int, [_VP, C.POINTER(_I32), C.POINTER(_I32), C.POINTER(_I32)]),
     "psdr_render_forward": (C.c_int, [_VP, C.POINTER(PsdrConfig), C.POINTER(PsdrCamera), _VP, _I32, _VP]),
+    "psdr_render_forward_multi": (C.c_int, [_VP, C.POINTER(PsdrConfig), C.POINTER(PsdrCamera), _I32,
+                                            C.POINTER(C.c_void_p), _I32, _VP]),
     "psdr_render_backward": (C.c_int, [_VP, C.POINTER(PsdrConfig), C.POINTER(PsdrCamera), _VP, _VP, _VP, _VP, _VP,
                                        _I32, _VP]),
     "psdr_render_forward_plain": (C.c_int, [_VP, C.POINTER(PsdrConfig), C.POINTER(PsdrCamera), _VP, _I32, _VP]),

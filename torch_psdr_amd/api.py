@@ -343,13 +343,36 @@ class PathIntegrator:
                     int(primary_edge_preprocess_rounds), int(secondary_edge_preprocess_rounds),
                     int(primary_edge_hypercube_resolution)] + [int(r) for r in secondary_edge_hypercube_resolution]
 
+    # cameras per wavefront (the reference renders them one after another,
+    # src/path_integrator.cc:385-403; here up to PSDR_MAX_CAMERAS share every launch)
+    _CAMS_PER_LAUNCH = 4
+
+    def _groups(self, scene: Scene):
+        cams = scene.cameras
+        return [cams[i:i + self._CAMS_PER_LAUNCH] for i in range(0, len(cams), self._CAMS_PER_LAUNCH)]
+
     def renderC(self, scene: Scene) -> List[Tensor]:
         ns, flat = ops.entry(scene.token)
+        out: List[Tensor] = []
         with torch.no_grad():
-            return [ops.forward_native(ns, flat, cam.params(), self.cfg, False) for cam in scene.cameras]
+            for group in self._groups(scene):
+                if len(group) == 1:
+                    out.append(ops.forward_native(ns, flat, group[0].params(), self.cfg, False))
+                else:
+                    out.extend(ops.forward_native_multi(ns, flat, [c.params() for c in group], self.cfg, False))
+        return out
 
     def renderD(self, scene: Scene) -> List[Tensor]:
         f = scene.flat
-        return [torch.ops.psdr.render(scene.token, cam.params(), self.cfg, f.vertices, f.texels, f.light_emit,
-                                      cam.param_tensor(), True)
-                for cam in scene.cameras]
+        out: List[Tensor] = []
+        for group in self._groups(scene):
+            if len(group) == 1:
+                cam = group[0]
+                out.append(torch.ops.psdr.render(scene.token, cam.params(), self.cfg, f.vertices, f.texels,
+                                                 f.light_emit, cam.param_tensor(), True))
+            else:
+                out.extend(torch.ops.psdr.render_multi(scene.token, ops.pack_cams([c.params() for c in group]),
+                                                       self.cfg,
+                                                       f.vertices, f.texels, f.light_emit,
+                                                       [c.param_tensor() for c in group], True))
+        return out

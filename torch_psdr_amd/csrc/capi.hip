@@ -821,7 +821,33 @@ int psdr_render_forward(psdr_scene* s, const psdr_config* cfg, const psdr_camera
     return PSDR_ERR_ARG;
   }
   CaptureScope cap(as_stream(stream));
-  return render_forward(s, *cfg, *cam, out_rgb, keep, as_stream(stream));
+  return render_forward(s, *cfg, cam, 1, &out_rgb, keep, as_stream(stream));
+}
+
+int psdr_render_forward_multi(psdr_scene* s, const psdr_config* cfg, const psdr_camera* cams, int32_t n_cams,
+                              float* const* out_rgb, int32_t keep, void* stream) {
+  if (!cfg_ok(cfg) || !cams || n_cams < 1 || n_cams > PSDR_MAX_CAMERAS) {
+    set_error(PSDR_ERR_ARG, "render_forward_multi: bad argument (1 to PSDR_MAX_CAMERAS cameras)");
+    return PSDR_ERR_ARG;
+  }
+  for (int c = 0; c < n_cams; ++c) {
+    if (!cam_ok(&cams[c])) {
+      set_error(PSDR_ERR_ARG, "render_forward_multi: bad camera " + std::to_string(c));
+      return PSDR_ERR_ARG;
+    }
+    PSDR_TRY(check_render_sizes(*cfg, cams[c]));
+  }
+  if (!s || !out_rgb) {
+    set_error(PSDR_ERR_ARG, "render_forward_multi: bad argument");
+    return PSDR_ERR_ARG;
+  }
+  for (int c = 0; c < n_cams; ++c)
+    if (!out_rgb[c]) {
+      set_error(PSDR_ERR_ARG, "render_forward_multi: output " + std::to_string(c) + " is NULL");
+      return PSDR_ERR_ARG;
+    }
+  CaptureScope cap(as_stream(stream));
+  return render_forward(s, *cfg, cams, n_cams, out_rgb, keep, as_stream(stream));
 }
 
 int psdr_render_backward(psdr_scene* s, const psdr_config* cfg, const psdr_camera* cam, const float* dL_dI,
@@ -988,6 +1014,7 @@ int psdr_get_tuning(psdr_tuning* out) {
   out->shade_blocks_per_cu = t.shade_bpc.load();
   out->adjoint_blocks_per_cu = t.adjoint_bpc.load();
   out->direct_blocks_per_cu = t.direct_bpc.load();
+  out->tile_lanes = t.tile_lanes.load();
   return PSDR_OK;
 }
 
@@ -1007,12 +1034,17 @@ int psdr_set_tuning(const psdr_tuning* in) {
     set_error(PSDR_ERR_ARG, "set_tuning: trace_stack_lds must be 8, 16 or 32");
     return PSDR_ERR_ARG;
   }
+  if (in->tile_lanes != 0 && in->tile_lanes < PSDR_MIN_TILE_LANES) {
+    set_error(PSDR_ERR_ARG, "set_tuning: tile_lanes must be 0 (no tiling) or at least PSDR_MIN_TILE_LANES");
+    return PSDR_ERR_ARG;
+  }
   Tuning& t = tuning();
   t.trace_bpc = in->trace_blocks_per_cu;
   t.trace_stack = in->trace_stack_lds;
   t.shade_bpc = in->shade_blocks_per_cu;
   t.adjoint_bpc = in->adjoint_blocks_per_cu;
   t.direct_bpc = in->direct_blocks_per_cu;
+  t.tile_lanes = in->tile_lanes;
   return PSDR_OK;
 }
 
@@ -1097,8 +1129,13 @@ int psdr_last_render_stats(const psdr_scene* s, int64_t* counts, int32_t n_count
     set_error(PSDR_ERR_STATE, "last_render_stats: no forward render");
     return PSDR_ERR_STATE;
   }
-  std::vector<int> c(128, 0);
-  PSDR_HIP_CHECK(hipMemcpy(c.data(), s->ws->counters_r.p, 4 * 128, hipMemcpyDeviceToHost));
+  // one block of 128 counters per lane tile: sum them
+  const int tiles = std::max(s->ws->last_tiles, 1);
+  std::vector<int> raw(128ull * tiles, 0);
+  PSDR_HIP_CHECK(hipMemcpy(raw.data(), s->ws->counters_r.p, 4ull * 128 * tiles, hipMemcpyDeviceToHost));
+  std::vector<int64_t> c(128, 0);
+  for (int t = 0; t < tiles; ++t)
+    for (int i = 0; i < 128; ++i) c[i] += raw[128ull * t + i];
   for (int i = 0; i < n_counts && i < 64; ++i) counts[i] = c[i];
   if (rays_traced) {
     // camera rays + per depth d < D: alive paths x (light + BSDF samples, + 1
@@ -1106,7 +1143,7 @@ int psdr_last_render_stats(const psdr_scene* s, int64_t* counts, int32_t n_count
     const Workspace& w = *s->ws;
     const int64_t nLB = static_cast<int64_t>(w.last_cfg.mis_light_samples) + w.last_cfg.mis_bsdf_samples;
     int64_t r = c[0];
-    for (int d = 0; d < w.last_D; ++d) r += static_cast<int64_t>(c[d + 1]) * (nLB + (d < w.last_D - 1 ? 1 : 0));
+    for (int d = 0; d < w.last_D; ++d) r += c[d + 1] * (nLB + (d < w.last_D - 1 ? 1 : 0));
     *rays_traced = r;
   }
   return PSDR_OK;

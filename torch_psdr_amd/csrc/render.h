@@ -10,6 +10,7 @@ namespace psdr {
 // blocks per CU, and the traversal stack entries kept in LDS per lane.
 struct Tuning {
   std::atomic<int> trace_bpc{8}, trace_stack{16}, shade_bpc{8}, adjoint_bpc{16}, direct_bpc{8};
+  std::atomic<int> tile_lanes{0};  // lanes per wavefront tile of a forward render, 0 = all
 };
 Tuning& tuning();
 constexpr int TUNE_BPC_MAX = 64;
@@ -31,8 +32,9 @@ int trace_counters(int enable, int64_t* out, int n_out);
 size_t spill_bytes(int grid);
 int lane_draws(uint32_t lane0, int n_lanes, int n_draws, uint32_t* out, hipStream_t st);
 int camera_rays(const CameraK& cam, uint32_t spp, float* o, float* d, int* pix, hipStream_t st);
-int render_forward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& cam, float* out, int keep,
-                   hipStream_t st);
+// The images of n_cams (1..4) cameras of one scene in one wavefront.
+int render_forward(psdr_scene* s, const psdr_config& cfg, const psdr_camera* cams, int n_cams, float* const* outs,
+                   int keep, hipStream_t st);
 // grad_cam: PSDR_CAMERA_GRAD floats or null; cam_acc: camera matrix adjoints
 // (cam_grad.h) or null
 int render_backward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& cam, const float* dL_dI,

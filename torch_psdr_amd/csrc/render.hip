@@ -151,7 +151,9 @@ __global__ void k_span_init(unsigned long long* span, int n) {
 
 struct ShadeParams {
   SceneView sv;
-  CameraK cam;
+  CameraK cam;  // one camera; with njobs > 1 the cameras of `jobs`
+  int njobs;
+  const LaneJobs* jobs;
   int depth;  // vertex depth being generated (resolve of depth-1 first when > 0)
   int D;
   uint32_t nL, nB;
@@ -181,23 +183,38 @@ __global__ void k_zero_i32(int* p, int n) {
   if (i < n) p[i] = 0;
 }
 
-__global__ void k_cam(CameraK cam, uint32_t spp, int N, QueueView q, RaysView rays, int* count_out) {
-  const int lane = blockIdx.x * blockDim.x + threadIdx.x;
-  if (lane == 0) *count_out = N;
-  if (lane >= N) return;
-  Rng r = lane_rng(static_cast<uint32_t>(lane));
+// Camera rays of the n lanes [lane0, lane0 + n) of the (concatenated) lane
+// space into queue / ray slots 0 .. n-1.
+__global__ void k_cam(CameraK cam, int njobs, const LaneJobs* __restrict__ jobs, uint32_t spp, uint32_t lane0, int n,
+                      QueueView q, RaysView rays, int* count_out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) *count_out = n;
+  if (i >= n) return;
+  const uint32_t g = lane0 + static_cast<uint32_t>(i);
+  uint32_t lane = g;
+  const CameraK* c = nullptr;
+  if (njobs > 1) {
+    const int jb = lane_job(jobs, g);
+    lane = g - jobs->off[jb];
+    c = &jobs->cam[jb];
+  }
+  Rng r = lane_rng(lane);
   const float u0 = r.next(), u1 = r.next();
   f3 o, d;
   int p;
-  camera_ray(cam, static_cast<uint32_t>(lane), spp, u0, u1, &o, &d, &p);
-  put_ray(rays, lane, o, d);
-  q.lane[lane] = static_cast<uint32_t>(lane);
-  q.rng[lane] = r.state;
+  if (njobs > 1) camera_ray(*c, lane, spp, u0, u1, &o, &d, &p);
+  else camera_ray(cam, lane, spp, u0, u1, &o, &d, &p);
+  put_ray(rays, i, o, d);
+  q.lane[i] = g;
+  q.rng[i] = r.state;
 }
 
+// MULTI: the lanes belong to several cameras (camera and forward-state
+// buffers per lane from P.jobs); otherwise to P.cam and cache0.
+template <bool MULTI>
 __global__ __launch_bounds__(256) void k_shade(ShadeParams P, QueueView qi, const int* __restrict__ count_in,
                                                RaysView rin, QueueView qo, int* __restrict__ count_out,
-                                               RaysView rout, CacheView cache, float* __restrict__ lane_L,
+                                               RaysView rout, CacheView cache0, float* __restrict__ lane_L,
                                                uint64_t* __restrict__ lane_rng_out) {
   const SceneView& sv = P.sv;
   const int cnt = *count_in;
@@ -208,31 +225,42 @@ __global__ __launch_bounds__(256) void k_shade(ShadeParams P, QueueView qi, cons
     bool alive = false;
     bool light0 = false;   // camera vertex on a light, counted with enable_light_visable
     bool dead_in = false;  // plain shape: slot of a path that finished earlier
-    uint32_t lane = 0;
+    uint32_t lane = 0;  // in the concatenated lane space
+    uint32_t loc = 0;   // the lane within its camera's render
     Rng rng{0, 0};
     f3 T = splat(1.f), L = splat(0.f);
     float J0 = 1.f;
     Its its;
     float cu = 0.f, cv = 0.f;
     f3 O = P.cam.pos;
+    int cam_ortho = P.cam.ortho;
+    CacheView cache = cache0;
     if (have && P.plain && (qi.lane[j] & LANE_DEAD)) {
       dead_in = true;
       have = false;
     }
     if (have) {
       lane = qi.lane[j];
+      loc = lane;
+      if (MULTI) {
+        const int jb = lane_job(P.jobs, lane);
+        loc = lane - P.jobs->off[jb];
+        cache = P.jobs->cache[jb];
+        O = P.jobs->cam[jb].pos;
+        cam_ortho = P.jobs->cam[jb].ortho;
+      }
       rng.state = qi.rng[j];
-      rng.inc = lane_inc(lane);
+      rng.inc = lane_inc(loc);
       if (d == 0) {
         const int ht = rin.tri[j];
         const float hu = rin.u[j], hv = rin.v[j];
         if (P.keep) {
-          cache.tri[lane] = ht;
-          cache.u[lane] = hu;
-          cache.v[lane] = hv;
+          cache.tri[loc] = ht;
+          cache.u[loc] = hu;
+          cache.v[loc] = hv;
         }
         // orthographic rays start on the camera plane, one origin per ray
-        if (P.cam.ortho) O = f3{rin.o[3 * j], rin.o[3 * j + 1], rin.o[3 * j + 2]};
+        if (cam_ortho) O = f3{rin.o[3 * j], rin.o[3 * j + 1], rin.o[3 * j + 2]};
         its = reconstruct(sv, ht, hu, hv, O, true);
         cu = hu;
         cv = hv;
@@ -254,7 +282,7 @@ __global__ __launch_bounds__(256) void k_shade(ShadeParams P, QueueView qi, cons
           const int ht = rin.tri[k * rs + j];
           const float hu = rin.u[k * rs + j], hv = rin.v[k * rs + j];
           if (P.keep) {
-            const int64_t ci = (cslot + k) * cache.N + lane;
+            const int64_t ci = (cslot + k) * cache.N + loc;
             cache.tri[ci] = ht;
             cache.u[ci] = hu;
             cache.v[ci] = hv;
@@ -287,7 +315,7 @@ __global__ __launch_bounds__(256) void k_shade(ShadeParams P, QueueView qi, cons
           const int ht = rin.tri[k * rs + j];
           const float hu = rin.u[k * rs + j], hv = rin.v[k * rs + j];
           if (P.keep) {
-            const int64_t ci = (cslot + k) * cache.N + lane;
+            const int64_t ci = (cslot + k) * cache.N + loc;
             cache.tri[ci] = ht;
             cache.u[ci] = hu;
             cache.v[ci] = hv;
@@ -318,7 +346,7 @@ __global__ __launch_bounds__(256) void k_shade(ShadeParams P, QueueView qi, cons
             const int ht = rin.tri[k * rs + j];
             const float hu = rin.u[k * rs + j], hv = rin.v[k * rs + j];
             if (P.keep) {
-              const int64_t ci = (cslot + k) * cache.N + lane;
+              const int64_t ci = (cslot + k) * cache.N + loc;
               cache.tri[ci] = ht;
               cache.u[ci] = hu;
               cache.v[ci] = hv;
@@ -370,10 +398,10 @@ __global__ __launch_bounds__(256) void k_shade(ShadeParams P, QueueView qi, cons
       const uint64_t rng_start = rng.state;
       if (P.keep) {
         const int64_t N = cache.N;
-        cache.T[(3ll * d) * N + lane] = T.x;
-        cache.T[(3ll * d + 1) * N + lane] = T.y;
-        cache.T[(3ll * d + 2) * N + lane] = T.z;
-        cache.rs[d * N + lane] = rng_start;
+        cache.T[(3ll * d) * N + loc] = T.x;
+        cache.T[(3ll * d + 1) * N + loc] = T.y;
+        cache.T[(3ll * d + 2) * N + loc] = T.z;
+        cache.rs[d * N + loc] = rng_start;
       }
       const int rs = qo.cap;
       int k = 0;
@@ -420,10 +448,10 @@ __global__ __launch_bounds__(256) void k_shade(ShadeParams P, QueueView qi, cons
       const bool finite = isfinite(sqnorm(L));
       if (P.keep) {
         const int64_t N = cache.N;
-        cache.L[lane] = L.x;
-        cache.L[N + lane] = L.y;
-        cache.L[2 * N + lane] = L.z;
-        cache.info[lane] = d | (light0 ? 256 : 0) | (finite ? 0 : 512);
+        cache.L[loc] = L.x;
+        cache.L[N + loc] = L.y;
+        cache.L[2 * N + loc] = L.z;
+        cache.info[loc] = d | (light0 ? 256 : 0) | (finite ? 0 : 512);
       }
       if (!finite) L = splat(0.f);
       L = L * J0;
@@ -732,12 +760,17 @@ int run_depths(psdr_scene* s, const WaveArgs& a, const QueueView& q0, const Rays
   const int sgrid = std::max(1, std::min(blocks(n, 256), num_cus() * tuning().shade_bpc.load()));
   QueueView qin = q0;
   RaysView rin = r0;
+  // ray k of queue entry j is at k * cap + j (k_shade); the queues of one run
+  // share a capacity >= n
+  const int cap = q_s[0].cap;
   for (int d = 0; d <= a.D; ++d) {
     const QueueView qout = q_s[d & 1];
     const RaysView rout = r_s[d & 1];
     ShadeParams P;
     P.sv = sv;
     P.cam = a.cam;
+    P.njobs = a.njobs;
+    P.jobs = a.jobs;
     P.depth = d;
     P.D = a.D;
     P.nL = a.nL;
@@ -749,15 +782,19 @@ int run_depths(psdr_scene* s, const WaveArgs& a, const QueueView& q0, const Rays
     P.Kmax = Kmax;
     P.keep = a.keep;
     P.plain = a.plain;
-    hipLaunchKernelGGL(k_shade, dim3(sgrid), dim3(256), 0, st, P, qin, a.counters + d, rin, qout, a.counters + d + 1,
-                       rout, a.cache, a.lane_L, a.lane_rng_out);
+    if (a.njobs > 1)
+      hipLaunchKernelGGL(k_shade<true>, dim3(sgrid), dim3(256), 0, st, P, qin, a.counters + d, rin, qout,
+                         a.counters + d + 1, rout, a.cache, a.lane_L, a.lane_rng_out);
+    else
+      hipLaunchKernelGGL(k_shade<false>, dim3(sgrid), dim3(256), 0, st, P, qin, a.counters + d, rin, qout,
+                         a.counters + d + 1, rout, a.cache, a.lane_L, a.lane_rng_out);
     PSDR_HIP_CHECK(hipGetLastError());
     if (d < a.D && P.K_out > 0) {
       if (a.plain) {
         // one full-width launch per ray kind, each followed by a host sync
         const int g1 = trace_grid(n);
         for (int k = 0; k < P.K_out; ++k) {
-          const int64_t o = static_cast<int64_t>(k) * n;
+          const int64_t o = static_cast<int64_t>(k) * cap;
           const int rc = trace_rays(sv, rout.o + 3 * o, rout.d + 3 * o, n, nullptr, 1, 0, rout.tl + o,
                                     k < static_cast<int>(a.nL) ? 1 : 0, rout.tri + o, nullptr, rout.u + o,
                                     rout.v + o, spill.as<int>(), g1, st);
@@ -765,7 +802,7 @@ int run_depths(psdr_scene* s, const WaveArgs& a, const QueueView& q0, const Rays
           PSDR_HIP_CHECK(hipStreamSynchronize(st));
         }
       } else {
-        const int rc = trace_rays(sv, rout.o, rout.d, n, a.counters + d + 1, P.K_out, n, rout.tl,
+        const int rc = trace_rays(sv, rout.o, rout.d, n, a.counters + d + 1, P.K_out, cap, rout.tl,
                                   static_cast<int>(a.nL), rout.tri, nullptr, rout.u, rout.v, spill.as<int>(), tgrid,
                                   st);
         if (rc) return rc;
@@ -777,30 +814,73 @@ int run_depths(psdr_scene* s, const WaveArgs& a, const QueueView& q0, const Rays
   return PSDR_OK;
 }
 
-int render_forward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& camd, float* out, int keep,
-                   hipStream_t st) {
-  const CameraK cam = make_camera(camd);
-  const int64_t N64 = static_cast<int64_t>(cfg.spp_interior) * cam.H * cam.W;
-  if (N64 <= 0 || N64 > 0x7fffffff / 4 || cfg.max_bounce > 60) {
-    set_error(PSDR_ERR_ARG, "render: spp*H*W out of range");
+// The camera's forward-state buffers (hit cache and replay state) in `slot`.
+static int carve_cache(RenderSlot* slot, int64_t N, int D, int Kmax, CacheView* cache) {
+  const int64_t slots = 1 + static_cast<int64_t>(D) * Kmax;
+  *cache = CacheView{nullptr, nullptr, nullptr, N, nullptr, nullptr, nullptr, nullptr};
+  if (!slot) return PSDR_OK;
+  if (slot->hit_cache.ensure(12ull * slots * N) || slot->replay.ensure((20ull * std::max(D, 1) + 16ull) * N))
+    return grow_failed();
+  cache->tri = slot->hit_cache.as<int>();
+  cache->u = reinterpret_cast<float*>(cache->tri + slots * N);
+  cache->v = cache->u + slots * N;
+  cache->rs = slot->replay.as<uint64_t>();
+  cache->T = reinterpret_cast<float*>(cache->rs + static_cast<int64_t>(std::max(D, 1)) * N);
+  cache->L = cache->T + 3ll * std::max(D, 1) * N;
+  cache->info = reinterpret_cast<int*>(cache->L + 3ll * N);
+  return PSDR_OK;
+}
+
+static_assert(MAX_CAMS == PSDR_MAX_CAMERAS, "cameras per wavefront");
+
+__global__ void k_set_jobs(LaneJobs J, LaneJobs* dst) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *dst = J;
+}
+
+// Cameras in one wavefront: their lanes are concatenated and processed in tiles
+// of at most `tile` lanes (tuning; 0 = all in one), so that the queue, ray and
+// hit buffers hold one tile while the per-lane radiance and the forward states
+// hold every lane.  Each tile uses its own block of counters.
+int render_forward(psdr_scene* s, const psdr_config& cfg, const psdr_camera* camd, int n_cams, float* const* outs,
+                   int keep, hipStream_t st) {
+  if (n_cams < 1 || n_cams > MAX_CAMS) {
+    set_error(PSDR_ERR_ARG, "render: 1 to 4 cameras per wavefront");
     return PSDR_ERR_ARG;
   }
-  const int N = static_cast<int>(N64);
+  LaneJobs J;
+  std::memset(&J, 0, sizeof(J));
+  J.n = n_cams;
+  int64_t total = 0;
+  for (int c = 0; c < n_cams; ++c) {
+    J.cam[c] = make_camera(camd[c]);
+    const int64_t n64 = static_cast<int64_t>(cfg.spp_interior) * J.cam[c].H * J.cam[c].W;
+    if (n64 <= 0 || n64 > 0x7fffffff / 4 || cfg.max_bounce > 60) {
+      set_error(PSDR_ERR_ARG, "render: spp*H*W out of range");
+      return PSDR_ERR_ARG;
+    }
+    J.off[c] = static_cast<uint32_t>(total);
+    total += n64;
+  }
+  J.off[n_cams] = static_cast<uint32_t>(total);
+  if (total > 0x7fffffff) {
+    set_error(PSDR_ERR_ARG, "render: the cameras of one wavefront hold more than 2^31 - 1 lanes");
+    return PSDR_ERR_ARG;
+  }
   const int D = static_cast<int>(cfg.max_bounce);
   const uint32_t nL = cfg.mis_light_samples, nB = cfg.mis_bsdf_samples;
   const int Kmax = static_cast<int>(nL + nB + 1);
-  const int64_t ray_cap = static_cast<int64_t>(N) * std::max(Kmax, 1);
   // a previous boundary prepass reads buffers this render rewrites or the
   // caller may change afterwards: order it before this render
   int rcj = aux_join(s, st);
   if (rcj) return rcj;
-  // renderD: the forward state goes to this (config, camera)'s slot
-  RenderSlot* slot = nullptr;
+  // renderD: each camera's forward state goes to its (config, camera) slot
+  RenderSlot* slots[MAX_CAMS] = {};
   if (keep) {
-    slot = s->ws->claim_slot(s, cfg, camd);
-    slot->have_cache = false;
-    slot->bprep = false;
-    s->ws->cur = slot;
+    for (int c = 0; c < n_cams; ++c) {
+      slots[c] = s->ws->claim_slot(s, cfg, camd[c]);
+      slots[c]->have_cache = false;
+      slots[c]->bprep = false;
+    }
   }
   const bool prep = keep && !s->plain && s->n_edges > 0 && (cfg.spp_primary_edge > 0 || cfg.spp_secondary_edge > 0);
   if (prep) {
@@ -814,84 +894,105 @@ int render_forward(psdr_scene* s, const psdr_config& cfg, const psdr_camera& cam
     if (rca) return rca;
     PSDR_HIP_CHECK(hipEventRecord(s->ws->aux.fork, st));
   }
-  // buffers
-  if (s->ws->lane_L.ensure(12ull * N) || s->ws->q_a.ensure(QUEUE_BYTES_PER * N) || s->ws->q_b.ensure(QUEUE_BYTES_PER * N) ||
-      s->ws->rays.ensure(RAY_BYTES_PER * ray_cap) || s->ws->hits.ensure(12ull * ray_cap) || s->ws->counters_r.ensure(4 * COUNTERS))
+  // tiles (the reference launch shape renders every lane at once); a tile's
+  // rays are indexed as 3 * i in 32-bit arithmetic, i < tile * Kmax
+  int64_t T = tuning().tile_lanes.load();
+  if (T <= 0 || T > total || s->plain) T = total;
+  T = std::min<int64_t>(T, 0x7fffffff / (3ll * std::max(Kmax, 1)));
+  const int n_tiles = static_cast<int>((total + T - 1) / T);
+  const int64_t ray_cap = T * std::max(Kmax, 1);
+  if (s->ws->lane_L.ensure(12ull * total) || s->ws->q_a.ensure(QUEUE_BYTES_PER * T) ||
+      s->ws->q_b.ensure(QUEUE_BYTES_PER * T) || s->ws->rays.ensure(RAY_BYTES_PER * ray_cap) ||
+      s->ws->hits.ensure(12ull * ray_cap) || s->ws->rays2.ensure(RAY_BYTES_PER * ray_cap) ||
+      s->ws->hits2.ensure(12ull * ray_cap) || s->ws->counters_r.ensure(4ull * COUNTERS * n_tiles) ||
+      s->ws->jobs.ensure(sizeof(LaneJobs)))
     return grow_failed();
   const int tgrid = trace_grid(ray_cap);
   if (s->ws->spill.ensure(spill_bytes(tgrid))) return grow_failed();
-  const int64_t slots = 1 + static_cast<int64_t>(D) * Kmax;
-  CacheView cache{nullptr, nullptr, nullptr, N, nullptr, nullptr, nullptr, nullptr};
-  if (slot) {
-    if (slot->hit_cache.ensure(12ull * slots * N) || slot->replay.ensure((20ull * std::max(D, 1) + 16ull) * N))
-      return grow_failed();
-    cache.tri = slot->hit_cache.as<int>();
-    cache.u = reinterpret_cast<float*>(cache.tri + slots * N);
-    cache.v = cache.u + slots * N;
-    cache.rs = slot->replay.as<uint64_t>();
-    cache.T = reinterpret_cast<float*>(cache.rs + static_cast<int64_t>(std::max(D, 1)) * N);
-    cache.L = cache.T + 3ll * std::max(D, 1) * N;
-    cache.info = reinterpret_cast<int*>(cache.L + 3ll * N);
+  for (int c = 0; c < n_cams; ++c) {
+    const int rc = carve_cache(slots[c], static_cast<int64_t>(J.off[c + 1]) - J.off[c], D, Kmax, &J.cache[c]);
+    if (rc) return rc;
   }
-  QueueView qa = carve_queue(s->ws->q_a, N), qb = carve_queue(s->ws->q_b, N);
+  LaneJobs* jobs_dev = s->ws->jobs.as<LaneJobs>();
+  if (n_cams > 1) {
+    hipLaunchKernelGGL(k_set_jobs, dim3(1), dim3(1), 0, st, J, jobs_dev);
+    PSDR_HIP_CHECK(hipGetLastError());
+  }
+  const int Ti = static_cast<int>(T);
+  QueueView qa = carve_queue(s->ws->q_a, Ti), qb = carve_queue(s->ws->q_b, Ti);
+  // the shade kernel reads the hits of the previous depth while writing the
+  // next depth's rays: rays and hits are double-buffered
   RaysView rays_a = carve_rays(s->ws->rays, s->ws->hits, ray_cap);
-  // second ray set: the shade kernel reads the hits of the previous depth while
-  // writing the next depth's rays; rays and hits are double-buffered by halves.
+  RaysView rays_b = carve_rays(s->ws->rays2, s->ws->hits2, ray_cap);
   int* counters = s->ws->counters_r.as<int>();
-  hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(COUNTERS), 0, st, counters, COUNTERS);
+  hipLaunchKernelGGL(k_zero_i32, dim3(blocks(COUNTERS * n_tiles, 256)), dim3(256), 0, st, counters,
+                     COUNTERS * n_tiles);
   const SceneView sv = s->view();
   float* lane_L = s->ws->lane_L.as<float>();
-
-  // stage 0: camera rays + primary hits
-  hipLaunchKernelGGL(k_cam, dim3(blocks(N, 256)), dim3(256), 0, st, cam, cfg.spp_interior, N, qa, rays_a,
-                     counters + 0);
-  PSDR_HIP_CHECK(hipGetLastError());
-  int rc = trace_rays(sv, rays_a.o, rays_a.d, N, counters + 0, 1, N, nullptr, 0, rays_a.tri, nullptr, rays_a.u,
-                      rays_a.v, s->ws->spill.as<int>(), tgrid, st);
-  if (rc) return rc;
-  if (s->plain) PSDR_HIP_CHECK(hipStreamSynchronize(st));
-  if (s->ws->rays2.ensure(RAY_BYTES_PER * ray_cap) || s->ws->hits2.ensure(12ull * ray_cap)) return grow_failed();
-  RaysView rays_b = carve_rays(s->ws->rays2, s->ws->hits2, ray_cap);
   WaveArgs wa;
-  wa.cam = cam;
+  wa.cam = J.cam[0];
   wa.D = D;
   wa.nL = nL;
   wa.nB = nB;
   wa.elv = cfg.enable_light_visable ? 1 : 0;
   wa.ad_jac = keep ? 1 : 0;
   wa.keep = keep;
-  wa.cache = cache;
+  wa.cache = J.cache[0];
   wa.lane_L = lane_L;
   wa.lane_rng_out = nullptr;
-  wa.counters = counters;
   wa.plain = s->plain ? 1 : 0;
+  wa.njobs = n_cams;
+  wa.jobs = n_cams > 1 ? jobs_dev : nullptr;
   const QueueView qs[2] = {qb, qa};
   const RaysView rs[2] = {rays_b, rays_a};
-  rc = run_depths(s, wa, qa, rays_a, qs, rs, N, st);
-  if (rc) return rc;
-  const int P_ = cam.H * cam.W;
+  for (int t = 0; t < n_tiles; ++t) {
+    const int64_t l0 = static_cast<int64_t>(t) * T;
+    const int n = static_cast<int>(std::min<int64_t>(T, total - l0));
+    int* cnt = counters + COUNTERS * t;
+    // stage 0: camera rays + primary hits
+    hipLaunchKernelGGL(k_cam, dim3(blocks(n, 256)), dim3(256), 0, st, J.cam[0], n_cams, wa.jobs, cfg.spp_interior,
+                       static_cast<uint32_t>(l0), n, qa, rays_a, cnt);
+    PSDR_HIP_CHECK(hipGetLastError());
+    int rc = trace_rays(sv, rays_a.o, rays_a.d, n, cnt, 1, Ti, nullptr, 0, rays_a.tri, nullptr, rays_a.u, rays_a.v,
+                        s->ws->spill.as<int>(), tgrid, st);
+    if (rc) return rc;
+    if (s->plain) PSDR_HIP_CHECK(hipStreamSynchronize(st));
+    wa.counters = cnt;
+    rc = run_depths(s, wa, qa, rays_a, qs, rs, n, st);
+    if (rc) return rc;
+  }
   const int ppb = cfg.spp_interior <= static_cast<uint32_t>(FILM_LANES) ? FILM_LANES / static_cast<int>(cfg.spp_interior)
                                                                         : 1;
-  hipLaunchKernelGGL(k_film, dim3(blocks(P_, ppb)), dim3(256), 0, st, lane_L, P_, cfg.spp_interior, cfg.n_pass, out);
-  PSDR_HIP_CHECK(hipGetLastError());
+  for (int c = 0; c < n_cams; ++c) {
+    const int P_ = J.cam[c].H * J.cam[c].W;
+    hipLaunchKernelGGL(k_film, dim3(blocks(P_, ppb)), dim3(256), 0, st, lane_L + 3ll * J.off[c], P_, cfg.spp_interior,
+                       cfg.n_pass, outs[c]);
+    PSDR_HIP_CHECK(hipGetLastError());
+  }
   if (prep) {
-    const int rcp = boundary_prepare_async(s, cfg, camd);
-    if (rcp) return rcp;
+    for (int c = 0; c < n_cams; ++c) {
+      s->ws->cur = slots[c];
+      const int rcp = boundary_prepare_async(s, cfg, camd[c]);
+      if (rcp) return rcp;
+    }
   }
   s->ws->last_owner = s;
-  s->ws->last_N = N;
+  s->ws->last_N = static_cast<uint64_t>(total);
   s->ws->last_K = Kmax;
   s->ws->last_D = D;
+  s->ws->last_tiles = n_tiles;
   s->ws->last_cfg = cfg;
-  s->ws->last_cam = camd;
-  if (slot) {
-    slot->N = N;
-    slot->K = Kmax;
-    slot->D = D;
-    slot->cfg = cfg;
-    slot->cam = camd;
-    slot->have_cache = true;
+  s->ws->last_cam = camd[n_cams - 1];
+  for (int c = 0; c < n_cams; ++c) {
+    if (!slots[c]) continue;
+    slots[c]->N = J.off[c + 1] - J.off[c];
+    slots[c]->K = Kmax;
+    slots[c]->D = D;
+    slots[c]->cfg = cfg;
+    slots[c]->cam = camd[c];
+    slots[c]->have_cache = true;
   }
+  if (keep) s->ws->cur = slots[n_cams - 1];
   return PSDR_OK;
 }
 

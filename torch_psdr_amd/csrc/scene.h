@@ -156,7 +156,7 @@ constexpr int RENDER_SLOTS = 4;
 // for the largest render.  Calls on scenes sharing a workspace are ordered on
 // the caller's stream as calls on one scene are.
 struct Workspace {
-  DevBuf lane_L, q_a, q_b, rays, hits, rays2, hits2, spill, counters_r, stats, cam_acc;
+  DevBuf lane_L, q_a, q_b, rays, hits, rays2, hits2, spill, counters_r, stats, cam_acc, jobs;
   // forward state per (scene, config, camera); cur = the slot the current call works on
   RenderSlot slots[RENDER_SLOTS];
   RenderSlot* cur = &slots[0];
@@ -167,8 +167,9 @@ struct Workspace {
   bool aux_open = false, aux_captured = false;
   // last forward render (lane radiance and statistics entry points)
   const psdr_scene* last_owner = nullptr;
-  uint64_t last_N = 0;
+  uint64_t last_N = 0;  // lanes of every camera of the last forward render
   int last_K = 0, last_D = 0;
+  int last_tiles = 1;   // lane tiles it ran (one block of counters each)
   psdr_config last_cfg{};
   psdr_camera last_cam{};
   int refs = 1;  // scenes using this workspace

@@ -42,6 +42,24 @@ struct CacheView {
   int* info;     // [N] depths with light/BSDF work | 256 light camera vertex | 512 radiance not finite
 };
 
+// Cameras rendered by one wavefront: lanes [off[j], off[j+1]) of the
+// concatenated lane space are camera j's lanes 0 .. N_j - 1 (each camera's
+// sampler starts at lane 0, as a render of that camera alone does).  Kept in
+// device memory; a queue entry holds its concatenated lane.
+constexpr int MAX_CAMS = 4;
+struct LaneJobs {
+  int n;
+  uint32_t off[MAX_CAMS + 1];
+  CameraK cam[MAX_CAMS];
+  CacheView cache[MAX_CAMS];
+};
+__device__ __forceinline__ int lane_job(const LaneJobs* J, uint32_t g) {
+  int j = 0;
+  for (int k = 1; k < MAX_CAMS; ++k)
+    if (k < J->n && g >= J->off[k]) j = k;
+  return j;
+}
+
 QueueView carve_queue(DevBuf& b, int cap);
 RaysView carve_rays(DevBuf& rb, DevBuf& hb, int64_t cap);
 RaysView offset_rays(const RaysView& r, int64_t off);
@@ -52,7 +70,7 @@ RaysView offset_rays(const RaysView& r, int64_t off);
 // if lane_rng_out is set, the final PCG state per lane.  Uses q_s[2], r_s[2] as
 // ping-pong scratch (capacities >= n paths and n*Kmax rays).
 struct WaveArgs {
-  CameraK cam;
+  CameraK cam;  // the camera (one camera), else cameras come from `jobs`
   int D;
   uint32_t nL, nB;
   int elv, ad_jac, keep;
@@ -62,6 +80,8 @@ struct WaveArgs {
   int* counters;  // 128 ints, zeroed by the callee except counters[0] = *count0
   int plain = 0;  // reference launch shape (see psdr_scene::plain)
   DevBuf* spill = nullptr;  // traversal stack overflow area (grown as needed)
+  int njobs = 1;                   // cameras in the lane space
+  const LaneJobs* jobs = nullptr;  // device table when njobs > 1 (cam / cache unused then)
 };
 int run_depths(psdr_scene* s, const WaveArgs& a, const QueueView& q0, const RaysView& r0, const QueueView* q_s,
                const RaysView* r_s, int n, hipStream_t st);

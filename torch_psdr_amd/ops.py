@@ -220,6 +220,20 @@ def forward_native(ns: NativeScene, flat: FlatScene, cam: Sequence[float], cfg: 
     return out
 
 
+def forward_native_multi(ns: NativeScene, flat: FlatScene, cams: Sequence[Sequence[float]], cfg: Sequence[int],
+                         keep: bool) -> List[torch.Tensor]:
+    """The images of up to N.PSDR_MAX_CAMERAS cameras of one scene, rendered in
+    one wavefront (psdr_render_forward_multi)."""
+    ns.bind(flat)
+    outs = [torch.empty((int(c[10]), int(c[11]), 3), dtype=torch.float32, device=flat.vertices.device) for c in cams]
+    c = make_config(cfg)
+    ks = (N.PsdrCamera * len(cams))(*[make_camera(cam) for cam in cams])
+    ptrs = (N.C.c_void_p * len(cams))(*[o.data_ptr() for o in outs])
+    N.check(ns.lib.psdr_render_forward_multi(ns.handle, N.C.byref(c), ks, len(cams), ptrs, int(keep),
+                                             N.current_stream()))
+    return outs
+
+
 @torch.library.custom_op("psdr::render", mutates_args=())
 def render(token: int, cam: List[float], cfg: List[int], vertices: torch.Tensor, texels: torch.Tensor,
            emit: torch.Tensor, camera: torch.Tensor, keep: bool) -> torch.Tensor:
@@ -291,3 +305,76 @@ def _backward(ctx, grad):
 
 
 render.register_autograd(_backward, setup_context=_setup)
+
+
+def pack_cams(cams: Sequence[Sequence[float]]) -> List[float]:
+    """Camera parameter lists as one list: each one preceded by its length."""
+    out: List[float] = []
+    for c in cams:
+        out.append(float(len(c)))
+        out.extend(float(x) for x in c)
+    return out
+
+
+def unpack_cams(packed: Sequence[float]) -> List[List[float]]:
+    cams, i = [], 0
+    while i < len(packed):
+        n = int(packed[i])
+        cams.append(list(packed[i + 1:i + 1 + n]))
+        i += 1 + n
+    return cams
+
+
+@torch.library.custom_op("psdr::render_multi", mutates_args=())
+def render_multi(token: int, cams: List[float], cfg: List[int], vertices: torch.Tensor, texels: torch.Tensor,
+                 emit: torch.Tensor, cameras: List[torch.Tensor], keep: bool) -> List[torch.Tensor]:
+    """Several cameras of one scene in one wavefront.  `cams`: their parameter
+    lists packed by pack_cams; `cameras`: their parameter vectors (inputs only
+    so that autograd routes their gradients)."""
+    ns, flat = entry(token)
+    return forward_native_multi(ns, flat, unpack_cams(cams), cfg, keep)
+
+
+@render_multi.register_fake
+def _render_multi_fake(token, cams, cfg, vertices, texels, emit, cameras, keep):
+    return [vertices.new_empty((int(c[10]), int(c[11]), 3)) for c in unpack_cams(cams)]
+
+
+def _setup_multi(ctx, inputs, output):
+    token, cams, cfg, vertices, texels, emit, cameras, keep = inputs
+    ctx.token, ctx.cams, ctx.cfg = token, unpack_cams(cams), cfg
+    ctx.entry = entry(token)
+    ctx.camera_meta = [(c.device, c.dtype) for c in cameras]
+    ctx.save_for_backward(vertices, texels, emit)
+
+
+def _backward_multi(ctx, grads):
+    """Each camera's backward finds its own forward state; gradients add up."""
+    vertices, texels, emit = ctx.saved_tensors
+    with _lock:
+        added = ctx.token not in _entries
+        if added:
+            _entries[ctx.token] = ctx.entry
+    cam_needs = ctx.needs_input_grad[6]
+    gv = gt = ge = None
+    gcams = []
+    try:
+        for k, (cam, g) in enumerate(zip(ctx.cams, grads)):
+            want_camera = bool(cam_needs[k]) if isinstance(cam_needs, (list, tuple)) else bool(cam_needs)
+            if g is None:
+                gcams.append(None)
+                continue
+            v, t, e, c = render_backward(ctx.token, cam, ctx.cfg, g, vertices, texels, emit, want_camera)
+            gv = v if gv is None else gv + v
+            gt = t if gt is None else gt + t
+            ge = e if ge is None else ge + e
+            dev, dt = ctx.camera_meta[k]
+            gcams.append(c.to(device=dev, dtype=dt) if want_camera else None)
+    finally:
+        if added:
+            with _lock:
+                _entries.pop(ctx.token, None)
+    return None, None, None, gv, gt, ge, gcams, None
+
+
+render_multi.register_autograd(_backward_multi, setup_context=_setup_multi)

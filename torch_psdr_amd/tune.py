@@ -2,9 +2,9 @@
 
 The native library exposes its run-time launch shapes through
 ``psdr_get_tuning`` / ``psdr_set_tuning`` (include/psdr_hip.h): the persistent
-grid of the closest-hit kernel, the traversal stack entries kept in LDS, and the
+grid of the closest-hit kernel, the traversal stack entries kept in LDS, the
 grids of the wavefront shading, path-replay adjoint and direct-boundary
-kernels.  None of them changes a result (images are bit-identical, gradients
+kernels, and the lane tile of the forward render (the size of its queues).  None of them changes a result (images are bit-identical, gradients
 differ only in the order of their float atomics), so the tuner may pick any of
 them by time alone.
 
@@ -27,10 +27,10 @@ from typing import Callable, Dict, Iterable, Optional
 from . import _native as N
 
 FIELDS = ("trace_blocks_per_cu", "trace_stack_lds", "shade_blocks_per_cu", "adjoint_blocks_per_cu",
-          "direct_blocks_per_cu")
+          "direct_blocks_per_cu", "tile_lanes")
 
 DEFAULTS = {"trace_blocks_per_cu": 8, "trace_stack_lds": 16, "shade_blocks_per_cu": 8, "adjoint_blocks_per_cu": 16,
-            "direct_blocks_per_cu": 8}
+            "direct_blocks_per_cu": 8, "tile_lanes": 0}
 
 # candidate values per parameter, swept in this order (the traversal first: it
 # holds most of a step's kernel time)
@@ -40,6 +40,9 @@ SPACE: Dict[str, tuple] = {
     "shade_blocks_per_cu": (2, 4, 8, 16),
     "adjoint_blocks_per_cu": (4, 8, 16, 32),
     "direct_blocks_per_cu": (2, 4, 8, 16),
+    # lanes per wavefront tile of the forward render (0: every lane at once);
+    # tiles bound the queue / ray / hit buffers to O(tile) lanes
+    "tile_lanes": (0, 1 << 20, 1 << 21),
 }
 
 
@@ -162,7 +165,7 @@ def load(key: str, path: Optional[Path] = None, apply: bool = True) -> Optional[
     entry = json.loads(path.read_text()).get(key)
     if entry is None:
         return None
-    tuning = {f: int(entry[f]) for f in FIELDS}
+    tuning = {f: int(entry.get(f, DEFAULTS[f])) for f in FIELDS}
     if apply:
         set_tuning(tuning)
     return tuning
