@@ -119,9 +119,25 @@ def test_c1_texture_step_matches_oracle_jvp(gpu_device, oracle):
     assert abs(lhs - rhs) <= 2e-3 * max(abs(lhs), abs(rhs))
 
 
-def test_c4_terrain_full_scale(gpu_device, oracle):
+@pytest.mark.parametrize("tile", [0, 1 << 22])
+def test_c4_terrain_full_scale(gpu_device, oracle, tile):
+    """The whole renderD step in one wavefront (tile 0) and in lane tiles of
+    4 M lanes (16 tiles: the queue, ray and hit buffers hold one tile), each
+    within its memory bound, with the same lane parity."""
+    import gc
+    from torch_psdr_amd import tune
+    gc.collect()
     torch.cuda.synchronize()
     free0 = torch.cuda.mem_get_info()[0]
+    prev = tune.get_tuning()
+    tune.set_tuning(tile_lanes=tile)
+    try:
+        _c4_step(gpu_device, oracle, tile, free0)
+    finally:
+        tune.set_tuning(prev)
+
+
+def _c4_step(gpu_device, oracle, tile, free0):
     p = W.c4_parts(dr, gpu_device)
     h = p["heights"].clone().requires_grad_(True)
     tex = p["tex"].clone().requires_grad_(True)
@@ -132,8 +148,8 @@ def test_c4_terrain_full_scale(gpu_device, oracle):
     ((img - 0.5) ** 2).mean().backward()
     torch.cuda.synchronize()
     used = free0 - torch.cuda.mem_get_info()[0]
-    print(f"C4: device memory in use after the step {used / 2**30:.1f} GiB")
-    assert used < 64 * 2**30
+    print(f"C4 (tile {tile}): device memory in use after the step {used / 2**30:.1f} GiB", flush=True)
+    assert used < (64 if tile == 0 else 32) * 2**30
     im = img.detach()
     assert torch.isfinite(im).all() and im.mean() > 0
     for name, t in (("heights", h), ("texels", tex)):
