@@ -562,19 +562,11 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adjoint(AdjParams P) {
     // at each vertex, and the vertices themselves (the extension hits)
     const int n = min(info & 255, DMAX);  // depths with light/BSDF work
     const bool light0 = (info & 256) != 0;  // camera vertex on a light, enable_light_visable
-    uint64_t rs[DMAX];
-    f3 Ts[DMAX];
-    int vt[DMAX + 1];
-    float vu[DMAX + 1], vv[DMAX + 1];
-    vt[0] = t0;
-    vu[0] = u0;
-    vv[0] = v0;
-    for (int d = 0; d < n; ++d) {
-      rs[d] = P.crs[d * P.N + lane];
-      Ts[d] = f3{P.cT[(3ll * d) * P.N + lane], P.cT[(3ll * d + 1) * P.N + lane], P.cT[(3ll * d + 2) * P.N + lane]};
-      // extension hit of depth d (used only where depth d's extension is valid)
-      draw_cached(P, 1 + static_cast<int64_t>(d) * P.Kmax + P.nL + P.nB, lane, &vt[d + 1], &vu[d + 1], &vv[d + 1]);
-    }
+    // vertex k's cached hit: the camera hit (k = 0) or depth k-1's extension hit
+    auto vtx = [&](int k, int* t, float* u, float* v) {
+      if (k == 0) { *t = t0; *u = u0; *v = v0; return; }
+      draw_cached(P, 1 + static_cast<int64_t>(k - 1) * P.Kmax + P.nL + P.nB, lane, t, u, v);
+    };
     const f3 L = f3{P.cL[lane], P.cL[P.N + lane], P.cL[2 * P.N + lane]};
     const Its& cur = its;
     // enable_light_visable: L += T0 * emit at a light camera vertex; an SH9
@@ -592,24 +584,34 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adjoint(AdjParams P) {
     f3 pendO = splat(0.f);
     for (int d = n - 1; d >= 0; --d) {
       f3 O = O0;
-      if (d > 0) O = reconstruct(sv, vt[d - 1], vu[d - 1], vv[d - 1], O0, true).p;
-      const Its vx = reconstruct(sv, vt[d], vu[d], vv[d], O, true);
-      Rng r{rs[d], rng.inc};
+      int ta, tb;
+      float ua, va, ub, vb;
+      if (d > 0) {
+        vtx(d - 1, &ta, &ua, &va);
+        O = reconstruct(sv, ta, ua, va, O0, true).p;
+      }
+      vtx(d, &tb, &ub, &vb);
+      const Its vx = reconstruct(sv, tb, ub, vb, O, true);
+      Rng r{P.crs[d * P.N + lane], rng.inc};
+      const f3 Tsd = f3{P.cT[(3ll * d) * P.N + lane], P.cT[(3ll * d + 1) * P.N + lane], P.cT[(3ll * d + 2) * P.N + lane]};
       // A == 0: nothing after this depth reached L.  The throughput of an
       // extension whose sample has pdf 0 (cos = sqrt(u) = 0, or a grazing next
       // hit) is 0 / 0; the forward multiplies it only into terms it never
       // adds (else L is not finite and the lane is dropped), so its adjoint is
       // 0, not 0 * NaN.
       const bool tail = nonzero3(A);
-      const DepthOut o = depth_step<ENV>(P, hot, vx, r, d, lane, true, g * Ts[d], tail ? A * Ts[d] : splat(0.f));
+      const DepthOut o = depth_step<ENV>(P, hot, vx, r, d, lane, true, g * Tsd, tail ? A * Tsd : splat(0.f));
       if (o.ext) {
         VertexAdj nx = pend;
         nx.p = nx.p + o.next.p;
         nx.n = nx.n + o.next.n;
         nx.area = nx.area + o.next.area;
         if (nonzero(nx)) {
-          const Its nv = reconstruct(sv, vt[d + 1], vu[d + 1], vv[d + 1], vx.p, true);
-          tri_vjp(P, vt[d + 1], vu[d + 1], vv[d + 1], flip_of(sv, nv), nx);
+          int tn;
+          float un, vn;
+          vtx(d + 1, &tn, &un, &vn);
+          const Its nv = reconstruct(sv, tn, un, vn, vx.p, true);
+          tri_vjp(P, tn, un, vn, flip_of(sv, nv), nx);
         }
       }
       pend = o.cur;
@@ -619,7 +621,7 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adjoint(AdjParams P) {
     }
     // camera vertex: its own adjoints plus render_interior's L * J0 (J0 = A / detach(A))
     pend.area = pend.area + dot(g, L) / its.area;
-    tri_vjp(P, vt[0], vu[0], vv[0], flip_of(sv, its), pend);
+    tri_vjp(P, t0, u0, v0, flip_of(sv, its), pend);
     // the camera ray origin, through the camera vertex's wi
     pendO = pendO + gO0;
     if (P.cam_acc && nonzero3(pendO)) (void)cam_origin_vjp(P.cam, pendO, film, P.cam.r2c, CAM_R, cacc);

@@ -589,8 +589,12 @@ static int primary_pass(psdr_scene* s, const psdr_config& cfg, const CameraK& ca
   PSDR_HIP_CHECK(hipGetLastError());
   const int tgrid = trace_grid(2ll * n);
   if (slot.p_spill.ensure(std::max(slot.p_spill.bytes, spill_bytes(tgrid)))) return grow_failed();
+  // the primary term's chain (preprocess, sort, both sides' paths) is the
+  // longest of a renderD step: its traversal and shading waves take issue
+  // priority over the concurrent interior and direct-boundary kernels
+  const int prio = s->plain ? 0 : 1;
   rc = trace_rays(sv, r0.o, r0.d, 2 * n, nullptr, 1, 0, nullptr, 0, r0.tri, nullptr, r0.u, r0.v, slot.p_spill.as<int>(),
-                  tgrid, st);
+                  tgrid, st, 0, prio);
   if (rc) return rc;
   if (s->plain) PSDR_HIP_CHECK(hipStreamSynchronize(st));
   hipLaunchKernelGGL(k_prim_check, dim3(blocks(n, BT)), dim3(BT), 0, st, sv, cam, n, q0, rec, r0);
@@ -607,6 +611,7 @@ static int primary_pass(psdr_scene* s, const psdr_config& cfg, const CameraK& ca
   wa.lane_rng_out = slot.b_rngA.as<uint64_t>();
   wa.counters = cnt;
   wa.plain = s->plain ? 1 : 0;
+  wa.prio = prio;
   wa.spill = &slot.p_spill;
   rc = run_depths(s, wa, q0, r0, qs, rs, n, st);
   if (rc) return rc;

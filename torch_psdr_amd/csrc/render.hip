@@ -73,7 +73,10 @@ __global__ __launch_bounds__(TRACE_THREADS) void k_trace(SceneView sv, const flo
                                                          float* __restrict__ ov, int* __restrict__ spill,
                                                          unsigned long long* __restrict__ timed_bytes,
                                                          unsigned long long* __restrict__ span,
-                                                         unsigned long long* __restrict__ counters) {
+                                                         unsigned long long* __restrict__ counters, int prio) {
+  // launches on the step's critical path issue ahead of concurrent kernels'
+  // waves on the same SIMD (wave priority, not stream priority)
+  if (prio) __builtin_amdgcn_s_setprio(3);
   __shared__ int stack[STACK_LDS * TRACE_THREADS];
   __shared__ unsigned long long cnt_lds[TRACE_COUNTERS];
   if (STATS) {
@@ -150,6 +153,7 @@ __global__ void k_span_init(unsigned long long* span, int n) {
 }
 
 struct ShadeParams {
+  int prio;  // raise the waves' issue priority (the boundary chain on the step's critical path)
   SceneView sv;
   CameraK cam;  // one camera; with njobs > 1 the cameras of `jobs`
   int njobs;
@@ -216,6 +220,7 @@ __global__ __launch_bounds__(256) void k_shade(ShadeParams P, QueueView qi, cons
                                                RaysView rin, QueueView qo, int* __restrict__ count_out,
                                                RaysView rout, CacheView cache0, float* __restrict__ lane_L,
                                                uint64_t* __restrict__ lane_rng_out) {
+  if (P.prio) __builtin_amdgcn_s_setprio(3);
   const SceneView& sv = P.sv;
   const int cnt = *count_in;
   const int d = P.depth;
@@ -607,7 +612,7 @@ constexpr size_t TIMER_PAIRS = 4096;
 
 int trace_rays(const SceneView& sv, const float* ro, const float* rd, int n, const int* count_dev, int K, int seg,
                const float* tl, int nvis, int* tri, float* t, float* u, float* v, int* spill, int grid,
-               hipStream_t st, int stack_lds) {
+               hipStream_t st, int stack_lds, int prio) {
   TraceTimer& tt = trace_timer();
   std::lock_guard<std::mutex> lock(tt.mu);
   const bool timed = tt.on && tt.used < TIMER_PAIRS;
@@ -618,7 +623,7 @@ int trace_rays(const SceneView& sv, const float* ro, const float* rd, int n, con
   unsigned long long* cn = tt.count_on ? tt.counters : nullptr;
 #define PSDR_LAUNCH_TRACE(S, STATS)                                                                                  \
   hipLaunchKernelGGL((k_trace<S, STATS>), dim3(grid), dim3(TRACE_THREADS), 0, st, sv, ro, rd, n, count_dev, K, seg, tl, \
-                     nvis, tri, t, u, v, spill, tb, sp, cn)
+                     nvis, tri, t, u, v, spill, tb, sp, cn, prio)
   switch (stack_lds ? stack_lds : tuning().trace_stack.load()) {
     case 8:
       if (cn) PSDR_LAUNCH_TRACE(8, true);
@@ -767,6 +772,7 @@ int run_depths(psdr_scene* s, const WaveArgs& a, const QueueView& q0, const Rays
     const QueueView qout = q_s[d & 1];
     const RaysView rout = r_s[d & 1];
     ShadeParams P;
+    P.prio = a.prio;
     P.sv = sv;
     P.cam = a.cam;
     P.njobs = a.njobs;
@@ -797,14 +803,14 @@ int run_depths(psdr_scene* s, const WaveArgs& a, const QueueView& q0, const Rays
           const int64_t o = static_cast<int64_t>(k) * cap;
           const int rc = trace_rays(sv, rout.o + 3 * o, rout.d + 3 * o, n, nullptr, 1, 0, rout.tl + o,
                                     k < static_cast<int>(a.nL) ? 1 : 0, rout.tri + o, nullptr, rout.u + o,
-                                    rout.v + o, spill.as<int>(), g1, st);
+                                    rout.v + o, spill.as<int>(), g1, st, 0, a.prio);
           if (rc) return rc;
           PSDR_HIP_CHECK(hipStreamSynchronize(st));
         }
       } else {
         const int rc = trace_rays(sv, rout.o, rout.d, n, a.counters + d + 1, P.K_out, cap, rout.tl,
                                   static_cast<int>(a.nL), rout.tri, nullptr, rout.u, rout.v, spill.as<int>(), tgrid,
-                                  st);
+                                  st, 0, a.prio);
         if (rc) return rc;
       }
     }

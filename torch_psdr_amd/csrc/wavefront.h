@@ -80,6 +80,7 @@ struct WaveArgs {
   int* counters;  // 128 ints, zeroed by the callee except counters[0] = *count0
   int plain = 0;  // reference launch shape (see psdr_scene::plain)
   DevBuf* spill = nullptr;  // traversal stack overflow area (grown as needed)
+  int prio = 0;                    // raise the launches' wave priority (critical path)
   int njobs = 1;                   // cameras in the lane space
   const LaneJobs* jobs = nullptr;  // device table when njobs > 1 (cam / cache unused then)
 };
