@@ -521,10 +521,30 @@ __device__ __forceinline__ DepthOut depth_step(const AdjParams& P, float* hot, c
   return out;
 }
 
+// Lanes are replayed in chunks of ADJ_CHUNK lanes of the workgroup's range,
+// reordered inside the chunk by their number of depths (counting sort into
+// ADJ_BINS bins): a wave's loop over depths runs as long as its longest path,
+// so waves of paths of one length idle least.  The order within a bin is the
+// lanes' own order, so the reordering is deterministic.
+constexpr int ADJ_SUB = 4;                          // lanes per thread per chunk
+constexpr int ADJ_CHUNK = ADJ_SUB * ADJ_THREADS;
+constexpr int ADJ_WAVES = ADJ_THREADS / 64;
+constexpr int ADJ_BINS = 8;
+
+template <int DMAX>
+__device__ __forceinline__ int adj_depth_key(const AdjParams& P, int64_t lane, int64_t lane_end) {
+  if (lane >= lane_end) return 0;
+  const int info = P.cinfo[lane];
+  if (info & 512) return 0;
+  return min(min(info & 255, DMAX), ADJ_BINS - 1);
+}
+
 template <int DMAX, bool ENV>
 __global__ __launch_bounds__(ADJ_THREADS) void k_adjoint(AdjParams P) {
   __shared__ float hot[HOT_MAX];
   __shared__ float cacc[CAM_ACC];
+  __shared__ int order[ADJ_CHUNK];
+  __shared__ int bins[ADJ_SUB * ADJ_WAVES * ADJ_BINS];
   for (int i = threadIdx.x; i < P.hot_n; i += blockDim.x) hot[i] = 0.f;
   cam_acc_init(cacc);
   adj_cache().init();
@@ -534,97 +554,134 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adjoint(AdjParams P) {
   // triangles and texels, so their scatters aggregate in LDS
   const int64_t per = ((P.N + gridDim.x - 1) / gridDim.x + ADJ_THREADS - 1) / ADJ_THREADS * ADJ_THREADS;
   const int64_t lane_end = min(P.N, static_cast<int64_t>(blockIdx.x) * per + per);
-  for (int64_t lane = static_cast<int64_t>(blockIdx.x) * per + threadIdx.x; lane < lane_end; lane += ADJ_THREADS) {
-    const int64_t pix = P.spp > 1 ? lane / P.spp : lane;
-    f3 g = f3{P.dLdI[3 * pix], P.dLdI[3 * pix + 1], P.dLdI[3 * pix + 2]};
-    if (P.spp > 1) g = g / static_cast<float>(P.spp);
-    if (g.x == 0.f && g.y == 0.f && g.z == 0.f) continue;
-    const int info = P.cinfo[lane];
-    if (info & 512) continue;  // the forward dropped this lane's non-finite radiance
-    const Rng rng{0, lane_inc(static_cast<uint32_t>(lane))};
-    int t0;
-    float u0, v0;
-    draw_cached(P, 0, lane, &t0, &u0, &v0);
-    // camera ray origin: the camera position, or (orthographic) the film
-    // point's spot on the camera plane, from the lane's first two draws
-    f3 O0 = P.cam.pos, film = splat(0.f);
-    if (P.cam.ortho) {
-      Rng r0 = lane_rng(static_cast<uint32_t>(lane));
-      const float a = r0.next(), b = r0.next();
-      int pixel;
-      film = film_point(P.cam, static_cast<uint32_t>(lane), P.spp, a, b, &pixel);
-      const f3 fp = xform_pos(P.cam.r2c, film);
-      O0 = xform_pos(P.cam.c2w, f3{fp.x, fp.y, 0.0f});
-    }
-    const Its its = reconstruct(sv, t0, u0, v0, O0, true);
-    if (!its.valid) continue;
-    // per-depth state recorded by the forward render: PCG state and throughput
-    // at each vertex, and the vertices themselves (the extension hits)
-    const int n = min(info & 255, DMAX);  // depths with light/BSDF work
-    const bool light0 = (info & 256) != 0;  // camera vertex on a light, enable_light_visable
-    // vertex k's cached hit: the camera hit (k = 0) or depth k-1's extension hit
-    auto vtx = [&](int k, int* t, float* u, float* v) {
-      if (k == 0) { *t = t0; *u = u0; *v = v0; return; }
-      draw_cached(P, 1 + static_cast<int64_t>(k - 1) * P.Kmax + P.nL + P.nB, lane, t, u, v);
-    };
-    const f3 L = f3{P.cL[lane], P.cL[P.N + lane], P.cL[2 * P.N + lane]};
-    const Its& cur = its;
-    // enable_light_visable: L += T0 * emit at a light camera vertex; an SH9
-    // light's radiance there depends on the camera origin through wi
-    f3 gO0 = splat(0.f);
-    if (light0) {
-      const f3 gw0 = radiance_vjp<ENV>(P, hot, cur.light, cur, g);
-      if (nonzero3(gw0)) gO0 = -vjp_normalize(cur.p - O0, -gw0);
-    }
-    // reverse sweep.  pend = adjoint of vertex d+1 gathered so far (its own
-    // depth's terms and, through its wi, vertex d+2's); pendO = adjoint of
-    // vertex d's point from vertex d+1's wi.
-    f3 A = splat(0.f);  // adjoint of T_{d+1}
-    VertexAdj pend = zero_adj();
-    f3 pendO = splat(0.f);
-    for (int d = n - 1; d >= 0; --d) {
-      f3 O = O0;
-      int ta, tb;
-      float ua, va, ub, vb;
-      if (d > 0) {
-        vtx(d - 1, &ta, &ua, &va);
-        O = reconstruct(sv, ta, ua, va, O0, true).p;
+  const int wave = threadIdx.x >> 6;
+  const uint64_t lt = (1ull << (threadIdx.x & 63)) - 1ull;
+  for (int64_t c0 = static_cast<int64_t>(blockIdx.x) * per; c0 < lane_end; c0 += ADJ_CHUNK) {
+    // rank of each lane within its bin: lanes of earlier sub-rows, then
+    // earlier waves of this sub-row, then earlier lanes of this wave
+    int key[ADJ_SUB], rank[ADJ_SUB];
+#pragma unroll
+    for (int k = 0; k < ADJ_SUB; ++k) {
+      key[k] = adj_depth_key<DMAX>(P, c0 + k * ADJ_THREADS + threadIdx.x, lane_end);
+      rank[k] = 0;
+#pragma unroll
+      for (int b = 0; b < ADJ_BINS; ++b) {
+        const uint64_t m = __ballot(key[k] == b);
+        if (key[k] == b) rank[k] = __popcll(m & lt);
+        if ((threadIdx.x & 63) == 0) bins[(k * ADJ_WAVES + wave) * ADJ_BINS + b] = __popcll(m);
       }
-      vtx(d, &tb, &ub, &vb);
-      const Its vx = reconstruct(sv, tb, ub, vb, O, true);
-      Rng r{P.crs[d * P.N + lane], rng.inc};
-      const f3 Tsd = f3{P.cT[(3ll * d) * P.N + lane], P.cT[(3ll * d + 1) * P.N + lane], P.cT[(3ll * d + 2) * P.N + lane]};
-      // A == 0: nothing after this depth reached L.  The throughput of an
-      // extension whose sample has pdf 0 (cos = sqrt(u) = 0, or a grazing next
-      // hit) is 0 / 0; the forward multiplies it only into terms it never
-      // adds (else L is not finite and the lane is dropped), so its adjoint is
-      // 0, not 0 * NaN.
-      const bool tail = nonzero3(A);
-      const DepthOut o = depth_step<ENV>(P, hot, vx, r, d, lane, true, g * Tsd, tail ? A * Tsd : splat(0.f));
-      if (o.ext) {
-        VertexAdj nx = pend;
-        nx.p = nx.p + o.next.p;
-        nx.n = nx.n + o.next.n;
-        nx.area = nx.area + o.next.area;
-        if (nonzero(nx)) {
-          int tn;
-          float un, vn;
-          vtx(d + 1, &tn, &un, &vn);
-          const Its nv = reconstruct(sv, tn, un, vn, vx.p, true);
-          tri_vjp(P, tn, un, vn, flip_of(sv, nv), nx);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {  // exclusive scan, bins outermost
+      int run = 0;
+      for (int b = 0; b < ADJ_BINS; ++b)
+        for (int r = 0; r < ADJ_SUB * ADJ_WAVES; ++r) {
+          const int c = bins[r * ADJ_BINS + b];
+          bins[r * ADJ_BINS + b] = run;
+          run += c;
         }
-      }
-      pend = o.cur;
-      pend.p = pend.p + pendO;
-      pendO = o.gO;
-      A = tail ? g * o.c + A * o.t : g * o.c;
     }
-    // camera vertex: its own adjoints plus render_interior's L * J0 (J0 = A / detach(A))
-    pend.area = pend.area + dot(g, L) / its.area;
-    tri_vjp(P, t0, u0, v0, flip_of(sv, its), pend);
-    // the camera ray origin, through the camera vertex's wi
-    pendO = pendO + gO0;
-    if (P.cam_acc && nonzero3(pendO)) (void)cam_origin_vjp(P.cam, pendO, film, P.cam.r2c, CAM_R, cacc);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < ADJ_SUB; ++k)
+      order[bins[(k * ADJ_WAVES + wave) * ADJ_BINS + key[k]] + rank[k]] = k * ADJ_THREADS + threadIdx.x;
+    __syncthreads();
+    for (int s = 0; s < ADJ_SUB; ++s) {
+      const int64_t lane = c0 + order[s * ADJ_THREADS + threadIdx.x];
+      if (lane >= lane_end) continue;
+      const int64_t pix = P.spp > 1 ? lane / P.spp : lane;
+      f3 g = f3{P.dLdI[3 * pix], P.dLdI[3 * pix + 1], P.dLdI[3 * pix + 2]};
+      if (P.spp > 1) g = g / static_cast<float>(P.spp);
+      if (g.x == 0.f && g.y == 0.f && g.z == 0.f) continue;
+      const int info = P.cinfo[lane];
+      if (info & 512) continue;  // the forward dropped this lane's non-finite radiance
+      const Rng rng{0, lane_inc(static_cast<uint32_t>(lane))};
+      int t0;
+      float u0, v0;
+      draw_cached(P, 0, lane, &t0, &u0, &v0);
+      // camera ray origin: the camera position, or (orthographic) the film
+      // point's spot on the camera plane, from the lane's first two draws
+      f3 O0 = P.cam.pos, film = splat(0.f);
+      if (P.cam.ortho) {
+        Rng r0 = lane_rng(static_cast<uint32_t>(lane));
+        const float a = r0.next(), b = r0.next();
+        int pixel;
+        film = film_point(P.cam, static_cast<uint32_t>(lane), P.spp, a, b, &pixel);
+        const f3 fp = xform_pos(P.cam.r2c, film);
+        O0 = xform_pos(P.cam.c2w, f3{fp.x, fp.y, 0.0f});
+      }
+      const Its its = reconstruct(sv, t0, u0, v0, O0, true);
+      if (!its.valid) continue;
+      // per-depth state recorded by the forward render: PCG state and throughput
+      // at each vertex, and the vertices themselves (the extension hits)
+      const int n = min(info & 255, DMAX);  // depths with light/BSDF work
+      const bool light0 = (info & 256) != 0;  // camera vertex on a light, enable_light_visable
+      // vertex k's cached hit: the camera hit (k = 0) or depth k-1's extension hit
+      auto vtx = [&](int k, int* t, float* u, float* v) {
+        if (k == 0) { *t = t0; *u = u0; *v = v0; return; }
+        draw_cached(P, 1 + static_cast<int64_t>(k - 1) * P.Kmax + P.nL + P.nB, lane, t, u, v);
+      };
+      const f3 L = f3{P.cL[lane], P.cL[P.N + lane], P.cL[2 * P.N + lane]};
+      const Its& cur = its;
+      // enable_light_visable: L += T0 * emit at a light camera vertex; an SH9
+      // light's radiance there depends on the camera origin through wi
+      f3 gO0 = splat(0.f);
+      if (light0) {
+        const f3 gw0 = radiance_vjp<ENV>(P, hot, cur.light, cur, g);
+        if (nonzero3(gw0)) gO0 = -vjp_normalize(cur.p - O0, -gw0);
+      }
+      // reverse sweep.  pend = adjoint of vertex d+1 gathered so far (its own
+      // depth's terms and, through its wi, vertex d+2's); pendO = adjoint of
+      // vertex d's point from vertex d+1's wi.
+      f3 A = splat(0.f);  // adjoint of T_{d+1}
+      VertexAdj pend = zero_adj();
+      f3 pendO = splat(0.f);
+      for (int d = n - 1; d >= 0; --d) {
+        f3 O = O0;
+        int ta, tb;
+        float ua, va, ub, vb;
+        if (d > 0) {
+          vtx(d - 1, &ta, &ua, &va);
+          O = reconstruct(sv, ta, ua, va, O0, true).p;
+        }
+        vtx(d, &tb, &ub, &vb);
+        const Its vx = reconstruct(sv, tb, ub, vb, O, true);
+        Rng r{P.crs[d * P.N + lane], rng.inc};
+        const f3 Tsd =
+            f3{P.cT[(3ll * d) * P.N + lane], P.cT[(3ll * d + 1) * P.N + lane], P.cT[(3ll * d + 2) * P.N + lane]};
+        // A == 0: nothing after this depth reached L.  The throughput of an
+        // extension whose sample has pdf 0 (cos = sqrt(u) = 0, or a grazing next
+        // hit) is 0 / 0; the forward multiplies it only into terms it never
+        // adds (else L is not finite and the lane is dropped), so its adjoint is
+        // 0, not 0 * NaN.
+        const bool tail = nonzero3(A);
+        const DepthOut o = depth_step<ENV>(P, hot, vx, r, d, lane, true, g * Tsd, tail ? A * Tsd : splat(0.f));
+        if (o.ext) {
+          VertexAdj nx = pend;
+          nx.p = nx.p + o.next.p;
+          nx.n = nx.n + o.next.n;
+          nx.area = nx.area + o.next.area;
+          if (nonzero(nx)) {
+            int tn;
+            float un, vn;
+            vtx(d + 1, &tn, &un, &vn);
+            const Its nv = reconstruct(sv, tn, un, vn, vx.p, true);
+            tri_vjp(P, tn, un, vn, flip_of(sv, nv), nx);
+          }
+        }
+        pend = o.cur;
+        pend.p = pend.p + pendO;
+        pendO = o.gO;
+        A = tail ? g * o.c + A * o.t : g * o.c;
+      }
+      // camera vertex: its own adjoints plus render_interior's L * J0 (J0 = A / detach(A))
+      pend.area = pend.area + dot(g, L) / its.area;
+      tri_vjp(P, t0, u0, v0, flip_of(sv, its), pend);
+      // the camera ray origin, through the camera vertex's wi
+      pendO = pendO + gO0;
+      if (P.cam_acc && nonzero3(pendO)) (void)cam_origin_vjp(P.cam, pendO, film, P.cam.r2c, CAM_R, cacc);
+    }
+    __syncthreads();  // order[] and bins[] are rewritten by the next chunk
   }
   __syncthreads();
   adj_cache().flush(P.grad_v, P.grad_t);
