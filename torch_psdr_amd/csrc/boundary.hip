@@ -14,11 +14,14 @@
 //   k_prim_finish  preprocess: hypercube mass |dL / pdf|; render: the VJP of
 //                  [(x_e - detach x_e) . n] dL / detach(pdf) into the edge's vertices
 // Direct (shadow) edges, per lane of Sampler(spp_secondary_edge*H*W):
-//   k_direct       3 draws, 3-D hypercube pick (10k x 6 x 6 cells), edge and
-//                  light points, 3 closest hits (toward / away from the light,
-//                  camera visibility), boundary_term with its hit_triangle
-//                  re-parameterisation; preprocess accumulates |value|, render
-//                  pushes the VJP through hit_triangle into the edge's vertices.
+//   k_direct_setup 3 draws, 3-D hypercube pick (10k x 6 x 6 cells), edge and
+//                  light points; queues the samples whose edge is a silhouette
+//                  seen from the light point
+//   k_direct       per queued sample: 3 closest hits (toward / away from the
+//                  light, camera visibility), boundary_term with its
+//                  hit_triangle re-parameterisation; preprocess accumulates
+//                  |value|, render pushes the VJP through hit_triangle into the
+//                  edge's vertices.
 // The hypercube preprocessing passes reuse one Sampler(cells) stream across
 // rounds, exactly as the reference does.
 #include <hip/hip_runtime.h>
@@ -288,6 +291,41 @@ struct DirectOut {
   f3 value;  // preprocess value (non-differentiable form)
 };
 
+// The first steps of eval_direct_boundary (src/path_integrator.cc:287-299):
+// edge point, light point, and whether the edge is a silhouette seen from the
+// light point.  Most samples of the preprocessing pass end here, before any
+// traversal, so a first kernel runs only this part and queues the samples that
+// pass for the traversing kernel, which repeats it (bit for bit).
+struct DirectSetup {
+  bool valid;
+  int e;
+  float ue, pdf_ep, pdf_light;
+  EdgeRec E;
+  f3 ep, dir;
+  LightPoint lp;
+};
+
+__device__ __forceinline__ DirectSetup direct_setup(const SceneView& sv, float sx, float sy, float sz) {
+  DirectSetup d;
+  d.valid = false;
+  if (sv.n_edges == 0) return d;
+  d.ue = sx;
+  float pdf_e;
+  d.e = discrete_sample(sv.edge_pmf, sv.edge_cmf, sv.n_edges, &d.ue, &pdf_e);
+  d.E = load_edge(sv, d.e);
+  d.ep = d.E.p0 + d.ue * (d.E.p1 - d.E.p0);
+  d.pdf_ep = pdf_e / d.E.len;
+  float y = sy, pdf_choose;
+  const int obj = discrete_sample(sv.obj_pmf, sv.obj_cmf, sv.n_objects, &y, &pdf_choose);
+  const int l = sv.objs[obj].light;
+  if (l < 0) return d;
+  d.lp = sample_light_point(sv, l, y, sz);
+  d.pdf_light = d.lp.pdf * pdf_choose;
+  d.dir = normalize(d.ep - d.lp.p);
+  d.valid = (dot(d.E.nl, d.dir) * dot(d.E.nr, d.dir) < 0.f) && norm(d.E.nl - d.E.nr) > REAL_EPS;
+  return d;
+}
+
 // eval_direct_boundary (src/path_integrator.cc:287-329) + boundary_term
 // (src/path_integrator.cc:31-78).  Render pass (!PRE): the adjoint of the value
 // is dLdI[pixel] * gscale and is pushed into the edge's vertices.
@@ -298,21 +336,14 @@ __device__ DirectOut eval_direct(const SceneView& sv, const CameraK& cam, float 
   r.valid = false;
   r.pix = -1;
   r.value = splat(0.f);
-  if (sv.n_edges == 0) return r;
-  float ue = sx, pdf_e;
-  const int e = discrete_sample(sv.edge_pmf, sv.edge_cmf, sv.n_edges, &ue, &pdf_e);
-  const EdgeRec E = load_edge(sv, e);
-  const f3 ep = E.p0 + ue * (E.p1 - E.p0);
-  const float pdf_ep = pdf_e / E.len;
-  float y = sy, pdf_choose;
-  const int obj = discrete_sample(sv.obj_pmf, sv.obj_cmf, sv.n_objects, &y, &pdf_choose);
-  const int l = sv.objs[obj].light;
-  if (l < 0) return r;
-  const LightPoint lp = sample_light_point(sv, l, y, sz);
-  const float pdf_light = lp.pdf * pdf_choose;
-  const f3 dir = normalize(ep - lp.p);
-  bool valid = (dot(E.nl, dir) * dot(E.nr, dir) < 0.f) && norm(E.nl - E.nr) > REAL_EPS;
-  if (!valid) return r;
+  const DirectSetup ds = direct_setup(sv, sx, sy, sz);
+  if (!ds.valid) return r;
+  const int e = ds.e;
+  const float ue = ds.ue, pdf_ep = ds.pdf_ep, pdf_light = ds.pdf_light;
+  const EdgeRec& E = ds.E;
+  const f3 ep = ds.ep, dir = ds.dir;
+  const LightPoint& lp = ds.lp;
+  bool valid;
   const float tl = norm(ep - lp.p);
   HitRec h = closest_hit<BSTACK>(sv.node, sv.leaf, sv.n_bvh, sv.n_tris, ep, -dir, stk, spill, spill_stride,
                                  vis_tmax(tl), vis_tocc(tl));
@@ -395,31 +426,31 @@ __device__ DirectOut eval_direct(const SceneView& sv, const CameraK& cam, float 
   return r;
 }
 
-// PRE: preprocess round over the cells (accumulates |value| into mass).
-// Render pass: position p evaluates lane order[p] and writes its 12-float
-// record (the 3x3 map from the pixel adjoint to the edge-point adjoint, pixel,
-// edge, edge parameter; pixel -1 when the sample is invalid).
+// Samples of a direct-boundary pass: PRE, one per cell (accumulates |value|
+// into mass); render pass, position p of the processing order draws lane
+// order[p] and owns the 12-float record at p (the 3x3 map from the pixel
+// adjoint to the edge-point adjoint, pixel, edge, edge parameter; pixel -1 when
+// the sample is invalid).  k_direct_setup draws each sample and queues those
+// whose edge is a silhouette from the light point (p, draws, gradient scale),
+// or every sample when compact is 0 (the reference's launch shape);
+// k_direct traces the queued ones.
 template <bool PRE>
-__global__ __launch_bounds__(BT) void k_direct(SceneView sv, CameraK cam, int n, uint64_t* __restrict__ state,
-                                               int r0, int r1, int r2, const float* __restrict__ cpmf,
-                                               const float* __restrict__ ccmf, float* __restrict__ mass,
-                                               uint32_t spp, float* __restrict__ drec, int* __restrict__ spill,
-                                               const uint32_t* __restrict__ order) {
-  __shared__ int stk[BSTACK * BT];
-  const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int stride = gridDim.x * blockDim.x;
-  const int cells = r0 * r1 * r2;
-  const float u0 = 1.0f / static_cast<float>(r0), u1 = 1.0f / static_cast<float>(r1),
-              u2 = 1.0f / static_cast<float>(r2);
-  // each workgroup takes a contiguous range of positions (neighbouring samples
-  // in the processing order share edges, so their gradients aggregate in LDS)
-  const int per = ((n + static_cast<int>(gridDim.x) - 1) / static_cast<int>(gridDim.x) + BT - 1) / BT * BT;
-  const int p_end = min(n, static_cast<int>(blockIdx.x) * per + per);
-  for (int p = static_cast<int>(blockIdx.x) * per + static_cast<int>(threadIdx.x); p < p_end; p += BT) {
+__global__ __launch_bounds__(BT) void k_direct_setup(SceneView sv, int n, uint64_t* __restrict__ state, int r0,
+                                                     int r1, int r2, const float* __restrict__ cpmf,
+                                                     const float* __restrict__ ccmf, uint32_t spp,
+                                                     float* __restrict__ drec, const uint32_t* __restrict__ order,
+                                                     int* __restrict__ qidx, float4* __restrict__ qval,
+                                                     int* __restrict__ qcount, int compact) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  bool want = false;
+  float s[3] = {0.f, 0.f, 0.f};
+  float gscale = 0.f;
+  if (p < n) {
+    const int cells = r0 * r1 * r2;
+    const float u0 = 1.0f / static_cast<float>(r0), u1 = 1.0f / static_cast<float>(r1),
+                u2 = 1.0f / static_cast<float>(r2);
     const int i = order ? static_cast<int>(order[p]) : p;
     Rng rng;
-    float s[3];
-    float pdf_cube = 1.f;
     if (PRE) {
       rng.state = state[i];
       rng.inc = lane_inc(static_cast<uint32_t>(i));
@@ -432,27 +463,48 @@ __global__ __launch_bounds__(BT) void k_direct(SceneView sv, CameraK cam, int n,
     } else {
       rng = lane_rng(static_cast<uint32_t>(i));
       for (int k = 0; k < 3; ++k) s[k] = rng.next();
+      float pdf_cube = 1.f;
       const int c = discrete_sample(cpmf, ccmf, cells, &s[2], &pdf_cube);
       const int c0 = c / (r1 * r2), c1 = (c / r2) % r1, c2 = c % r2;
       s[0] = (s[0] + static_cast<float>(c0)) * u0;
       s[1] = (s[1] + static_cast<float>(c1)) * u1;
       s[2] = (s[2] + static_cast<float>(c2)) * u2;
+      gscale = 1.0f / pdf_cube;
+      if (spp > 1) gscale = gscale / static_cast<float>(spp);
+      drec[12ll * p + 9] = __int_as_float(-1);
     }
+    want = !compact || direct_setup(sv, s[0], s[1], s[2]).valid;
+  }
+  const int q = wave_alloc(want, qcount);
+  if (want) {
+    qidx[q] = p;
+    qval[q] = make_float4(s[0], s[1], s[2], gscale);
+  }
+}
+
+template <bool PRE>
+__global__ __launch_bounds__(BT) void k_direct(SceneView sv, CameraK cam, const int* __restrict__ qcount,
+                                               const int* __restrict__ qidx, const float4* __restrict__ qval,
+                                               float* __restrict__ mass, float* __restrict__ drec,
+                                               int* __restrict__ spill) {
+  __shared__ int stk[BSTACK * BT];
+  const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int stride = gridDim.x * blockDim.x;
+  const int nq = *qcount;
+  for (int q = gtid; q < nq; q += stride) {
+    const int p = qidx[q];
+    const float4 v = qval[q];
     if (PRE) {
-      DirectOut o = eval_direct<true>(sv, cam, s[0], s[1], s[2], stk, spill + gtid, stride, 0.f, nullptr);
+      DirectOut o = eval_direct<true>(sv, cam, v.x, v.y, v.z, stk, spill + gtid, stride, 0.f, nullptr);
       if (o.valid) {
-        f3 v = o.value;
-        v.x = v.x < REAL_EPS ? REAL_EPS : v.x;
-        v.y = v.y < REAL_EPS ? REAL_EPS : v.y;
-        v.z = v.z < REAL_EPS ? REAL_EPS : v.z;
-        mass[i] = mass[i] + norm(v);
+        f3 w = o.value;
+        w.x = w.x < REAL_EPS ? REAL_EPS : w.x;
+        w.y = w.y < REAL_EPS ? REAL_EPS : w.y;
+        w.z = w.z < REAL_EPS ? REAL_EPS : w.z;
+        mass[p] = mass[p] + norm(w);
       }
     } else {
-      float gscale = 1.0f / pdf_cube;
-      if (spp > 1) gscale = gscale / static_cast<float>(spp);
-      float* rec = drec + 12ll * p;
-      rec[9] = __int_as_float(-1);
-      eval_direct<false>(sv, cam, s[0], s[1], s[2], stk, spill + gtid, stride, gscale, rec);
+      eval_direct<false>(sv, cam, v.x, v.y, v.z, stk, spill + gtid, stride, v.w, drec + 12ll * p);
     }
   }
 }
@@ -503,6 +555,8 @@ __global__ void k_fill_f32(float* p, int n, float v) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
 }
+__global__ void k_zero_count(int* c) { *c = 0; }
+
 __global__ void k_seed_states(uint64_t* st, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) st[i] = lane_rng(static_cast<uint32_t>(i)).state;
@@ -701,14 +755,22 @@ int boundary_direct_prepare(psdr_scene* s, const psdr_config& cfg, const psdr_ca
       slot.d_cmf.ensure(4ull * cells) || slot.d_tmp.ensure(4ull * (2 * (cells / 256 + 2) + 16)) ||
       slot.d_spill.ensure(static_cast<size_t>(grid) * BT * (STACK_TOTAL - BSTACK) * 4) ||
       slot.d_keys.ensure(8ull * n) || slot.d_vals.ensure(8ull * n) || slot.d_hist.ensure(4 * radix_hist_words(n)) ||
-      slot.d_rec.ensure(48ull * std::max(n, 1)))
+      slot.d_rec.ensure(48ull * std::max(n, 1)) || slot.d_qidx.ensure(4ull * std::max(cells, n)) ||
+      slot.d_qval.ensure(16ull * std::max(cells, n)) || slot.d_qcnt.ensure(4))
     return grow_failed();
   uint64_t* state = slot.d_state.as<uint64_t>();
+  int* qidx = slot.d_qidx.as<int>();
+  float4* qval = slot.d_qval.as<float4>();
+  int* qcnt = slot.d_qcnt.as<int>();
+  const int compact = s->plain ? 0 : 1;
   hipLaunchKernelGGL(k_seed_states, dim3(blocks(cells, BT)), dim3(BT), 0, st, state, cells);
   hipLaunchKernelGGL(k_fill_f32, dim3(blocks(cells, BT)), dim3(BT), 0, st, slot.d_mass.as<float>(), cells, 0.f);
   for (uint32_t r = 0; r < cfg.secondary_edge_preprocess_rounds; ++r) {
-    hipLaunchKernelGGL(k_direct<true>, dim3(grid), dim3(BT), 0, st, sv, cam, cells, state, r0, r1, r2, nullptr, nullptr,
-                       slot.d_mass.as<float>(), 0u, nullptr, slot.d_spill.as<int>(), nullptr);
+    hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(1), 0, st, qcnt);
+    hipLaunchKernelGGL(k_direct_setup<true>, dim3(blocks(cells, BT)), dim3(BT), 0, st, sv, cells, state, r0, r1, r2,
+                       nullptr, nullptr, 0u, nullptr, nullptr, qidx, qval, qcnt, compact);
+    hipLaunchKernelGGL(k_direct<true>, dim3(grid), dim3(BT), 0, st, sv, cam, qcnt, qidx, qval,
+                       slot.d_mass.as<float>(), nullptr, slot.d_spill.as<int>());
     PSDR_HIP_CHECK(hipGetLastError());
   }
   int rc = sampler_tables(slot.d_mass.as<float>(), cells, slot.d_pmf.as<float>(), slot.d_cmf.as<float>(), nullptr,
@@ -729,9 +791,12 @@ int boundary_direct_prepare(psdr_scene* s, const psdr_config& cfg, const psdr_ca
     if (rc) return rc;
     order = (passes & 1) ? v0 + n : v0;
   }
-  hipLaunchKernelGGL(k_direct<false>, dim3(grid), dim3(BT), 0, st, sv, cam, n, state, r0, r1, r2,
-                     slot.d_pmf.as<float>(), slot.d_cmf.as<float>(), nullptr, cfg.spp_secondary_edge,
-                     slot.d_rec.as<float>(), slot.d_spill.as<int>(), order);
+  hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(1), 0, st, qcnt);
+  hipLaunchKernelGGL(k_direct_setup<false>, dim3(blocks(n, BT)), dim3(BT), 0, st, sv, n, state, r0, r1, r2,
+                     slot.d_pmf.as<float>(), slot.d_cmf.as<float>(), cfg.spp_secondary_edge, slot.d_rec.as<float>(),
+                     order, qidx, qval, qcnt, compact);
+  hipLaunchKernelGGL(k_direct<false>, dim3(grid), dim3(BT), 0, st, sv, cam, qcnt, qidx, qval, nullptr,
+                     slot.d_rec.as<float>(), slot.d_spill.as<int>());
   PSDR_HIP_CHECK(hipGetLastError());
   return PSDR_OK;
 }

@@ -139,6 +139,7 @@ struct RenderSlot {
   DevBuf p_qa, p_qb, p_rays, p_hits, p_rays2, p_hits2, p_spill;
   // direct-boundary scratch (separate from the primary pass's, so both can run at once)
   DevBuf d_state, d_mass, d_pmf, d_cmf, d_tmp, d_spill, d_keys, d_vals, d_hist, d_rec;
+  DevBuf d_qidx, d_qval, d_qcnt;  // samples queued for the traversing kernel
   bool have_cache = false;  // hit_cache / replay hold the forward render of (cfg, cam)
   bool bprep = false;       // the boundary prepass for (cfg, cam) has been issued
   uint64_t N = 0;
