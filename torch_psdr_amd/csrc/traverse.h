@@ -211,7 +211,11 @@ __device__ __forceinline__ HitRec closest_hit(const float4* __restrict__ node, c
     } else {
       if (sp == 0) break;
       --sp;
-      ni = (sp < STACK_LDS) ? stack_lds[sp * nthr + lane] : spill[(sp - STACK_LDS) * spill_stride];
+      // the spill read is volatile so that the compiler keeps two loads: merged,
+      // they become one flat (generic address) load on every pop, which is
+      // slower than the LDS read taken by all but the deepest entries
+      if (sp < STACK_LDS) ni = stack_lds[sp * nthr + lane];
+      else ni = *reinterpret_cast<volatile const int*>(spill + (sp - STACK_LDS) * spill_stride);
     }
   }
   extra_leaves(leaf, n_tris, n_all, o, d, tocc, h, st);
