@@ -409,6 +409,8 @@ SceneView psdr_scene::view() const {
   v.edge_pmf = edge_pmf.as<float>();
   v.edge_cmf = edge_cmf.as<float>();
   v.n_edges = n_edges;
+  v.seek_leaf = seek_leaf.as<float4>();
+  v.n_seek = n_seek;
   return v;
 }
 
@@ -540,6 +542,19 @@ static int scene_create_impl(const psdr_scene_desc* d, psdr_scene* s, hipStream_
   // objects / lights
   s->objs_host.resize(d->n_objects);
   for (int o = 0; o < d->n_objects; ++o) s->objs_host[o] = object_k(d->objects[o]);
+  // the emitting triangles a BSDF-sampled ray tests before it traverses
+  s->seek_ranges.clear();
+  s->n_seek = -1;
+  if (s->n_bvh == nt) {
+    int cnt = 0;
+    for (int o = 0; o < d->n_objects; ++o)
+      if (d->objects[o].light >= 0 && d->objects[o].tri_count > 0) {
+        s->seek_ranges.emplace_back(d->objects[o].tri_begin, d->objects[o].tri_count);
+        cnt += d->objects[o].tri_count;
+      }
+    if (cnt <= SEEK_MAX) s->n_seek = cnt;
+  }
+  PSDR_TRY(s->seek_leaf.ensure(48ull * SEEK_MAX));
   std::vector<LightK> lk(d->n_lights);
   int off = 0;
   for (int l = 0; l < d->n_lights; ++l) {

@@ -73,7 +73,8 @@ __global__ __launch_bounds__(TRACE_THREADS) void k_trace(SceneView sv, const flo
                                                          float* __restrict__ ov, int* __restrict__ spill,
                                                          unsigned long long* __restrict__ timed_bytes,
                                                          unsigned long long* __restrict__ span,
-                                                         unsigned long long* __restrict__ counters, int prio) {
+                                                         unsigned long long* __restrict__ counters, int prio,
+                                                         int nseek) {
   // launches on the step's critical path issue ahead of concurrent kernels'
   // waves on the same SIMD (wave priority, not stream priority)
   if (prio) __builtin_amdgcn_s_setprio(3);
@@ -104,19 +105,27 @@ __global__ __launch_bounds__(TRACE_THREADS) void k_trace(SceneView sv, const flo
     }
     const f3 o = f3{ro[3 * i], ro[3 * i + 1], ro[3 * i + 2]};
     const f3 d = f3{rd[3 * i], rd[3 * i + 1], rd[3 * i + 2]};
-    // segments [0, nvis) are light-sample rays: visibility queries (see closest_hit)
+    // segments [0, nvis) are light-sample rays: visibility queries (see
+    // closest_hit); [nvis, nvis + nseek) BSDF-sampled rays whose hit matters
+    // only when it is an emitting triangle (see light_seeking_hit)
     float tmax = F_INF, tocc = -F_INF;
     if (k < nvis) {
       const float t = rtl[i];
       tmax = vis_tmax(t);
       tocc = vis_tocc(t);
     }
+    const bool seek = sv.n_seek >= 0 && k >= nvis && k < nvis + nseek;
     HitRec h;
     if (STATS) {
       RayStats st;
-      h = closest_hit<STACK_LDS>(sv.node, sv.leaf, sv.n_bvh, sv.n_tris, o, d, stack, spill + gtid, stride, tmax, tocc,
-                                 st);
-      const bool closest = k >= nvis;  // visibility queries stop early: no second distance
+      bool skip = false;
+      const HitRec seed = seek_lights(sv.seek_leaf, sv.n_seek, seek, o, d, &skip, st);
+      h = skip ? seed
+               : closest_hit<STACK_LDS>(sv.node, sv.leaf, sv.n_bvh, sv.n_tris, o, d, stack, spill + gtid, stride,
+                                        tmax, tocc, st, &seed);
+      // visibility queries stop early (no second distance); a light-seeking ray
+      // that hit no emitting triangle was not traced
+      const bool closest = k >= nvis && !skip;
       atomicAdd(&cnt_lds[0], 1ull);
       atomicAdd(&cnt_lds[1], static_cast<unsigned long long>(st.nodes));
       atomicAdd(&cnt_lds[2], static_cast<unsigned long long>(st.leaves));
@@ -126,7 +135,12 @@ __global__ __launch_bounds__(TRACE_THREADS) void k_trace(SceneView sv, const flo
       if (h.tri >= 0) atomicAdd(&cnt_lds[6], 1ull);
       if (closest) atomicAdd(&cnt_lds[7], 1ull);
     } else {
-      h = closest_hit<STACK_LDS>(sv.node, sv.leaf, sv.n_bvh, sv.n_tris, o, d, stack, spill + gtid, stride, tmax, tocc);
+      NoStats st;
+      bool skip = false;
+      const HitRec seed = seek_lights(sv.seek_leaf, sv.n_seek, seek, o, d, &skip, st);
+      h = skip ? seed
+               : closest_hit<STACK_LDS>(sv.node, sv.leaf, sv.n_bvh, sv.n_tris, o, d, stack, spill + gtid, stride,
+                                        tmax, tocc, st, &seed);
     }
     otri[i] = h.tri;
     if (ot) ot[i] = h.tri >= 0 ? h.t : -1.f;
@@ -612,7 +626,7 @@ constexpr size_t TIMER_PAIRS = 4096;
 
 int trace_rays(const SceneView& sv, const float* ro, const float* rd, int n, const int* count_dev, int K, int seg,
                const float* tl, int nvis, int* tri, float* t, float* u, float* v, int* spill, int grid,
-               hipStream_t st, int stack_lds, int prio) {
+               hipStream_t st, int stack_lds, int prio, int nseek) {
   TraceTimer& tt = trace_timer();
   std::lock_guard<std::mutex> lock(tt.mu);
   const bool timed = tt.on && tt.used < TIMER_PAIRS;
@@ -623,7 +637,7 @@ int trace_rays(const SceneView& sv, const float* ro, const float* rd, int n, con
   unsigned long long* cn = tt.count_on ? tt.counters : nullptr;
 #define PSDR_LAUNCH_TRACE(S, STATS)                                                                                  \
   hipLaunchKernelGGL((k_trace<S, STATS>), dim3(grid), dim3(TRACE_THREADS), 0, st, sv, ro, rd, n, count_dev, K, seg, tl, \
-                     nvis, tri, t, u, v, spill, tb, sp, cn, prio)
+                     nvis, tri, t, u, v, spill, tb, sp, cn, prio, nseek)
   switch (stack_lds ? stack_lds : tuning().trace_stack.load()) {
     case 8:
       if (cn) PSDR_LAUNCH_TRACE(8, true);
@@ -801,16 +815,19 @@ int run_depths(psdr_scene* s, const WaveArgs& a, const QueueView& q0, const Rays
         const int g1 = trace_grid(n);
         for (int k = 0; k < P.K_out; ++k) {
           const int64_t o = static_cast<int64_t>(k) * cap;
-          const int rc = trace_rays(sv, rout.o + 3 * o, rout.d + 3 * o, n, nullptr, 1, 0, rout.tl + o,
-                                    k < static_cast<int>(a.nL) ? 1 : 0, rout.tri + o, nullptr, rout.u + o,
-                                    rout.v + o, spill.as<int>(), g1, st, 0, a.prio);
+          const int kind_vis = k < static_cast<int>(a.nL) ? 1 : 0;
+          const int kind_seek = !kind_vis && k < static_cast<int>(a.nL + a.nB) ? 1 : 0;
+          const int rc = trace_rays(sv, rout.o + 3 * o, rout.d + 3 * o, n, nullptr, 1, 0, rout.tl + o, kind_vis,
+                                    rout.tri + o, nullptr, rout.u + o, rout.v + o, spill.as<int>(), g1, st, 0, a.prio,
+                                    kind_seek);
           if (rc) return rc;
           PSDR_HIP_CHECK(hipStreamSynchronize(st));
         }
       } else {
+        // segments: nL light-sample rays, nB BSDF-sampled rays, the extension
         const int rc = trace_rays(sv, rout.o, rout.d, n, a.counters + d + 1, P.K_out, cap, rout.tl,
                                   static_cast<int>(a.nL), rout.tri, nullptr, rout.u, rout.v, spill.as<int>(), tgrid,
-                                  st, 0, a.prio);
+                                  st, 0, a.prio, static_cast<int>(a.nB));
         if (rc) return rc;
       }
     }

@@ -96,6 +96,10 @@ struct LightK {
 };
 
 // Read-only view of a scene passed by value to render kernels.
+// Most emitting triangles for which a BSDF-sampled ray tests the lights first
+// (see SceneView::seek_leaf).
+constexpr int SEEK_MAX = 16;
+
 struct SceneView {
   const float4* tri_geom;
   const float* tri_uv;
@@ -124,6 +128,12 @@ struct SceneView {
   const float* edge_pmf;
   const float* edge_cmf;
   int n_edges;
+  // leaf records of every emitting triangle in the LBVH (n_seek of them), or
+  // n_seek = -1: BSDF-sampled rays then resolve "which light, if any, is the
+  // closest hit" by a full traversal (an environment light, whose cube every ray
+  // hits, or more than SEEK_MAX emitting triangles)
+  const float4* seek_leaf;
+  int n_seek;
 };
 
 // Forward state of one renderD image kept for its backward pass: the
@@ -220,6 +230,11 @@ struct psdr_scene {
   // lane order and the gradient terms run one after another.
   bool plain = false;
   int light_tri_total = 0;
+  // emitting triangles' ranges (tri_begin, count) for the seek leaves, and their
+  // total, or -1 (see SceneView::n_seek)
+  std::vector<std::pair<int, int>> seek_ranges;
+  int n_seek = -1;
+  psdr::DevBuf seek_leaf;
 
   psdr_scene() = default;
   psdr_scene(const psdr_scene&) = delete;

@@ -760,6 +760,16 @@ int radix_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, uin
 
 int build_lbvh(psdr_scene* s, hipStream_t st) {
   const int n = s->n_bvh;
+  if (s->n_seek > 0) {
+    // the emitting triangles' leaf records again, in id order (seek leaves)
+    int off = 0;
+    for (const auto& r : s->seek_ranges) {
+      hipLaunchKernelGGL(k_leaf, dim3(1), dim3(64), 0, st, s->tri_geom.as<float4>() + 5ll * r.first, nullptr,
+                         r.second, r.first, s->seek_leaf.as<float4>() + 3ll * off, nullptr, nullptr);
+      PSDR_HIP_CHECK(hipGetLastError());
+      off += r.second;
+    }
+  }
   if (s->n_tris > n) {
     // leaves tested by every ray (an environment cube), after the tree's own
     hipLaunchKernelGGL(k_leaf, dim3(1), dim3(64), 0, st, s->tri_geom.as<float4>() + 5ll * n, nullptr, s->n_tris - n,

@@ -158,13 +158,19 @@ __device__ __forceinline__ void extra_leaves(const float4* __restrict__ leaf, in
   for (int k = n_tris; k < n_all; ++k) leaf_test(leaf, k, o, d, h, st);
 }
 
+// seed: a hit the ray is known to have (or null); the traversal then prunes
+// against it from the start.  The result is the same as without it: the
+// closest hit is the least (t, id) over all hit triangles, whatever the order
+// in which they are found.
 template <int STACK_LDS, class S = NoStats>
 __device__ __forceinline__ HitRec closest_hit(const float4* __restrict__ node, const float4* __restrict__ leaf,
                                               int n_tris, int n_all, f3 o, f3 d, int* stack_lds, int* spill,
-                                              int spill_stride, float tmax, float tocc, S& st) {
+                                              int spill_stride, float tmax, float tocc, S& st,
+                                              const HitRec* seed = nullptr) {
   HitRec h{-1, F_INF, -1.f, -1.f};
   if (!(d.x == d.x) || n_all <= 0) return h;
   h.t = tmax;
+  if (seed && seed->tri >= 0) h = *seed;
   if (n_tris <= 1) {
     if (n_tris == 1) leaf_test(leaf, 0, o, d, h, st);
     extra_leaves(leaf, n_tris, n_all, o, d, tocc, h, st);
@@ -229,6 +235,24 @@ __device__ __forceinline__ HitRec closest_hit(const float4* __restrict__ node, c
                                               int spill_stride, float tmax = F_INF, float tocc = -F_INF) {
   NoStats st;
   return closest_hit<STACK_LDS>(node, leaf, n_tris, n_all, o, d, stack_lds, spill, spill_stride, tmax, tocc, st);
+}
+
+// A BSDF-sampled ray only needs its closest hit when that hit is an emitting
+// triangle (src/path_integrator.cc:134-160 adds light only then).  For such a
+// ray (seek) the scene's few emitting triangles (seek leaves) are tested first:
+// a ray that hits none of them cannot have an emitting closest hit, so *skip is
+// set and the returned miss (tri -1) stands for its hit; otherwise the nearest
+// emitting hit is returned as the seed of the ordinary closest-hit traversal.
+// Without seek the result is an empty seed.
+template <class S>
+__device__ __forceinline__ HitRec seek_lights(const float4* __restrict__ seek_leaf, int n_seek, bool seek, f3 o,
+                                              f3 d, bool* skip, S& st) {
+  HitRec h{-1, F_INF, -1.f, -1.f};
+  *skip = false;
+  if (!seek || !(d.x == d.x)) return h;
+  for (int m = 0; m < n_seek; ++m) leaf_test(seek_leaf, m, o, d, h, st);
+  *skip = h.tri < 0;
+  return h;
 }
 
 // A closest hit within 4 ulp of the next triangle the ray hits (an order
