@@ -131,7 +131,7 @@ __device__ __forceinline__ void leaf_test(const float4* __restrict__ leaf, int k
       h.tri = id;
       h.u = u;
       h.v = v;
-    } else {
+    } else if (id != h.tri) {  // a seeded traversal meets its seed's triangle again
       st.second(t);
     }
   }
