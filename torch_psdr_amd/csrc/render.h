@@ -22,13 +22,15 @@ int num_cus();
 constexpr int TRACE_BLOCK_THREADS = 256;
 // Rays are K segments (segment k at k * seg); segments [0, nvis) are
 // visibility queries toward the distances tl, [nvis, nvis + nseek) BSDF-sampled
-// rays that need their hit only when it is an emitting triangle.
+// rays that need their hit only when it is an emitting triangle; with
+// seek_count (device) the one such segment holds only its first *seek_count
+// entries.
 // stack_lds: LDS stack entries per lane (8, 16 or 32; 0 = the tuned value);
 // prio != 0: the launch's waves take issue priority over concurrent kernels'.
 // spill must hold grid * TRACE_BLOCK_THREADS * (STACK_TOTAL - stack_lds) ints.
 int trace_rays(const SceneView& sv, const float* ro, const float* rd, int n, const int* count_dev, int K, int seg,
                const float* tl, int nvis, int* tri, float* t, float* u, float* v, int* spill, int grid,
-               hipStream_t st, int stack_lds = 0, int prio = 0, int nseek = 0);
+               hipStream_t st, int stack_lds = 0, int prio = 0, int nseek = 0, const int* seek_count = nullptr);
 constexpr unsigned long long RAY_BYTES_PER = 28;  // ray buffers: origin, direction, light distance
 int trace_grid(int64_t max_rays);
 int trace_timing(int enable, double* total_ms, int64_t* launches, int64_t* bytes, double* busy_ms);

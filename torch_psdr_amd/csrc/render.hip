@@ -32,7 +32,7 @@ namespace psdr {
 namespace {
 
 constexpr int TRACE_THREADS = TRACE_BLOCK_THREADS;
-// render counters: [d] paths entering shade(d)
+// render counters of a tile's run (WaveArgs::counters)
 constexpr int COUNTERS = 128;
 
 __global__ void k_lane_draws(uint32_t lane0, int n_lanes, int n_draws, uint32_t* out) {
@@ -74,7 +74,7 @@ __global__ __launch_bounds__(TRACE_THREADS) void k_trace(SceneView sv, const flo
                                                          unsigned long long* __restrict__ timed_bytes,
                                                          unsigned long long* __restrict__ span,
                                                          unsigned long long* __restrict__ counters, int prio,
-                                                         int nseek) {
+                                                         int nseek, const int* __restrict__ seek_count) {
   // launches on the step's critical path issue ahead of concurrent kernels'
   // waves on the same SIMD (wave priority, not stream priority)
   if (prio) __builtin_amdgcn_s_setprio(3);
@@ -89,12 +89,16 @@ __global__ __launch_bounds__(TRACE_THREADS) void k_trace(SceneView sv, const flo
   // the launch's execution time without the time it waited for CUs
   if (span && threadIdx.x == 0) atomicMin(&span[0], static_cast<unsigned long long>(wall_clock64()));
   // rays are K segments of `per` rays each, segment k starting at k * seg
-  // (one segment per ray kind, so a wave traces rays of one kind)
+  // (one segment per ray kind, so a wave traces rays of one kind); with
+  // seek_count, the one light-seeking segment holds only its first *seek_count
+  // entries (the BSDF-sampled rays that can hit an emitting triangle, packed)
   const int per = count ? *count : n;
   const int cnt = per * K;
+  const int mseek = seek_count ? *seek_count : per;
   // algorithmic bytes of this launch: ray read (24 B) + tri, u, v (+ t) written
   if (timed_bytes && blockIdx.x == 0 && threadIdx.x == 0)
-    atomicAdd(timed_bytes, static_cast<unsigned long long>(cnt) * (ot ? 40ull : 36ull));
+    atomicAdd(timed_bytes, static_cast<unsigned long long>(cnt - (seek_count ? per - mseek : 0)) *
+                               (ot ? 40ull : 36ull));
   const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
   const int stride = gridDim.x * blockDim.x;
   for (int ii = gtid; ii < cnt; ii += stride) {
@@ -103,6 +107,7 @@ __global__ __launch_bounds__(TRACE_THREADS) void k_trace(SceneView sv, const flo
       k = ii / per;
       i = k * seg + (ii - k * per);
     }
+    if (seek_count && k == nvis && ii - k * per >= mseek) continue;
     const f3 o = f3{ro[3 * i], ro[3 * i + 1], ro[3 * i + 2]};
     const f3 d = f3{rd[3 * i], rd[3 * i + 1], rd[3 * i + 2]};
     // segments [0, nvis) are light-sample rays: visibility queries (see
@@ -180,6 +185,10 @@ struct ShadeParams {
   int K_in, K_out, Kmax;
   int keep;
   int plain;  // reference launch shape: no compaction, queue entry j stays at slot j
+  // the BSDF-sampled rays (nB == 1) that can hit an emitting triangle are packed
+  // at the front of their segment; the segment's tl entry of queue slot j holds
+  // the index of slot j's ray there, or -1 (no emitting triangle in its way)
+  int pack_mis;
 };
 
 // plain launch shape: a finished path keeps its queue slot with this bit set
@@ -233,7 +242,7 @@ template <bool MULTI>
 __global__ __launch_bounds__(256) void k_shade(ShadeParams P, QueueView qi, const int* __restrict__ count_in,
                                                RaysView rin, QueueView qo, int* __restrict__ count_out,
                                                RaysView rout, CacheView cache0, float* __restrict__ lane_L,
-                                               uint64_t* __restrict__ lane_rng_out) {
+                                               uint64_t* __restrict__ lane_rng_out, int* __restrict__ mis_count_out) {
   if (P.prio) __builtin_amdgcn_s_setprio(3);
   const SceneView& sv = P.sv;
   const int cnt = *count_in;
@@ -331,8 +340,9 @@ __global__ __launch_bounds__(256) void k_shade(ShadeParams P, QueueView qi, cons
         for (uint32_t i = 0; i < P.nB; ++i, ++k) {
           const float ru = rng.next();
           const float rv = rng.next();
-          const int ht = rin.tri[k * rs + j];
-          const float hu = rin.u[k * rs + j], hv = rin.v[k * rs + j];
+          const int mi = P.pack_mis ? __float_as_int(rin.tl[k * rs + j]) : j;
+          const int ht = mi >= 0 ? rin.tri[k * rs + mi] : -1;
+          const float hu = mi >= 0 ? rin.u[k * rs + mi] : -1.f, hv = mi >= 0 ? rin.v[k * rs + mi] : -1.f;
           if (P.keep) {
             const int64_t ci = (cslot + k) * cache.N + loc;
             cache.tri[ci] = ht;
@@ -413,8 +423,11 @@ __global__ __launch_bounds__(256) void k_shade(ShadeParams P, QueueView qi, cons
     } else {
       slot = wave_alloc(alive, count_out);
     }
+    bool mis_want = false;  // packed BSDF-sampled ray to write
+    f3 mis_d = splat(0.f);
+    uint64_t rng_start = 0;
     if (have && alive) {
-      const uint64_t rng_start = rng.state;
+      rng_start = rng.state;
       if (P.keep) {
         const int64_t N = cache.N;
         cache.T[(3ll * d) * N + loc] = T.x;
@@ -444,8 +457,26 @@ __global__ __launch_bounds__(256) void k_shade(ShadeParams P, QueueView qi, cons
         const float ru = rng.next();
         const float rv = rng.next();
         const Sampled smp = mat_sample(sv, its, ru, rv);
-        put_ray(rout, k * rs + slot, its.p, smp.wo);
+        if (P.pack_mis) {
+          bool skip;
+          NoStats nst;
+          (void)seek_lights(sv.seek_leaf, sv.n_seek, true, its.p, smp.wo, &skip, nst);
+          mis_want = !skip;
+          mis_d = smp.wo;
+        } else {
+          put_ray(rout, k * rs + slot, its.p, smp.wo);
+        }
       }
+    }
+    if (P.pack_mis) {
+      const int seg = static_cast<int>(P.nL) * qo.cap;
+      const int m = wave_alloc(mis_want, mis_count_out);
+      if (mis_want) put_ray(rout, seg + m, its.p, mis_d);
+      if (have && alive) rout.tl[seg + slot] = __int_as_float(mis_want ? m : -1);
+    }
+    if (have && alive) {
+      const int rs = qo.cap;
+      const int k = static_cast<int>(P.nL + P.nB);
       {
         const float ru = rng.next();
         const float rv = rng.next();
@@ -626,7 +657,7 @@ constexpr size_t TIMER_PAIRS = 4096;
 
 int trace_rays(const SceneView& sv, const float* ro, const float* rd, int n, const int* count_dev, int K, int seg,
                const float* tl, int nvis, int* tri, float* t, float* u, float* v, int* spill, int grid,
-               hipStream_t st, int stack_lds, int prio, int nseek) {
+               hipStream_t st, int stack_lds, int prio, int nseek, const int* seek_count) {
   TraceTimer& tt = trace_timer();
   std::lock_guard<std::mutex> lock(tt.mu);
   const bool timed = tt.on && tt.used < TIMER_PAIRS;
@@ -637,7 +668,7 @@ int trace_rays(const SceneView& sv, const float* ro, const float* rd, int n, con
   unsigned long long* cn = tt.count_on ? tt.counters : nullptr;
 #define PSDR_LAUNCH_TRACE(S, STATS)                                                                                  \
   hipLaunchKernelGGL((k_trace<S, STATS>), dim3(grid), dim3(TRACE_THREADS), 0, st, sv, ro, rd, n, count_dev, K, seg, tl, \
-                     nvis, tri, t, u, v, spill, tb, sp, cn, prio, nseek)
+                     nvis, tri, t, u, v, spill, tb, sp, cn, prio, nseek, seek_count)
   switch (stack_lds ? stack_lds : tuning().trace_stack.load()) {
     case 8:
       if (cn) PSDR_LAUNCH_TRACE(8, true);
@@ -782,6 +813,9 @@ int run_depths(psdr_scene* s, const WaveArgs& a, const QueueView& q0, const Rays
   // ray k of queue entry j is at k * cap + j (k_shade); the queues of one run
   // share a capacity >= n
   const int cap = q_s[0].cap;
+  // one BSDF-sampled ray per vertex and few emitting triangles: only the rays
+  // that can hit one are traced, packed (their counts after the depth counts)
+  const int pack = !a.plain && sv.n_seek >= 0 && a.nB == 1 && a.D < MIS_COUNT_OFF ? 1 : 0;
   for (int d = 0; d <= a.D; ++d) {
     const QueueView qout = q_s[d & 1];
     const RaysView rout = r_s[d & 1];
@@ -802,12 +836,14 @@ int run_depths(psdr_scene* s, const WaveArgs& a, const QueueView& q0, const Rays
     P.Kmax = Kmax;
     P.keep = a.keep;
     P.plain = a.plain;
+    P.pack_mis = pack;
+    int* mis_count = pack ? a.counters + MIS_COUNT_OFF + d : nullptr;
     if (a.njobs > 1)
       hipLaunchKernelGGL(k_shade<true>, dim3(sgrid), dim3(256), 0, st, P, qin, a.counters + d, rin, qout,
-                         a.counters + d + 1, rout, a.cache, a.lane_L, a.lane_rng_out);
+                         a.counters + d + 1, rout, a.cache, a.lane_L, a.lane_rng_out, mis_count);
     else
       hipLaunchKernelGGL(k_shade<false>, dim3(sgrid), dim3(256), 0, st, P, qin, a.counters + d, rin, qout,
-                         a.counters + d + 1, rout, a.cache, a.lane_L, a.lane_rng_out);
+                         a.counters + d + 1, rout, a.cache, a.lane_L, a.lane_rng_out, mis_count);
     PSDR_HIP_CHECK(hipGetLastError());
     if (d < a.D && P.K_out > 0) {
       if (a.plain) {
@@ -827,7 +863,7 @@ int run_depths(psdr_scene* s, const WaveArgs& a, const QueueView& q0, const Rays
         // segments: nL light-sample rays, nB BSDF-sampled rays, the extension
         const int rc = trace_rays(sv, rout.o, rout.d, n, a.counters + d + 1, P.K_out, cap, rout.tl,
                                   static_cast<int>(a.nL), rout.tri, nullptr, rout.u, rout.v, spill.as<int>(), tgrid,
-                                  st, 0, a.prio, static_cast<int>(a.nB));
+                                  st, 0, a.prio, static_cast<int>(a.nB), mis_count);
         if (rc) return rc;
       }
     }

@@ -60,6 +60,10 @@ __device__ __forceinline__ int lane_job(const LaneJobs* J, uint32_t g) {
   return j;
 }
 
+// offset of the packed BSDF-sampled ray counts in a run's counter block (runs
+// with D < MIS_COUNT_OFF pack them)
+constexpr int MIS_COUNT_OFF = 64;
+
 QueueView carve_queue(DevBuf& b, int cap);
 RaysView carve_rays(DevBuf& rb, DevBuf& hb, int64_t cap);
 RaysView offset_rays(const RaysView& r, int64_t off);
@@ -77,7 +81,9 @@ struct WaveArgs {
   CacheView cache;
   float* lane_L;
   uint64_t* lane_rng_out;
-  int* counters;  // 128 ints, zeroed by the callee except counters[0] = *count0
+  // 128 ints, zeroed by the caller except counters[0] = *count0: [d] paths
+  // entering depth d, [MIS_COUNT_OFF + d] packed BSDF-sampled rays of depth d
+  int* counters;
   int plain = 0;  // reference launch shape (see psdr_scene::plain)
   DevBuf* spill = nullptr;  // traversal stack overflow area (grown as needed)
   int prio = 0;                    // raise the launches' wave priority (critical path)
