@@ -387,8 +387,7 @@ def plain_baseline(torch, ops, step, verts, tex, n_steps, rays, grads):
 # shapes named here; another shape reports null.  Commands and per-launch table:
 # profiles/k_trace_gradient_step_512px_16spp.md.
 TRACE_TRAFFIC = [
-    {"shape": {"trace_blocks_per_cu": 16, "trace_stack_lds": 16}, "bytes_per_launch": 63144380},
-    {"shape": {"trace_blocks_per_cu": 32, "trace_stack_lds": 8}, "bytes_per_launch": 63579410},
+    {"shape": {"trace_blocks_per_cu": 16, "trace_stack_lds": 16}, "bytes_per_launch": 48756206},
 ]
 
 
