@@ -176,13 +176,19 @@ def test_all_terms_equal_sum_of_terms(gpu_device):
             assert np.abs(tot[b] - ref).max() <= 1e-4 * scale, (rep, b, np.abs(tot[b] - ref).max(), scale)
 
 
-def test_plain_launch_shape_matches_fused(gpu_device):
+@pytest.mark.parametrize("n_light", [1, 2, 0])
+def test_plain_launch_shape_matches_fused(gpu_device, n_light):
     """The reference-shaped baseline (no compaction, one synchronised trace per
-    ray kind, unsorted boundary samples, sequential terms) renders the same image
-    bit for bit and the same gradients up to accumulation order."""
+    ray kind, unsorted boundary samples, sequential terms, the primary edges'
+    hypercube preprocessing as wavefront launches) renders the same image bit
+    for bit and the same gradients up to accumulation order.  The fused step
+    runs that preprocessing as one kernel (a lane per ray of a depth with one
+    light sample per vertex, a thread per cell otherwise); a different cell
+    mass would move the boundary samples and change the gradients."""
     meshes, lights, leaves = H.objects(gpu_device, level=2, tex_res=8, requires_grad=True)
     scene = dr.Scene([H.camera(gpu_device, 24)], meshes, lights)
-    integ = dr.PathIntegrator(spp_interior=2, max_bounce=3, spp_primary_edge=2, spp_secondary_edge=2)
+    integ = dr.PathIntegrator(spp_interior=2, max_bounce=3, spp_primary_edge=2, spp_secondary_edge=2,
+                              mis_light_samples=n_light)
     target = torch.rand((24, 24, 3), generator=torch.Generator().manual_seed(3)).to(gpu_device)
     out = {}
     for shape in ("fused", "plain", "fused"):

@@ -30,6 +30,7 @@
 
 #include "cam_grad.h"
 #include "grad_cache.h"
+#include "path.h"
 #include "render.h"
 #include "shading.h"
 #include "traverse.h"
@@ -278,6 +279,260 @@ __global__ __launch_bounds__(BT) void k_prim_finish(SceneView sv, CameraK cam, i
   }
 }
 
+// The preprocessing round of the primary boundary in one launch, one hypercube
+// cell per thread: the steps of k_prim_setup<true>, k_prim_rays, the side
+// rays' traversal, k_prim_check, trace2light from each side (run_depths with
+// B = 1 and no light visibility) and prim_finish_one<true>, with every ray
+// traced where it is made.  A cell's work is a chain of dependent traversals
+// over ~10^4 cells; as separate wavefront launches (~30 of them) each level
+// waits for its slowest ray and for free CUs beside the concurrent interior
+// render, which made this pass the step's critical path.  Values are the
+// wavefront's: the same draws in the same order, the same traversals and the
+// shared path functions of path.h.
+constexpr int PRE_THREADS = 64;
+constexpr int PRE_STACK = 16;
+
+// trace2light<false> (src/path_integrator.cc:80-186) from vertex `its` with
+// PCG state *rng, as run_depths computes it for one queue entry: returns the
+// lane radiance and leaves *rng after the path's last draw.
+__device__ __forceinline__ f3 pre_path(const SceneView& sv, int D, uint32_t nL, Its its, f3 O, Rng* rng, int* stk,
+                                       int* spill, int spill_stride) {
+  f3 T = splat(1.f), L = splat(0.f);
+  bool alive = its.valid;
+  for (int d = 0; alive && d < D && !its.is_light; ++d) {
+    const Its ip = its;
+    // next-event estimation
+    for (uint32_t i = 0; i < nL; ++i) {
+      const float ru = rng->next();
+      const float rv = rng->next();
+      f3 dir;
+      float tl;
+      light_sample_ray(sv, ip.p, ru, rv, &dir, &tl);
+      const HitRec h = closest_hit<PRE_STACK>(sv.node, sv.leaf, sv.n_bvh, sv.n_tris, ip.p, dir, stk, spill,
+                                              spill_stride, vis_tmax(tl), vis_tocc(tl));
+      light_sample_add(sv, ip, ru, rv, h.tri, h.u, h.v, nL, 1, 0, T, &L);
+    }
+    // BSDF sample with MIS: only an emitting closest hit matters
+    {
+      const float ru = rng->next();
+      const float rv = rng->next();
+      const Sampled smp = mat_sample(sv, ip, ru, rv);
+      NoStats nst;
+      bool skip = false;
+      const HitRec seed = seek_lights(sv.seek_leaf, sv.n_seek, sv.n_seek >= 0, ip.p, smp.wo, &skip, nst);
+      const HitRec h = skip ? seed
+                            : closest_hit<PRE_STACK>(sv.node, sv.leaf, sv.n_bvh, sv.n_tris, ip.p, smp.wo, stk, spill,
+                                                     spill_stride, F_INF, -F_INF, nst, &seed);
+      bsdf_sample_add(sv, ip, ru, rv, h.tri, h.u, h.v, nL, 1, 0, T, &L);
+    }
+    // extension (none after the last depth; its draws are made all the same)
+    const float ru = rng->next();
+    const float rv = rng->next();
+    if (d >= D - 1) break;
+    const Sampled smp = mat_sample(sv, ip, ru, rv);
+    const HitRec h = closest_hit<PRE_STACK>(sv.node, sv.leaf, sv.n_bvh, sv.n_tris, ip.p, smp.wo, stk, spill,
+                                            spill_stride);
+    alive = extend_path(sv, ip, ru, rv, h.tri, h.u, h.v, 0, &its, &T, &O);
+  }
+  if (!isfinite(sqnorm(L))) L = splat(0.f);
+  return L * 1.0f;  // J0 = 1 without renderD's area jacobians
+}
+
+// k_prim_setup<true> for cell i: its draw, edge point and the side rays'
+// direction; false when the edge point is not in view.
+struct PreCell {
+  uint64_t rng0;
+  f3 ep, o, d, nn;
+  float pdf_ep;
+};
+__device__ __forceinline__ bool pre_cell(const SceneView& sv, const CameraK& cam, int n, int i,
+                                         const uint64_t* __restrict__ state, PreCell* c) {
+  Rng rng;
+  rng.state = state[i];
+  rng.inc = lane_inc(static_cast<uint32_t>(i));
+  const float s = (static_cast<float>(i) + rng.next()) * (1.0f / static_cast<float>(n));
+  c->rng0 = rng.state;
+  float u = s, pdf_e;
+  const int e = discrete_sample(sv.edge_pmf, sv.edge_cmf, sv.n_edges, &u, &pdf_e);
+  const EdgeRec E = load_edge(sv, e);
+  c->ep = E.p0 + u * (E.p1 - E.p0);
+  int px, py;
+  const bool valid = camera_point_ray(cam, c->ep, &px, &py, &c->o, &c->d);
+  c->nn = cross(c->d, normalize(E.p1 - E.p0));
+  c->pdf_ep = pdf_e / E.len;
+  return valid;
+}
+
+// k_prim_check: both side rays hit and one of them lies on the edge point
+__device__ __forceinline__ bool pre_check(const SceneView& sv, const PreCell& c, const HitRec& a, const HitRec& b) {
+  const Its ia = reconstruct(sv, a.tri, a.u, a.v, c.o, true);
+  const Its ib = reconstruct(sv, b.tri, b.u, b.v, c.o, true);
+  return ia.valid && ib.valid && (norm(ia.p - c.ep) < 10.0f * REAL_EPS || norm(ib.p - c.ep) < 10.0f * REAL_EPS);
+}
+
+// One cell per thread (any number of light samples per vertex).
+__global__ __launch_bounds__(PRE_THREADS) void k_prim_pre(SceneView sv, CameraK cam, int n, int D, uint32_t nL,
+                                                           uint64_t* __restrict__ state, float* __restrict__ mass,
+                                                           int* __restrict__ spill) {
+  // the primary boundary's chain is the step's critical path
+  __builtin_amdgcn_s_setprio(3);
+  __shared__ int stk[PRE_STACK * PRE_THREADS];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int stride = gridDim.x * blockDim.x;
+  if (i >= n) return;
+  int* sp = spill + i;
+  PreCell c;
+  bool ok = pre_cell(sv, cam, n, i, state, &c);
+  HitRec ha{-1, F_INF, -1.f, -1.f}, hb = ha;
+  if (ok) {
+    ha = closest_hit<PRE_STACK>(sv.node, sv.leaf, sv.n_bvh, sv.n_tris, c.o, c.d - c.nn * REAL_EPS, stk, sp, stride);
+    hb = closest_hit<PRE_STACK>(sv.node, sv.leaf, sv.n_bvh, sv.n_tris, c.o, c.d + c.nn * REAL_EPS, stk, sp, stride);
+    ok = pre_check(sv, c, ha, hb);
+  }
+  if (!ok) {
+    // the paths of an invalid sample end at once: no draws, no radiance
+    state[i] = c.rng0;
+    return;
+  }
+  // both sides' paths from the camera-side vertex (origin: the camera, or the
+  // orthographic ray's own origin), side B continuing side A's PCG stream
+  const f3 O = cam.ortho ? c.o : cam.pos;
+  Rng r{c.rng0, lane_inc(static_cast<uint32_t>(i))};
+  const f3 LA = pre_path(sv, D, nL, reconstruct(sv, ha.tri, ha.u, ha.v, O, true), O, &r, stk, sp, stride);
+  const f3 LB = pre_path(sv, D, nL, reconstruct(sv, hb.tri, hb.u, hb.v, O, true), O, &r, stk, sp, stride);
+  // prim_finish_one<true>
+  state[i] = r.state;
+  const f3 value = (LA - LB) / c.pdf_ep;
+  if (isfinite(sqnorm(value))) mass[i] = mass[i] + norm(value);
+}
+
+// One light sample per vertex: four lanes per cell.  A round traces the
+// cell's rays of one depth on separate lanes (the two side rays; then the
+// light-sample, BSDF-sample and extension rays of each depth of side A, then
+// of side B); lane 0 of the group draws, shades and makes the next round's
+// rays.  A cell's latency is then one traversal per depth instead of three.
+constexpr int PRE_GROUP = 4;
+struct PreRay {
+  f3 o, d;
+  float tmax, tocc;
+  HitRec h;  // in: the seed (tri -1: none); out: the hit
+  int on;    // trace this slot in the round
+};
+
+__global__ __launch_bounds__(PRE_THREADS) void k_prim_pre_group(SceneView sv, CameraK cam, int n, int D,
+                                                                 uint64_t* __restrict__ state,
+                                                                 float* __restrict__ mass, int* __restrict__ spill) {
+  __builtin_amdgcn_s_setprio(3);
+  __shared__ int stk[PRE_STACK * PRE_THREADS];
+  __shared__ PreRay slot[PRE_THREADS];
+  const int lane = threadIdx.x, sub = lane % PRE_GROUP;
+  const int i = (blockIdx.x * blockDim.x + lane) / PRE_GROUP;  // the cell
+  const int gtid = blockIdx.x * blockDim.x + lane;
+  const int stride = gridDim.x * blockDim.x;
+  PreRay* my = &slot[lane - sub];  // the group's slots
+  const bool lead = sub == 0 && i < n;
+  // lane 0's cell state
+  PreCell c;
+  Rng rng{0, 0};
+  f3 T = splat(1.f), L = splat(0.f), LA = splat(0.f), O = splat(0.f);
+  Its its;
+  int phase = 0;  // 0 side rays, 1 side A, 2 side B, 3 done
+  int d = 0;
+  float ru[3] = {0.f, 0.f, 0.f}, rv[3] = {0.f, 0.f, 0.f};
+  slot[lane].on = 0;
+  __syncthreads();
+  if (lead) {
+    const bool ok = pre_cell(sv, cam, n, i, state, &c);
+    rng = Rng{c.rng0, lane_inc(static_cast<uint32_t>(i))};
+    if (ok) {
+      my[0] = PreRay{c.o, c.d - c.nn * REAL_EPS, F_INF, -F_INF, HitRec{-1, F_INF, -1.f, -1.f}, 1};
+      my[1] = PreRay{c.o, c.d + c.nn * REAL_EPS, F_INF, -F_INF, HitRec{-1, F_INF, -1.f, -1.f}, 1};
+    } else {
+      state[i] = c.rng0;  // an invalid sample's paths end at once
+      phase = 3;
+    }
+  }
+  NoStats nst;
+  const f3 nan3 = splat(__builtin_nanf(""));
+  for (;;) {
+    __syncthreads();
+    // trace this lane's slot
+    PreRay& s = slot[lane];
+    if (s.on) {
+      const HitRec seed = s.h;
+      s.h = closest_hit<PRE_STACK>(sv.node, sv.leaf, sv.n_bvh, sv.n_tris, s.o, s.d, stk, spill + gtid, stride, s.tmax,
+                                   s.tocc, nst, &seed);
+    }
+    __syncthreads();
+    bool more = false;
+    if (lead && phase < 3) {
+      if (phase == 0) {
+        const HitRec ha = my[0].h, hb = my[1].h;
+        if (!pre_check(sv, c, ha, hb)) {
+          state[i] = c.rng0;
+          phase = 3;
+        } else {
+          O = cam.ortho ? c.o : cam.pos;
+          its = reconstruct(sv, ha.tri, ha.u, ha.v, O, true);
+          phase = 1;
+          d = 0;
+          my[3].h = hb;  // side B's first hit, kept for its path (slot 3 is never traced)
+        }
+      } else {
+        // resolve the depth traced in this round
+        const Its ip = its;
+        light_sample_add(sv, ip, ru[0], rv[0], my[0].h.tri, my[0].h.u, my[0].h.v, 1, 1, 0, T, &L);
+        bsdf_sample_add(sv, ip, ru[1], rv[1], my[1].h.tri, my[1].h.u, my[1].h.v, 1, 1, 0, T, &L);
+        bool alive = false;
+        if (d < D - 1) alive = extend_path(sv, ip, ru[2], rv[2], my[2].h.tri, my[2].h.u, my[2].h.v, 0, &its, &T, &O);
+        ++d;
+        if (!alive) its.valid = false;
+      }
+      // paths that end here: side A hands over to side B, side B finishes
+      while (phase == 1 || phase == 2) {
+        if (its.valid && d < D && !its.is_light) break;
+        if (!isfinite(sqnorm(L))) L = splat(0.f);
+        L = L * 1.0f;
+        if (phase == 1) {
+          LA = L;
+          L = splat(0.f);
+          T = splat(1.f);
+          O = cam.ortho ? c.o : cam.pos;
+          its = reconstruct(sv, my[3].h.tri, my[3].h.u, my[3].h.v, O, true);
+          d = 0;
+          phase = 2;
+        } else {
+          state[i] = rng.state;
+          const f3 value = (LA - L) / c.pdf_ep;
+          if (isfinite(sqnorm(value))) mass[i] = mass[i] + norm(value);
+          phase = 3;
+        }
+      }
+      my[0].on = my[1].on = my[2].on = 0;
+      if (phase == 1 || phase == 2) {
+        // the next depth's rays from `its`
+        for (int k = 0; k < 3; ++k) {
+          ru[k] = rng.next();
+          rv[k] = rng.next();
+        }
+        float tl;
+        f3 dir;
+        light_sample_ray(sv, its.p, ru[0], rv[0], &dir, &tl);
+        my[0] = PreRay{its.p, dir, vis_tmax(tl), vis_tocc(tl), HitRec{-1, F_INF, -1.f, -1.f}, 1};
+        const Sampled sb = mat_sample(sv, its, ru[1], rv[1]);
+        bool skip = false;
+        const HitRec seed = seek_lights(sv.seek_leaf, sv.n_seek, sv.n_seek >= 0, its.p, sb.wo, &skip, nst);
+        my[1] = PreRay{its.p, sb.wo, F_INF, -F_INF, seed, skip ? 0 : 1};
+        const bool ext = d < D - 1;
+        my[2] = PreRay{its.p, ext ? mat_sample(sv, its, ru[2], rv[2]).wo : nan3, F_INF, -F_INF,
+                       HitRec{-1, F_INF, -1.f, -1.f}, ext ? 1 : 0};
+        more = true;
+      }
+    }
+    if (!__syncthreads_or(more)) break;
+  }
+}
+
 // ---------------------------------------------------------------- direct
 __device__ __forceinline__ f3 vjp_norm(f3 x, f3 gy) {
   const float r = 1.0f / sqrtf(sqnorm(x));
@@ -439,6 +694,7 @@ __global__ __launch_bounds__(BT) void k_direct_setup(SceneView sv, int n, uint64
                                                      int r1, int r2, const float* __restrict__ cpmf,
                                                      const float* __restrict__ ccmf, uint32_t spp,
                                                      float* __restrict__ drec, const uint32_t* __restrict__ order,
+                                                     const uint32_t* __restrict__ sorted_cells,
                                                      int* __restrict__ qidx, float4* __restrict__ qval,
                                                      int* __restrict__ qcount, int compact) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -464,7 +720,14 @@ __global__ __launch_bounds__(BT) void k_direct_setup(SceneView sv, int n, uint64
       rng = lane_rng(static_cast<uint32_t>(i));
       for (int k = 0; k < 3; ++k) s[k] = rng.next();
       float pdf_cube = 1.f;
-      const int c = discrete_sample(cpmf, ccmf, cells, &s[2], &pdf_cube);
+      // the cell k_direct_keys picked for this lane (the sort key), else the search
+      int c;
+      if (sorted_cells) {
+        c = static_cast<int>(sorted_cells[p]);
+        discrete_reuse(cpmf, ccmf, cells, c, &s[2], &pdf_cube);
+      } else {
+        c = discrete_sample(cpmf, ccmf, cells, &s[2], &pdf_cube);
+      }
       const int c0 = c / (r1 * r2), c1 = (c / r2) % r1, c2 = c % r2;
       s[0] = (s[0] + static_cast<float>(c0)) * u0;
       s[1] = (s[1] + static_cast<float>(c1)) * u1;
@@ -707,9 +970,29 @@ int boundary_primary_prepare(psdr_scene* s, const psdr_config& cfg, const psdr_c
     return grow_failed();
   hipLaunchKernelGGL(k_seed_states, dim3(blocks(R, BT)), dim3(BT), 0, st, slot.b_state.as<uint64_t>(), R);
   hipLaunchKernelGGL(k_fill_f32, dim3(blocks(R, BT)), dim3(BT), 0, st, slot.hc_mass.as<float>(), R, 0.f);
-  for (uint32_t r = 0; r < cfg.primary_edge_preprocess_rounds; ++r) {
-    const int rc = primary_pass(s, cfg, cam, true, R, R, nullptr, nullptr, slot.hc_mass.as<float>(), st);
-    if (rc) return rc;
+  if (s->plain) {
+    for (uint32_t r = 0; r < cfg.primary_edge_preprocess_rounds; ++r) {
+      const int rc = primary_pass(s, cfg, cam, true, R, R, nullptr, nullptr, slot.hc_mass.as<float>(), st);
+      if (rc) return rc;
+    }
+  } else {
+    const int group_blocks = blocks(static_cast<int64_t>(R) * PRE_GROUP, PRE_THREADS);
+    const size_t pre_threads = static_cast<size_t>(group_blocks) * PRE_THREADS;  // >= the one-per-cell grid
+    if (slot.p_spill.ensure(std::max<size_t>(slot.p_spill.bytes, 4ull * (STACK_TOTAL - PRE_STACK) * pre_threads)))
+      return grow_failed();
+    // one light sample per vertex: a lane per ray of a depth (k_prim_pre_group)
+    const bool group = cfg.mis_light_samples == 1;
+    for (uint32_t r = 0; r < cfg.primary_edge_preprocess_rounds; ++r) {
+      if (group)
+        hipLaunchKernelGGL(k_prim_pre_group, dim3(group_blocks), dim3(PRE_THREADS), 0, st, s->view(), cam, R,
+                           static_cast<int>(cfg.max_bounce), slot.b_state.as<uint64_t>(), slot.hc_mass.as<float>(),
+                           slot.p_spill.as<int>());
+      else
+        hipLaunchKernelGGL(k_prim_pre, dim3(blocks(R, PRE_THREADS)), dim3(PRE_THREADS), 0, st, s->view(), cam, R,
+                           static_cast<int>(cfg.max_bounce), cfg.mis_light_samples, slot.b_state.as<uint64_t>(),
+                           slot.hc_mass.as<float>(), slot.p_spill.as<int>());
+      PSDR_HIP_CHECK(hipGetLastError());
+    }
   }
   int rc = sampler_tables(slot.hc_mass.as<float>(), R, slot.hc_pmf.as<float>(), slot.hc_cmf.as<float>(), nullptr,
                           slot.b_tmp.as<float>(), st);
@@ -768,7 +1051,7 @@ int boundary_direct_prepare(psdr_scene* s, const psdr_config& cfg, const psdr_ca
   for (uint32_t r = 0; r < cfg.secondary_edge_preprocess_rounds; ++r) {
     hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(1), 0, st, qcnt);
     hipLaunchKernelGGL(k_direct_setup<true>, dim3(blocks(cells, BT)), dim3(BT), 0, st, sv, cells, state, r0, r1, r2,
-                       nullptr, nullptr, 0u, nullptr, nullptr, qidx, qval, qcnt, compact);
+                       nullptr, nullptr, 0u, nullptr, nullptr, nullptr, qidx, qval, qcnt, compact);
     hipLaunchKernelGGL(k_direct<true>, dim3(grid), dim3(BT), 0, st, sv, cam, qcnt, qidx, qval,
                        slot.d_mass.as<float>(), nullptr, slot.d_spill.as<int>());
     PSDR_HIP_CHECK(hipGetLastError());
@@ -780,6 +1063,7 @@ int boundary_direct_prepare(psdr_scene* s, const psdr_config& cfg, const psdr_ca
   // process the lanes in cell order (edge position major): neighbouring lanes
   // of a wave then trace from neighbouring edge points
   const uint32_t* order = nullptr;
+  const uint32_t* sorted_cells = nullptr;  // the sort keys: each position's cell
   if (!s->plain) {
     uint32_t* k0 = slot.d_keys.as<uint32_t>();
     uint32_t* v0 = slot.d_vals.as<uint32_t>();
@@ -790,11 +1074,12 @@ int boundary_direct_prepare(psdr_scene* s, const psdr_config& cfg, const psdr_ca
     rc = radix_sort_pairs(k0, v0, k0 + n, v0 + n, slot.d_hist.as<uint32_t>(), n, passes, st);
     if (rc) return rc;
     order = (passes & 1) ? v0 + n : v0;
+    sorted_cells = (passes & 1) ? k0 + n : k0;
   }
   hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(1), 0, st, qcnt);
   hipLaunchKernelGGL(k_direct_setup<false>, dim3(blocks(n, BT)), dim3(BT), 0, st, sv, n, state, r0, r1, r2,
                      slot.d_pmf.as<float>(), slot.d_cmf.as<float>(), cfg.spp_secondary_edge, slot.d_rec.as<float>(),
-                     order, qidx, qval, qcnt, compact);
+                     order, sorted_cells, qidx, qval, qcnt, compact);
   hipLaunchKernelGGL(k_direct<false>, dim3(grid), dim3(BT), 0, st, sv, cam, qcnt, qidx, qval, nullptr,
                      slot.d_rec.as<float>(), slot.d_spill.as<int>());
   PSDR_HIP_CHECK(hipGetLastError());

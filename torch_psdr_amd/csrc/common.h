@@ -238,9 +238,21 @@ PSDR_HD bool camera_project(const CameraK& c, f3 pt, int* px, int* py) {
 // DiscreteSampler::sample_reuse (src/sampler.h:98-117): binary search of the
 // smallest i in [0, cnt-1] with cmf[i] >= u in floor(log2(cnt-1))+1 steps.
 // ---------------------------------------------------------------------------
-PSDR_HD int discrete_sample(const float* pmf, const float* cmf, int cnt, float* u, float* pdf) {
+// The reuse of u and the pdf once entry i of cnt is picked (i the result of
+// the search below).
+PSDR_HD void discrete_reuse(const float* pmf, const float* cmf, int cnt, int i, float* u, float* pdf) {
   if (cnt == 1) {
     *pdf = 1.f;
+    return;
+  }
+  const float p = pmf[i];
+  *u = (*u - (i > 0 ? cmf[i - 1] : 0.f)) / p;
+  *pdf = p;
+}
+
+PSDR_HD int discrete_sample(const float* pmf, const float* cmf, int cnt, float* u, float* pdf) {
+  if (cnt == 1) {
+    discrete_reuse(pmf, cmf, cnt, 0, u, pdf);
     return 0;
   }
   int start = 0, end = cnt - 1;
@@ -251,9 +263,7 @@ PSDR_HD int discrete_sample(const float* pmf, const float* cmf, int cnt, float* 
     if (cmf[mid] < *u) start = min(mid + 1, end);
     else end = mid;
   }
-  const float p = pmf[start];
-  *u = (*u - (start > 0 ? cmf[start - 1] : 0.f)) / p;
-  *pdf = p;
+  discrete_reuse(pmf, cmf, cnt, start, u, pdf);
   return start;
 }
 

@@ -22,6 +22,7 @@
 #include <mutex>
 #include <vector>
 
+#include "path.h"
 #include "render.h"
 #include "shading.h"
 #include "traverse.h"
@@ -305,7 +306,7 @@ __global__ __launch_bounds__(256) void k_shade(ShadeParams P, QueueView qi, cons
         int k = 0;
         // next-event estimation (src/path_integrator.cc:103-131)
         for (uint32_t i = 0; i < P.nL; ++i, ++k) {
-          float ru = rng.next();
+          const float ru = rng.next();
           const float rv = rng.next();
           const int ht = rin.tri[k * rs + j];
           const float hu = rin.u[k * rs + j], hv = rin.v[k * rs + j];
@@ -315,26 +316,7 @@ __global__ __launch_bounds__(256) void k_shade(ShadeParams P, QueueView qi, cons
             cache.u[ci] = hu;
             cache.v[ci] = hv;
           }
-          float pdf_choose;
-          const int obj = discrete_sample(sv.obj_pmf, sv.obj_cmf, sv.n_objects, &ru, &pdf_choose);
-          const int l = sv.objs[obj].light;
-          if (l < 0) continue;
-          const LightPoint lp = sample_light_point(sv, l, ru, rv);
-          const float pdf_light = lp.pdf * pdf_choose;
-          const f3 dir = normalize(lp.p - ip.p);
-          const Its itl = reconstruct(sv, ht, hu, hv, ip.p, true);
-          const bool is_hit = itl.valid && itl.is_light && sqnorm(itl.p - lp.p) < REAL_EPS;
-          if (!is_hit) continue;
-          const f3 emit = light_radiance(sv, l, itl);
-          const f3 bsdf = mat_eval(sv, ip, dir);
-          const float tmp = dot(itl.n, -dir) / itl.d2;
-          const float G = dot(ip.n, dir) * tmp;
-          f3 value = (emit * G) * bsdf;
-          if (P.ad_jac) value = value * (itl.area / itl.area);
-          value = value / pdf_light;
-          float weight = 1.0f / static_cast<float>(P.nL);
-          if (P.nB > 0) weight = weight * mis_weight(pdf_light, mat_pdf(ip, dir) * tmp, P.nL, P.nB);
-          L = L + (T * weight) * value;
+          light_sample_add(sv, ip, ru, rv, ht, hu, hv, P.nL, P.nB, P.ad_jac, T, &L);
         }
         // BSDF sampling with MIS (src/path_integrator.cc:134-160)
         for (uint32_t i = 0; i < P.nB; ++i, ++k) {
@@ -349,23 +331,7 @@ __global__ __launch_bounds__(256) void k_shade(ShadeParams P, QueueView qi, cons
             cache.u[ci] = hu;
             cache.v[ci] = hv;
           }
-          const Sampled smp = mat_sample(sv, ip, ru, rv);
-          const Its itl = reconstruct(sv, ht, hu, hv, ip.p, true);
-          if (!(itl.valid && itl.is_light)) continue;
-          const int l = itl.light;
-          const f3 emit = light_radiance(sv, l, itl);
-          const float tmp = dot(itl.n, -smp.wo) / itl.d2;
-          const float G = dot(ip.n, smp.wo) * tmp;
-          const float pdf_bsdf = smp.pdf * tmp;
-          f3 value = (emit * G) * smp.bsdf;
-          if (P.ad_jac) value = value * (itl.area / itl.area);
-          value = value / pdf_bsdf;
-          float weight = 1.0f / static_cast<float>(P.nB);
-          if (P.nL > 0) {
-            const float pdf_light = (sv.light_power[l] / *sv.lights_total) * (1.0f / sv.light_area[l]);
-            weight = weight * mis_weight(pdf_bsdf, pdf_light, P.nB, P.nL);
-          }
-          L = L + (T * weight) * value;
+          bsdf_sample_add(sv, ip, ru, rv, ht, hu, hv, P.nL, P.nB, P.ad_jac, T, &L);
         }
         // extension (src/path_integrator.cc:163-181)
         {
@@ -380,20 +346,9 @@ __global__ __launch_bounds__(256) void k_shade(ShadeParams P, QueueView qi, cons
               cache.u[ci] = hu;
               cache.v[ci] = hv;
             }
-            const Sampled smp = mat_sample(sv, ip, ru, rv);
-            its = reconstruct(sv, ht, hu, hv, ip.p, true);
+            alive = extend_path(sv, ip, ru, rv, ht, hu, hv, P.ad_jac, &its, &T, &O);
             cu = hu;
             cv = hv;
-            alive = smp.valid && its.valid;
-            if (alive) {
-              const float tmp = dot(its.n, -smp.wo) / its.d2;
-              const float G = dot(ip.n, smp.wo) * tmp;
-              const float pdf = smp.pdf * tmp;
-              f3 f = smp.bsdf;
-              if (P.ad_jac) f = f * (its.area / its.area);
-              T = T * ((f * G) / pdf);
-              O = ip.p;
-            }
           }
         }
       }
@@ -438,18 +393,11 @@ __global__ __launch_bounds__(256) void k_shade(ShadeParams P, QueueView qi, cons
       const int rs = qo.cap;
       int k = 0;
       for (uint32_t i = 0; i < P.nL; ++i, ++k) {
-        float ru = rng.next();
+        const float ru = rng.next();
         const float rv = rng.next();
-        float pdf_choose;
-        const int obj = discrete_sample(sv.obj_pmf, sv.obj_cmf, sv.n_objects, &ru, &pdf_choose);
-        const int l = sv.objs[obj].light;
-        f3 dir = splat(__builtin_nanf(""));
-        float tl = F_INF;
-        if (l >= 0) {
-          const LightPoint lp = sample_light_point(sv, l, ru, rv);
-          dir = normalize(lp.p - its.p);
-          tl = norm(lp.p - its.p);
-        }
+        f3 dir;
+        float tl;
+        light_sample_ray(sv, its.p, ru, rv, &dir, &tl);
         put_ray(rout, k * rs + slot, its.p, dir);
         rout.tl[k * rs + slot] = tl;
       }
