@@ -4,9 +4,12 @@
 //   k_prim_setup   1 draw, hypercube cell pick (10k cells), edge point, camera
 //                  projection; sort key = Morton code of the pixel
 //   sort           render pass only: lanes are processed in pixel-Morton order
-//                  (results stay per lane; only the order of the queue changes,
-//                  so neighbouring lanes of a wave trace neighbouring rays)
-//   k_prim_rays    rays at d -/+ 1e-5 * edge normal, queue in processing order
+//                  (each lane keeps its PCG stream and result; only the order
+//                  changes, so neighbouring lanes of a wave trace neighbouring
+//                  rays); every per-sample array from here on is indexed by
+//                  the position in that order
+//   k_prim_rays    the sample of each position (drawn again), rays at
+//                  d -/+ 1e-5 * edge normal, queue in processing order
 //   trace          both rays
 //   k_prim_check   both hits valid and one of them on the edge point
 //   run_depths x2  trace2light from each hit (B = 1, no light visibility),
@@ -116,12 +119,21 @@ __device__ __forceinline__ uint32_t spread_bits16(uint32_t x) {
   return x;
 }
 
+// One primary-edge sample (src/path_integrator.cc:197-214): the setup draw of
+// lane i (PRE: hypercube cell i of a preprocessing round, continuing its
+// state), the edge point (Scene::sample_edge_point, src/scene.cc:189-206),
+// its pixel and camera ray, and the edge normal nn that offsets the side rays.
+struct PrimDraw {
+  uint64_t rng;  // the lane's PCG state after the draw
+  int e;
+  float t, pdf_ep, pdf_cube;
+  bool valid;
+  int px, py;
+  f3 ep, o, d, nn;
+};
 template <bool PRE>
-__global__ void k_prim_setup(SceneView sv, CameraK cam, int n, uint64_t* __restrict__ state, int res,
-                             const float* __restrict__ cpmf, const float* __restrict__ ccmf, PrimRecs rec,
-                             uint64_t* __restrict__ rng0, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+__device__ __forceinline__ PrimDraw prim_draw(const SceneView& sv, const CameraK& cam, int i, const uint64_t* state,
+                                              int res, const float* cpmf, const float* ccmf) {
   Rng rng;
   float s, pdf_cube = 1.f;
   const float unit = 1.0f / static_cast<float>(res);
@@ -135,51 +147,34 @@ __global__ void k_prim_setup(SceneView sv, CameraK cam, int n, uint64_t* __restr
     const int c = discrete_sample(cpmf, ccmf, res, &s, &pdf_cube);
     s = (s + static_cast<float>(c)) * unit;
   }
-  rng0[i] = rng.state;
-  // Scene::sample_edge_point (src/scene.cc:189-206)
+  PrimDraw w;
+  w.rng = rng.state;
   float u = s, pdf_e;
-  const int e = discrete_sample(sv.edge_pmf, sv.edge_cmf, sv.n_edges, &u, &pdf_e);
-  const EdgeRec E = load_edge(sv, e);
-  const f3 ep = E.p0 + u * (E.p1 - E.p0);
-  int px, py;
-  f3 o, d;
-  const bool valid = camera_point_ray(cam, ep, &px, &py, &o, &d);
-  const f3 nn = cross(d, normalize(E.p1 - E.p0));
-  if (keys) {
-    keys[i] = valid ? (spread_bits16(static_cast<uint32_t>(px)) | (spread_bits16(static_cast<uint32_t>(py)) << 1))
-                    : 0xffffffffu;
-    vals[i] = static_cast<uint32_t>(i);
-  }
-  rec.edge[i] = e;
-  rec.t[i] = u;
-  rec.pdf_ep[i] = pdf_e / E.len;
-  rec.pdf_cube[i] = pdf_cube;
-  rec.pix[i] = valid ? py * cam.W + px : -1;
-  rec.n[i] = nn.x;
-  rec.n[rec.cap + i] = nn.y;
-  rec.n[2 * rec.cap + i] = nn.z;
+  w.e = discrete_sample(sv.edge_pmf, sv.edge_cmf, sv.n_edges, &u, &pdf_e);
+  const EdgeRec E = load_edge(sv, w.e);
+  w.ep = E.p0 + u * (E.p1 - E.p0);
+  w.valid = camera_point_ray(cam, w.ep, &w.px, &w.py, &w.o, &w.d);
+  w.nn = cross(w.d, normalize(E.p1 - E.p0));
+  w.t = u;
+  w.pdf_ep = pdf_e / E.len;
+  w.pdf_cube = pdf_cube;
+  return w;
 }
-
-// Position p of the processing order handles lane order[p] (order = null: p):
-// its two rays go to p and n + p and its queue entry to p.
-__global__ void k_prim_rays(SceneView sv, CameraK cam, int n, const uint32_t* __restrict__ order, PrimRecs rec,
-                            const uint64_t* __restrict__ rng0, RaysView rays, QueueView q0,
-                            int* __restrict__ count0) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p == 0) *count0 = n;
-  if (p >= n) return;
-  const int i = order ? static_cast<int>(order[p]) : p;
-  q0.lane[p] = static_cast<uint32_t>(i);
-  q0.rng[p] = rng0[i];
-  // the same expressions as k_prim_setup, so the rays are bit-identical
-  const EdgeRec E = load_edge(sv, rec.edge[i]);
-  const f3 ep = E.p0 + rec.t[i] * (E.p1 - E.p0);
-  int px, py;
-  f3 o, d;
-  (void)camera_point_ray(cam, ep, &px, &py, &o, &d);
-  const f3 nn = f3{rec.n[i], rec.n[rec.cap + i], rec.n[2 * rec.cap + i]};
+__device__ __forceinline__ void put_prim(PrimRecs& rec, int p, const PrimDraw& w, const CameraK& cam) {
+  rec.edge[p] = w.e;
+  rec.t[p] = w.t;
+  rec.pdf_ep[p] = w.pdf_ep;
+  rec.pdf_cube[p] = w.pdf_cube;
+  rec.pix[p] = w.valid ? w.py * cam.W + w.px : -1;
+  rec.n[p] = w.nn.x;
+  rec.n[rec.cap + p] = w.nn.y;
+  rec.n[2 * rec.cap + p] = w.nn.z;
+}
+// the side rays at d -/+ 1e-5 * nn (src/path_integrator.cc:215-218) to
+// positions p and n + p; NaN directions (no traversal) for an invalid sample
+__device__ __forceinline__ void put_side_rays(RaysView& rays, int n, int p, f3 o, f3 d, f3 nn, bool valid) {
   f3 da = d - nn * REAL_EPS, db = d + nn * REAL_EPS;
-  if (rec.pix[i] < 0) da = db = splat(__builtin_nanf(""));
+  if (!valid) da = db = splat(__builtin_nanf(""));
   rays.o[3 * p] = o.x; rays.o[3 * p + 1] = o.y; rays.o[3 * p + 2] = o.z;
   rays.d[3 * p] = da.x; rays.d[3 * p + 1] = da.y; rays.d[3 * p + 2] = da.z;
   const int j = n + p;
@@ -187,14 +182,65 @@ __global__ void k_prim_rays(SceneView sv, CameraK cam, int n, const uint32_t* __
   rays.d[3 * j] = db.x; rays.d[3 * j + 1] = db.y; rays.d[3 * j + 2] = db.z;
 }
 
+// In lane order (preprocessing rounds and the plain launch shape): the sample
+// records and the post-draw PCG states of lanes [0, n).  With keys: only the
+// sort keys (Morton code of the pixel, all ones when the edge point is off
+// screen) and lane indices; the sample is drawn again in k_prim_rays.
+template <bool PRE>
+__global__ void k_prim_setup(SceneView sv, CameraK cam, int n, uint64_t* __restrict__ state, int res,
+                             const float* __restrict__ cpmf, const float* __restrict__ ccmf, PrimRecs rec,
+                             uint64_t* __restrict__ rng0, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const PrimDraw w = prim_draw<PRE>(sv, cam, i, state, res, cpmf, ccmf);
+  if (keys) {
+    keys[i] = w.valid ? (spread_bits16(static_cast<uint32_t>(w.px)) | (spread_bits16(static_cast<uint32_t>(w.py)) << 1))
+                      : 0xffffffffu;
+    vals[i] = static_cast<uint32_t>(i);
+    return;
+  }
+  rng0[i] = w.rng;
+  put_prim(rec, i, w, cam);
+}
+
+// Position p of the processing order: its two rays go to p and n + p, its
+// queue entry (lane field = p) to p.  Without order the records of lane p
+// come from k_prim_setup; with order, position p draws lane order[p] itself
+// and stores its record at p, so that every later per-sample kernel reads and
+// writes position-indexed arrays.
+__global__ void k_prim_rays(SceneView sv, CameraK cam, int n, const uint32_t* __restrict__ order, int res,
+                            const float* __restrict__ cpmf, const float* __restrict__ ccmf, PrimRecs rec,
+                            const uint64_t* __restrict__ rng0, RaysView rays, QueueView q0,
+                            int* __restrict__ count0) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p == 0) *count0 = n;
+  if (p >= n) return;
+  q0.lane[p] = static_cast<uint32_t>(p);
+  if (order) {
+    const PrimDraw w = prim_draw<false>(sv, cam, static_cast<int>(order[p]), nullptr, res, cpmf, ccmf);
+    put_prim(rec, p, w, cam);
+    q0.rng[p] = w.rng;
+    put_side_rays(rays, n, p, w.o, w.d, w.nn, w.valid);
+    return;
+  }
+  q0.rng[p] = rng0[p];
+  // the same expressions as prim_draw, so the rays are bit-identical
+  const EdgeRec E = load_edge(sv, rec.edge[p]);
+  const f3 ep = E.p0 + rec.t[p] * (E.p1 - E.p0);
+  int px, py;
+  f3 o, d;
+  (void)camera_point_ray(cam, ep, &px, &py, &o, &d);
+  const f3 nn = f3{rec.n[p], rec.n[rec.cap + p], rec.n[2 * rec.cap + p]};
+  put_side_rays(rays, n, p, o, d, nn, rec.pix[p] >= 0);
+}
+
 // both hits valid and one of them within 1e-4 of the edge point (src/path_integrator.cc:227-230)
-__global__ void k_prim_check(SceneView sv, CameraK cam, int n, QueueView q0, PrimRecs rec, RaysView rays) {
+__global__ void k_prim_check(SceneView sv, CameraK cam, int n, PrimRecs rec, RaysView rays) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
-  const int i = static_cast<int>(q0.lane[p]);
-  if (rec.pix[i] < 0) return;
-  const EdgeRec E = load_edge(sv, rec.edge[i]);
-  const f3 ep = E.p0 + rec.t[i] * (E.p1 - E.p0);
+  if (rec.pix[p] < 0) return;
+  const EdgeRec E = load_edge(sv, rec.edge[p]);
+  const f3 ep = E.p0 + rec.t[p] * (E.p1 - E.p0);
   const f3 o = f3{rays.o[3 * p], rays.o[3 * p + 1], rays.o[3 * p + 2]};
   const Its a = reconstruct(sv, rays.tri[p], rays.u[p], rays.v[p], o, true);
   const Its b = reconstruct(sv, rays.tri[n + p], rays.u[n + p], rays.v[n + p], o, true);
@@ -202,15 +248,16 @@ __global__ void k_prim_check(SceneView sv, CameraK cam, int n, QueueView q0, Pri
   if (!ok) {
     rays.tri[p] = -1;
     rays.tri[n + p] = -1;
-    rec.pix[i] = -1;
+    rec.pix[p] = -1;
   }
 }
 
+// side B starts where side A's PCG stream ended (outputs are position-indexed)
 __global__ void k_prim_requeue(int n, QueueView q0, const uint64_t* __restrict__ rng_a, int* __restrict__ count0) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p == 0) *count0 = n;
   if (p >= n) return;
-  q0.rng[p] = rng_a[q0.lane[p]];
+  q0.rng[p] = rng_a[p];
 }
 
 template <bool PRE>
@@ -250,11 +297,11 @@ __device__ __forceinline__ void prim_finish_one(const SceneView& sv, const Camer
   scatter_edge(sv, grad_v, E, rec.t[i], gep, gc);
 }
 
-// Position p handles lane order[p] (the processing order of the pass), so a
-// workgroup's samples are neighbours and share edges.
+// Position p of the processing order (records, side radiance and PCG states
+// are position-indexed), so a workgroup's samples are neighbours and share
+// edges.  A preprocessing round runs in lane order: position p is cell p.
 template <bool PRE>
-__global__ __launch_bounds__(BT) void k_prim_finish(SceneView sv, CameraK cam, int n, const uint32_t* __restrict__ order,
-                                                    PrimRecs rec, const float* __restrict__ LA,
+__global__ __launch_bounds__(BT) void k_prim_finish(SceneView sv, CameraK cam, int n, PrimRecs rec, const float* __restrict__ LA,
                                                     const float* __restrict__ LB, const uint64_t* __restrict__ rng_b,
                                                     uint64_t* __restrict__ state, float* __restrict__ mass,
                                                     const float* __restrict__ dLdI, uint32_t spp,
@@ -270,7 +317,7 @@ __global__ __launch_bounds__(BT) void k_prim_finish(SceneView sv, CameraK cam, i
   }
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p < n)
-    prim_finish_one<PRE>(sv, cam, static_cast<int>(order[p]), rec, LA, LB, rng_b, state, mass, dLdI, spp, grad_v,
+    prim_finish_one<PRE>(sv, cam, p, rec, LA, LB, rng_b, state, mass, dLdI, spp, grad_v,
                          &gc, cam_acc ? cacc : nullptr);
   if (!PRE) {
     __syncthreads();
@@ -347,20 +394,14 @@ struct PreCell {
 };
 __device__ __forceinline__ bool pre_cell(const SceneView& sv, const CameraK& cam, int n, int i,
                                          const uint64_t* __restrict__ state, PreCell* c) {
-  Rng rng;
-  rng.state = state[i];
-  rng.inc = lane_inc(static_cast<uint32_t>(i));
-  const float s = (static_cast<float>(i) + rng.next()) * (1.0f / static_cast<float>(n));
-  c->rng0 = rng.state;
-  float u = s, pdf_e;
-  const int e = discrete_sample(sv.edge_pmf, sv.edge_cmf, sv.n_edges, &u, &pdf_e);
-  const EdgeRec E = load_edge(sv, e);
-  c->ep = E.p0 + u * (E.p1 - E.p0);
-  int px, py;
-  const bool valid = camera_point_ray(cam, c->ep, &px, &py, &c->o, &c->d);
-  c->nn = cross(c->d, normalize(E.p1 - E.p0));
-  c->pdf_ep = pdf_e / E.len;
-  return valid;
+  const PrimDraw w = prim_draw<true>(sv, cam, i, state, n, nullptr, nullptr);
+  c->rng0 = w.rng;
+  c->ep = w.ep;
+  c->o = w.o;
+  c->d = w.d;
+  c->nn = w.nn;
+  c->pdf_ep = w.pdf_ep;
+  return w.valid;
 }
 
 // k_prim_check: both side rays hit and one of them lies on the edge point
@@ -902,7 +943,8 @@ static int primary_pass(psdr_scene* s, const psdr_config& cfg, const CameraK& ca
     if (rc) return rc;
     order = (passes & 1) ? v0 + n : v0;
   }
-  hipLaunchKernelGGL(k_prim_rays, dim3(blocks(n, BT)), dim3(BT), 0, st, sv, cam, n, order, rec, rng0, r0, q0, cnt);
+  hipLaunchKernelGGL(k_prim_rays, dim3(blocks(n, BT)), dim3(BT), 0, st, sv, cam, n, order, res, cpmf, ccmf, rec, rng0,
+                     r0, q0, cnt);
   PSDR_HIP_CHECK(hipGetLastError());
   const int tgrid = trace_grid(2ll * n);
   if (slot.p_spill.ensure(std::max(slot.p_spill.bytes, spill_bytes(tgrid)))) return grow_failed();
@@ -914,7 +956,7 @@ static int primary_pass(psdr_scene* s, const psdr_config& cfg, const CameraK& ca
                   tgrid, st, 0, prio);
   if (rc) return rc;
   if (s->plain) PSDR_HIP_CHECK(hipStreamSynchronize(st));
-  hipLaunchKernelGGL(k_prim_check, dim3(blocks(n, BT)), dim3(BT), 0, st, sv, cam, n, q0, rec, r0);
+  hipLaunchKernelGGL(k_prim_check, dim3(blocks(n, BT)), dim3(BT), 0, st, sv, cam, n, rec, r0);
   WaveArgs wa;
   wa.cam = cam;
   wa.D = static_cast<int>(cfg.max_bounce);
@@ -930,6 +972,7 @@ static int primary_pass(psdr_scene* s, const psdr_config& cfg, const CameraK& ca
   wa.plain = s->plain ? 1 : 0;
   wa.prio = prio;
   wa.spill = &slot.p_spill;
+  wa.lane_of = order;  // queue lanes are positions; the PCG streams those of order[p]
   rc = run_depths(s, wa, q0, r0, qs, rs, n, st);
   if (rc) return rc;
   hipLaunchKernelGGL(k_prim_requeue, dim3(blocks(n, BT)), dim3(BT), 0, st, n, q0, slot.b_rngA.as<uint64_t>(),
@@ -940,7 +983,7 @@ static int primary_pass(psdr_scene* s, const psdr_config& cfg, const CameraK& ca
   rc = run_depths(s, wa, q0, offset_rays(r0, n), qs, rs, n, st);
   if (rc) return rc;
   if (pre) {
-    hipLaunchKernelGGL(k_prim_finish<true>, dim3(blocks(n, BT)), dim3(BT), 0, st, sv, cam, n, q0.lane, rec,
+    hipLaunchKernelGGL(k_prim_finish<true>, dim3(blocks(n, BT)), dim3(BT), 0, st, sv, cam, n, rec,
                        slot.b_LA.as<float>(), slot.b_LB.as<float>(), slot.b_rngB.as<uint64_t>(), state, mass, nullptr,
                        cfg.spp_primary_edge, nullptr, nullptr);
     PSDR_HIP_CHECK(hipGetLastError());
@@ -1009,8 +1052,7 @@ int boundary_primary_finish(psdr_scene* s, const psdr_config& cfg, const psdr_ca
   if (n < 0) return PSDR_ERR_ARG;
   if (n == 0) return PSDR_OK;
   const PrimRecs rec = carve_prim(slot.b_rec, n);
-  const QueueView q0 = carve_queue(slot.b_q0, n);
-  hipLaunchKernelGGL(k_prim_finish<false>, dim3(blocks(n, BT)), dim3(BT), 0, st, s->view(), cam, n, q0.lane, rec,
+  hipLaunchKernelGGL(k_prim_finish<false>, dim3(blocks(n, BT)), dim3(BT), 0, st, s->view(), cam, n, rec,
                      slot.b_LA.as<float>(), slot.b_LB.as<float>(), slot.b_rngB.as<uint64_t>(), nullptr, nullptr, dL_dI,
                      cfg.spp_primary_edge, grad_v, cam_acc);
   PSDR_HIP_CHECK(hipGetLastError());

@@ -190,6 +190,9 @@ struct ShadeParams {
   // at the front of their segment; the segment's tl entry of queue slot j holds
   // the index of slot j's ray there, or -1 (no emitting triangle in its way)
   int pack_mis;
+  // queue lane fields are positions and lane_of[position] the PCG stream's
+  // lane (one camera, no forward state kept); null: lane fields are lanes
+  const uint32_t* lane_of;
 };
 
 // plain launch shape: a finished path keeps its queue slot with this bit set
@@ -279,7 +282,7 @@ __global__ __launch_bounds__(256) void k_shade(ShadeParams P, QueueView qi, cons
         cam_ortho = P.jobs->cam[jb].ortho;
       }
       rng.state = qi.rng[j];
-      rng.inc = lane_inc(loc);
+      rng.inc = lane_inc(P.lane_of ? P.lane_of[lane] : loc);
       if (d == 0) {
         const int ht = rin.tri[j];
         const float hu = rin.u[j], hv = rin.v[j];
@@ -750,6 +753,10 @@ int camera_rays(const CameraK& cam, uint32_t spp, float* o, float* d, int* pix, 
 int run_depths(psdr_scene* s, const WaveArgs& a, const QueueView& q0, const RaysView& r0, const QueueView* q_s,
                const RaysView* r_s, int n, hipStream_t st) {
   const SceneView sv = s->view();
+  if (a.lane_of && (a.keep || a.njobs > 1)) {
+    set_error(PSDR_ERR_ARG, "run_depths: position-indexed lanes need one camera and no kept forward state");
+    return PSDR_ERR_ARG;
+  }
   const int Kmax = static_cast<int>(a.nL + a.nB + 1);
   const int64_t ray_cap = static_cast<int64_t>(n) * Kmax;
   const int tgrid = trace_grid(ray_cap);
@@ -785,6 +792,7 @@ int run_depths(psdr_scene* s, const WaveArgs& a, const QueueView& q0, const Rays
     P.keep = a.keep;
     P.plain = a.plain;
     P.pack_mis = pack;
+    P.lane_of = a.lane_of;
     int* mis_count = pack ? a.counters + MIS_COUNT_OFF + d : nullptr;
     if (a.njobs > 1)
       hipLaunchKernelGGL(k_shade<true>, dim3(sgrid), dim3(256), 0, st, P, qin, a.counters + d, rin, qout,

@@ -89,6 +89,10 @@ struct WaveArgs {
   int prio = 0;                    // raise the launches' wave priority (critical path)
   int njobs = 1;                   // cameras in the lane space
   const LaneJobs* jobs = nullptr;  // device table when njobs > 1 (cam / cache unused then)
+  // when set (one camera, keep = 0): queue lane fields are positions of the
+  // caller's processing order, lane_of[position] is the lane whose PCG stream
+  // the path draws from, and lane_L / lane_rng_out are indexed by position
+  const uint32_t* lane_of = nullptr;
 };
 int run_depths(psdr_scene* s, const WaveArgs& a, const QueueView& q0, const RaysView& r0, const QueueView* q_s,
                const RaysView* r_s, int n, hipStream_t st);
