@@ -38,8 +38,9 @@ int main(int argc, char** argv) {
     if (it % 17 == 0) o = target - d * RAY_TMIN;
     const float tbest = (it % 3 == 0) ? INFINITY : std::fabs(U(g)) * 5.f;
     float t, u, v;
-    const bool hit = ray_tri(o, d, p0, e1, e2, &t, &u, &v) && t <= tbest;
-    const bool maybe = tri_maybe_hit(o, d, p0, e1, e2, tbest);
+    const TriTerms q = tri_terms(o, d, p0, e1, e2);
+    const bool hit = tri_exact(q, &t, &u, &v) && t <= tbest;
+    const bool maybe = tri_maybe_hit(q, tbest);
     hits += hit;
     rejected += !maybe;
     if (hit && !maybe) {

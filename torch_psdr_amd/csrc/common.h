@@ -142,25 +142,27 @@ PSDR_HD Rng lane_rng(uint32_t lane) {
 
 // ---------------------------------------------------------------------------
 // Ray / triangle: Moller-Trumbore, fixed operation order; u, v are the weights of
-// p1, p2 (device_programs.cu:45).  Hit iff t >= tmin (1e-3).
+// p1, p2 (device_programs.cu:45).  Hit iff t >= tmin (1e-3).  Split in two so
+// that traversal can reject most triangles from the terms alone, without the
+// division: tri_terms gives det = e1 . (d x e2) and the numerators of u, v, t;
+// tri_exact divides and decides.
 // ---------------------------------------------------------------------------
-PSDR_HD bool ray_tri(f3 o, f3 d, f3 p0, f3 e1, f3 e2, float* t, float* u, float* v) {
+struct TriTerms {
+  float det, du, dv, dt;
+};
+PSDR_HD TriTerms tri_terms(f3 o, f3 d, f3 p0, f3 e1, f3 e2) {
   const f3 pv = cross(d, e2);
-  const float det = dot(e1, pv);
-  if (!(det != 0.f)) return false;
-  const float inv = 1.0f / det;
   const f3 tv = o - p0;
-  const float uu = dot(tv, pv) * inv;
-  if (!(uu >= 0.f && uu <= 1.f)) return false;
   const f3 qv = cross(tv, e1);
-  const float vv = dot(d, qv) * inv;
-  if (!(vv >= 0.f && uu + vv <= 1.f)) return false;
-  const float tt = dot(e2, qv) * inv;
-  if (!(tt >= RAY_TMIN)) return false;
-  *t = tt;
-  *u = uu;
-  *v = vv;
-  return true;
+  return TriTerms{dot(e1, pv), dot(tv, pv), dot(d, qv), dot(e2, qv)};
+}
+PSDR_HD bool tri_exact(const TriTerms& q, float* t, float* u, float* v) {
+  const float inv = 1.0f / q.det;
+  *u = q.du * inv;
+  *v = q.dv * inv;
+  *t = q.dt * inv;
+  // every condition evaluated (no early exit)
+  return (q.det != 0.f) & (*u >= 0.f) & (*u <= 1.f) & (*v >= 0.f) & (*u + *v <= 1.f) & (*t >= RAY_TMIN);
 }
 
 // ---------------------------------------------------------------------------

@@ -64,29 +64,26 @@ PSDR_HD float slab_entry(float4 lo_hi_a, float4 lo_hi_b, float4 lo_hi_c, int whi
   return hit ? tn : F_INF;
 }
 
-// Division-free rejection ahead of ray_tri: with s = sign(det), the exact
-// test's u = un / det, v = vn / det, t = tn / det (each rounded twice) can only
-// pass if s*un, s*vn >= 0 (up to values too small to round away from zero),
-// s*(un + vn) <= |det|, |det| * tmin <= s*tn and s*tn <= |det| * tbest.  The
-// margins (2^-60 relative near zero, 2^-18 elsewhere) exceed every rounding
-// error of both forms (for |det| < 2^126, i.e. coordinates far below 2^63), so a
-// triangle rejected here is rejected by ray_tri too; NaN compares false, leaving
-// such cases to ray_tri.
-PSDR_HD bool tri_maybe_hit(f3 o, f3 d, f3 p0, f3 e1, f3 e2, float tbest) {
-  const f3 pv = cross(d, e2);
-  const float det = dot(e1, pv);
-  const float adet = fabsf(det);
-  const float sg = det < 0.f ? -1.f : 1.f;
-  const f3 tv = o - p0;
-  const float su = dot(tv, pv) * sg;
+// Division-free rejection ahead of tri_exact (from tri_terms: det, and the
+// numerators du, dv, dt of u, v, t): with s = sign(det), the
+// exact test's u = du / det, v = dv / det, t = dt / det (each rounded twice)
+// can only pass if s*du, s*dv >= 0 (up to values too small to round away from
+// zero), s*(du + dv) <= |det|, |det| * tmin <= s*dt and s*dt <= |det| * tbest.
+// The margins (2^-60 relative near zero, 2^-18 elsewhere) exceed every
+// rounding error of both forms (for |det| < 2^126, i.e. coordinates far below
+// 2^63), so a triangle rejected here is rejected by the exact test too; NaN
+// compares false, leaving such cases to the exact test.  All six comparisons
+// are evaluated (no early exit): a wave runs them for its slowest lane anyway,
+// and one branch costs less than six.
+PSDR_HD bool tri_maybe_hit(const TriTerms& q, float tbest) {
+  const float adet = fabsf(q.det);
+  const float sg = q.det < 0.f ? -1.f : 1.f;
+  const float su = q.du * sg, sv = q.dv * sg, st = q.dt * sg;
   const float tiny = adet * 0x1p-60f;
-  if (su < -tiny || su > adet * (1.0f + 0x1p-18f)) return false;
-  const f3 qv = cross(tv, e1);
-  const float sv = dot(d, qv) * sg;
-  if (sv < -tiny || su + sv > adet * (1.0f + 0x1p-18f)) return false;
-  const float st = dot(e2, qv) * sg;
-  if (st < adet * (RAY_TMIN * (1.0f - 0x1p-18f)) || st > adet * (tbest * (1.0f + 0x1p-18f))) return false;
-  return true;
+  const float hi = adet * (1.0f + 0x1p-18f);
+  const bool out = (su < -tiny) | (su > hi) | (sv < -tiny) | (su + sv > hi) |
+                   (st < adet * (RAY_TMIN * (1.0f - 0x1p-18f))) | (st > adet * (tbest * (1.0f + 0x1p-18f)));
+  return !out;
 }
 
 // Traversal statistics of a profiling launch.  NoStats compiles to nothing.
@@ -114,16 +111,19 @@ struct RayStats {
   __device__ __forceinline__ void second(float t) { t2 = fminf(t2, t); }
 };
 
+// One leaf (triangle k): the prefilter, then the exact test for the lanes it
+// keeps.
 template <class S = NoStats>
 __device__ __forceinline__ void leaf_test(const float4* __restrict__ leaf, int k, f3 o, f3 d, HitRec& h,
                                           S& st) {
   const float4 a = leaf[3 * k], b = leaf[3 * k + 1], c = leaf[3 * k + 2];
-  float t, u, v;
   const f3 p0{a.x, a.y, a.z}, e1{b.x, b.y, b.z}, e2{c.x, c.y, c.z};
   st.leaf();
-  if (!tri_maybe_hit(o, d, p0, e1, e2, st.prune(h.t))) return;
+  const TriTerms q = tri_terms(o, d, p0, e1, e2);
+  if (!tri_maybe_hit(q, st.prune(h.t))) return;
   st.exact();
-  if (ray_tri(o, d, p0, e1, e2, &t, &u, &v)) {
+  float t, u, v;
+  if (tri_exact(q, &t, &u, &v)) {
     const int id = __float_as_int(a.w);
     if (t < h.t || (t == h.t && id < h.tri)) {
       st.second(h.t);
@@ -190,14 +190,20 @@ __device__ __forceinline__ HitRec closest_hit(const float4* __restrict__ node, c
     int cl = __float_as_int(e.x), cr = __float_as_int(e.y);
     float tl = slab_entry(a, b, c, 0, oi, inv, st.prune(h.t));
     float tr = slab_entry(a, b, c, 1, oi, inv, st.prune(h.t));
-    if (tl != F_INF && cl < 0) {
-      if ((cl & 0x7fffffff) < n_tris) leaf_test(leaf, cl & 0x7fffffff, o, d, h, st);
-      tl = F_INF;
+    // leaf children the ray enters: a lane's first one (left or right) in one
+    // pass, so lanes entering only a left leaf and lanes entering only a right
+    // one test together; the right one of lanes entering both in a second pass.
+    // The closest hit is the least (t, id) however the leaves are ordered.
+    const bool ll = tl != F_INF && cl < 0, lr = tr != F_INF && cr < 0;
+    if (ll || lr) {
+      const int k0 = (ll ? cl : cr) & 0x7fffffff;
+      if (k0 < n_tris) leaf_test(leaf, k0, o, d, h, st);
     }
-    if (tr != F_INF && cr < 0) {
+    if (ll && lr) {
       if ((cr & 0x7fffffff) < n_tris) leaf_test(leaf, cr & 0x7fffffff, o, d, h, st);
-      tr = F_INF;
     }
+    if (ll) tl = F_INF;
+    if (lr) tr = F_INF;
     if (h.t < tocc && h.tri >= 0) break;  // occluded before the target
     const bool hl = tl != F_INF, hr = tr != F_INF;
     if (hl && hr) {
