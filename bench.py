@@ -381,13 +381,13 @@ def plain_baseline(torch, ops, step, verts, tex, n_steps, rays, grads):
 
 
 # HBM bytes per k_trace launch on the default workload from two rocprofv3 --pmc
-# runs (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch, summed over one step's 18
+# runs (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch, summed over one step's 11
 # traversal launches), each scaled by what the counter reads for a 1 GiB
 # device-to-device copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0), at the launch
 # shapes named here; another shape reports null.  Commands and per-launch table:
 # profiles/k_trace_gradient_step_512px_16spp.md.
 TRACE_TRAFFIC = [
-    {"shape": {"trace_blocks_per_cu": 16, "trace_stack_lds": 16}, "bytes_per_launch": 48756206},
+    {"shape": {"trace_blocks_per_cu": 16, "trace_stack_lds": 16}, "bytes_per_launch": 75304774},
 ]
 
 

@@ -151,6 +151,11 @@ const char* psdr_last_error(void);
 int psdr_abi_version(void);
 
 /* ---- scene ---------------------------------------------------------------- */
+/* Most triangles a scene may hold: triangle records are indexed as 5 * t in
+ * 32-bit arithmetic and LBVH nodes carry child ids in 30 bits.  A larger
+ * n_tris is refused with PSDR_ERR_ARG. */
+#define PSDR_MAX_TRIS 268435456
+
 /* Replaces pypsdr.TriangleMesh / AreaLight / Scene construction
  * (src/psdr.cc:79-106 -> src/mesh.cc:5-95, src/emitter.cc:4-14, src/scene.cc:3-104,
  * src/scene_optix.cc:10-238).  Copies the topology, builds the edge list, derives

@@ -45,6 +45,11 @@ def test_argument_errors_without_gpu():
     assert b"NULL" in lib.psdr_last_error()
     cam = N.PsdrCamera()
     assert lib.psdr_camera_matrices(ctypes.byref(cam), None, None, None, None, None) == -1  # height 0
+    # more triangles than node child ids and 32-bit record offsets allow
+    desc = N.PsdrSceneDesc()
+    desc.n_tris = N.PSDR_MAX_TRIS + 1
+    assert lib.psdr_scene_create(ctypes.byref(desc), ctypes.byref(out), None) == N.PSDR_ERR_ARG
+    assert b"PSDR_MAX_TRIS" in lib.psdr_last_error()
 
 
 def _cfg(**kw):

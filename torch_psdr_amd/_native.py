@@ -37,6 +37,7 @@ PSDR_CAMERA_PERSPECTIVE = 0
 PSDR_CAMERA_ORTHOGRAPHIC = 1
 PSDR_CAMERA_GRAD = 10
 PSDR_NOALLOC_MAX_RAYS = 715827882
+PSDR_MAX_TRIS = 268435456
 PSDR_TRACE_COUNTERS = 8
 PSDR_MAX_CAMERAS = 4
 PSDR_MIN_TILE_LANES = 65536

@@ -264,6 +264,10 @@ static int check_desc(const psdr_scene_desc* d) {
     set_error(PSDR_ERR_ARG, "negative count in scene desc");
     return PSDR_ERR_ARG;
   }
+  if (d->n_tris > PSDR_MAX_TRIS) {
+    set_error(PSDR_ERR_ARG, "scene desc: more than PSDR_MAX_TRIS triangles");
+    return PSDR_ERR_ARG;
+  }
   if (d->n_tris > 0 && (!d->vertices || !d->faces || !d->tri_uv || !d->tri_obj)) {
     set_error(PSDR_ERR_ARG, "scene desc: missing geometry buffer");
     return PSDR_ERR_ARG;
@@ -595,7 +599,7 @@ static int scene_create_impl(const psdr_scene_desc* d, psdr_scene* s, hipStream_
   PSDR_TRY(s->parent.ensure(8 * n1));
   PSDR_TRY(s->box.ensure(48 * n1));
   PSDR_TRY(s->leaf.ensure(48 * n1));
-  PSDR_TRY(s->node.ensure(64 * n1));
+  PSDR_TRY(s->node.ensure(48 * n1));
   PSDR_TRY(s->counters.ensure(4 * n1));
   PSDR_TRY(s->sort_keys.ensure(8 * n1));
   PSDR_TRY(s->sort_vals.ensure(8 * n1));
@@ -710,6 +714,7 @@ int psdr_scene_create(const psdr_scene_desc* desc, psdr_scene** out, void* strea
     return PSDR_ERR_ARG;
   }
   *out = nullptr;
+  PSDR_TRY(check_desc(desc));  // before anything is allocated or the device is touched
   psdr_scene* s = new (std::nothrow) psdr_scene();
   if (s) {
     s->ws = new (std::nothrow) Workspace();

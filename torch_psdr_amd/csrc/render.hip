@@ -63,9 +63,14 @@ __global__ void k_camera_rays(CameraK cam, uint32_t spp, int64_t N, float* o, fl
 // triangles tested (division-free prefilter, then the exact test), closest-hit
 // rays whose hit is a near tie or grazes an edge (traverse.h near_tie /
 // grazing), and hits; the results equal those of the plain instance.
+// Eight waves per SIMD (the gfx950 maximum; the compiler keeps the kernel
+// within the SGPR budget by spilling a few scalars to VGPR lanes outside the
+// node loop): the traversal waits on dependent node fetches, and the extra wave
+// hides more of that latency than the spills cost.
 constexpr int TRACE_COUNTERS = 8;
 template <int STACK_LDS, bool STATS>
-__global__ __launch_bounds__(TRACE_THREADS) void k_trace(SceneView sv, const float* __restrict__ ro,
+__global__ __launch_bounds__(TRACE_THREADS) __attribute__((amdgpu_waves_per_eu(8))) void k_trace(
+    SceneView sv, const float* __restrict__ ro,
                                                          const float* __restrict__ rd, int n,
                                                          const int* __restrict__ count, int K, int seg,
                                                          const float* __restrict__ rtl, int nvis,

@@ -4,8 +4,8 @@
 //   tri_geom  [n_tris*5] float4 : {p0, area} {p1, -} {p2, -} {normal, -} {tangent, -}
 //   tri_uv    [n_tris*6] float  : corner uvs (non-differentiable, src/mesh.cc:29-36)
 //   leaf      [n_tris*3] float4 : LBVH-ordered {p0, tri id} {p1-p0, -} {p2-p0, -}
-//   node      [(n-1)*4]  float4 : {lo_l.xyz, hi_l.x} {hi_l.yz, lo_r.xy} {lo_r.z, hi_r.xyz}
-//                                 {left, right, -, -} (child ids; leaf k -> k | 0x80000000)
+//   node      [(n-1)*3]  float4 : {lo_l.xyz, hi_l.x} {hi_l.yz, lo_r.xy} {lo_r.z, hi_r.xyz}, the
+//                                 child ids in the first four planes' low bytes (traverse.h)
 //   edge_geom [n_edges*16] float: p0 p1 out n_left n_right length (src/mesh.h:28-42)
 #pragma once
 

@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "scene.h"
+#include "traverse.h"
 
 namespace psdr {
 
@@ -570,7 +571,8 @@ __global__ __launch_bounds__(256) void k_node_boxes(const float* __restrict__ lb
     }
 }
 
-// node i = {left box, right box, child ids}; box keeps the same 12 floats
+// node i = the 48-byte traversal node of its two child boxes (pack_node);
+// box keeps the exact 12 planes
 __global__ void k_pack_nodes(const float* __restrict__ lbox, const float* __restrict__ nbox,
                              const int* __restrict__ child, int n, float* __restrict__ box,
                              float4* __restrict__ node) {
@@ -583,10 +585,7 @@ __global__ void k_pack_nodes(const float* __restrict__ lbox, const float* __rest
     for (int a = 0; a < 6; ++a) b[6 * s + a] = src[a];
   }
   for (int a = 0; a < 12; ++a) box[12 * i + a] = b[a];
-  node[4 * i] = make_float4(b[0], b[1], b[2], b[3]);
-  node[4 * i + 1] = make_float4(b[4], b[5], b[6], b[7]);
-  node[4 * i + 2] = make_float4(b[8], b[9], b[10], b[11]);
-  node[4 * i + 3] = make_float4(__int_as_float(child[2 * i]), __int_as_float(child[2 * i + 1]), 0.f, 0.f);
+  pack_node(b, b + 6, child[2 * i], child[2 * i + 1], node + 3 * i);
 }
 
 inline int blocks(long long n, int t) { return static_cast<int>((n + t - 1) / t); }

@@ -1,9 +1,10 @@
 // Closest-hit LBVH traversal for gfx950 (replaces optixTrace with the
 // raygen/closesthit/miss programs of src/cuda/device_programs.cu:13-47).
 //
-// BVH2 with both child boxes stored in the parent (one 64-byte node fetch tests
-// two boxes); leaves are single triangles stored in LBVH order as
-// {p0, id}{e1}{e2}.  The traversal stack lives in LDS (STACK_LDS entries per
+// BVH2 with both child boxes stored in the parent: a 48-byte node (three
+// 16-byte loads) holds the two boxes' 12 planes, and the child ids ride in the
+// low mantissa bytes of four of them (node_children); leaves are single
+// triangles stored in LBVH order as {p0, id}{e1}{e2}.  The traversal stack lives in LDS (STACK_LDS entries per
 // lane, laid out [entry][lane] so a wave's pushes hit distinct banks) and
 // overflows into a per-thread global area.  The tree is a Karras hierarchy over
 // 64-bit keys (31-bit Morton code << 32 | sorted position; the top bit is always
@@ -62,6 +63,61 @@ PSDR_HD float slab_entry(float4 lo_hi_a, float4 lo_hi_b, float4 lo_hi_c, int whi
   const float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
   const bool hit = tn <= tf * 1.000001f && tn <= tbest * 1.000001f;
   return hit ? tn : F_INF;
+}
+
+// Child ids of a node.  A Karras node's children are gamma and gamma + 1 (each
+// an internal node or a leaf, leaf k encoded k | 0x80000000); gamma (30 bits)
+// and the two leaf flags are the low bytes of the node's first four planes
+// (left lo x, y, z, left hi x), which plane_with_byte rounded outward so that
+// the planes stay conservative whatever byte they carry.  The fourth 16-byte
+// load a separate child-id word would cost is the saving: the traversal is
+// bound as much by its vector-memory instructions as by its arithmetic.
+PSDR_HD void node_children(float4 a, int* cl, int* cr) {
+  const uint32_t x = __builtin_bit_cast(uint32_t, a.x), y = __builtin_bit_cast(uint32_t, a.y),
+                 z = __builtin_bit_cast(uint32_t, a.z), w = __builtin_bit_cast(uint32_t, a.w);
+  const int g = static_cast<int>((x & 0xffu) | ((y & 0xffu) << 8) | ((z & 0xffu) << 16) | ((w & 0x3fu) << 24));
+  *cl = g | ((w & 0x40u) ? static_cast<int>(0x80000000u) : 0);
+  *cr = (g + 1) | ((w & 0x80u) ? static_cast<int>(0x80000000u) : 0);
+}
+
+// A box plane p carrying one payload byte in its low mantissa bits, rounded
+// outward first so that any byte keeps the plane conservative: the result is
+// <= p for a low plane (hi = false) and >= p for a high one.  In the bit
+// pattern of |p| (m), moving away from zero rounds m up to a multiple of 256
+// (the byte then only adds); moving toward zero rounds m down to a multiple of
+// 256 at least 255 below it (the byte cannot pass m); a plane within 255 ulps
+// of zero becomes a tiny value of the outward sign.  The planes move by at
+// most 511 ulps (3e-5 relative).  Planes are below 2^127 in magnitude (a
+// triangle box padded by 1e-6 relative; larger coordinates would overflow the
+// traversal's arithmetic long before).
+PSDR_HD float plane_with_byte(float p, uint32_t byte, bool hi) {
+  const uint32_t bits = __builtin_bit_cast(uint32_t, p);
+  const uint32_t m = bits & 0x7fffffffu;
+  const bool neg = (bits >> 31) != 0u && m != 0u;
+  const bool away = hi ? !neg : neg;  // outward is away from zero
+  uint32_t sign, mag;
+  if (away) {
+    sign = neg ? 0x80000000u : 0u;
+    mag = (m + 255u) & ~0xffu;
+  } else if (m >= 255u) {
+    sign = neg ? 0x80000000u : 0u;
+    mag = (m - 255u) & ~0xffu;
+  } else {
+    sign = hi ? 0u : 0x80000000u;
+    mag = 0u;
+  }
+  return __builtin_bit_cast(float, sign | mag | (byte & 0xffu));
+}
+
+// The 48-byte node of child boxes bl, br ({lo xyz, hi xyz}, padded) and child
+// ids cl, cr ((cr & 0x7fffffff) == (cl & 0x7fffffff) + 1, gamma < 2^30).
+PSDR_HD void pack_node(const float* bl, const float* br, int cl, int cr, float4* out) {
+  const uint32_t g = static_cast<uint32_t>(cl) & 0x7fffffffu;
+  const uint32_t top = ((g >> 24) & 0x3fu) | (cl < 0 ? 0x40u : 0u) | (cr < 0 ? 0x80u : 0u);
+  out[0] = make_float4(plane_with_byte(bl[0], g & 0xffu, false), plane_with_byte(bl[1], (g >> 8) & 0xffu, false),
+                       plane_with_byte(bl[2], (g >> 16) & 0xffu, false), plane_with_byte(bl[3], top, true));
+  out[1] = make_float4(bl[4], bl[5], br[0], br[1]);
+  out[2] = make_float4(br[2], br[3], br[4], br[5]);
 }
 
 // Division-free rejection ahead of tri_exact (from tri_terms: det, and the
@@ -185,9 +241,10 @@ __device__ __forceinline__ HitRec closest_hit(const float4* __restrict__ node, c
   const int n_internal = n_tris - 1;
   for (;;) {
     if (static_cast<unsigned>(ni) >= static_cast<unsigned>(n_internal)) break;  // malformed tree guard
-    const float4 a = node[4 * ni], b = node[4 * ni + 1], c = node[4 * ni + 2], e = node[4 * ni + 3];
+    const float4 a = node[3 * ni], b = node[3 * ni + 1], c = node[3 * ni + 2];
     st.node();
-    int cl = __float_as_int(e.x), cr = __float_as_int(e.y);
+    int cl, cr;
+    node_children(a, &cl, &cr);
     float tl = slab_entry(a, b, c, 0, oi, inv, st.prune(h.t));
     float tr = slab_entry(a, b, c, 1, oi, inv, st.prune(h.t));
     // leaf children the ray enters: a lane's first one (left or right) in one
