@@ -61,9 +61,10 @@ def main():
     ap.add_argument("--graph", choices=("value", "report", "off"), default="value",
                     help="the step captured into a HIP graph and replayed: reported as the value (the eager "
                          "step's time beside it), reported beside the eager value, or not run")
-    ap.add_argument("--workload", choices=("c2", "c3"), default="c2",
+    ap.add_argument("--workload", choices=("c2", "c3", "trace"), default="c2",
                     help="c2: the 512^2 / 16 spp vertex + texture step (default); c3: 64 scenes of uneven size, "
-                         "rendered and differentiated in turn on shared buffers")
+                         "rendered and differentiated in turn on shared buffers; trace: the closest-hit "
+                         "traversal alone on the c2 scene (camera, diffuse-bounce, toward-light and random rays)")
     ap.add_argument("--no-grad-error", action="store_true",
                     help="skip the gradient-error leg (directional central differences of the loss)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -92,6 +93,9 @@ def main():
     dev = torch.device("cuda:0")
     if args.workload == "c3":
         bench_c3(args, torch, dr, W, dev)
+        return
+    if args.workload == "trace":
+        bench_trace(args, torch, dr, W, N, ops, dev)
         return
     parts = W.c2_parts(dr, dev, res=args.res, spp=args.spp)
     integ = parts["integrator"]
@@ -275,6 +279,84 @@ def bench_c3(args, torch, dr, W, dev):
         "graph_step": graph}))
 
 
+def bench_trace(args, torch, dr, W, N, ops, dev):
+    """The closest-hit traversal alone (psdr_scene_trace, one launch per ray
+    set) on the 512x512 / 16 spp step's scene: camera rays of 12 spp, diffuse
+    bounce rays from their hits (cosine-weighted about the sphere's or the
+    ground's normal), rays from those hits toward random points of the light,
+    and 3 M random rays through the scene.  Median of --steps launches per set
+    after one warm-up; the checksums (sum of hit ids and of hit distances) let
+    two builds be compared for equal results."""
+    import ctypes
+    parts = W.c2_parts(dr, dev)
+    scene = parts["scene"](parts["Vp"], parts["tex_opt"])
+    ns, flat = ops.entry(scene.token)
+    ns.bind(flat)
+    lib = ns.lib
+    cam = ops.make_camera(scene.cameras[0].params())
+    spp = 12
+    n = 512 * 512 * spp
+    o = torch.empty((n, 3), device=dev)
+    d = torch.empty((n, 3), device=dev)
+    pix = torch.empty((n,), dtype=torch.int32, device=dev)
+    N.check(lib.psdr_camera_rays(ctypes.byref(cam), spp, o.data_ptr(), d.data_ptr(), pix.data_ptr(),
+                                 N.current_stream()))
+    if args.tuning:
+        from torch_psdr_amd import tune
+        tune.set_tuning({k: int(v) for k, v in (kv.split("=") for kv in args.tuning.split(","))})
+    out = {}
+
+    def run(name, ro, rd):
+        m = ro.shape[0]
+        tri = torch.empty((m,), dtype=torch.int32, device=dev)
+        t, u, v = (torch.empty((m,), device=dev) for _ in range(3))
+
+        def f():
+            N.check(lib.psdr_scene_trace(ns.handle, ro.data_ptr(), rd.data_ptr(), m, tri.data_ptr(), t.data_ptr(),
+                                         u.data_ptr(), v.data_ptr(), N.current_stream()))
+        f()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(max(args.steps, 1)):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            f()
+            b.record()
+            torch.cuda.synchronize()
+            ts.append(a.elapsed_time(b))
+        med = sorted(ts)[len(ts) // 2]
+        hit = tri >= 0
+        out[name] = {"rays": m, "median_us": round(med * 1e3, 1), "grays_per_s": round(m / med / 1e6, 3),
+                     "hit_fraction": round(float(hit.float().mean()), 4),
+                     "checksum": [int(tri.long().sum()), float(t[hit].double().sum())]}
+        print(f"trace {name}: {m} rays, {med * 1e3:.1f} us", file=sys.stderr, flush=True)
+        return tri, t
+
+    tri, t = run("camera", o, d)
+    hit = tri >= 0
+    p = o + d * t[:, None]
+    g = torch.Generator(device=dev).manual_seed(5)
+    up = torch.tensor([0.0, 1.0, 0.0], device=dev)
+    nrm = torch.where((p[:, 1] < -0.999)[:, None], up, torch.nn.functional.normalize(p, dim=1))
+    dirs = torch.nn.functional.normalize(nrm + torch.nn.functional.normalize(
+        torch.randn((n, 3), device=dev, generator=g), dim=1), dim=1)
+    sel = hit.nonzero().squeeze(1)
+    ob = (p + nrm * 1e-3)[sel].contiguous()
+    run("bounce", ob, dirs[sel].contiguous())
+    lp = torch.rand((sel.shape[0], 3), device=dev, generator=g) * 2 - 1
+    lp[:, 1] = 4.0
+    run("tolight", ob, torch.nn.functional.normalize(lp - ob, dim=1).contiguous())
+    gc = torch.Generator().manual_seed(1)
+    orr = (torch.rand((3_000_000, 3), generator=gc) * 2 - 1) * torch.tensor([6.0, 3.0, 6.0]) + \
+        torch.tensor([0.0, 2.0, 0.0])
+    tg = (torch.rand((3_000_000, 3), generator=gc) * 2 - 1) * 1.5
+    run("random", orr.to(dev).contiguous(), torch.nn.functional.normalize(tg - orr, dim=1).to(dev).contiguous())
+    print(json.dumps({"metric": "closest-hit traversal Grays/s per ray set (c2 scene)", "unit": "Grays/s",
+                      "n_gpus": 1, "steps": args.steps, "data": "synthetic",
+                      "config": {"workload": "closest-hit traversal, icosphere L5 (20480 tris) + ground + light",
+                                 "tuning": args.tuning}, "sets": out}))
+
+
 def graph_step_timing(torch, step, n_steps, rays):
     """The same step (Scene rebuild with its LBVH, renderD, loss, backward)
     captured once into a HIP graph and replayed n_steps times."""
@@ -388,6 +470,9 @@ def plain_baseline(torch, ops, step, verts, tex, n_steps, rays, grads):
 # profiles/k_trace_gradient_step_512px_16spp.md.
 TRACE_TRAFFIC = [
     {"shape": {"trace_blocks_per_cu": 16, "trace_stack_lds": 16}, "bytes_per_launch": 75304774},
+    {"shape": {"trace_blocks_per_cu": 16, "trace_stack_lds": 8}, "bytes_per_launch": 75868454},
+    {"shape": {"trace_blocks_per_cu": 8, "trace_stack_lds": 16}, "bytes_per_launch": 75896143},
+    {"shape": {"trace_blocks_per_cu": 8, "trace_stack_lds": 8}, "bytes_per_launch": 76040765},
 ]
 
 
