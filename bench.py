@@ -61,10 +61,12 @@ def main():
     ap.add_argument("--graph", choices=("value", "report", "off"), default="value",
                     help="the step captured into a HIP graph and replayed: reported as the value (the eager "
                          "step's time beside it), reported beside the eager value, or not run")
-    ap.add_argument("--workload", choices=("c2", "c3", "trace"), default="c2",
-                    help="c2: the 512^2 / 16 spp vertex + texture step (default); c3: 64 scenes of uneven size, "
-                         "rendered and differentiated in turn on shared buffers; trace: the closest-hit "
-                         "traversal alone on the c2 scene (camera, diffuse-bounce, toward-light and random rays)")
+    ap.add_argument("--workload", choices=("c2", "c0", "c1", "c3", "c4", "trace"), default="c2",
+                    help="c2: the 512^2 / 16 spp vertex + texture step (default); c0: the 256^2 / 4 spp forward "
+                         "render; c1: the 256^2 / 8 spp texture step; c3: 64 scenes of uneven size, rendered and "
+                         "differentiated in turn on shared buffers; c4: the 1024^2 / 64 spp terrain step; trace: "
+                         "the closest-hit traversal alone on the c2 scene (camera, diffuse-bounce, toward-light and "
+                         "random rays)")
     ap.add_argument("--no-grad-error", action="store_true",
                     help="skip the gradient-error leg (directional central differences of the loss)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -96,6 +98,9 @@ def main():
         return
     if args.workload == "trace":
         bench_trace(args, torch, dr, W, N, ops, dev)
+        return
+    if args.workload in ("c0", "c1", "c4"):
+        bench_config(args, torch, dr, W, dev)
         return
     parts = W.c2_parts(dr, dev, res=args.res, spp=args.spp)
     integ = parts["integrator"]
@@ -277,6 +282,74 @@ def bench_c3(args, torch, dr, W, dev):
                    "form": form},
         "forms_ms_per_step": {k: round(v, 3) for k, v in forms.items()},
         "graph_step": graph}))
+
+
+def bench_config(args, torch, dr, W, dev):
+    """The other configurations, each timed as --steps eager steps after
+    --warmup (synchronised around the timed region):
+    c0: icosphere L3 + ground + light, 256^2, 4 spp, depth 2, renderC;
+    c1: the same scene with a 256^2 texture (seed 1) on the sphere, 256^2, 8 spp,
+        depth 2, renderD + L2 loss against the seed-2 texture's image + backward
+        (texel gradients);
+    c4: 1024^2 fBm terrain (2.09 M triangles, two lights, 1024^2 texture),
+        1024^2, 64 spp, depth 4, renderD + L2 loss + backward (height and texel
+        gradients)."""
+    D, L, B = 2, 1, 1
+    spp_edge = 0
+    if args.workload == "c0":
+        scene, integ, _ = W.build_scene(dr, dev, level=3, res=256, spp=4, max_bounce=2)
+        res, spp = 256, 4
+
+        def step():
+            integ.renderC(scene)
+    elif args.workload == "c1":
+        tex = W.texture(1, 256, dev).requires_grad_(True)
+        scene, integ, _ = W.build_scene(dr, dev, level=3, res=256, tex=tex, spp=8, max_bounce=2)
+        with torch.no_grad():
+            tgt_scene, _, _ = W.build_scene(dr, dev, level=3, res=256, tex=W.texture(2, 256, dev), spp=8,
+                                            max_bounce=2)
+            target = integ.renderC(tgt_scene)[0]
+        res, spp, spp_edge = 256, 8, 1
+
+        def step():
+            tex.grad = None
+            img = integ.renderD(scene)[0]
+            ((img - target) ** 2).mean().backward()
+    else:
+        p = W.c4_parts(dr, dev)
+        h = p["heights"].clone().requires_grad_(True)
+        tex = p["tex"].clone().requires_grad_(True)
+        integ = p["integrator"]
+        res, spp, spp_edge, D = 1024, 64, 1, 4
+
+        def step():
+            h.grad = None
+            tex.grad = None
+            img = integ.renderD(p["scene"](h, tex))[0]
+            ((img - 0.5) ** 2).mean().backward()
+    for _ in range(max(args.warmup, 1)):
+        step()
+    torch.cuda.synchronize()
+    steps = max(args.steps, 1)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    R = 1 + D * (L + B + 1)
+    n = res * res * spp
+    n_e = res * res * spp_edge
+    rays = R * n + 23 * n_e
+    names = {"c0": "forward render, icosphere L3 (1280 tris) + ground + light, 256x256, 4 spp, depth 2",
+             "c1": "texture-gradient step, icosphere L3 + 256^2 texture, 256x256, 8 spp, depth 2, edge spp 1",
+             "c4": "height + texture gradient step, 1024^2 fBm terrain (2.09M tris), two lights, 1024x1024, "
+                   "64 spp, depth 4, edge spp 1"}
+    print(json.dumps({"metric": f"{'fwd' if args.workload == 'c0' else 'fwd+bwd'} ms/iter & Mrays/s",
+                      "value": round(rays / (ms * 1e-3) / 1e6, 3), "unit": "Mrays/s", "n_gpus": 1, "steps": steps,
+                      "warmup": max(args.warmup, 1), "ms_per_step": round(ms, 3), "higher_is_better": True,
+                      "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+                      "config": {"workload": names[args.workload], "lanes": n, "boundary_samples_per_term": n_e,
+                                 "rays_per_step": rays, "form": "eager"}}))
 
 
 def bench_trace(args, torch, dr, W, N, ops, dev):
