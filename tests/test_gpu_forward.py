@@ -189,6 +189,12 @@ CASES = [
          cfg=dict(spp_interior=2, max_bounce=3, enable_light_visable=True)),
     dict(name="env-sh9-only", kw=dict(level=2, env="sh9", area_light=False), res=24,
          cfg=dict(spp_interior=2, max_bounce=2)),
+    # the reference examples' integrator: 3 light and 3 BSDF samples per depth, roughness 0.5
+    dict(name="examples-mis3x3", kw=dict(level=2, bsdf_rough=0.5), res=24,
+         cfg=dict(spp_interior=2, max_bounce=3, mis_light_samples=3, mis_bsdf_samples=3)),
+    # deeper paths than the configurations use (the depth-8 and general adjoint instances)
+    dict(name="deep-d6", kw=dict(level=2), res=16, cfg=dict(spp_interior=2, max_bounce=6)),
+    dict(name="deep-d10", kw=dict(level=1), res=16, cfg=dict(spp_interior=2, max_bounce=10)),
 ]
 
 

@@ -84,6 +84,10 @@ INTERIOR_CASES = [
     dict(name="env-map", kw=dict(level=2, env="map"), res=16, cfg=dict(spp_interior=2, max_bounce=2)),
     dict(name="env-sh9", kw=dict(level=2, env="sh9", bsdf_rough=0.5), res=16,
          cfg=dict(spp_interior=2, max_bounce=2, enable_light_visable=True)),
+    dict(name="examples-mis3x3", kw=dict(level=1, bsdf_rough=0.5, tex_res=8), res=16,
+         cfg=dict(spp_interior=2, max_bounce=3, mis_light_samples=3, mis_bsdf_samples=3)),
+    dict(name="deep-d6", kw=dict(level=1, tex_res=8), res=16, cfg=dict(spp_interior=2, max_bounce=6)),
+    dict(name="deep-d10", kw=dict(level=1), res=12, cfg=dict(spp_interior=2, max_bounce=10)),
 ]
 
 
@@ -97,12 +101,17 @@ def test_interior_vjp_matches_oracle_jvp(gpu_device, oracle, case):
         _check(oracle, scene, integ, grads, g, N.PSDR_TERM_INTERIOR, block, 100 + i, 2e-3)
 
 
-@pytest.mark.parametrize("term,env", [(N.PSDR_TERM_PRIMARY, None), (N.PSDR_TERM_DIRECT, None),
-                                      (N.PSDR_TERM_PRIMARY, "sh9"), (N.PSDR_TERM_DIRECT, "map")],
-                         ids=["primary", "direct", "primary-env-sh9", "direct-env-map"])
-def test_boundary_vjp_matches_oracle_jvp(gpu_device, oracle, term, env):
+@pytest.mark.parametrize("term,env,mis", [(N.PSDR_TERM_PRIMARY, None, 1), (N.PSDR_TERM_DIRECT, None, 1),
+                                          (N.PSDR_TERM_PRIMARY, "sh9", 1), (N.PSDR_TERM_DIRECT, "map", 1),
+                                          (N.PSDR_TERM_PRIMARY, None, 3), (N.PSDR_TERM_DIRECT, None, 3)],
+                         ids=["primary", "direct", "primary-env-sh9", "direct-env-map", "primary-mis3",
+                              "direct-mis3"])
+def test_boundary_vjp_matches_oracle_jvp(gpu_device, oracle, term, env, mis):
+    # mis 3: the reference examples' light-sample count (the primary term's
+    # preprocessing then runs one cell per thread instead of four lanes per cell)
     scene, integ = _setup(gpu_device, dict(level=2, env=env), 32,
-                          dict(spp_interior=1, max_bounce=2, spp_primary_edge=2, spp_secondary_edge=2))
+                          dict(spp_interior=1, max_bounce=2, spp_primary_edge=2, spp_secondary_edge=2,
+                               mis_light_samples=mis, mis_bsdf_samples=mis))
     g = torch.randn((32, 32, 3), generator=torch.Generator().manual_seed(11)).to(gpu_device)
     grads = _gpu_vjp(scene, integ, g, term)
     assert np.isfinite(grads[0]).all()
