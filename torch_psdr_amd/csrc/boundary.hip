@@ -779,7 +779,8 @@ __global__ __launch_bounds__(BT) void k_direct_setup(SceneView sv, int n, uint64
     }
     want = !compact || direct_setup(sv, s[0], s[1], s[2]).valid;
   }
-  const int q = wave_alloc(want, qcount);
+  __shared__ int alloc_lds[BT / 64 + 1];
+  const int q = block_alloc(want, qcount, alloc_lds);
   if (want) {
     qidx[q] = p;
     qval[q] = make_float4(s[0], s[1], s[2], gscale);

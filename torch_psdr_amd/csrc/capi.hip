@@ -404,6 +404,7 @@ SceneView psdr_scene::view() const {
   v.lights_total = lights_total.as<float>();
   v.light_tri_pmf = light_tri_pmf.as<float>();
   v.light_tri_cmf = light_tri_cmf.as<float>();
+  v.n_light_tris = light_tri_total;
   v.obj_pmf = obj_pmf.as<float>();
   v.obj_cmf = obj_cmf.as<float>();
   v.texels = texels;

@@ -247,13 +247,19 @@ __global__ void k_cam(CameraK cam, int njobs, const LaneJobs* __restrict__ jobs,
 
 // MULTI: the lanes belong to several cameras (camera and forward-state
 // buffers per lane from P.jobs); otherwise to P.cam and cache0.
-template <bool MULTI>
+// LDS: the scene's small tables are staged in LDS (lds_tables_fit).
+template <bool MULTI, bool LDS>
 __global__ __launch_bounds__(256) void k_shade(ShadeParams P, QueueView qi, const int* __restrict__ count_in,
                                                RaysView rin, QueueView qo, int* __restrict__ count_out,
                                                RaysView rout, CacheView cache0, float* __restrict__ lane_L,
                                                uint64_t* __restrict__ lane_rng_out, int* __restrict__ mis_count_out) {
   if (P.prio) __builtin_amdgcn_s_setprio(3);
-  const SceneView& sv = P.sv;
+  __shared__ int alloc_lds[8];
+  SceneView sv = P.sv;
+  if constexpr (LDS) {
+    __shared__ LdsTables tabs;
+    stage_tables(tabs, &sv);
+  }
   const int cnt = *count_in;
   const int d = P.depth;
   for (int base = blockIdx.x * blockDim.x; base < cnt; base += gridDim.x * blockDim.x) {
@@ -384,7 +390,7 @@ __global__ __launch_bounds__(256) void k_shade(ShadeParams P, QueueView qi, cons
         for (int k = 0; k < P.K_out; ++k) put_ray(rout, k * qo.cap + j, nan3, nan3);
       }
     } else {
-      slot = wave_alloc(alive, count_out);
+      slot = block_alloc(alive, count_out, alloc_lds);
     }
     bool mis_want = false;  // packed BSDF-sampled ray to write
     f3 mis_d = splat(0.f);
@@ -426,7 +432,7 @@ __global__ __launch_bounds__(256) void k_shade(ShadeParams P, QueueView qi, cons
     }
     if (P.pack_mis) {
       const int seg = static_cast<int>(P.nL) * qo.cap;
-      const int m = wave_alloc(mis_want, mis_count_out);
+      const int m = block_alloc(mis_want, mis_count_out, alloc_lds);
       if (mis_want) put_ray(rout, seg + m, its.p, mis_d);
       if (have && alive) rout.tl[seg + slot] = __int_as_float(mis_want ? m : -1);
     }
@@ -799,12 +805,11 @@ int run_depths(psdr_scene* s, const WaveArgs& a, const QueueView& q0, const Rays
     P.pack_mis = pack;
     P.lane_of = a.lane_of;
     int* mis_count = pack ? a.counters + MIS_COUNT_OFF + d : nullptr;
-    if (a.njobs > 1)
-      hipLaunchKernelGGL(k_shade<true>, dim3(sgrid), dim3(256), 0, st, P, qin, a.counters + d, rin, qout,
-                         a.counters + d + 1, rout, a.cache, a.lane_L, a.lane_rng_out, mis_count);
-    else
-      hipLaunchKernelGGL(k_shade<false>, dim3(sgrid), dim3(256), 0, st, P, qin, a.counters + d, rin, qout,
-                         a.counters + d + 1, rout, a.cache, a.lane_L, a.lane_rng_out, mis_count);
+    const bool lds = lds_tables_fit(sv);
+    auto kern = a.njobs > 1 ? (lds ? k_shade<true, true> : k_shade<true, false>)
+                            : (lds ? k_shade<false, true> : k_shade<false, false>);
+    hipLaunchKernelGGL(kern, dim3(sgrid), dim3(256), 0, st, P, qin, a.counters + d, rin, qout, a.counters + d + 1,
+                       rout, a.cache, a.lane_L, a.lane_rng_out, mis_count);
     PSDR_HIP_CHECK(hipGetLastError());
     if (d < a.D && P.K_out > 0) {
       if (a.plain) {

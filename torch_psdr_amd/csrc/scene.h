@@ -117,8 +117,9 @@ struct SceneView {
   const float* light_power;  // [n_lights]
   const float* light_area;   // [n_lights]
   const float* lights_total; // [1]
-  const float* light_tri_pmf;
+  const float* light_tri_pmf;  // [n_light_tris]
   const float* light_tri_cmf;
+  int n_light_tris;
   const float* obj_pmf;  // light pick over objects
   const float* obj_cmf;
   const float* texels;

@@ -18,6 +18,62 @@ struct Its {
 
 __device__ __forceinline__ f3 xyz4(float4 a) { return f3{a.x, a.y, a.z}; }
 
+// The scene's small tables (objects, lights, the light-pick and light-triangle
+// samplers, the emitting triangles' leaves) staged in a workgroup's LDS.  A
+// shading step looks them up several times in a chain of dependent reads (the
+// hit's object, then the picked light, its triangle sampler, the sampled
+// triangle); from LDS each read costs a fraction of a cached global load's
+// latency.  Values are copied bit for bit, so results do not change.
+constexpr int LDS_OBJS = 16, LDS_LIGHTS = 8, LDS_LIGHT_TRIS = 64;
+struct LdsTables {
+  ObjectK objs[LDS_OBJS];
+  LightK lights[LDS_LIGHTS];
+  float obj_pmf[LDS_OBJS], obj_cmf[LDS_OBJS];
+  float light_emit[3 * LDS_LIGHTS], light_power[LDS_LIGHTS], light_area[LDS_LIGHTS], lights_total[4];
+  float ltri_pmf[LDS_LIGHT_TRIS], ltri_cmf[LDS_LIGHT_TRIS];
+  float4 seek[3 * SEEK_MAX];
+};
+// whether a scene's tables fit LdsTables (host side: picks the kernel instance)
+inline bool lds_tables_fit(const SceneView& v) {
+  return v.n_objects <= LDS_OBJS && v.n_lights <= LDS_LIGHTS && v.n_light_tris <= LDS_LIGHT_TRIS &&
+         v.n_seek <= SEEK_MAX;
+}
+template <class T>
+__device__ __forceinline__ void lds_copy_words(T* dst, const T* src, int n) {
+  static_assert(sizeof(T) % 4 == 0, "word copy");
+  int* d = reinterpret_cast<int*>(dst);
+  const int* s = reinterpret_cast<const int*>(src);
+  const int words = n * static_cast<int>(sizeof(T) / 4);
+  for (int i = threadIdx.x; i < words; i += blockDim.x) d[i] = s[i];
+}
+// Every thread of the block calls this (it ends with a barrier); v then reads
+// the staged tables.  The caller checked lds_tables_fit.
+__device__ __forceinline__ void stage_tables(LdsTables& t, SceneView* v) {
+  lds_copy_words(t.objs, v->objs, v->n_objects);
+  lds_copy_words(t.lights, v->lights, v->n_lights);
+  lds_copy_words(t.obj_pmf, v->obj_pmf, v->n_objects);
+  lds_copy_words(t.obj_cmf, v->obj_cmf, v->n_objects);
+  lds_copy_words(t.light_emit, v->light_emit, 3 * v->n_lights);
+  lds_copy_words(t.light_power, v->light_power, v->n_lights);
+  lds_copy_words(t.light_area, v->light_area, v->n_lights);
+  lds_copy_words(t.lights_total, v->lights_total, 1);
+  lds_copy_words(t.ltri_pmf, v->light_tri_pmf, v->n_light_tris);
+  lds_copy_words(t.ltri_cmf, v->light_tri_cmf, v->n_light_tris);
+  if (v->n_seek > 0) lds_copy_words(t.seek, v->seek_leaf, 3 * v->n_seek);
+  __syncthreads();
+  v->objs = t.objs;
+  v->lights = t.lights;
+  v->obj_pmf = t.obj_pmf;
+  v->obj_cmf = t.obj_cmf;
+  v->light_emit = t.light_emit;
+  v->light_power = t.light_power;
+  v->light_area = t.light_area;
+  v->lights_total = t.lights_total;
+  v->light_tri_pmf = t.ltri_pmf;
+  v->light_tri_cmf = t.ltri_cmf;
+  v->seek_leaf = t.seek;
+}
+
 // Scene::hit reconstruction (src/scene.cc:106-161) from (tri, u, v) and the ray
 // origin O.
 __device__ __forceinline__ Its reconstruct(const SceneView& sv, int tri, float u, float v, f3 O, bool valid_in) {
@@ -246,6 +302,33 @@ __device__ __forceinline__ int wave_alloc(bool want, int* counter) {
   }
   const uint64_t lt = lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
   return base + __popcll(m & lt);
+}
+
+// Workgroup-aggregated slot allocation: one atomic per workgroup instead of
+// one per wave.  Atomic adds to one address are serialised by the memory
+// system (about 11 ns each on MI355X, measured), so a queue filled by tens of
+// thousands of waves spends most of its time there.  Every thread of the
+// block must call it (it holds three barriers); lds: (blockDim / 64 + 1) ints.
+// Slots are dense: wave w's lanes follow those of waves 0 .. w-1.
+__device__ __forceinline__ int block_alloc(bool want, int* counter, int* lds) {
+  const uint64_t m = __ballot(want);
+  const int lane = __lane_id(), w = static_cast<int>(threadIdx.x) >> 6, nw = static_cast<int>(blockDim.x) >> 6;
+  if (lane == 0) lds[w] = __popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int s = 0;
+    for (int k = 0; k < nw; ++k) {
+      const int c = lds[k];
+      lds[k] = s;
+      s += c;
+    }
+    lds[nw] = s != 0 ? atomicAdd(counter, s) : 0;
+  }
+  __syncthreads();
+  const uint64_t lt = lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
+  const int slot = lds[nw] + lds[w] + __popcll(m & lt);
+  __syncthreads();  // lds is reused by the next call
+  return slot;
 }
 
 }  // namespace psdr
