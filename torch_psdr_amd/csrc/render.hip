@@ -68,6 +68,11 @@ __global__ void k_camera_rays(CameraK cam, uint32_t spp, int64_t N, float* o, fl
 // node loop): the traversal waits on dependent node fetches, and the extra wave
 // hides more of that latency than the spills cost.
 constexpr int TRACE_COUNTERS = 8;
+// A timed launch's workgroups stamp their start and end into SPAN_WAYS
+// (min, max) pairs picked by workgroup index, reduced on the host: atomics on
+// one address are serialised by the memory system (about 11 ns each), and one
+// pair for thousands of workgroups would lengthen the launch it measures.
+constexpr int SPAN_WAYS = 64;
 template <int STACK_LDS, bool STATS>
 __global__ __launch_bounds__(TRACE_THREADS) __attribute__((amdgpu_waves_per_eu(8))) void k_trace(
     SceneView sv, const float* __restrict__ ro,
@@ -90,10 +95,12 @@ __global__ __launch_bounds__(TRACE_THREADS) __attribute__((amdgpu_waves_per_eu(8
     if (threadIdx.x < TRACE_COUNTERS) cnt_lds[threadIdx.x] = 0;
     __syncthreads();
   }
-  // timed launch: span[0] = earliest workgroup start, span[1] = latest
-  // workgroup end on the device's constant-rate clock, so span[1] - span[0] is
+  // timed launch: min over span[2w] = earliest workgroup start, max over
+  // span[2w + 1] = latest workgroup end (w < SPAN_WAYS) on the device's
+  // constant-rate clock, so the difference is
   // the launch's execution time without the time it waited for CUs
-  if (span && threadIdx.x == 0) atomicMin(&span[0], static_cast<unsigned long long>(wall_clock64()));
+  unsigned long long* way = span ? span + 2 * (blockIdx.x % SPAN_WAYS) : nullptr;
+  if (way && threadIdx.x == 0) atomicMin(&way[0], static_cast<unsigned long long>(wall_clock64()));
   // rays are K segments of `per` rays each, segment k starting at k * seg
   // (one segment per ray kind, so a wave traces rays of one kind); with
   // seek_count, the one light-seeking segment holds only its first *seek_count
@@ -160,7 +167,7 @@ __global__ __launch_bounds__(TRACE_THREADS) __attribute__((amdgpu_waves_per_eu(8
   }
   if (span) {
     __syncthreads();
-    if (threadIdx.x == 0) atomicMax(&span[1], static_cast<unsigned long long>(wall_clock64()));
+    if (threadIdx.x == 0) atomicMax(&way[1], static_cast<unsigned long long>(wall_clock64()));
   }
   if (STATS) {
     __syncthreads();
@@ -605,7 +612,7 @@ struct TraceTimer {
   size_t used = 0;             // pairs recorded since the last start
   int64_t dropped = 0;         // launches beyond the event pool
   unsigned long long* bytes = nullptr;
-  unsigned long long* span = nullptr;  // [pair][start, end] device clock of each launch
+  unsigned long long* span = nullptr;  // [pair][way][start, end] device clock of each launch
   double clock_khz = 0.0;
   bool count_on = false;                   // launches use the counting instance
   unsigned long long* counters = nullptr;  // TRACE_COUNTERS totals
@@ -626,7 +633,7 @@ int trace_rays(const SceneView& sv, const float* ro, const float* rd, int n, con
   if (tt.on && !timed) ++tt.dropped;
   if (timed) PSDR_HIP_CHECK(hipEventRecord(tt.ev[2 * tt.used], st));
   unsigned long long* tb = timed ? tt.bytes : nullptr;
-  unsigned long long* sp = timed ? tt.span + 2 * tt.used : nullptr;
+  unsigned long long* sp = timed ? tt.span + 2 * SPAN_WAYS * tt.used : nullptr;
   unsigned long long* cn = tt.count_on ? tt.counters : nullptr;
 #define PSDR_LAUNCH_TRACE(S, STATS)                                                                                  \
   hipLaunchKernelGGL((k_trace<S, STATS>), dim3(grid), dim3(TRACE_THREADS), 0, st, sv, ro, rd, n, count_dev, K, seg, tl, \
@@ -670,12 +677,12 @@ int trace_timing(int enable, double* total_ms, int64_t* launches, int64_t* bytes
       tt.ev.resize(2 * TIMER_PAIRS);
       for (auto& e : tt.ev) PSDR_HIP_CHECK(hipEventCreate(&e));
       PSDR_HIP_CHECK(hipMalloc(&tt.bytes, sizeof(unsigned long long)));
-      PSDR_HIP_CHECK(hipMalloc(&tt.span, 2 * TIMER_PAIRS * sizeof(unsigned long long)));
+      PSDR_HIP_CHECK(hipMalloc(&tt.span, 2 * SPAN_WAYS * TIMER_PAIRS * sizeof(unsigned long long)));
     }
     PSDR_HIP_CHECK(hipDeviceSynchronize());
     PSDR_HIP_CHECK(hipMemset(tt.bytes, 0, sizeof(unsigned long long)));
-    hipLaunchKernelGGL(k_span_init, dim3(blocks(TIMER_PAIRS, 256)), dim3(256), 0, nullptr, tt.span,
-                       static_cast<int>(TIMER_PAIRS));
+    hipLaunchKernelGGL(k_span_init, dim3(blocks(TIMER_PAIRS * SPAN_WAYS, 256)), dim3(256), 0, nullptr, tt.span,
+                       static_cast<int>(TIMER_PAIRS * SPAN_WAYS));
     PSDR_HIP_CHECK(hipGetLastError());
     PSDR_HIP_CHECK(hipDeviceSynchronize());
     tt.used = 0;
@@ -697,12 +704,18 @@ int trace_timing(int enable, double* total_ms, int64_t* launches, int64_t* bytes
   }
   unsigned long long b = 0;
   PSDR_HIP_CHECK(hipMemcpy(&b, tt.bytes, sizeof(b), hipMemcpyDeviceToHost));
-  std::vector<unsigned long long> span(2 * tt.used);
+  std::vector<unsigned long long> span(2 * SPAN_WAYS * tt.used);
   if (tt.used) PSDR_HIP_CHECK(hipMemcpy(span.data(), tt.span, span.size() * sizeof(unsigned long long),
                                         hipMemcpyDeviceToHost));
   double ticks = 0.0;
-  for (size_t i = 0; i < tt.used; ++i)
-    if (span[2 * i + 1] >= span[2 * i]) ticks += static_cast<double>(span[2 * i + 1] - span[2 * i]);
+  for (size_t i = 0; i < tt.used; ++i) {
+    unsigned long long t0 = ~0ull, t1 = 0ull;  // ways no workgroup used keep (max, 0)
+    for (int w = 0; w < SPAN_WAYS; ++w) {
+      t0 = std::min(t0, span[2 * (SPAN_WAYS * i + w)]);
+      t1 = std::max(t1, span[2 * (SPAN_WAYS * i + w) + 1]);
+    }
+    if (t1 >= t0) ticks += static_cast<double>(t1 - t0);
+  }
   if (tt.dropped) {
     set_error(PSDR_ERR_STATE, "trace_timing: more launches than the event pool holds");
     return PSDR_ERR_STATE;
