@@ -1,5 +1,5 @@
 // Host-only check (built with hipcc, runs on the CPU) of the 48-byte LBVH node
-// (traverse.h pack_node / node_children / plane_with_byte):
+// (traverse.h pack_node / node_children / node_kids / plane_with_byte / slab_hit):
 //  * plane_with_byte never moves a plane inward, whatever byte it carries: a low
 //    plane comes out <= p, a high one >= p (normal, tiny, denormal, signed-zero
 //    planes of magnitude below 2^127), and never becomes inf or NaN;
@@ -72,6 +72,8 @@ int main(int argc, char** argv) {
     int dl, dr;
     node_children(nd[0], &dl, &dr);
     if (dl != cl || dr != cr) ++bad_ids;
+    const NodeKids kk = node_kids(nd[0]);
+    if (kk.g != (cl & 0x7fffffff) || kk.left_leaf != (cl < 0) || kk.right_leaf != (cr < 0)) ++bad_ids;
     const float tail[8] = {nd[1].x, nd[1].y, nd[1].z, nd[1].w, nd[2].x, nd[2].y, nd[2].z, nd[2].w};
     const float want[8] = {bl[4], bl[5], br[0], br[1], br[2], br[3], br[4], br[5]};
     if (std::memcmp(tail, want, sizeof tail) != 0) ++bad_copy;
@@ -86,11 +88,12 @@ int main(int argc, char** argv) {
     const float4 eb = make_float4(bl[4], bl[5], br[0], br[1]);
     const float4 ec = make_float4(br[2], br[3], br[4], br[5]);
     for (int w = 0; w < 2; ++w) {
-      const float te = slab_entry(ea, eb, ec, w, oi, inv, tbest);
-      const float tp = slab_entry(nd[0], nd[1], nd[2], w, oi, inv, tbest);
-      if (te != INFINITY) {
+      float tn_e, tn_p;
+      const bool te = slab_hit(ea, eb, ec, w, oi, inv, tbest * 1.000001f, &tn_e);
+      const bool tp = slab_hit(nd[0], nd[1], nd[2], w, oi, inv, tbest * 1.000001f, &tn_p);
+      if (te) {
         ++entered;
-        if (tp == INFINITY) ++bad_slab;
+        if (!tp) ++bad_slab;
       }
     }
   }

@@ -1,5 +1,5 @@
 // Host-only check (built with hipcc, runs on the CPU): the traversal's slab
-// test (slab_inverse + slab_entry in traverse.h) never rejects a node box whose
+// test (slab_inverse + slab_hit in traverse.h) never rejects a node box whose
 // content the ray meets at some t in [0, tbest].  Node boxes are the union of
 // triangle boxes padded by (|lo| + |hi|) * 1e-6 + 1e-30 per axis (scene.hip), so
 // the content lies in the box shrunk by that pad; the reference is an exact
@@ -70,7 +70,8 @@ int main(int argc, char** argv) {
     const float4 a4 = make_float4(lo[0], lo[1], lo[2], hi[0]);
     const float4 b4 = make_float4(hi[1], hi[2], 0.f, 0.f);
     const float4 c4 = make_float4(0.f, 0.f, 0.f, 0.f);
-    const float te = slab_entry(a4, b4, c4, 0, oi, inv, tbest);
+    float tn_e;
+    const bool te = slab_hit(a4, b4, c4, 0, oi, inv, tbest * 1.000001f, &tn_e);
     double lod[3], hid[3], od[3], dd[3];
     for (int a = 0; a < 3; ++a) {
       lod[a] = clo[a];
@@ -81,13 +82,13 @@ int main(int argc, char** argv) {
     const bool ex = exact_hit(lod, hid, od, dd, tbest);
     if (ex) {
       ++met;
-      if (!(te < INFINITY)) {
+      if (!te) {
         ++bad;
         if (bad <= 5)
           std::printf("rejected: o %g %g %g d %g %g %g lo %g %g %g hi %g %g %g tbest %g\n", o[0], o[1], o[2], d[0],
                       d[1], d[2], lo[0], lo[1], lo[2], hi[0], hi[1], hi[2], tbest);
       }
-    } else if (!(te < INFINITY)) {
+    } else if (!te) {
       ++pruned;
     }
   }

@@ -45,7 +45,13 @@ PSDR_HD f3 slab_inverse(f3 d) {
   return f3{1.0f / x, 1.0f / y, 1.0f / z};
 }
 
-PSDR_HD float slab_entry(float4 lo_hi_a, float4 lo_hi_b, float4 lo_hi_c, int which, f3 oi, f3 inv, float tbest) {
+// Slab test of child box `which` (0: planes a.xyz, a.w b.xy; 1: b.zw c.x,
+// c.yzw): whether the ray enters it at some t in [0, tbest], with the entry
+// distance in *tn_out.  tbw = tbest * 1.000001 (the caller keeps it, since
+// tbest changes only with the leaf tests); the far side is widened by the same
+// factor.
+PSDR_HD bool slab_hit(float4 lo_hi_a, float4 lo_hi_b, float4 lo_hi_c, int which, f3 oi, f3 inv, float tbw,
+                      float* tn_out) {
   float lx, ly, lz, hx, hy, hz;
   if (which == 0) {
     lx = lo_hi_a.x; ly = lo_hi_a.y; lz = lo_hi_a.z;
@@ -61,8 +67,8 @@ PSDR_HD float slab_entry(float4 lo_hi_a, float4 lo_hi_b, float4 lo_hi_c, int whi
   const float t0z = __builtin_fmaf(lz, inv.z, oi.z), t1z = __builtin_fmaf(hz, inv.z, oi.z);
   const float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), 0.f));
   const float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
-  const bool hit = tn <= tf * 1.000001f && tn <= tbest * 1.000001f;
-  return hit ? tn : F_INF;
+  *tn_out = tn;
+  return tn <= tf * 1.000001f && tn <= tbw;
 }
 
 // Child ids of a node.  A Karras node's children are gamma and gamma + 1 (each
@@ -78,6 +84,31 @@ PSDR_HD void node_children(float4 a, int* cl, int* cr) {
   const int g = static_cast<int>((x & 0xffu) | ((y & 0xffu) << 8) | ((z & 0xffu) << 16) | ((w & 0x3fu) << 24));
   *cl = g | ((w & 0x40u) ? static_cast<int>(0x80000000u) : 0);
   *cr = (g + 1) | ((w & 0x80u) ? static_cast<int>(0x80000000u) : 0);
+}
+
+// node_children's fields as the traversal uses them: gamma and the two leaf
+// flags (children gamma and gamma + 1).  On the device two byte permutes
+// gather the four payload bytes, lo = (y.b0 << 8) | x.b0 and
+// hi = (w.b0 << 24) | (z.b0 << 16), whose top two bits are the flags.
+struct NodeKids {
+  int g;
+  bool left_leaf, right_leaf;
+};
+PSDR_HD NodeKids node_kids(float4 a) {
+  const uint32_t x = __builtin_bit_cast(uint32_t, a.x), y = __builtin_bit_cast(uint32_t, a.y),
+                 z = __builtin_bit_cast(uint32_t, a.z), w = __builtin_bit_cast(uint32_t, a.w);
+#if defined(__HIP_DEVICE_COMPILE__)
+  const uint32_t lo = __builtin_amdgcn_perm(y, x, 0x0c0c0400u);
+  const uint32_t hi = __builtin_amdgcn_perm(w, z, 0x04000c0cu);
+#else
+  const uint32_t lo = (x & 0xffu) | ((y & 0xffu) << 8);
+  const uint32_t hi = ((z & 0xffu) << 16) | ((w & 0xffu) << 24);
+#endif
+  NodeKids k;
+  k.g = static_cast<int>((hi & 0x3fffffffu) | lo);
+  k.left_leaf = (hi & 0x40000000u) != 0u;
+  k.right_leaf = static_cast<int>(hi) < 0;
+  return k;
 }
 
 // A box plane p carrying one payload byte in its low mantissa bits, rounded
@@ -239,30 +270,31 @@ __device__ __forceinline__ HitRec closest_hit(const float4* __restrict__ node, c
   int sp = 0;
   int ni = 0;
   const int n_internal = n_tris - 1;
+  float tbw = st.prune(h.t) * 1.000001f;  // changes only with the leaf tests
   for (;;) {
     if (static_cast<unsigned>(ni) >= static_cast<unsigned>(n_internal)) break;  // malformed tree guard
     const float4 a = node[3 * ni], b = node[3 * ni + 1], c = node[3 * ni + 2];
     st.node();
-    int cl, cr;
-    node_children(a, &cl, &cr);
-    float tl = slab_entry(a, b, c, 0, oi, inv, st.prune(h.t));
-    float tr = slab_entry(a, b, c, 1, oi, inv, st.prune(h.t));
+    const NodeKids kid = node_kids(a);
+    const int cl = kid.g, cr = kid.g + 1;
+    float tl, tr;
+    const bool el = slab_hit(a, b, c, 0, oi, inv, tbw, &tl);
+    const bool er = slab_hit(a, b, c, 1, oi, inv, tbw, &tr);
     // leaf children the ray enters: a lane's first one (left or right) in one
     // pass, so lanes entering only a left leaf and lanes entering only a right
     // one test together; the right one of lanes entering both in a second pass.
     // The closest hit is the least (t, id) however the leaves are ordered.
-    const bool ll = tl != F_INF && cl < 0, lr = tr != F_INF && cr < 0;
+    const bool ll = el && kid.left_leaf, lr = er && kid.right_leaf;
     if (ll || lr) {
-      const int k0 = (ll ? cl : cr) & 0x7fffffff;
+      const int k0 = ll ? cl : cr;
       if (k0 < n_tris) leaf_test(leaf, k0, o, d, h, st);
+      if (ll && lr) {
+        if (cr < n_tris) leaf_test(leaf, cr, o, d, h, st);
+      }
+      tbw = st.prune(h.t) * 1.000001f;
     }
-    if (ll && lr) {
-      if ((cr & 0x7fffffff) < n_tris) leaf_test(leaf, cr & 0x7fffffff, o, d, h, st);
-    }
-    if (ll) tl = F_INF;
-    if (lr) tr = F_INF;
     if (h.t < tocc && h.tri >= 0) break;  // occluded before the target
-    const bool hl = tl != F_INF, hr = tr != F_INF;
+    const bool hl = el && !kid.left_leaf, hr = er && !kid.right_leaf;
     if (hl && hr) {
       int nearc = cl, farc = cr;
       if (tr < tl) {
