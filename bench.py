@@ -538,14 +538,25 @@ def plain_baseline(torch, ops, step, verts, tex, n_steps, rays, grads):
 # HBM bytes per k_trace launch on the default workload from two rocprofv3 --pmc
 # runs (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch, summed over one step's 11
 # traversal launches), each scaled by what the counter reads for a 1 GiB
-# device-to-device copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0), at the launch
-# shapes named here; another shape reports null.  Commands and per-launch table:
-# profiles/k_trace_gradient_step_512px_16spp.md.
+# device-to-device copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0), at every launch
+# shape the autotuner can pick (tune.SPACE); another shape reports null.
+# Commands and per-launch table: profiles/k_trace_gradient_step_512px_16spp.md.
 TRACE_TRAFFIC = [
-    {"shape": {"trace_blocks_per_cu": 16, "trace_stack_lds": 16}, "bytes_per_launch": 75304774},
-    {"shape": {"trace_blocks_per_cu": 16, "trace_stack_lds": 8}, "bytes_per_launch": 75868454},
-    {"shape": {"trace_blocks_per_cu": 8, "trace_stack_lds": 16}, "bytes_per_launch": 75896143},
-    {"shape": {"trace_blocks_per_cu": 8, "trace_stack_lds": 8}, "bytes_per_launch": 76040765},
+    {"shape": {"trace_blocks_per_cu": 2, "trace_stack_lds": 8}, "bytes_per_launch": 76078333},
+    {"shape": {"trace_blocks_per_cu": 2, "trace_stack_lds": 16}, "bytes_per_launch": 75866333},
+    {"shape": {"trace_blocks_per_cu": 2, "trace_stack_lds": 32}, "bytes_per_launch": 75945836},
+    {"shape": {"trace_blocks_per_cu": 4, "trace_stack_lds": 8}, "bytes_per_launch": 75486316},
+    {"shape": {"trace_blocks_per_cu": 4, "trace_stack_lds": 16}, "bytes_per_launch": 74916919},
+    {"shape": {"trace_blocks_per_cu": 4, "trace_stack_lds": 32}, "bytes_per_launch": 75034810},
+    {"shape": {"trace_blocks_per_cu": 8, "trace_stack_lds": 8}, "bytes_per_launch": 75108820},
+    {"shape": {"trace_blocks_per_cu": 8, "trace_stack_lds": 16}, "bytes_per_launch": 74892576},
+    {"shape": {"trace_blocks_per_cu": 8, "trace_stack_lds": 32}, "bytes_per_launch": 74995843},
+    {"shape": {"trace_blocks_per_cu": 16, "trace_stack_lds": 8}, "bytes_per_launch": 75082877},
+    {"shape": {"trace_blocks_per_cu": 16, "trace_stack_lds": 16}, "bytes_per_launch": 74857536},
+    {"shape": {"trace_blocks_per_cu": 16, "trace_stack_lds": 32}, "bytes_per_launch": 75008445},
+    {"shape": {"trace_blocks_per_cu": 32, "trace_stack_lds": 8}, "bytes_per_launch": 75659017},
+    {"shape": {"trace_blocks_per_cu": 32, "trace_stack_lds": 16}, "bytes_per_launch": 75488189},
+    {"shape": {"trace_blocks_per_cu": 32, "trace_stack_lds": 32}, "bytes_per_launch": 76134781},
 ]
 
 
