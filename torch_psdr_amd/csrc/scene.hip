@@ -94,12 +94,10 @@ __global__ void k_edge_geom(const float* __restrict__ V, const int* __restrict__
 // --- discrete sampler tables ------------------------------------------------
 // One block per segment [seg_begin, seg_begin + seg_count) of w.  tmp holds two
 // floats per chunk per segment (at 2*seg_begin/SCAN_CHUNK + ...).
-__global__ void k_sampler(const float* __restrict__ w, const int* __restrict__ seg_begin,
-                          const int* __restrict__ seg_count, int n_static, float* __restrict__ pmf,
-                          float* __restrict__ cmf, float* __restrict__ totals, float* __restrict__ tmp) {
-  const int sgi = blockIdx.x;
-  const int b = seg_begin ? seg_begin[sgi] : 0;
-  const int n = seg_count ? seg_count[sgi] : n_static;
+// The tables of segment sgi ([b, b + n) of w) by the calling block; every
+// thread of the block calls it (it holds barriers).
+__device__ void sampler_segment(const float* __restrict__ w, int b, int n, int sgi, float* __restrict__ pmf,
+                                float* __restrict__ cmf, float* __restrict__ totals, float* __restrict__ tmp) {
   const int nchunk = (n + SCAN_CHUNK - 1) / SCAN_CHUNK;
   float* csum = tmp + 2 * ((b + SCAN_CHUNK - 1) / SCAN_CHUNK + sgi);
   float* coff = csum + nchunk;
@@ -155,6 +153,14 @@ __global__ void k_sampler(const float* __restrict__ w, const int* __restrict__ s
       cmf[b + i] = off + s;
     }
   }
+  __syncthreads();  // the caller may reuse tmp and s_total for another segment
+}
+__global__ void k_sampler(const float* __restrict__ w, const int* __restrict__ seg_begin,
+                          const int* __restrict__ seg_count, int n_static, float* __restrict__ pmf,
+                          float* __restrict__ cmf, float* __restrict__ totals, float* __restrict__ tmp) {
+  const int sgi = blockIdx.x;
+  sampler_segment(w, seg_begin ? seg_begin[sgi] : 0, seg_count ? seg_count[sgi] : n_static, sgi, pmf, cmf, totals,
+                  tmp);
 }
 
 // Three-level fixed order for long segments: chunks of 256 (sequential), groups
@@ -271,6 +277,43 @@ __global__ void k_light_power(const ObjectK* __restrict__ objs, int n_objects, c
   obj_w[o] = w;
 }
 
+// The light tables of a scene with few lights in one workgroup, in the order
+// of the separate launches (k_light_areas, per-light k_sampler, k_light_power,
+// the object k_sampler): same values, four launches fewer on the path every
+// renderD step starts with.  w: light-triangle areas, then object weights.
+__global__ void k_light_tables(const float4* __restrict__ geom, const LightK* __restrict__ lights, int n_lights,
+                               const ObjectK* __restrict__ objs, int n_objects, const float* __restrict__ emit,
+                               float* __restrict__ w, float* __restrict__ ltri_pmf, float* __restrict__ ltri_cmf,
+                               float* __restrict__ light_area, float* __restrict__ light_power,
+                               float* __restrict__ obj_pmf, float* __restrict__ obj_cmf,
+                               float* __restrict__ lights_total, float* __restrict__ tmp) {
+  for (int l = 0; l < n_lights; ++l) {
+    const LightK L = lights[l];
+    for (int k = threadIdx.x; k < L.tri_count; k += blockDim.x) w[L.pmf_off + k] = geom[5 * (L.tri_begin + k)].w;
+  }
+  __syncthreads();
+  for (int l = 0; l < n_lights; ++l) {
+    const LightK L = lights[l];
+    sampler_segment(w, L.pmf_off, L.tri_count, l, ltri_pmf, ltri_cmf, light_area, tmp);
+  }
+  for (int o = threadIdx.x; o < n_objects; o += blockDim.x) {
+    const int l = objs[o].light;
+    float v = 0.f;
+    if (l >= 0) {
+      if (objs[o].mat == PSDR_MAT_ENV_MAP || objs[o].mat == PSDR_MAT_ENV_SH9) {
+        v = REAL_4PI;
+      } else {
+        const float lum = (emit[3 * l] * 0.2126f + emit[3 * l + 1] * 0.7152f) + emit[3 * l + 2] * 0.0722f;
+        v = (REAL_2PI * lum) * light_area[l];
+      }
+      light_power[l] = v;
+    }
+    w[o] = v;
+  }
+  __syncthreads();
+  sampler_segment(w, 0, n_objects, 0, obj_pmf, obj_cmf, lights_total, tmp);
+}
+
 // --- LBVH --------------------------------------------------------------------
 __device__ __forceinline__ void tri_box(const float4* geom, int f, float lo[3], float hi[3]) {
   const float4 a = geom[5 * f], b = geom[5 * f + 1], c = geom[5 * f + 2];
@@ -339,10 +382,9 @@ __device__ __forceinline__ uint32_t expand_bits(uint32_t v) {
   return v;
 }
 
-__global__ void k_morton(const float4* __restrict__ geom, int n, const float* __restrict__ bounds,
-                         uint32_t* __restrict__ morton, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
-  const int f = blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= n) return;
+__device__ __forceinline__ void morton_one(const float4* __restrict__ geom, int f, const float* bounds,
+                                           uint32_t* __restrict__ morton, uint32_t* __restrict__ keys,
+                                           uint32_t* __restrict__ vals) {
   float c[3];
   tri_center(geom, f, c);
   uint32_t q[3];
@@ -367,6 +409,11 @@ __global__ void k_morton(const float4* __restrict__ geom, int n, const float* __
   morton[f] = m;
   keys[f] = m;
   vals[f] = static_cast<uint32_t>(f);
+}
+__global__ void k_morton(const float4* __restrict__ geom, int n, const float* __restrict__ bounds,
+                         uint32_t* __restrict__ morton, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f < n) morton_one(geom, f, bounds, morton, keys, vals);
 }
 
 __global__ __launch_bounds__(SORT_THREADS) void k_radix_hist(const uint32_t* __restrict__ keys, int n, int shift,
@@ -644,8 +691,17 @@ int scene_derive(psdr_scene* s, hipStream_t st) {
   const size_t wmax = std::max<size_t>({static_cast<size_t>(s->light_tri_total), static_cast<size_t>(ne),
                                         static_cast<size_t>(nobj), 1});
   float* tmp = wbuf + wmax;
+  // few lights: the light tables in one workgroup
+  const bool light_fused = nl > 0 && nl <= 16 && s->light_tri_total <= 16384 && nobj <= 16384;
+  if (light_fused) {
+    hipLaunchKernelGGL(k_light_tables, dim3(1), dim3(256), 0, st, s->tri_geom.as<float4>(), s->lights.as<LightK>(),
+                       nl, s->objs_dev, nobj, s->light_emit, wbuf, s->light_tri_pmf.as<float>(),
+                       s->light_tri_cmf.as<float>(), s->light_area.as<float>(), s->light_power.as<float>(),
+                       s->obj_pmf.as<float>(), s->obj_cmf.as<float>(), s->lights_total.as<float>(), tmp);
+    PSDR_HIP_CHECK(hipGetLastError());
+  }
   // lights: areas -> per-light tri samplers and total areas
-  if (nl > 0) {
+  if (nl > 0 && !light_fused) {
     hipLaunchKernelGGL(k_light_areas, dim3(nl), dim3(256), 0, st, s->tri_geom.as<float4>(),
                        s->lights.as<LightK>(), nl, wbuf, segi, segi + nl);
     PSDR_HIP_CHECK(hipGetLastError());
@@ -654,7 +710,7 @@ int scene_derive(psdr_scene* s, hipStream_t st) {
     if (rc) return rc;
   }
   // light power + light pick over objects
-  if (nobj > 0) {
+  if (nobj > 0 && !light_fused) {
     hipLaunchKernelGGL(k_light_power, dim3(blocks(nobj, 64)), dim3(64), 0, st, s->objs_dev, nobj,
                        s->light_emit, s->light_area.as<float>(), s->light_power.as<float>(), wbuf);
     PSDR_HIP_CHECK(hipGetLastError());
@@ -783,14 +839,14 @@ int build_lbvh(psdr_scene* s, hipStream_t st) {
     }
     return PSDR_OK;
   }
-  const int nb = std::min(blocks(n, 256), 1024);
-  float* part = s->bounds.as<float>() + 8;
-  hipLaunchKernelGGL(k_center_bounds, dim3(nb), dim3(256), 0, st, s->tri_geom.as<float4>(), n, part);
-  hipLaunchKernelGGL(k_center_bounds2, dim3(1), dim3(64), 0, st, part, nb, s->bounds.as<float>());
   uint32_t* k0 = s->sort_keys.as<uint32_t>();
   uint32_t* k1 = k0 + n;
   uint32_t* v0 = s->sort_vals.as<uint32_t>();
   uint32_t* v1 = v0 + n;
+  const int nb = std::min(blocks(n, 256), 1024);
+  float* part = s->bounds.as<float>() + 8;
+  hipLaunchKernelGGL(k_center_bounds, dim3(nb), dim3(256), 0, st, s->tri_geom.as<float4>(), n, part);
+  hipLaunchKernelGGL(k_center_bounds2, dim3(1), dim3(64), 0, st, part, nb, s->bounds.as<float>());
   hipLaunchKernelGGL(k_morton, dim3(blocks(n, 256)), dim3(256), 0, st, s->tri_geom.as<float4>(), n,
                      s->bounds.as<float>(), s->morton.as<uint32_t>(), k0, v0);
   PSDR_HIP_CHECK(hipGetLastError());
