@@ -364,10 +364,10 @@ int psdr_trace_counters(int32_t enable, int64_t* counts, int32_t n_counts);
  * the shapes: images are bit-identical, gradients differ only by the order of
  * their float atomics. */
 typedef struct psdr_tuning {
-  int32_t trace_blocks_per_cu;   /* closest-hit kernel grid, blocks of 256 per CU, 1..64 (default 8) */
+  int32_t trace_blocks_per_cu;   /* closest-hit kernel grid, blocks of 256 per CU, 1..64 (default 16) */
   int32_t trace_stack_lds;       /* traversal stack entries per lane kept in LDS: 8, 16 or 32 (default 16) */
   int32_t shade_blocks_per_cu;   /* wavefront shading kernel grid, 1..64 (default 8) */
-  int32_t adjoint_blocks_per_cu; /* path-replay adjoint kernel grid, 1..64 (default 16) */
+  int32_t adjoint_blocks_per_cu; /* path-replay adjoint kernel grid, 1..64 (default 4) */
   int32_t direct_blocks_per_cu;  /* direct-boundary kernel grid, 1..64 (default 8) */
   int32_t tile_lanes;            /* lanes per wavefront tile of a forward render: 0 = all lanes at once
                                     (default), else >= PSDR_MIN_TILE_LANES; the queue, ray and hit buffers

@@ -65,7 +65,9 @@ def test_autotune_picks_from_the_space_and_applies_it(gpu_device, restore_tuning
         img = integ.renderD(scene)[0]
         (img ** 2).sum().backward()
 
-    space = {"trace_blocks_per_cu": (4, 8), "trace_stack_lds": (8, 16), "shade_blocks_per_cu": (4, 8)}
+    # each field's space holds its default (the start of the sweep)
+    space = {"trace_blocks_per_cu": (8, 16), "trace_stack_lds": (8, 16), "shade_blocks_per_cu": (4, 8)}
+    assert all(tune.DEFAULTS[k] in vals for k, vals in space.items())
     res = tune.autotune(step, space=space, repeats=2, rounds=1)
     best = res["tuning"]
     assert tune.get_tuning() == best

@@ -9,7 +9,7 @@ namespace psdr {
 // Launch shapes chosen at run time (psdr_set_tuning), process-wide: grids in
 // blocks per CU, and the traversal stack entries kept in LDS per lane.
 struct Tuning {
-  std::atomic<int> trace_bpc{8}, trace_stack{16}, shade_bpc{8}, adjoint_bpc{16}, direct_bpc{8};
+  std::atomic<int> trace_bpc{16}, trace_stack{16}, shade_bpc{8}, adjoint_bpc{4}, direct_bpc{8};
   std::atomic<int> tile_lanes{0};  // lanes per wavefront tile of a forward render, 0 = all
 };
 Tuning& tuning();

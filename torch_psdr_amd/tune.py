@@ -29,7 +29,7 @@ from . import _native as N
 FIELDS = ("trace_blocks_per_cu", "trace_stack_lds", "shade_blocks_per_cu", "adjoint_blocks_per_cu",
           "direct_blocks_per_cu", "tile_lanes")
 
-DEFAULTS = {"trace_blocks_per_cu": 8, "trace_stack_lds": 16, "shade_blocks_per_cu": 8, "adjoint_blocks_per_cu": 16,
+DEFAULTS = {"trace_blocks_per_cu": 16, "trace_stack_lds": 16, "shade_blocks_per_cu": 8, "adjoint_blocks_per_cu": 4,
             "direct_blocks_per_cu": 8, "tile_lanes": 0}
 
 # candidate values per parameter, swept in this order (the traversal first: it
