@@ -7,7 +7,7 @@
 //   k_shade(d) : resolve depth d-1 (NEE + BSDF-MIS contributions, throughput
 //                update from the extension hit) and generate depth d's rays
 //                (light-sample, BSDF-sample and extension rays); paths that stay
-//                alive are compacted into the next queue with one atomic per wave
+//                alive are compacted into the next queue with one atomic per workgroup
 //   k_film     : per-pixel sum over the pixel's spp lanes in lane order, / spp
 // The last depth draws its extension sample (the stream advances exactly as in
 // the reference) but traces no extension ray: its hit cannot change L.

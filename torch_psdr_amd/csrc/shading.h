@@ -290,25 +290,12 @@ __device__ __forceinline__ float mis_weight(float w1, float w2, uint32_t n1, uin
   return sa / (sa + sb);
 }
 
-// Wave-aggregated slot allocation (all lanes of the wave must call it).
-__device__ __forceinline__ int wave_alloc(bool want, int* counter) {
-  const uint64_t m = __ballot(want);
-  const int lane = __lane_id();
-  int base = 0;
-  if (m != 0ull) {
-    const int leader = __ffsll(static_cast<long long>(m)) - 1;
-    if (lane == leader) base = atomicAdd(counter, __popcll(m));
-    base = __shfl(base, leader);
-  }
-  const uint64_t lt = lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
-  return base + __popcll(m & lt);
-}
-
-// Workgroup-aggregated slot allocation: one atomic per workgroup instead of
-// one per wave.  Atomic adds to one address are serialised by the memory
-// system (about 11 ns each on MI355X, measured), so a queue filled by tens of
-// thousands of waves spends most of its time there.  Every thread of the
-// block must call it (it holds three barriers); lds: (blockDim / 64 + 1) ints.
+// Workgroup-aggregated slot allocation: one atomic per workgroup (ballot
+// counts per wave, their prefix in LDS).  Atomic adds to one address are
+// serialised by the memory system (about 11 ns each on MI355X, measured), so
+// a queue filled with one atomic per wave by tens of thousands of waves spends
+// most of its time there.  Every thread of the block must call it (it holds
+// three barriers); lds: (blockDim / 64 + 1) ints.
 // Slots are dense: wave w's lanes follow those of waves 0 .. w-1.
 __device__ __forceinline__ int block_alloc(bool want, int* counter, int* lds) {
   const uint64_t m = __ballot(want);
