@@ -13,6 +13,7 @@ import torch
 import torch_psdr_amd as dr
 import torch_psdr_amd._native as N
 from torch_psdr_amd import ops
+from torch_psdr_amd import workloads as W
 from tests import helpers as H
 
 pytestmark = pytest.mark.gpu
@@ -285,3 +286,32 @@ def test_cameras_in_one_wavefront_match_single_renders(gpu_device, oracle, tile)
         _compare_images(together[2].cpu().numpy(), ref, "ortho camera in a 4-camera wavefront")
     finally:
         tune.set_tuning(prev)
+
+
+def test_forward_with_more_lights_than_the_shading_lds_tables_hold(gpu_device, oracle):
+    """Ten area lights (20 emitting triangles): more lights than k_shade stages
+    in LDS and more emitting triangles than BSDF-sampled rays test before
+    traversing, so the shading kernel reads the scene tables from global
+    memory and every BSDF-sampled ray traverses; lanes and image equal the
+    oracle's as in the other cases."""
+    meshes, lights, _ = H.objects(gpu_device, level=2)
+    for k in range(9):
+        Vl, Fl = W.quad(0.3, 3.0 + 0.1 * k, facing_down=True)
+        Vl = Vl.copy()
+        Vl[:, 0] += -2.0 + 0.5 * k
+        Vl[:, 2] += 0.25 * (k % 3)
+        lights.append(dr.AreaLight(torch.tensor(Vl, device=gpu_device), torch.tensor(Fl, device=gpu_device),
+                                   torch.tensor([2.0 + k, 1.0 + 0.5 * k, 3.0], device=gpu_device)))
+    cam = H.camera(gpu_device, 24)
+    scene = dr.Scene([cam], meshes, lights)
+    integ = dr.PathIntegrator(spp_interior=2, max_bounce=3)
+    img = integ.renderC(scene)[0]
+    ns, flat = _native(scene)
+    N_ = integ.cfg[1] * 24 ** 2
+    lane = torch.empty((N_, 3), device=gpu_device)
+    N.check(ns.lib.psdr_last_lane_radiance(ns.handle, lane.data_ptr(), _stream()))
+    torch.cuda.synchronize()
+    w_img, w_lane = oracle.render(oracle.Scene(flat), integ.cfg, scene.cameras[0].params(), mode=0, lane_L=True)
+    _compare_images(lane.cpu().numpy(), w_lane, "ten lights lanes")
+    _compare_images(img.cpu().numpy(), w_img, "ten lights image")
+    assert np.isfinite(img.cpu().numpy()).all() and img.abs().sum() > 0
